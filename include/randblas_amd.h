@@ -1,0 +1,293 @@
+/*
+ * randblas_amd.h -- C ABI of the MI355X (gfx950) sketching library.
+ *
+ * This is the drop-in boundary for RandBLAS's sketching hot path.  RandBLAS
+ * itself is a header-only C++ template library whose boundary is
+ * namespace RandBLAS (RandBLAS.hh:33-41); every entry point below replaces one
+ * template instantiation, named with the overload-free convention
+ * [s|d][l|r]skge[3|s] / [s|d][l|r]sksp3 described in rtd/source/FAQ.rst:154-186.
+ * A header-only C++ layer with the reference's own type and function names
+ * (randblas_amd/cpp/RandBLAS.hh) forwards to these functions.
+ *
+ * Conventions
+ *   - All matrix / array pointers are DEVICE pointers (hipMalloc'd or torch
+ *     tensors on the GPU).  Descriptors (rbx_*_dist, rbx_*_skop, rbx_rng_state)
+ *     are HOST structs passed by pointer.
+ *   - layout is 'C' (blas::Layout::ColMajor) or 'R' (RowMajor);
+ *     op is 'N' (blas::Op::NoTrans) or 'T' (Trans).
+ *   - stream is a hipStream_t passed as void*; NULL = the default stream.
+ *     Every call is stream-ordered and asynchronous unless stated otherwise.
+ *   - No exceptions cross the ABI: every function returns an rbx_status and
+ *     leaves a description in a thread-local slot read by
+ *     rbx_last_error_message().  This replaces RandBLAS::Error and
+ *     std::overflow_error (RandBLAS/exceptions.hh:57-95,152-162,
+ *     RandBLAS/base.hh:260-274).
+ *   - alpha == 0 leaves A unread; beta == 0 leaves B unread (BLAS semantics,
+ *     RandBLAS/util.hh:55-62).
+ *   - Results do not depend on the launch configuration: every operator entry
+ *     is a pure function of (key, counter, row, column).
+ */
+#ifndef RANDBLAS_AMD_H
+#define RANDBLAS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RBX_OK = 0,
+    RBX_INVALID_ARG = 1,  /* randblas_require / randblas_error_if failures */
+    RBX_OVERFLOW = 2,     /* safe_int_product overflow (base.hh:260-274)   */
+    RBX_HIP_ERROR = 3,    /* a HIP runtime call failed                      */
+    RBX_UNSUPPORTED = 4,  /* valid in the reference, not offered here       */
+    RBX_OUT_OF_MEMORY = 5 /* device allocation failed                       */
+} rbx_status;
+
+/* RNGState<Philox4x32> (RandBLAS/base.hh:64-164): a 128-bit counter in four
+ * 32-bit words (word 0 least significant) and a 64-bit key in two words. */
+typedef struct {
+    uint32_t counter[4];
+    uint32_t key[2];
+} rbx_rng_state;
+
+/* DenseDist (RandBLAS/dense_skops.hh:231-350).  family 'G' (Gaussian) or
+ * 'U' (uniform on [-sqrt3, sqrt3]); major_axis 'L' (Long) or 'S' (Short). */
+typedef struct {
+    int64_t n_rows;
+    int64_t n_cols;
+    char family;
+    char major_axis;
+} rbx_dense_dist;
+
+/* DenseSkOp (RandBLAS/dense_skops.hh:357-478).  buff == NULL means the operator
+ * is never materialised: kernels regenerate its entries from (key, counter)
+ * inside the GEMM tiles.  A non-NULL buff is a device buffer in `layout`
+ * holding the full n_rows x n_cols operator. */
+typedef struct {
+    rbx_dense_dist dist;
+    rbx_rng_state seed_state;
+    rbx_rng_state next_state;
+    void *buff;
+    char layout;
+    int own_memory; /* 1: buff was allocated by rbx_[sd]fill_dense_skop */
+} rbx_dense_skop;
+
+/* SparseDist (RandBLAS/sparse_skops.hh:131-246).  major_axis 'S' (SASO) or
+ * 'L' (LASO). */
+typedef struct {
+    int64_t n_rows;
+    int64_t n_cols;
+    int64_t vec_nnz;
+    char major_axis;
+} rbx_sparse_dist;
+
+/* SparseSkOp<T, Philox4x32, int64_t> (RandBLAS/sparse_skops.hh:289-450).
+ * nnz < 0 means "not sampled yet": the sketch entry points then sample it
+ * into library scratch for the duration of the call (skge.hh:483-487).
+ * rows / cols / vals are device arrays of length >= dist.vec_nnz * dim_minor. */
+typedef struct {
+    rbx_sparse_dist dist;
+    rbx_rng_state seed_state;
+    rbx_rng_state next_state;
+    int64_t nnz;
+    void *vals;
+    int64_t *rows;
+    int64_t *cols;
+    int own_memory; /* 1: arrays were allocated by rbx_[sd]fill_sparse_skop */
+} rbx_sparse_skop;
+
+/* CSRMatrix<T, int64_t> (RandBLAS/sparse_data/csr_matrix.hh:73-245), zero-based,
+ * device arrays. */
+typedef struct {
+    int64_t n_rows;
+    int64_t n_cols;
+    int64_t nnz;
+    void *vals;
+    int64_t *rowptr;
+    int64_t *colidxs;
+} rbx_csr_matrix;
+
+/* ---------------------------------------------------------------- errors */
+const char *rbx_last_error_message(void);
+/* Version string of this library. */
+const char *rbx_version(void);
+
+/* ---------------------------------------------------------------- RNG
+ * Raw Philox4x32-10 words for counters state.counter + i, i in [0, n_counters):
+ * out[4*i + w] = word w.  Replaces r123::Philox4x32::operator() as called at
+ * RandBLAS/random_gen.hh:107,135. */
+rbx_status rbx_philox4x32_10(const rbx_rng_state *state, int64_t n_counters, uint32_t *out,
+                             void *stream);
+
+/* Host-side helpers (synchronous, no device work).
+ * RNGState(uint64 k) (base.hh:119): counter 0, key {k mod 2^32, k >> 32}. */
+rbx_status rbx_rng_state_from_key(uint64_t key, rbx_rng_state *out);
+/* counter.incr(n) (Random123 r123array::incr; semantics pinned by
+ * test/test_basic_rng/test_r123.cc:706-797). */
+rbx_status rbx_rng_state_incr(rbx_rng_state *state, uint64_t n);
+
+/* ---------------------------------------------------------------- dense operators
+ * DenseSkOp(dist, seed_state) (dense_skops.hh:396-406): fills in next_state via
+ * dense::compute_next_state (dense_skops.hh:172-185); buff = NULL. */
+rbx_status rbx_dense_skop_init(const rbx_dense_dist *dist, const rbx_rng_state *seed,
+                               rbx_dense_skop *out);
+/* dim_major / dim_minor / natural layout / isometry scale of a DenseDist
+ * (dense_skops.hh:313-329, 187-199). */
+rbx_status rbx_dense_dist_props(const rbx_dense_dist *dist, int64_t *dim_major,
+                                int64_t *dim_minor, char *natural_layout,
+                                double *isometry_scale);
+
+/* fill_dense_unpacked (dense_skops.hh:563-606): writes the n_rows x n_cols
+ * submatrix at (ro_s, co_s) of an operator drawn from D into buff (compact,
+ * in `layout`) and returns the state the reference returns. */
+rbx_status rbx_sfill_dense_unpacked(char layout, const rbx_dense_dist *D, int64_t n_rows,
+                                    int64_t n_cols, int64_t ro_s, int64_t co_s, float *buff,
+                                    const rbx_rng_state *seed, rbx_rng_state *next_state,
+                                    void *stream);
+rbx_status rbx_dfill_dense_unpacked(char layout, const rbx_dense_dist *D, int64_t n_rows,
+                                    int64_t n_cols, int64_t ro_s, int64_t co_s, double *buff,
+                                    const rbx_rng_state *seed, rbx_rng_state *next_state,
+                                    void *stream);
+/* fill_dense(S) (dense_skops.hh:649-658): allocates S->buff (device) if it is
+ * NULL and marks the operator as owning it; fills in S->layout. */
+rbx_status rbx_sfill_dense_skop(rbx_dense_skop *S, void *stream);
+rbx_status rbx_dfill_dense_skop(rbx_dense_skop *S, void *stream);
+/* ~DenseSkOp (dense_skops.hh:472-477): frees buff if owned. */
+rbx_status rbx_dense_skop_free(rbx_dense_skop *S);
+
+/* ---------------------------------------------------------------- dense sketches
+ * lskge3 (skge.hh:154-202):  B = alpha * op(submat(S)) * op(A) + beta * B,
+ *   B is d x n, op(A) is m x n, op(submat(S)) is d x m taken at (ro_s, co_s).
+ * rskge3 (skge.hh:307-355):  B = alpha * op(A) * op(submat(S)) + beta * B,
+ *   B is m x d, op(A) is m x n, op(submat(S)) is n x d.
+ * With S->buff == NULL the operator is generated inside the GEMM tiles and
+ * never written to memory (the reference materialises it, skge.hh:175-181). */
+rbx_status rbx_slskge3(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m,
+                       float alpha, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
+                       const float *A, int64_t lda, float beta, float *B, int64_t ldb,
+                       void *stream);
+rbx_status rbx_dlskge3(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m,
+                       double alpha, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
+                       const double *A, int64_t lda, double beta, double *B, int64_t ldb,
+                       void *stream);
+rbx_status rbx_srskge3(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                       float alpha, const float *A, int64_t lda, const rbx_dense_skop *S,
+                       int64_t ro_s, int64_t co_s, float beta, float *B, int64_t ldb,
+                       void *stream);
+rbx_status rbx_drskge3(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                       double alpha, const double *A, int64_t lda, const rbx_dense_skop *S,
+                       int64_t ro_s, int64_t co_s, double beta, double *B, int64_t ldb,
+                       void *stream);
+
+/* Adjoints (no reference counterpart as a separate routine; the reference
+ * obtains them as lskge3/rskge3 with the operator transposed, skge.hh:195-200).
+ *   [sd]lskge3_adjoint: Abar = alpha * op(submat(S))^T * Bbar + beta * Abar,
+ *     Bbar is d x n, Abar is m x n (the gradient of B = op(S)*A with respect to A).
+ *   [sd]rskge3_adjoint: Abar = alpha * Bbar * op(submat(S))^T + beta * Abar,
+ *     Bbar is m x d, Abar is m x n.
+ * S is regenerated from its counters; nothing is stored between the forward
+ * and the adjoint. */
+rbx_status rbx_slskge3_adjoint(char layout, char opS, int64_t d, int64_t n, int64_t m,
+                               float alpha, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
+                               const float *Bbar, int64_t ldbb, float beta, float *Abar,
+                               int64_t ldab, void *stream);
+rbx_status rbx_dlskge3_adjoint(char layout, char opS, int64_t d, int64_t n, int64_t m,
+                               double alpha, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
+                               const double *Bbar, int64_t ldbb, double beta, double *Abar,
+                               int64_t ldab, void *stream);
+rbx_status rbx_srskge3_adjoint(char layout, char opS, int64_t m, int64_t d, int64_t n,
+                               float alpha, const float *Bbar, int64_t ldbb,
+                               const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, float beta,
+                               float *Abar, int64_t ldab, void *stream);
+rbx_status rbx_drskge3_adjoint(char layout, char opS, int64_t m, int64_t d, int64_t n,
+                               double alpha, const double *Bbar, int64_t ldbb,
+                               const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, double beta,
+                               double *Abar, int64_t ldab, void *stream);
+
+/* BF16-split form of slskge3 (no reference counterpart).  Computes
+ *   B = alpha * op(bf16(submat(S))) * op(A) + beta * B
+ * on the BF16 matrix cores: every entry of S is rounded to bfloat16
+ * (round-to-nearest-even) and A is split into n_terms bfloat16 terms
+ * (A = A_1 + ... + A_n_terms + residual, n_terms in {1,2,3}); products
+ * accumulate in FP32 and are flushed to FP64 every K-chunk.  The operator
+ * is a valid sketch in its own right (a rounded Gaussian); its distortion
+ * and its error against the FP64 product are reported by the bench.
+ * Only S->buff == NULL (generated operators) is supported. */
+rbx_status rbx_slskge3_bf16split(char layout, char opS, char opA, int64_t d, int64_t n,
+                                 int64_t m, float alpha, const rbx_dense_skop *S, int64_t ro_s,
+                                 int64_t co_s, const float *A, int64_t lda, float beta,
+                                 float *B, int64_t ldb, int n_terms, void *stream);
+
+/* ---------------------------------------------------------------- sparse operators
+ * SparseSkOp(dist, seed_state) (sparse_skops.hh:356-365) with next_state from
+ * compute_next_state (sparse_skops.hh:266-283); nnz = -1, arrays NULL. */
+rbx_status rbx_sparse_skop_init(const rbx_sparse_dist *dist, const rbx_rng_state *seed,
+                                rbx_sparse_skop *out);
+/* dim_major / dim_minor / full_nnz / isometry scale (sparse_skops.hh:108-114,
+ * 197-223). */
+rbx_status rbx_sparse_dist_props(const rbx_sparse_dist *dist, int64_t *dim_major,
+                                 int64_t *dim_minor, int64_t *full_nnz,
+                                 double *isometry_scale);
+/* fill_sparse_unpacked_nosub (sparse_skops.hh:515-565): writes nnz entries into
+ * the device arrays (capacity full_nnz) and returns the next state.  *nnz is
+ * written on the host, so this call synchronises `stream`.  SASO index arrays
+ * match the reference element for element; LASO vectors that contain
+ * duplicates are merged to sqrt(count)*first_sign in first-occurrence order
+ * (the reference's order there is that of an std::unordered_map). */
+rbx_status rbx_sfill_sparse_unpacked_nosub(const rbx_sparse_dist *D, int64_t *nnz, float *vals,
+                                           int64_t *rows, int64_t *cols,
+                                           const rbx_rng_state *seed, rbx_rng_state *next_state,
+                                           void *stream);
+rbx_status rbx_dfill_sparse_unpacked_nosub(const rbx_sparse_dist *D, int64_t *nnz, double *vals,
+                                           int64_t *rows, int64_t *cols,
+                                           const rbx_rng_state *seed, rbx_rng_state *next_state,
+                                           void *stream);
+/* fill_sparse(S) (sparse_skops.hh:587-603): allocates the arrays if NULL. */
+rbx_status rbx_sfill_sparse_skop(rbx_sparse_skop *S, void *stream);
+rbx_status rbx_dfill_sparse_skop(rbx_sparse_skop *S, void *stream);
+/* ~SparseSkOp (sparse_skops.hh:442-449) */
+rbx_status rbx_sparse_skop_free(rbx_sparse_skop *S);
+
+/* ---------------------------------------------------------------- sparse sketches
+ * lskges (skge.hh:465-492) and rskges (skge.hh:598-626): as lskge3/rskge3 with a
+ * SparseSkOp.  Each output entry is accumulated in the reference's order
+ * (ascending column of the operator, csc_spmm_impl.hh:74-96), so results with
+ * +-1 operator values are bit-identical to the reference's fallback kernels. */
+rbx_status rbx_slskges(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m,
+                       float alpha, const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s,
+                       const float *A, int64_t lda, float beta, float *B, int64_t ldb,
+                       void *stream);
+rbx_status rbx_dlskges(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m,
+                       double alpha, const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s,
+                       const double *A, int64_t lda, double beta, double *B, int64_t ldb,
+                       void *stream);
+rbx_status rbx_srskges(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                       float alpha, const float *A, int64_t lda, const rbx_sparse_skop *S,
+                       int64_t ro_s, int64_t co_s, float beta, float *B, int64_t ldb,
+                       void *stream);
+rbx_status rbx_drskges(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                       double alpha, const double *A, int64_t lda, const rbx_sparse_skop *S,
+                       int64_t ro_s, int64_t co_s, double beta, double *B, int64_t ldb,
+                       void *stream);
+
+/* ---------------------------------------------------------------- sparse data
+ * rsksp3 (sparse_data/sksp.hh:277-326): B = alpha * op(A) * op(submat(S)) + beta * B
+ * with A a CSR matrix (opA = 'N', ro_a = co_a = 0) and S a DenseSkOp.  A
+ * generated operator is materialised once into library scratch (as the
+ * reference does, sksp.hh:298-305) and the CSR product accumulates each output
+ * entry in the reference's order (csr_spmm_impl.hh:51-119). */
+rbx_status rbx_srsksp3(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                       float alpha, const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a,
+                       const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, float beta,
+                       float *B, int64_t ldb, void *stream);
+rbx_status rbx_drsksp3(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                       double alpha, const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a,
+                       const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, double beta,
+                       double *B, int64_t ldb, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RANDBLAS_AMD_H */

@@ -1,0 +1,588 @@
+"""randblas_amd -- MI355X (gfx950) sketching hot path of RandBLAS.
+
+Python mirror of the reference's C++ API over the C ABI in
+include/randblas_amd.h (librandblas_amd.so, loaded with ctypes).  Names,
+argument order and error behaviour follow namespace RandBLAS:
+
+    RNGState, DenseDist, DenseSkOp, SparseDist, SparseSkOp, ScalarDist, Axis,
+    fill_dense, fill_dense_unpacked, fill_sparse, lskge3, rskge3, lskges,
+    rskges, rsksp3, sketch_general, sketch_vector, sketch_symmetric
+
+Matrices are torch tensors on the GPU, passed as flat buffers with explicit
+leading dimensions exactly like the reference's raw pointers; torch only
+provides device memory and the current HIP stream.  Errors raised by the
+library come back as randblas_amd.Error (RandBLAS::Error) or OverflowError
+(std::overflow_error).  There is no CPU fallback: if the shared library is
+missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "librandblas_amd.so")
+
+
+class Error(RuntimeError):
+    """RandBLAS::Error (RandBLAS/exceptions.hh:57-95)."""
+
+
+class Unsupported(Error):
+    pass
+
+
+# ---------------------------------------------------------------- enums
+class Layout(str, enum.Enum):
+    ColMajor = "C"
+    RowMajor = "R"
+
+
+class Op(str, enum.Enum):
+    NoTrans = "N"
+    Trans = "T"
+
+
+class ScalarDist(str, enum.Enum):
+    Gaussian = "G"
+    Uniform = "U"
+
+
+class Axis(str, enum.Enum):
+    Short = "S"
+    Long = "L"
+
+
+def _c(x) -> bytes:
+    if isinstance(x, enum.Enum):
+        x = x.value
+    return str(x).encode()
+
+
+def _flip(op) -> str:
+    return "T" if _c(op) == b"N" else "N"
+
+
+# ---------------------------------------------------------------- C structs
+class c_rng_state(C.Structure):
+    _fields_ = [("counter", C.c_uint32 * 4), ("key", C.c_uint32 * 2)]
+
+
+class c_dense_dist(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("n_cols", C.c_int64), ("family", C.c_char),
+                ("major_axis", C.c_char)]
+
+
+class c_dense_skop(C.Structure):
+    _fields_ = [("dist", c_dense_dist), ("seed_state", c_rng_state), ("next_state", c_rng_state),
+                ("buff", C.c_void_p), ("layout", C.c_char), ("own_memory", C.c_int)]
+
+
+class c_sparse_dist(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("n_cols", C.c_int64), ("vec_nnz", C.c_int64),
+                ("major_axis", C.c_char)]
+
+
+class c_sparse_skop(C.Structure):
+    _fields_ = [("dist", c_sparse_dist), ("seed_state", c_rng_state), ("next_state", c_rng_state),
+                ("nnz", C.c_int64), ("vals", C.c_void_p), ("rows", C.c_void_p),
+                ("cols", C.c_void_p), ("own_memory", C.c_int)]
+
+
+class c_csr_matrix(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("n_cols", C.c_int64), ("nnz", C.c_int64),
+                ("vals", C.c_void_p), ("rowptr", C.c_void_p), ("colidxs", C.c_void_p)]
+
+
+RBX_OK, RBX_INVALID_ARG, RBX_OVERFLOW, RBX_HIP_ERROR, RBX_UNSUPPORTED, RBX_OUT_OF_MEMORY = range(6)
+
+# symbol -> argtypes (restype is always rbx_status unless listed in _RESTYPE)
+_I64, _F, _D, _P, _CH = C.c_int64, C.c_float, C.c_double, C.c_void_p, C.c_char
+_RS, _DD, _DS = C.POINTER(c_rng_state), C.POINTER(c_dense_dist), C.POINTER(c_dense_skop)
+_SD, _SS, _CSR = C.POINTER(c_sparse_dist), C.POINTER(c_sparse_skop), C.POINTER(c_csr_matrix)
+_SIG = {
+    "rbx_last_error_message": [],
+    "rbx_version": [],
+    "rbx_philox4x32_10": [_RS, _I64, _P, _P],
+    "rbx_rng_state_from_key": [C.c_uint64, _RS],
+    "rbx_rng_state_incr": [_RS, C.c_uint64],
+    "rbx_dense_skop_init": [_DD, _RS, _DS],
+    "rbx_dense_dist_props": [_DD, C.POINTER(_I64), C.POINTER(_I64), C.c_char_p, C.POINTER(_D)],
+    "rbx_sfill_dense_unpacked": [_CH, _DD, _I64, _I64, _I64, _I64, _P, _RS, _RS, _P],
+    "rbx_dfill_dense_unpacked": [_CH, _DD, _I64, _I64, _I64, _I64, _P, _RS, _RS, _P],
+    "rbx_sfill_dense_skop": [_DS, _P],
+    "rbx_dfill_dense_skop": [_DS, _P],
+    "rbx_dense_skop_free": [_DS],
+    "rbx_slskge3": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _DS, _I64, _I64, _P, _I64, _F, _P, _I64, _P],
+    "rbx_dlskge3": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _DS, _I64, _I64, _P, _I64, _D, _P, _I64, _P],
+    "rbx_srskge3": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _P, _I64, _DS, _I64, _I64, _F, _P, _I64, _P],
+    "rbx_drskge3": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _P, _I64, _DS, _I64, _I64, _D, _P, _I64, _P],
+    "rbx_slskge3_adjoint": [_CH, _CH, _I64, _I64, _I64, _F, _DS, _I64, _I64, _P, _I64, _F, _P, _I64, _P],
+    "rbx_dlskge3_adjoint": [_CH, _CH, _I64, _I64, _I64, _D, _DS, _I64, _I64, _P, _I64, _D, _P, _I64, _P],
+    "rbx_srskge3_adjoint": [_CH, _CH, _I64, _I64, _I64, _F, _P, _I64, _DS, _I64, _I64, _F, _P, _I64, _P],
+    "rbx_drskge3_adjoint": [_CH, _CH, _I64, _I64, _I64, _D, _P, _I64, _DS, _I64, _I64, _D, _P, _I64, _P],
+    "rbx_slskge3_bf16split": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _DS, _I64, _I64, _P, _I64, _F, _P,
+                              _I64, C.c_int, _P],
+    "rbx_sparse_skop_init": [_SD, _RS, _SS],
+    "rbx_sparse_dist_props": [_SD, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_D)],
+    "rbx_sfill_sparse_unpacked_nosub": [_SD, C.POINTER(_I64), _P, _P, _P, _RS, _RS, _P],
+    "rbx_dfill_sparse_unpacked_nosub": [_SD, C.POINTER(_I64), _P, _P, _P, _RS, _RS, _P],
+    "rbx_sfill_sparse_skop": [_SS, _P],
+    "rbx_dfill_sparse_skop": [_SS, _P],
+    "rbx_sparse_skop_free": [_SS],
+    "rbx_slskges": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _SS, _I64, _I64, _P, _I64, _F, _P, _I64, _P],
+    "rbx_dlskges": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _SS, _I64, _I64, _P, _I64, _D, _P, _I64, _P],
+    "rbx_srskges": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _P, _I64, _SS, _I64, _I64, _F, _P, _I64, _P],
+    "rbx_drskges": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _P, _I64, _SS, _I64, _I64, _D, _P, _I64, _P],
+    "rbx_srsksp3": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _CSR, _I64, _I64, _DS, _I64, _I64, _F, _P,
+                    _I64, _P],
+    "rbx_drsksp3": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _CSR, _I64, _I64, _DS, _I64, _I64, _D, _P,
+                    _I64, _P],
+}
+_RESTYPE = {"rbx_last_error_message": C.c_char_p, "rbx_version": C.c_char_p}
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load librandblas_amd.so.  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise Error(f"{LIB_PATH} is missing: build it with `python -m randblas_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIG.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+def _call(name: str, *args) -> None:
+    rc = getattr(lib(), name)(*args)
+    if rc != RBX_OK:
+        msg = lib().rbx_last_error_message().decode()
+        if rc == RBX_OVERFLOW:
+            raise OverflowError(msg)
+        if rc == RBX_UNSUPPORTED:
+            raise Unsupported(msg)
+        raise Error(f"{name} -> status {rc}: {msg}")
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t) -> C.c_void_p:
+    if t is None:
+        return C.c_void_p(0)
+    if isinstance(t, int):
+        return C.c_void_p(t)
+    if not t.is_cuda:
+        raise Error("randblas_amd operates on GPU tensors (device pointers)")
+    return C.c_void_p(t.data_ptr())
+
+
+def _prefix(t) -> str:
+    import torch
+    if t.dtype == torch.float64:
+        return "d"
+    if t.dtype == torch.float32:
+        return "s"
+    raise Error(f"unsupported dtype {t.dtype}")
+
+
+# ---------------------------------------------------------------- RNGState
+@dataclass
+class RNGState:
+    """RNGState<Philox4x32> (RandBLAS/base.hh:64-164).  RNGState(k) sets the key
+    to {k mod 2^32, k >> 32} and the counter to zero."""
+    key: int = 0
+    counter: Sequence[int] = (0, 0, 0, 0)
+
+    def c(self) -> c_rng_state:
+        s = c_rng_state()
+        k = int(self.key)
+        s.key[0], s.key[1] = k & 0xFFFFFFFF, (k >> 32) & 0xFFFFFFFF
+        for i in range(4):
+            s.counter[i] = int(self.counter[i]) & 0xFFFFFFFF
+        return s
+
+    @staticmethod
+    def from_c(s: c_rng_state) -> "RNGState":
+        return RNGState(key=int(s.key[0]) | (int(s.key[1]) << 32), counter=tuple(int(x) for x in s.counter))
+
+    @property
+    def key_words(self):
+        k = int(self.key)
+        return (k & 0xFFFFFFFF, k >> 32)
+
+    def incr(self, n: int = 1) -> "RNGState":
+        s = self.c()
+        _call("rbx_rng_state_incr", C.byref(s), n)
+        return RNGState.from_c(s)
+
+    def counter_int(self) -> int:
+        return sum(int(w) << (32 * i) for i, w in enumerate(self.counter))
+
+
+def philox4x32_10(state: RNGState, n_counters: int, out) -> None:
+    """Raw words for counters state.counter + i into a uint32/int32 GPU tensor of 4*n words."""
+    assert out.numel() >= 4 * n_counters
+    s = state.c()
+    _call("rbx_philox4x32_10", C.byref(s), n_counters, _ptr(out), _stream())
+
+
+# ---------------------------------------------------------------- dense
+@dataclass
+class DenseDist:
+    """DenseDist (RandBLAS/dense_skops.hh:231-350)."""
+    n_rows: int
+    n_cols: int
+    family: ScalarDist = ScalarDist.Gaussian
+    major_axis: Axis = Axis.Long
+
+    def c(self) -> c_dense_dist:
+        return c_dense_dist(self.n_rows, self.n_cols, _c(self.family), _c(self.major_axis))
+
+    def _props(self):
+        dmaj, dmin, iso = C.c_int64(), C.c_int64(), C.c_double()
+        lay = C.create_string_buffer(1)
+        _call("rbx_dense_dist_props", C.byref(self.c()), C.byref(dmaj), C.byref(dmin), lay, C.byref(iso))
+        return dmaj.value, dmin.value, lay.raw.decode(), iso.value
+
+    @property
+    def dim_major(self):
+        return self._props()[0]
+
+    @property
+    def dim_minor(self):
+        return self._props()[1]
+
+    @property
+    def natural_layout(self) -> str:
+        return self._props()[2]
+
+    @property
+    def isometry_scale(self) -> float:
+        return self._props()[3]
+
+
+class DenseSkOp:
+    """DenseSkOp<T> (RandBLAS/dense_skops.hh:357-478).  buff is None until
+    fill_dense(S) materialises it; sketches with buff None generate the operator
+    inside the GEMM kernels."""
+
+    def __init__(self, dist: DenseDist, seed_state: RNGState, buff=None, layout: Optional[str] = None):
+        self.dist = dist
+        self.seed_state = seed_state
+        out = c_dense_skop()
+        _call("rbx_dense_skop_init", C.byref(dist.c()), C.byref(seed_state.c()), C.byref(out))
+        self.next_state = RNGState.from_c(out.next_state)
+        self.n_rows, self.n_cols = dist.n_rows, dist.n_cols
+        self.layout = layout if layout is not None else out.layout.decode()
+        self.buff = buff
+
+    def c(self) -> c_dense_skop:
+        s = c_dense_skop()
+        s.dist = self.dist.c()
+        s.seed_state = self.seed_state.c()
+        s.next_state = self.next_state.c()
+        s.buff = None if self.buff is None else self.buff.data_ptr()
+        s.layout = _c(self.layout)
+        s.own_memory = 0
+        return s
+
+
+def fill_dense_unpacked(layout, D: DenseDist, n_rows, n_cols, ro_s, co_s, buff, seed: RNGState) -> RNGState:
+    """fill_dense_unpacked (dense_skops.hh:563-606); buff is a GPU tensor."""
+    nxt = c_rng_state()
+    _call(f"rbx_{_prefix(buff)}fill_dense_unpacked", _c(layout), C.byref(D.c()), n_rows, n_cols, ro_s, co_s,
+          _ptr(buff), C.byref(seed.c()), C.byref(nxt), _stream())
+    return RNGState.from_c(nxt)
+
+
+def fill_dense(D_or_S, buff=None, seed: Optional[RNGState] = None, dtype=None):
+    """fill_dense(D, buff, seed) -> next state, or fill_dense(S) which attaches
+    a torch buffer to S in its natural layout (dense_skops.hh:623-658)."""
+    import torch
+    if isinstance(D_or_S, DenseSkOp):
+        S = D_or_S
+        if S.buff is None:
+            S.buff = torch.empty(S.n_rows * S.n_cols, dtype=dtype or torch.float64, device="cuda")
+        fill_dense_unpacked(S.layout, S.dist, S.n_rows, S.n_cols, 0, 0, S.buff, S.seed_state)
+        return None
+    D = D_or_S
+    return fill_dense_unpacked(D.natural_layout, D, D.n_rows, D.n_cols, 0, 0, buff, seed)
+
+
+def lskge3(layout, opS, opA, d, n, m, alpha, S: DenseSkOp, ro_s, co_s, A, lda, beta, B, ldb):
+    """lskge3 (skge.hh:154-202)."""
+    _call(f"rbx_{_prefix(B)}lskge3", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
+          _ptr(A), lda, beta, _ptr(B), ldb, _stream())
+
+
+def rskge3(layout, opA, opS, m, d, n, alpha, A, lda, S: DenseSkOp, ro_s, co_s, beta, B, ldb):
+    """rskge3 (skge.hh:307-355)."""
+    _call(f"rbx_{_prefix(B)}rskge3", _c(layout), _c(opA), _c(opS), m, d, n, alpha, _ptr(A), lda, C.byref(S.c()),
+          ro_s, co_s, beta, _ptr(B), ldb, _stream())
+
+
+def lskge3_adjoint(layout, opS, d, n, m, alpha, S: DenseSkOp, ro_s, co_s, Bbar, ldbb, beta, Abar, ldab):
+    """Abar = alpha * op(submat(S))^T * Bbar + beta * Abar (operator regenerated)."""
+    _call(f"rbx_{_prefix(Abar)}lskge3_adjoint", _c(layout), _c(opS), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
+          _ptr(Bbar), ldbb, beta, _ptr(Abar), ldab, _stream())
+
+
+def rskge3_adjoint(layout, opS, m, d, n, alpha, Bbar, ldbb, S: DenseSkOp, ro_s, co_s, beta, Abar, ldab):
+    """Abar = alpha * Bbar * op(submat(S))^T + beta * Abar (operator regenerated)."""
+    _call(f"rbx_{_prefix(Abar)}rskge3_adjoint", _c(layout), _c(opS), m, d, n, alpha, _ptr(Bbar), ldbb,
+          C.byref(S.c()), ro_s, co_s, beta, _ptr(Abar), ldab, _stream())
+
+
+def lskge3_bf16split(layout, opS, opA, d, n, m, alpha, S: DenseSkOp, ro_s, co_s, A, lda, beta, B, ldb,
+                     n_terms: int = 2):
+    """B = alpha * op(bf16(submat S)) * op(A) + beta * B on the BF16 matrix cores."""
+    _call("rbx_slskge3_bf16split", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
+          _ptr(A), lda, beta, _ptr(B), ldb, n_terms, _stream())
+
+
+# ---------------------------------------------------------------- sparse
+@dataclass
+class SparseDist:
+    """SparseDist (RandBLAS/sparse_skops.hh:131-246)."""
+    n_rows: int
+    n_cols: int
+    vec_nnz: int = 4
+    major_axis: Axis = Axis.Short
+
+    def c(self) -> c_sparse_dist:
+        return c_sparse_dist(self.n_rows, self.n_cols, self.vec_nnz, _c(self.major_axis))
+
+    def _props(self):
+        a, b, f, iso = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+        _call("rbx_sparse_dist_props", C.byref(self.c()), C.byref(a), C.byref(b), C.byref(f), C.byref(iso))
+        return a.value, b.value, f.value, iso.value
+
+    @property
+    def dim_major(self):
+        return self._props()[0]
+
+    @property
+    def dim_minor(self):
+        return self._props()[1]
+
+    @property
+    def full_nnz(self):
+        return self._props()[2]
+
+    @property
+    def isometry_scale(self):
+        return self._props()[3]
+
+
+class SparseSkOp:
+    """SparseSkOp<T, Philox4x32, int64_t> (RandBLAS/sparse_skops.hh:289-450)."""
+
+    def __init__(self, dist: SparseDist, seed_state: RNGState, dtype=None):
+        import torch
+        self.dist = dist
+        self.seed_state = seed_state
+        out = c_sparse_skop()
+        _call("rbx_sparse_skop_init", C.byref(dist.c()), C.byref(seed_state.c()), C.byref(out))
+        self.next_state = RNGState.from_c(out.next_state)
+        self.n_rows, self.n_cols = dist.n_rows, dist.n_cols
+        self.nnz = -1
+        self.dtype = dtype or torch.float64
+        self.vals = self.rows = self.cols = None
+
+    def c(self) -> c_sparse_skop:
+        s = c_sparse_skop()
+        s.dist = self.dist.c()
+        s.seed_state = self.seed_state.c()
+        s.next_state = self.next_state.c()
+        s.nnz = self.nnz
+        s.vals = None if self.vals is None else self.vals.data_ptr()
+        s.rows = None if self.rows is None else self.rows.data_ptr()
+        s.cols = None if self.cols is None else self.cols.data_ptr()
+        s.own_memory = 0
+        return s
+
+
+def fill_sparse(S: SparseSkOp) -> None:
+    """fill_sparse(S) (sparse_skops.hh:587-603) into torch buffers."""
+    import torch
+    full = S.dist.full_nnz
+    if S.vals is None:
+        S.vals = torch.empty(full, dtype=S.dtype, device="cuda")
+        S.rows = torch.empty(full, dtype=torch.int64, device="cuda")
+        S.cols = torch.empty(full, dtype=torch.int64, device="cuda")
+    nnz = C.c_int64()
+    nxt = c_rng_state()
+    pre = "d" if S.dtype == torch.float64 else "s"
+    _call(f"rbx_{pre}fill_sparse_unpacked_nosub", C.byref(S.dist.c()), C.byref(nnz), _ptr(S.vals), _ptr(S.rows),
+          _ptr(S.cols), C.byref(S.seed_state.c()), C.byref(nxt), _stream())
+    S.nnz = nnz.value
+
+
+def fill_sparse_unpacked_nosub(D: SparseDist, vals, rows, cols, seed: RNGState):
+    """Returns (nnz, next_state)."""
+    nnz = C.c_int64()
+    nxt = c_rng_state()
+    _call(f"rbx_{_prefix(vals)}fill_sparse_unpacked_nosub", C.byref(D.c()), C.byref(nnz), _ptr(vals), _ptr(rows),
+          _ptr(cols), C.byref(seed.c()), C.byref(nxt), _stream())
+    return nnz.value, RNGState.from_c(nxt)
+
+
+def lskges(layout, opS, opA, d, n, m, alpha, S: SparseSkOp, ro_s, co_s, A, lda, beta, B, ldb):
+    """lskges (skge.hh:465-492)."""
+    _call(f"rbx_{_prefix(B)}lskges", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
+          _ptr(A), lda, beta, _ptr(B), ldb, _stream())
+
+
+def rskges(layout, opA, opS, m, d, n, alpha, A, lda, S: SparseSkOp, ro_s, co_s, beta, B, ldb):
+    """rskges (skge.hh:598-626)."""
+    _call(f"rbx_{_prefix(B)}rskges", _c(layout), _c(opA), _c(opS), m, d, n, alpha, _ptr(A), lda, C.byref(S.c()),
+          ro_s, co_s, beta, _ptr(B), ldb, _stream())
+
+
+@dataclass
+class CSRMatrix:
+    """CSRMatrix<T, int64_t> (sparse_data/csr_matrix.hh:73-245) over GPU tensors."""
+    n_rows: int
+    n_cols: int
+    vals: object
+    rowptr: object
+    colidxs: object
+
+    def c(self) -> c_csr_matrix:
+        return c_csr_matrix(self.n_rows, self.n_cols, int(self.vals.numel()), self.vals.data_ptr(),
+                            self.rowptr.data_ptr(), self.colidxs.data_ptr())
+
+
+def rsksp3(layout, opA, opS, m, d, n, alpha, A: CSRMatrix, ro_a, co_a, S: DenseSkOp, ro_s, co_s, beta, B, ldb):
+    """rsksp3 (sparse_data/sksp.hh:277-326)."""
+    _call(f"rbx_{_prefix(B)}rsksp3", _c(layout), _c(opA), _c(opS), m, d, n, alpha, C.byref(A.c()), ro_a, co_a,
+          C.byref(S.c()), ro_s, co_s, beta, _ptr(B), ldb, _stream())
+
+
+# ---------------------------------------------------------------- dispatching wrappers
+def sketch_general(layout, op1, op2, a1, a2, a3, alpha, *rest):
+    """sketch_general overloads (skge.hh:756-1199), dispatched on which argument
+    is the sketching operator, with or without (ro_s, co_s):
+
+      sketch_general(layout, opS, opA, d, n, m, alpha, S, [ro_s, co_s,] A, lda, beta, B, ldb)
+      sketch_general(layout, opA, opS, m, d, n, alpha, A, lda, S, [ro_s, co_s,] beta, B, ldb)
+    """
+    if isinstance(rest[0], (DenseSkOp, SparseSkOp)):
+        S = rest[0]
+        if len(rest) == 6:
+            A, lda, beta, B, ldb = rest[1:]
+            d, n, m = a1, a2, a3
+            _check_full(S, op1, d, m)
+            ro_s = co_s = 0
+        else:
+            ro_s, co_s, A, lda, beta, B, ldb = rest[1:]
+        f = lskge3 if isinstance(S, DenseSkOp) else lskges
+        return f(layout, op1, op2, a1, a2, a3, alpha, S, ro_s, co_s, A, lda, beta, B, ldb)
+    A, lda, S = rest[0], rest[1], rest[2]
+    if len(rest) == 6:
+        beta, B, ldb = rest[3:]
+        m, d, n = a1, a2, a3
+        _check_full(S, op2, n, d)
+        ro_s = co_s = 0
+    else:
+        ro_s, co_s, beta, B, ldb = rest[3:]
+    f = rskge3 if isinstance(S, DenseSkOp) else rskges
+    return f(layout, op1, op2, a1, a2, a3, alpha, A, lda, S, ro_s, co_s, beta, B, ldb)
+
+
+def _check_full(S, opS, r, c):
+    # full-operator overloads require exact dimensions (skge.hh:1089-1095, 1191-1197)
+    if _c(opS) == b"N":
+        ok = S.n_rows == r and S.n_cols == c
+    else:
+        ok = S.n_rows == c and S.n_cols == r
+    if not ok:
+        raise Error("sketch_general: operator dimensions do not match the full-operator overload")
+
+
+def sketch_vector(opS, *args):
+    """sketch_vector (skve.hh:140-164, 232-246).
+
+      sketch_vector(opS, d, m, alpha, S, ro_s, co_s, x, incx, beta, y, incy)
+      sketch_vector(opS, alpha, S, x, incx, beta, y, incy)
+
+    (d, m) are the dimensions of submat(S) before op; the call reduces to
+    sketch_general(RowMajor, opS, NoTrans, d', 1, m', ...) with the vector
+    increments as leading dimensions."""
+    if len(args) == 7:
+        alpha, S, x, incx, beta, y, incy = args
+        d, m, ro_s, co_s = S.n_rows, S.n_cols, 0, 0
+    else:
+        d, m, alpha, S, ro_s, co_s, x, incx, beta, y, incy = args
+    _d, _m = (m, d) if _c(opS) == b"T" else (d, m)
+    return sketch_general(Layout.RowMajor, opS, Op.NoTrans, _d, 1, _m, alpha, S, ro_s, co_s, x, incx, beta, y,
+                          incy)
+
+
+def sketch_symmetric(layout, *args, sym_check_tol=0.0):
+    """sketch_symmetric (sksy.hh:158-177, 293-312, 406-423, 513-530).
+
+      sketch_symmetric(layout, d, n, alpha, S, ro_s, co_s, A, lda, beta, B, ldb)   B = alpha*S*A + beta*B
+      sketch_symmetric(layout, n, d, alpha, A, lda, S, ro_s, co_s, beta, B, ldb)   B = alpha*A*S + beta*B
+      sketch_symmetric(layout, alpha, S, A, lda, beta, B, ldb)
+      sketch_symmetric(layout, alpha, A, lda, S, beta, B, ldb)
+
+    A (n x n, stored as a general matrix) is first checked for symmetry with
+    util::require_symmetric (util.hh:127-148)."""
+    if len(args) == 7:
+        if isinstance(args[1], (DenseSkOp, SparseSkOp)):
+            alpha, S, A, lda, beta, B, ldb = args
+            d, n = S.n_rows, S.n_cols
+            require_symmetric(layout, A, n, lda, sym_check_tol)
+            return sketch_general(layout, Op.NoTrans, Op.NoTrans, d, n, n, alpha, S, 0, 0, A, lda, beta, B, ldb)
+        alpha, A, lda, S, beta, B, ldb = args
+        n, d = S.n_rows, S.n_cols
+        require_symmetric(layout, A, n, lda, sym_check_tol)
+        return sketch_general(layout, Op.NoTrans, Op.NoTrans, n, d, n, alpha, A, lda, S, 0, 0, beta, B, ldb)
+    if isinstance(args[3], (DenseSkOp, SparseSkOp)):
+        d, n, alpha, S, ro_s, co_s, A, lda, beta, B, ldb = args
+        require_symmetric(layout, A, n, lda, sym_check_tol)
+        return sketch_general(layout, Op.NoTrans, Op.NoTrans, d, n, n, alpha, S, ro_s, co_s, A, lda, beta, B, ldb)
+    n, d, alpha, A, lda, S, ro_s, co_s, beta, B, ldb = args
+    require_symmetric(layout, A, n, lda, sym_check_tol)
+    return sketch_general(layout, Op.NoTrans, Op.NoTrans, n, d, n, alpha, A, lda, S, ro_s, co_s, beta, B, ldb)
+
+
+def raise_if(cond: bool, msg: str):
+    if cond:
+        raise Error(msg)
+
+
+def require_symmetric(layout, A, n, lda, tol):
+    """util::require_symmetric (util.hh:127-148), evaluated on the device."""
+    if tol < 0:
+        return
+    import torch
+    if _c(layout) == b"C":
+        M = A.as_strided((n, n), (1, lda))
+    else:
+        M = A.as_strided((n, n), (lda, 1))
+    viol = (M - M.T).abs()
+    rel = (M.abs() + M.T.abs() + 1) * tol
+    bad = torch.nonzero(torch.triu(viol > rel, diagonal=1))
+    if bad.numel():
+        i, j = (int(v) for v in bad[0])
+        raise Error(f"Symmetry check failed. |A({i},{j}) - A({j},{i})| was {float(viol[i, j]):e}, "
+                    f"which exceeds tolerance of {float(rel[i, j]):e}.")
+
+
+def version() -> str:
+    return lib().rbx_version().decode()

@@ -1,0 +1,243 @@
+// rbx_core.hip -- error slot, RNG-state helpers, raw Philox streams and the
+// materialising fill_dense kernel.
+#include "rbx_internal.hh"
+
+#include <math.h>
+#include <string.h>
+#include <mutex>
+
+namespace rbx {
+
+static thread_local char g_err[1024];
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+void clear_error() { g_err[0] = 0; }
+
+int num_cus() {
+    static int cached = 0;
+    if (cached == 0) {
+        int dev = 0;
+        hipDeviceProp_t p;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+            cached = p.multiProcessorCount;
+        else
+            cached = 256;
+    }
+    return cached;
+}
+
+// ---------------------------------------------------------------------------
+// Raw Philox words: one thread per counter, out[4*i + w].
+__global__ __launch_bounds__(256) void philox_stream_kernel(Ctr128 base, uint32_t k0, uint32_t k1,
+                                                            int64_t n, uint4 *out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = philox4x32_10(ctr_add(base, (uint64_t)i), k0, k1);
+}
+
+// ---------------------------------------------------------------------------
+// fill_dense_unpacked (dense_skops.hh:563-606) on the device.
+//
+// The submatrix is n_rows x n_cols at (ro_s, co_s); the generated matrix
+// G[p,q] = S[ro_s+p, co_s+q] is written to buff[p*rs + q*cs].  One thread
+// owns one Philox counter of the natural layout and writes the (up to) four
+// entries it produces, so every entry is computed exactly once.
+template <typename T>
+__global__ __launch_bounds__(256) void fill_dense_kernel(GenMap g, int64_t n_p, int64_t n_q, T *buff,
+                                                         int64_t rs, int64_t cs, int64_t cell_first,
+                                                         int64_t n_cells_per_vec, int64_t n_vecs) {
+    const int64_t total = n_cells_per_vec * n_vecs;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t bstart = g.b_off;
+    const int64_t n_b = g.b_along_rows ? n_p : n_q;
+    for (; idx < total; idx += stride) {
+        int64_t v = idx / n_cells_per_vec;          // offset along the minor (a) direction
+        int64_t cell = cell_first + idx % n_cells_per_vec;
+        float4 w = gen4(g.gauss != 0, ctr_add(g.ctr, (uint64_t)((g.a_off + v) * g.P + cell)), g.k0, g.k1);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            int64_t bb = 4 * cell + t - bstart;      // offset along the major (b) direction
+            if (bb < 0 || bb >= n_b) continue;
+            int64_t p = g.b_along_rows ? bb : v;
+            int64_t q = g.b_along_rows ? v : bb;
+            buff[p * rs + q * cs] = entry_value<T>(f4_get(w, t), g.gauss != 0);
+        }
+    }
+}
+
+template <typename T>
+static void fill_dense_impl(char layout, const rbx_dense_dist *D, int64_t n_rows, int64_t n_cols,
+                            int64_t ro_s, int64_t co_s, T *buff, const rbx_rng_state *seed,
+                            rbx_rng_state *next_state, hipStream_t stream) {
+    RBX_REQUIRE(D != nullptr && seed != nullptr);
+    check_layout(layout);
+    DenseGeom geo = dense_geom(*D);
+    RBX_REQUIRE(n_rows > 0 && n_cols > 0);
+    RBX_REQUIRE(ro_s >= 0 && co_s >= 0);
+    RBX_REQUIRE(D->n_rows >= n_rows + ro_s);
+    RBX_REQUIRE(D->n_cols >= n_cols + co_s);
+    RBX_REQUIRE(buff != nullptr);
+    safe_mul(geo.dim_minor, geo.P);
+    safe_mul(n_rows, n_cols);
+    rbx_dense_skop S;
+    S.dist = *D;
+    S.seed_state = *seed;
+    GenMap g = gen_map(S, 'N', ro_s, co_s, false);
+    // natural-orientation extents
+    int64_t n_b = g.b_along_rows ? n_rows : n_cols;
+    int64_t n_a = g.b_along_rows ? n_cols : n_rows;
+    int64_t cell_first = g.b_off / 4;
+    int64_t cell_last = (g.b_off + n_b - 1) / 4;
+    int64_t cells = cell_last - cell_first + 1;
+    int64_t rs = layout == 'C' ? 1 : n_cols, cs = layout == 'C' ? n_rows : 1;
+    int64_t total = safe_mul(cells, n_a);
+    int64_t blocks = (total + 255) / 256;
+    int64_t cap = (int64_t)num_cus() * 64;
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(fill_dense_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, g, n_rows,
+                       n_cols, buff, rs, cs, cell_first, cells, n_a);
+    RBX_HIP(hipGetLastError());
+    if (next_state) {
+        // fill_dense_submat_impl returns seed + ctr_mat_start + n_srows*stride
+        // (dense_skops.hh:109-169): ctr_mat_start = a_off*P + b_off/4.
+        Ctr128 c = ctr_add(g.ctr, (uint64_t)(g.a_off * g.P + cell_first));
+        c = ctr_add(c, (uint64_t)safe_mul(n_a, g.P));
+        next_state->key[0] = seed->key[0];
+        next_state->key[1] = seed->key[1];
+        ctr_to_words(c, next_state->counter);
+    }
+}
+
+template <typename T>
+static void fill_dense_skop_impl(rbx_dense_skop *S, hipStream_t stream) {
+    RBX_REQUIRE(S != nullptr);
+    DenseGeom geo = dense_geom(S->dist);
+    if (S->buff == nullptr) {
+        size_t bytes = (size_t)safe_mul(safe_mul(S->dist.n_rows, S->dist.n_cols), sizeof(T));
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) RBX_FAIL(RBX_OUT_OF_MEMORY, "fill_dense: hipMalloc of %zu bytes failed", bytes);
+        S->buff = p;
+        S->own_memory = 1;
+    }
+    if (S->layout != 'C' && S->layout != 'R') S->layout = geo.natural;
+    fill_dense_impl<T>(S->layout, &S->dist, S->dist.n_rows, S->dist.n_cols, 0, 0, (T *)S->buff,
+                       &S->seed_state, nullptr, stream);
+}
+
+}  // namespace rbx
+
+using namespace rbx;
+
+extern "C" {
+
+const char *rbx_last_error_message(void) { return g_err; }
+
+const char *rbx_version(void) { return "randblas_amd 0.1 (gfx950)"; }
+
+rbx_status rbx_rng_state_from_key(uint64_t key, rbx_rng_state *out) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(out != nullptr);
+    memset(out, 0, sizeof(*out));
+    out->key[0] = (uint32_t)key;
+    out->key[1] = (uint32_t)(key >> 32);
+    RBX_API_END
+}
+
+rbx_status rbx_rng_state_incr(rbx_rng_state *state, uint64_t n) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(state != nullptr);
+    ctr_to_words(ctr_add(ctr_from_words(state->counter), n), state->counter);
+    RBX_API_END
+}
+
+rbx_status rbx_philox4x32_10(const rbx_rng_state *state, int64_t n_counters, uint32_t *out, void *stream) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(state != nullptr);
+    RBX_REQUIRE(n_counters >= 0);
+    if (n_counters == 0) return RBX_OK;
+    RBX_REQUIRE(out != nullptr);
+    int64_t blocks = (n_counters + 255) / 256;
+    int64_t cap = (int64_t)num_cus() * 64;
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(philox_stream_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       ctr_from_words(state->counter), state->key[0], state->key[1], n_counters,
+                       (uint4 *)out);
+    RBX_HIP(hipGetLastError());
+    RBX_API_END
+}
+
+rbx_status rbx_dense_dist_props(const rbx_dense_dist *dist, int64_t *dim_major, int64_t *dim_minor,
+                                char *natural_layout, double *isometry_scale) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(dist != nullptr);
+    DenseGeom g = dense_geom(*dist);
+    if (dim_major) *dim_major = g.dim_major;
+    if (dim_minor) *dim_minor = g.dim_minor;
+    if (natural_layout) *natural_layout = g.natural;
+    if (isometry_scale) *isometry_scale = pow((double)g.dim_minor, -0.5);
+    RBX_API_END
+}
+
+rbx_status rbx_dense_skop_init(const rbx_dense_dist *dist, const rbx_rng_state *seed, rbx_dense_skop *out) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(dist != nullptr && seed != nullptr && out != nullptr);
+    DenseGeom g = dense_geom(*dist);
+    memset(out, 0, sizeof(*out));
+    out->dist = *dist;
+    out->seed_state = *seed;
+    out->next_state = *seed;
+    // dense::compute_next_state (dense_skops.hh:172-185)
+    ctr_to_words(ctr_add(ctr_from_words(seed->counter), (uint64_t)safe_mul(g.P, g.dim_minor)),
+                 out->next_state.counter);
+    out->buff = nullptr;
+    out->layout = g.natural;
+    out->own_memory = 0;
+    RBX_API_END
+}
+
+rbx_status rbx_sfill_dense_unpacked(char layout, const rbx_dense_dist *D, int64_t n_rows, int64_t n_cols,
+                                    int64_t ro_s, int64_t co_s, float *buff, const rbx_rng_state *seed,
+                                    rbx_rng_state *next_state, void *stream) {
+    RBX_API_BEGIN
+    fill_dense_impl<float>(layout, D, n_rows, n_cols, ro_s, co_s, buff, seed, next_state, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_dfill_dense_unpacked(char layout, const rbx_dense_dist *D, int64_t n_rows, int64_t n_cols,
+                                    int64_t ro_s, int64_t co_s, double *buff, const rbx_rng_state *seed,
+                                    rbx_rng_state *next_state, void *stream) {
+    RBX_API_BEGIN
+    fill_dense_impl<double>(layout, D, n_rows, n_cols, ro_s, co_s, buff, seed, next_state, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_sfill_dense_skop(rbx_dense_skop *S, void *stream) {
+    RBX_API_BEGIN
+    fill_dense_skop_impl<float>(S, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_dfill_dense_skop(rbx_dense_skop *S, void *stream) {
+    RBX_API_BEGIN
+    fill_dense_skop_impl<double>(S, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_dense_skop_free(rbx_dense_skop *S) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(S != nullptr);
+    if (S->own_memory && S->buff != nullptr) {
+        RBX_HIP(hipFree(S->buff));
+    }
+    S->buff = nullptr;
+    S->own_memory = 0;
+    RBX_API_END
+}
+
+}  // extern "C"

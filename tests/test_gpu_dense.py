@@ -1,0 +1,257 @@
+"""GPU parity: Philox streams, dense operator sampling and dense sketches
+(lskge3 / rskge3 / adjoints) through the C ABI against the CPU oracle.
+
+Bars (from the specification of this path):
+  * Philox words: bit-exact;
+  * operator entries: bit-exact against the oracle's portable math, and within
+    3 ulp (2 ulp for all but ~1e-6 of entries) of the host-libm evaluation;
+  * FP64 sketches: relative Frobenius error <= 1e-13 and the componentwise
+    bound of test/test_matmul_cores/linop_common.hh:260-266;
+  * FP32 sketches: relative Frobenius error <= 1e-5.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_fro, ulp_diff_f32
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(DEV)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def strided(buf, rows, cols, layout, ld):
+    if layout == "C":
+        return np.lib.stride_tricks.as_strided(buf, (rows, cols), (buf.itemsize, ld * buf.itemsize))
+    return np.lib.stride_tricks.as_strided(buf, (rows, cols), (ld * buf.itemsize, buf.itemsize))
+
+
+# ---------------------------------------------------------------- Philox
+@pytest.mark.parametrize("start,key,n", [([0, 0, 0, 0], 0, 1000), ([0xFFFFFF00, 0, 0, 0], 42, 4096),
+                                         ([0xFFFFFFFF, 0xFFFFFFFF, 5, 0], 2**33 + 7, 777),
+                                         ([1, 2, 3, 4], 2**64 - 1, 1 << 20)])
+def test_philox_words_bit_exact(rb, orc, start, key, n):
+    out = torch.empty(4 * n, dtype=torch.int32, device=DEV)
+    rb.philox4x32_10(rb.RNGState(key, start), n, out)
+    got = host(out).view(np.uint32).reshape(n, 4)
+    ref = orc.philox_stream(start, [key & 0xFFFFFFFF, key >> 32], n)
+    assert np.array_equal(got, ref)
+
+
+def test_philox_kat_on_device(rb):
+    import json, os
+    from conftest import ROOT
+    kat = json.load(open(os.path.join(ROOT, "tests", "golden", "philox4x32_10_kat.json")))
+    for row in kat["rows"]:
+        out = torch.empty(4, dtype=torch.int32, device=DEV)
+        key = row["key"][0] | (row["key"][1] << 32)
+        rb.philox4x32_10(rb.RNGState(key, row["counter"]), 1, out)
+        assert [int(x) for x in host(out).view(np.uint32)] == row["expected"]
+
+
+# ---------------------------------------------------------------- fill_dense
+FILL_CASES = [
+    ((400, 2000), "L", "G", "R", (400, 2000, 0, 0)),   # the C1 operator
+    ((400, 2000), "L", "G", "C", (37, 101, 5, 9)),
+    ((2000, 400), "L", "G", "C", (2000, 400, 0, 0)),
+    ((2000, 400), "L", "U", "R", (33, 17, 1, 3)),
+    ((301, 77), "S", "G", "R", (300, 70, 1, 6)),
+    ((77, 301), "S", "U", "C", (77, 301, 0, 0)),
+    ((5, 3), "L", "G", "R", (2, 1, 3, 2)),
+    ((1, 1), "L", "G", "C", (1, 1, 0, 0)),
+]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,axis,fam,layout,sub", FILL_CASES)
+def test_fill_dense_bit_exact(rb, orc, dtype, shape, axis, fam, layout, sub):
+    r, c, ro, co = sub
+    ctr, key = (7, 0, 1, 0), 42
+    D = rb.DenseDist(shape[0], shape[1], rb.ScalarDist(fam), rb.Axis(axis))
+    buf = torch.empty(r * c, dtype=torch.float64 if dtype == np.float64 else torch.float32, device=DEV)
+    nxt = rb.fill_dense_unpacked(layout, D, r, c, ro, co, buf, rb.RNGState(key, ctr))
+    got = host(buf)
+    ref, ref_nxt = orc.fill_dense_unpacked(layout, shape[0], shape[1], r, c, ro, co, list(ctr), [key, 0], fam,
+                                           axis, dtype)
+    ref_flat = ref.reshape(-1, order="F" if layout == "C" else "C")
+    assert np.array_equal(got, ref_flat)
+    assert list(nxt.counter) == [int(x) for x in ref_nxt]
+
+
+def test_gaussians_close_to_libm_reference(rb, orc):
+    # C1's operator, float: device values vs the host-libm Box-Muller
+    D = rb.DenseDist(400, 2000)
+    buf = torch.empty(400 * 2000, dtype=torch.float32, device=DEV)
+    rb.fill_dense(D, buf, rb.RNGState(42))
+    got = host(buf)
+    ref, _ = orc.fill_dense(400, 2000, [0, 0, 0, 0], [42, 0], dtype=np.float32, mathmode=1)
+    d = ulp_diff_f32(got, ref.reshape(-1))
+    assert d.max() <= 3
+    assert (d > 2).sum() <= 8
+
+
+# ---------------------------------------------------------------- sketches
+def run_lskge3(rb, orc, dtype, layout, opS, opA, d, n, m, alpha, beta, ro=0, co=0, pad=(0, 0), Sshape=None,
+               key=42, fam="G", axis="L", materialise=False, seed=0):
+    rng = np.random.default_rng(seed)
+    if Sshape is None:
+        Sshape = (d + ro, m + co) if opS == "N" else (m + ro, d + co)
+    rA, cA = (m, n) if opA == "N" else (n, m)
+    lda = (rA if layout == "C" else cA) + pad[0]
+    ldb = (d if layout == "C" else n) + pad[1]
+    A = rng.standard_normal(lda * (cA if layout == "C" else rA)).astype(dtype)
+    B0 = rng.standard_normal(ldb * (n if layout == "C" else d)).astype(dtype)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    S = rb.DenseSkOp(rb.DenseDist(Sshape[0], Sshape[1], rb.ScalarDist(fam), rb.Axis(axis)), rb.RNGState(key))
+    if materialise:
+        rb.fill_dense(S, dtype=tdt)
+    Bd = dev(B0)
+    rb.lskge3(layout, opS, opA, d, n, m, alpha, S, ro, co, dev(A), lda, beta, Bd, ldb)
+    got = host(Bd)
+    Bref = B0.astype(np.float64).copy()
+    orc.lskge3("d", layout, opS, opA, d, n, m, alpha, Sshape, [0, 0, 0, 0], [key, 0], ro, co,
+               A.astype(np.float64), lda, beta, Bref, ldb, family=fam, major_axis=axis)
+    return strided(got, d, n, layout, ldb), strided(Bref, d, n, layout, ldb)
+
+
+def test_lskge3_c1_fp64(rb, orc):
+    """C1: S = DenseDist(400, 2000) Gaussian key 42, A 2000 x 200 N(0,1), ColMajor."""
+    got, ref = run_lskge3(rb, orc, np.float64, "C", "N", "N", 400, 200, 2000, 1.0, 0.0)
+    err = rel_fro(got, ref)
+    print(f"C1 FP64 relative Frobenius error {err:.3e}")
+    assert err <= 1e-13
+
+
+LSK_CASES = [(layout, opS, opA) for layout in "CR" for opS in "NT" for opA in "NT"]
+
+
+@pytest.mark.parametrize("layout,opS,opA", LSK_CASES)
+def test_lskge3_fp64_variants(rb, orc, layout, opS, opA):
+    got, ref = run_lskge3(rb, orc, np.float64, layout, opS, opA, 71, 45, 133, 0.75, -1.25, ro=3, co=5, pad=(2, 3),
+                          Sshape=None)
+    assert rel_fro(got, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("layout,opS,opA", LSK_CASES)
+def test_lskge3_fp32_variants(rb, orc, layout, opS, opA):
+    got, ref = run_lskge3(rb, orc, np.float32, layout, opS, opA, 67, 49, 257, 1.0, 0.5, ro=1, co=2, pad=(1, 0))
+    assert rel_fro(got, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("axis,fam", [("S", "G"), ("L", "U"), ("S", "U")])
+def test_lskge3_axes_families(rb, orc, axis, fam):
+    got, ref = run_lskge3(rb, orc, np.float64, "C", "N", "N", 40, 30, 90, 1.0, 0.0, Sshape=(40, 90), axis=axis,
+                          fam=fam)
+    assert rel_fro(got, ref) <= 1e-13
+    got, ref = run_lskge3(rb, orc, np.float64, "R", "T", "N", 40, 30, 90, 1.0, 0.0, Sshape=(90, 40), axis=axis,
+                          fam=fam)
+    assert rel_fro(got, ref) <= 1e-13
+
+
+def test_materialised_operator_same_result(rb, orc):
+    a, ref = run_lskge3(rb, orc, np.float64, "C", "N", "T", 50, 20, 70, 1.0, 0.0, materialise=True)
+    b, _ = run_lskge3(rb, orc, np.float64, "C", "N", "T", 50, 20, 70, 1.0, 0.0, materialise=False)
+    assert np.array_equal(a, b)
+    assert rel_fro(a, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_sketch_of_identity_is_operator(rb, orc, dt):
+    d, m = 77, 130
+    tdt = torch.float64 if dt == np.float64 else torch.float32
+    S = rb.DenseSkOp(rb.DenseDist(d, m), rb.RNGState(3))
+    I = torch.eye(m, dtype=tdt, device=DEV)
+    B = torch.empty(d * m, dtype=tdt, device=DEV)
+    rb.lskge3("C", "N", "N", d, m, m, 1.0, S, 0, 0, I.reshape(-1), m, 0.0, B, d)
+    ref, _ = orc.fill_dense(d, m, [0, 0, 0, 0], [3, 0], dtype=dt)
+    assert np.array_equal(host(B).reshape(m, d).T, ref)
+
+
+def test_beta_zero_ignores_nan_and_alpha_zero_skips_A(rb):
+    S = rb.DenseSkOp(rb.DenseDist(8, 16), rb.RNGState(1))
+    A = torch.randn(16 * 4, dtype=torch.float64, device=DEV)
+    B = torch.full((8 * 4,), float("nan"), dtype=torch.float64, device=DEV)
+    rb.lskge3("C", "N", "N", 8, 4, 16, 1.0, S, 0, 0, A, 16, 0.0, B, 8)
+    assert torch.isfinite(B).all()
+    B2 = torch.ones(8 * 4, dtype=torch.float64, device=DEV)
+    rb.lskge3("C", "N", "N", 8, 4, 16, 0.0, S, 0, 0, None, 16, 3.0, B2, 8)
+    assert torch.equal(B2, torch.full_like(B2, 3.0))
+
+
+def test_deterministic(rb):
+    S = rb.DenseSkOp(rb.DenseDist(300, 5000), rb.RNGState(9))
+    A = torch.randn(5000 * 100, dtype=torch.float64, device=DEV)
+    outs = []
+    for _ in range(3):
+        B = torch.empty(300 * 100, dtype=torch.float64, device=DEV)
+        rb.lskge3("C", "N", "N", 300, 100, 5000, 1.0, S, 0, 0, A, 5000, 0.0, B, 300)
+        outs.append(host(B))
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("opS", ["N", "T"])
+@pytest.mark.parametrize("opA", ["N", "T"])
+def test_rskge3_variants(rb, orc, layout, opS, opA):
+    rng = np.random.default_rng(2)
+    m, d, n = 37, 29, 150
+    ro, co = 2, 1
+    Sshape = (n + ro, d + co) if opS == "N" else (d + ro, n + co)
+    rA, cA = (m, n) if opA == "N" else (n, m)
+    lda = (rA if layout == "C" else cA) + 1
+    ldb = (m if layout == "C" else d) + 2
+    A = rng.standard_normal(lda * (cA if layout == "C" else rA))
+    B0 = rng.standard_normal(ldb * (d if layout == "C" else m))
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape), rb.RNGState(17))
+    Bd = dev(B0)
+    rb.rskge3(layout, opA, opS, m, d, n, -0.5, dev(A), lda, S, ro, co, 2.0, Bd, ldb)
+    ref = B0.copy()
+    orc.rskge3("d", layout, opA, opS, m, d, n, -0.5, A, lda, Sshape, [0, 0, 0, 0], [17, 0], ro, co, 2.0, ref, ldb)
+    assert rel_fro(strided(host(Bd), m, d, layout, ldb), strided(ref, m, d, layout, ldb)) <= 1e-13
+
+
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("opS", ["N", "T"])
+def test_adjoint_dot_product(rb, layout, opS):
+    # <op(S) A, G> == <A, op(S)^T G> with the operator regenerated, and the
+    # adjoint equals lskge3 with the opposite op bit for bit.
+    d, n, m = 64, 33, 500
+    Sshape = (d, m) if opS == "N" else (m, d)
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape), rb.RNGState(5))
+    A = torch.randn(m * n, dtype=torch.float64, device=DEV)
+    G = torch.randn(d * n, dtype=torch.float64, device=DEV)
+    lda, ldg = (m, d) if layout == "C" else (n, n)
+    SA = torch.empty(d * n, dtype=torch.float64, device=DEV)
+    rb.lskge3(layout, opS, "N", d, n, m, 1.0, S, 0, 0, A, lda, 0.0, SA, ldg)
+    StG = torch.empty(m * n, dtype=torch.float64, device=DEV)
+    rb.lskge3_adjoint(layout, opS, d, n, m, 1.0, S, 0, 0, G, ldg, 0.0, StG, lda)
+    lhs = float((SA * G).sum())
+    rhs = float((A * StG).sum())
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1)
+    StG2 = torch.empty(m * n, dtype=torch.float64, device=DEV)
+    flip = "T" if opS == "N" else "N"
+    rb.lskge3(layout, flip, "N", m, n, d, 1.0, S, 0, 0, G, ldg, 0.0, StG2, lda)
+    assert torch.equal(StG, StG2)
+
+
+def test_right_adjoint_dot_product(rb):
+    m, d, n = 40, 24, 300
+    S = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(6))
+    A = torch.randn(m * n, dtype=torch.float64, device=DEV)
+    G = torch.randn(m * d, dtype=torch.float64, device=DEV)
+    AS = torch.empty(m * d, dtype=torch.float64, device=DEV)
+    rb.rskge3("C", "N", "N", m, d, n, 1.0, A, m, S, 0, 0, 0.0, AS, m)
+    GSt = torch.empty(m * n, dtype=torch.float64, device=DEV)
+    rb.rskge3_adjoint("C", "N", m, d, n, 1.0, G, m, S, 0, 0, 0.0, GSt, m)
+    lhs = float((AS * G).sum())
+    rhs = float((A * GSt).sum())
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1)
