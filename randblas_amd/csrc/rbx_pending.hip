@@ -10,12 +10,6 @@ using namespace rbx;
 
 extern "C" {
 
-rbx_status rbx_slskge3_bf16split(char, char, char, int64_t, int64_t, int64_t, float, const rbx_dense_skop *,
-                                 int64_t, int64_t, const float *, int64_t, float, float *, int64_t, int,
-                                 void *) {
-    PENDING("rbx_slskge3_bf16split")
-}
-
 rbx_status rbx_sparse_skop_init(const rbx_sparse_dist *, const rbx_rng_state *, rbx_sparse_skop *) {
     PENDING("rbx_sparse_skop_init")
 }

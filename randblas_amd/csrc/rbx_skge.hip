@@ -24,6 +24,8 @@ struct Mfma16<double> {
     static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    // output element r of lane l sits at row l/16 + 4r, column l%16
+    static __device__ __forceinline__ int out_row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
 struct Mfma16<float> {
@@ -31,6 +33,8 @@ struct Mfma16<float> {
     static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    // output element r of lane l sits at row 4*(l/16) + r, column l%16
+    static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int BM = 64, BN = 64, BK = 16, NTHR = 256;
@@ -87,7 +91,8 @@ __global__ __launch_bounds__(NTHR) void skge_kernel(GemmProblem pb) {
         __syncthreads();
     }
 
-    // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 block
+    // epilogue: element r of the lane's accumulator is row M16::out_row(lane, r),
+    // column lane%16 of each 16x16 block (the f64 and f32 forms differ)
     T *Cp = (T *)pb.C;
     const T alpha = (T)pb.alpha, beta = (T)pb.beta;
 #pragma unroll
@@ -96,7 +101,7 @@ __global__ __launch_bounds__(NTHR) void skge_kernel(GemmProblem pb) {
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int64_t gi = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+                int64_t gi = m0 + wm * 32 + i * 16 + M16::out_row(lane, r);
                 int64_t gj = n0 + wn * 32 + j * 16 + (lane & 15);
                 if (gi < pb.M && gj < pb.N) {
                     T *c = Cp + gi * pb.crs + gj * pb.ccs;
