@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused Gaussian sketch B = S * A on one MI355X.
+
+Workload (the large fused Gaussian sketch): A is 262144 x 1024 FP32,
+column-major, A = G * diag(10^(-6 j / 1023)) with G iid N(0,1) drawn by this
+library's own generator (DenseDist(262144, 1024), key 5), so its condition
+number is ~1e6; S = DenseDist(4096, 262144) Gaussian, key 42, counter 0,
+never materialised.  One step is one call of the BF16-split sketch
+(rbx_slskge3_bf16split: A split into bf16 terms, S rounded to bf16, FP32
+MFMA accumulation, FP64 combination of the K-splits), inputs resident in HBM.
+
+value = algorithmic 2*d*m*n / time of the whole step (TFLOP/s).  Beside it:
+  * fp64: the plain FP64 fused sketch (rbx_dlskge3 on A promoted to FP64);
+  * errors of the BF16 result against FP64 products (same bf16 operator, and
+    the unrounded operator) on a block of rows, and the subspace-embedding
+    distortion of (1/sqrt d) S Q for A = QR;
+  * roofline of the dominant kernel from HIP events around its launches;
+  * cpu_baseline: the CPU oracle (OpenMP counter-addressed fill, then an
+    OpenBLAS GEMM) on a row block of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+D, M, N = 4096, 262144, 1024
+KEY_S, KEY_A = 42, 5
+METRIC = "Gaussian sketch SA TFLOP/s & %roofline, BF16-split vs FP64, at stated rel err"
+PEAK_BF16 = 2500.0   # TFLOP/s, MI355X dense BF16 MFMA
+PEAK_FP64 = 78.6     # TFLOP/s, MI355X FP64 MFMA
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--terms", type=int, default=3, help="bf16 split terms of A (1-3)")
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-rows", type=int, default=2048, help="rows of S in the CPU baseline sample")
+    ap.add_argument("--err-rows", type=int, default=64, help="rows of B checked against the FP64 oracle")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip fp64 / error / distortion legs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the sketch shards)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import randblas_amd as rb
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    flops = 2.0 * D * M * N
+
+    # ---- inputs (resident in HBM before timing) ----
+    log("generating A")
+    A = torch.empty(M * N, dtype=torch.float32, device=dev)
+    rb.fill_dense(rb.DenseDist(M, N), A, rb.RNGState(KEY_A))   # natural ColMajor
+    scales = torch.pow(10.0, -6.0 * torch.arange(N, device=dev, dtype=torch.float64) / (N - 1)).float()
+    A.view(N, M).mul_(scales.view(N, 1))
+    S = rb.DenseSkOp(rb.DenseDist(D, M), rb.RNGState(KEY_S))
+    B = torch.empty(D * N, dtype=torch.float32, device=dev)
+
+    def step():
+        rb.lskge3_bf16split("C", "N", "N", D, N, M, 1.0, S, 0, 0, A, M, 0.0, B, D, n_terms=args.terms)
+
+    log("warmup")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    rb.profile_collect()
+    rb.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rb.profile_enable(False)
+    prof = rb.profile_collect()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = flops / (ms * 1e-3) / 1e12
+    log(f"bf16-split x{args.terms}: {ms:.3f} ms/step, {value:.1f} TFLOP/s; kernels {prof}")
+
+    cnt, tot = prof.get("bf16_skge_kernel", (1, float("nan")))
+    k_ms = tot / max(cnt, 1)
+    achieved = flops / (k_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": "bf16_skge_kernel", "achieved": round(achieved, 2),
+                "peak": PEAK_BF16, "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16, 4),
+                "executed_mfma_frac": round(achieved * args.terms / PEAK_BF16, 4),
+                "kernel_ms": round(k_ms, 4), "traffic": None,
+                "per_kernel_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}}
+
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "TFLOP/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+           "data": "synthetic (A and S from this library's Philox generator)",
+           "config": {"workload": "fused Gaussian sketch S(4096x262144, key 42) * A(262144x1024 FP32, "
+                                  "key 5, column scales to cond 1e6), BF16-split",
+                      "d": D, "m": M, "n": N, "split_terms": args.terms, "layout": "ColMajor"},
+           "roofline": roofline}
+
+    B_h = B.view(N, D).t().cpu().numpy().astype(np.float64)
+    B64_h = B64_rows = None
+    if not args.no_extras:
+        # ---- plain FP64 fused sketch ----
+        log("fp64 leg")
+        A64 = A.double()
+        B64 = torch.empty(D * N, dtype=torch.float64, device=dev)
+        rb.lskge3("C", "N", "N", D, N, M, 1.0, S, 0, 0, A64, M, 0.0, B64, D)
+        torch.cuda.synchronize()
+        rb.profile_collect()
+        rb.profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(args.fp64_steps):
+            rb.lskge3("C", "N", "N", D, N, M, 1.0, S, 0, 0, A64, M, 0.0, B64, D)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rb.profile_enable(False)
+        p64 = rb.profile_collect()
+        ms64 = (t1 - t0) * 1e3 / args.fp64_steps
+        tf64 = flops / (ms64 * 1e-3) / 1e12
+        c64, k64 = p64.get("skge_kernel_f64", (1, float("nan")))
+        B64_h = B64.view(N, D).t().cpu().numpy()
+        B64_rows = B64_h[:args.cpu_rows].copy()
+        out["fp64"] = {"ms_per_step": round(ms64, 3), "tflops": round(tf64, 3),
+                       "frac_of_fp64_peak": round(flops / (k64 / c64 * 1e-3) / 1e12 / PEAK_FP64, 4),
+                       "bf16_speedup": round(ms64 / ms, 2)}
+        log(f"fp64: {ms64:.2f} ms, {tf64:.2f} TFLOP/s, bf16 speedup {ms64 / ms:.1f}x")
+
+        del A64, B64
+        torch.cuda.synchronize()
+
+    # ---- CPU: FP64 oracle rows (also the timed CPU baseline), errors, distortion ----
+    # All reference linear algebra runs on the host (numpy/OpenBLAS, scipy).
+    if not args.no_cpu:
+        base, ref_rows, A_h = cpu_baseline(A, args.cpu_rows, np, torch)
+        out["cpu_baseline"] = base
+        if not args.no_extras:
+            import oracle
+            rows = min(args.err_rows, args.cpu_rows)
+            Sblk = ref_rows["S"][:rows]
+            got = B.view(N, D).t()[:rows].cpu().numpy().astype(np.float64)
+            got64 = B64_rows[:rows]
+            ref_op = ref_rows["B"][:rows]
+            Sb16 = oracle.bf16_round(Sblk.astype(np.float32)).astype(np.float64)
+            ref_same = Sb16 @ A_h
+            rel = lambda x, r: float(np.linalg.norm(x - r) / np.linalg.norm(r))
+            out["rel_err_vs_fp64_same_operator"] = rel(got, ref_same)
+            out["rel_err_vs_fp64_reference_operator"] = rel(got, ref_op)
+            out["fp64_rel_err_vs_oracle"] = rel(got64, ref_op)
+            out["error_sample"] = f"rows 0..{rows - 1} of B against the CPU oracle in FP64"
+            log(f"rel err: bf16 vs same-operator FP64 {out['rel_err_vs_fp64_same_operator']:.3e}, vs reference "
+                f"operator {out['rel_err_vs_fp64_reference_operator']:.3e}; fp64 path {out['fp64_rel_err_vs_oracle']:.3e}")
+            # distortion of (1/sqrt d) S Q with A = QR: the eigenvalues of the
+            # pencil (B^T B, A^T A) / d are the squared singular values of S Q / sqrt(d)
+            import scipy.linalg as sla
+            log("distortion (host)")
+            AtA = A_h.T @ A_h
+            for name, Bm in (("bf16", B_h), ("fp64", B64_h)):
+                ev = sla.eigh(Bm.T @ Bm / D, AtA, eigvals_only=True)
+                sv = np.sqrt(np.clip(ev, 0, None))
+                out[f"distortion_{name}"] = round(max(float(sv.max()) - 1.0, 1.0 - float(sv.min())), 6)
+            log(f"distortion bf16 {out['distortion_bf16']}, fp64 {out['distortion_fp64']}")
+
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(A, rows, np, torch):
+    """The CPU oracle on a row block of the same workload: counter-addressed
+    OpenMP fill of `rows` rows of S (dense_skops.hh:96-170 restated), then the
+    FP64 GEMM with OpenBLAS through numpy (blas::gemm at skge.hh:200).
+    Returns (baseline record, {"S", "B"} of that block, A on the host)."""
+    import oracle
+    log(f"cpu baseline: {rows} rows of S")
+    cores = os.cpu_count() or 1
+    A_h = A.view(N, M).cpu().numpy().astype(np.float64).T  # M x N
+    oracle.lib()
+    t0 = time.perf_counter()
+    Sblk, _ = oracle.fill_dense_unpacked("R", D, M, rows, M, 0, 0, [0, 0, 0, 0], [KEY_S, 0],
+                                         dtype=np.float64, nthreads=cores)
+    t1 = time.perf_counter()
+    Bblk = Sblk @ A_h
+    t2 = time.perf_counter()
+    fl = 2.0 * rows * M * N
+    rec = {"value": round(fl / (t2 - t0) / 1e12, 5), "unit": "TFLOP/s", "cores": cores, "kind": "port",
+           "sample": f"rows 0..{rows - 1} of S (of 4096) times the full A: fill {t1 - t0:.2f} s, "
+                     f"FP64 GEMM (OpenBLAS) {t2 - t1:.2f} s",
+           "fill_s": round(t1 - t0, 3), "gemm_s": round(t2 - t1, 3)}
+    log(f"cpu baseline {rec['value']} TFLOP/s ({rec['sample']})")
+    return rec, {"S": Sblk, "B": Bblk}, A_h
+
+
+if __name__ == "__main__":
+    main()

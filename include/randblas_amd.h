@@ -287,6 +287,16 @@ rbx_status rbx_drsksp3(char layout, char opA, char opS, int64_t m, int64_t d, in
                        const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, double beta,
                        double *B, int64_t ldb, void *stream);
 
+/* ---------------------------------------------------------------- diagnostics
+ * (no reference counterpart) Kernel timing for benchmarks: while enabled,
+ * every kernel the library launches is bracketed by HIP events on its stream.
+ * rbx_profile_collect synchronises those events and returns, per kernel name,
+ * the number of launches and their total duration in milliseconds (at most
+ * max_entries rows; names are NUL-terminated, name_len bytes apart), then
+ * clears the record.  Returns the number of rows written, or -1 on error. */
+rbx_status rbx_profile_enable(int on);
+int rbx_profile_collect(char *names, int name_len, int *counts, double *total_ms, int max_entries);
+
 #ifdef __cplusplus
 }
 #endif

@@ -262,3 +262,23 @@ def left_spmm_csr(dtype, layout, opB, d, n, m, alpha, vals, rowptr, colidxs, B, 
                                  np.ascontiguousarray(colidxs, np.int64), _ptr(B), ldb, beta,
                                  _ptr(Cbuf), ldc, nthreads)
     _check(rc, "left_spmm_csr")
+
+
+# ---------------------------------------------------------------- bfloat16 helpers
+def bf16_round(x) -> np.ndarray:
+    """Round float32 values to bfloat16 (nearest, ties to even); float32 result."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def bf16_split(a, terms: int):
+    """A = A_1 + ... + A_terms + residual with each A_t the bfloat16 rounding of
+    the previous remainder (remainders are exact in float32)."""
+    r = np.ascontiguousarray(a, dtype=np.float32)
+    out = []
+    for _ in range(terms):
+        h = bf16_round(r)
+        out.append(h)
+        r = (r - h).astype(np.float32)
+    return out

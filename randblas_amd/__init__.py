@@ -142,8 +142,10 @@ _SIG = {
                     _I64, _P],
     "rbx_drsksp3": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _CSR, _I64, _I64, _DS, _I64, _I64, _D, _P,
                     _I64, _P],
+    "rbx_profile_enable": [C.c_int],
+    "rbx_profile_collect": [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int],
 }
-_RESTYPE = {"rbx_last_error_message": C.c_char_p, "rbx_version": C.c_char_p}
+_RESTYPE = {"rbx_last_error_message": C.c_char_p, "rbx_version": C.c_char_p, "rbx_profile_collect": C.c_int}
 
 _lib: Optional[C.CDLL] = None
 
@@ -582,6 +584,24 @@ def require_symmetric(layout, A, n, lda, tol):
         i, j = (int(v) for v in bad[0])
         raise Error(f"Symmetry check failed. |A({i},{j}) - A({j},{i})| was {float(viol[i, j]):e}, "
                     f"which exceeds tolerance of {float(rel[i, j]):e}.")
+
+
+def profile_enable(on: bool = True) -> None:
+    """Bracket every library kernel launch with HIP events (diagnostics)."""
+    _call("rbx_profile_enable", 1 if on else 0)
+
+
+def profile_collect() -> dict:
+    """{kernel name: (launch count, total ms)} since the last collect."""
+    n_max, width = 64, 64
+    names = C.create_string_buffer(n_max * width)
+    counts = (C.c_int * n_max)()
+    tot = (C.c_double * n_max)()
+    n = lib().rbx_profile_collect(names, width, counts, tot, n_max)
+    if n < 0:
+        raise Error(lib().rbx_last_error_message().decode())
+    raw = names.raw
+    return {raw[i * width:(i + 1) * width].split(b"\0")[0].decode(): (counts[i], tot[i]) for i in range(n)}
 
 
 def version() -> str:

@@ -6,16 +6,23 @@
 //
 // Three kernels:
 //   1. split:  op(A) (FP32, any strides) -> T bfloat16 planes A_1..A_T with
-//      A = A_1 + ... + A_T + residual, each plane column-major (k contiguous),
-//      zero-padded to Kp x Np.  Every term is the round-to-nearest-even of the
-//      previous remainder, and each remainder is exact in FP32.
+//      A = A_1 + ... + A_T + residual.  Every term is the round-to-nearest-even
+//      of the previous remainder, and each remainder is exact in FP32.  The
+//      planes are stored fragment-packed: for a 32-column block and a 16-deep
+//      K step, the 64 lanes' MFMA B operands (8 bf16 each) are one contiguous
+//      1 KB run, so every load instruction of the main kernel is a full
+//      coalesced 1 KB read.  Zero padding to Kp (multiple of 32) x Np
+//      (multiple of 256).
 //   2. main:   one 256-thread workgroup per (64-row block, 1024-column block,
 //      K-split).  Each 32-wide K-chunk of the operator is generated from its
 //      Philox counters, rounded to bfloat16 and double-buffered in LDS; the
 //      four waves each own 256 columns and keep a 64 x 256 FP32 accumulator
-//      (2 x 8 blocks of v_mfma_f32_32x32x16_bf16).  Each operator entry is
-//      generated once per 1024 columns.  The K-split bounds each FP32
-//      accumulation chain; partial tiles go to a row-major scratch.
+//      (2 x 8 blocks of v_mfma_f32_32x32x16_bf16, 256 AGPRs).  Each operator
+//      entry is generated once per 1024 columns.  B operands rotate through
+//      registers: the load of block j for the next K step is issued right
+//      after block j's MFMAs of this step, so loads are in flight for a whole
+//      step.  The K-split bounds each FP32 accumulation chain; partial tiles
+//      go to a row-major scratch.
 //   3. reduce: B = alpha * sum_s partial_s (in FP64, fixed order) + beta * B.
 // WG-to-tile mapping is XCD-aware: workgroups that share a K-range are
 // dispatched to the same XCD back to back, so the A planes they stream are
@@ -23,8 +30,6 @@
 #include "rbx_internal.hh"
 #include "rbx_gemm_common.hh"
 
-#include <mutex>
-#include <vector>
 
 namespace rbx {
 
@@ -54,16 +59,25 @@ __device__ __forceinline__ uint16_t entry_value<uint16_t>(float v, bool gauss) {
 }
 
 // ---------------------------------------------------------------- split
+// Element (k, j) of plane t lives at
+//   t*plane_stride + (((j/32)*(Kp/16) + k/16)*64 + j%32 + 32*((k/8)%2))*8 + k%8.
+__device__ __forceinline__ int64_t packed_index(int64_t k, int64_t j, int64_t KS) {
+    return ((((j >> 5) * KS + (k >> 4)) * 64 + (j & 31) + 32 * ((k >> 3) & 1)) << 3) + (k & 7);
+}
+
 __global__ __launch_bounds__(256) void split_kernel(const float *A, int64_t rs, int64_t cs, int64_t K, int64_t N,
                                                     int64_t Kp, int64_t Np, int terms, uint16_t *planes,
                                                     int64_t plane_stride) {
-    // one thread per (8 consecutive k, one column j)
+    // one thread per (8 consecutive k, one column j); consecutive threads walk
+    // the unit-stride direction of op(A)
     const int64_t kg = Kp / 8;
     const int64_t total = kg * Np;
+    const int64_t KS = Kp / 16;
+    const bool k_fast = rs == 1;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = idx / kg;
-        const int64_t k0 = (idx - j * kg) * 8;
+        const int64_t j = k_fast ? idx / kg : idx % Np;
+        const int64_t k0 = (k_fast ? idx % kg : idx / Np) * 8;
         uint16_t out[3][8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -76,13 +90,14 @@ __global__ __launch_bounds__(256) void split_kernel(const float *A, int64_t rs, 
                 a = __fsub_rn(a, bf16_to_f32(h));  // exact
             }
         }
+        const int64_t dst = packed_index(k0, j, KS);
         for (int t = 0; t < terms; ++t) {
             u32x4 v;
             v.x = (uint32_t)out[t][0] | ((uint32_t)out[t][1] << 16);
             v.y = (uint32_t)out[t][2] | ((uint32_t)out[t][3] << 16);
             v.z = (uint32_t)out[t][4] | ((uint32_t)out[t][5] << 16);
             v.w = (uint32_t)out[t][6] | ((uint32_t)out[t][7] << 16);
-            *(u32x4 *)(planes + t * plane_stride + j * Kp + k0) = v;
+            *(u32x4 *)(planes + t * plane_stride + dst) = v;
         }
     }
 }
@@ -99,6 +114,32 @@ struct Bf16Problem {
 
 // 32x32 output block, element r of lane l: row 8*(r/4) + 4*(l/32) + r%4, column l%32.
 __device__ __forceinline__ int out_row32(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+// Operator chunk [m0, m0+64) x [k0, k0+32) -> LDS (row-major, BF_LDK).  When
+// the chunk is inside the operator and its major axis runs along K from a
+// counter boundary, each Philox counter fills four consecutive entries of a
+// row with one 8-byte store.
+__device__ __forceinline__ void gen_chunk(const GenMap &g, int64_t m0, int64_t k0, int64_t M, int64_t K,
+                                          uint16_t *lds, int tid) {
+    const bool fast = !g.b_along_rows && ((g.b_off + k0) & 3) == 0 && m0 + BF_MT <= M && k0 + BF_KC <= K;
+    if (!fast) {
+        gen_tile<uint16_t, BF_MT, BF_KC, BF_NTHR>(g, m0, k0, M, K, lds, BF_LDK, 1, tid);
+        return;
+    }
+    const bool gauss = g.gauss != 0;
+    const int64_t cell0 = (g.b_off + k0) >> 2;
+#pragma unroll
+    for (int it = 0; it < (BF_MT * BF_KC / 4) / BF_NTHR; ++it) {
+        const int idx = tid + it * BF_NTHR;
+        const int row = idx >> 3, cell = idx & 7;
+        const Ctr128 c = ctr_add(g.ctr, (uint64_t)((g.a_off + m0 + row) * g.P + cell0 + cell));
+        const float4 v = gen4(gauss, c, g.k0, g.k1);
+        uint2 pk;
+        pk.x = (uint32_t)entry_value<uint16_t>(v.x, gauss) | ((uint32_t)entry_value<uint16_t>(v.y, gauss) << 16);
+        pk.y = (uint32_t)entry_value<uint16_t>(v.z, gauss) | ((uint32_t)entry_value<uint16_t>(v.w, gauss) << 16);
+        *(uint2 *)(lds + row * BF_LDK + cell * 4) = pk;
+    }
+}
 
 template <int TERMS>
 __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
@@ -129,37 +170,50 @@ __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
     const int64_t wcol = n0 + wave * 256;  // first column of this wave
-    // number of 32-column blocks of this wave inside the padded planes
-    int nblk = (int)((pb.Np - wcol) / 32);
-    if (nblk > 8) nblk = 8;
-    if (nblk < 0) nblk = 0;
+    const bool active = wcol < pb.Np;     // Np is a multiple of 256: all 8 blocks or none
+    const int64_t KS = pb.Kp / 16;
+    // lane's B operand for block j, K step s, term t:
+    //   planes + t*plane_stride + (((wcol/32 + j)*KS + s)*64 + lane)*8
+    const uint16_t *bbase = pb.planes + ((((wcol >> 5) * KS) * 64 + lane) << 3);
+    const int64_t bstep_blk = KS * 512;  // elements between column blocks
+    const int64_t s_first = kbeg / 16, s_last = kend / 16 - 1;
 
-    if (nchunks > 0) gen_tile<uint16_t, BF_MT, BF_KC, BF_NTHR>(pb.g, m0, kbeg, pb.M, pb.K, Ss[0], BF_LDK, 1, tid);
+    bf16x8 bq[8][TERMS];
+    if (active) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int tt = 0; tt < TERMS; ++tt)
+                bq[j][tt] = __builtin_bit_cast(
+                    bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + tt * pb.plane_stride + s_first * 512));
+    }
+
+    if (nchunks > 0) gen_chunk(pb.g, m0, kbeg, pb.M, pb.K, Ss[0], tid);
     __syncthreads();
 
     for (int c = 0; c < nchunks; ++c) {
         const int64_t kc = kbeg + (int64_t)c * BF_KC;
-        if (c + 1 < nchunks)
-            gen_tile<uint16_t, BF_MT, BF_KC, BF_NTHR>(pb.g, m0, kc + BF_KC, pb.M, pb.K, Ss[(c + 1) & 1], BF_LDK, 1,
-                                                      tid);
+        if (c + 1 < nchunks) gen_chunk(pb.g, m0, kc + BF_KC, pb.M, pb.K, Ss[(c + 1) & 1], tid);
         const uint16_t *S = Ss[c & 1];
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
             const int koff = ks * 16 + 8 * (lane >> 5);
-            bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (lane & 31) * BF_LDK + koff));
-            bf16x8 a1 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (32 + (lane & 31)) * BF_LDK + koff));
-            const int64_t k = kc + koff;
+            const bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (lane & 31) * BF_LDK + koff));
+            const bf16x8 a1 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (32 + (lane & 31)) * BF_LDK + koff));
+            int64_t s_next = kc / 16 + ks + 1;
+            if (s_next > s_last) s_next = s_last;  // redundant in-range reload at the end
+            if (active) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                if (j < nblk) {
-                    const int64_t col = wcol + j * 32 + (lane & 31);
-                    const uint16_t *bp = pb.planes + col * pb.Kp + k;
+                for (int j = 0; j < 8; ++j) {
 #pragma unroll
                     for (int tt = 0; tt < TERMS; ++tt) {
-                        bf16x8 b = __builtin_bit_cast(bf16x8, *(const u32x4 *)(bp + tt * pb.plane_stride));
-                        acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][j], 0, 0, 0);
-                        acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][j], 0, 0, 0);
+                        acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bq[j][tt], acc[0][j], 0, 0, 0);
+                        acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bq[j][tt], acc[1][j], 0, 0, 0);
                     }
+#pragma unroll
+                    for (int tt = 0; tt < TERMS; ++tt)
+                        bq[j][tt] = __builtin_bit_cast(
+                            bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + tt * pb.plane_stride + s_next * 512));
                 }
             }
         }
@@ -167,6 +221,7 @@ __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
     }
 
     // partial tile -> scratch (row-major M x N per split; lanes walk columns)
+    if (!active) return;
     float *P = pb.part + (int64_t)sp * pb.M * pb.N;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -195,31 +250,6 @@ __global__ __launch_bounds__(256) void bf16_reduce_kernel(const float *part, int
         if (beta != 0.0) out += beta * (double)(*b);
         *b = (float)out;
     }
-}
-
-// Library-owned scratch, grown on demand (one buffer per device).
-static std::mutex g_ws_mutex;
-static std::vector<std::pair<void *, size_t>> g_ws;
-
-static void *workspace(size_t bytes) {
-    std::lock_guard<std::mutex> lk(g_ws_mutex);
-    int dev = 0;
-    RBX_HIP(hipGetDevice(&dev));
-    if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1, {nullptr, 0});
-    auto &w = g_ws[dev];
-    if (w.second < bytes) {
-        if (w.first) {
-            RBX_HIP(hipDeviceSynchronize());
-            RBX_HIP(hipFree(w.first));
-            w.first = nullptr;
-            w.second = 0;
-        }
-        void *p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) RBX_FAIL(RBX_OUT_OF_MEMORY, "bf16split: hipMalloc of %zu bytes failed", bytes);
-        w.first = p;
-        w.second = bytes;
-    }
-    return w.first;
 }
 
 static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, float alpha,
@@ -268,7 +298,7 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     pb.N = N;
     pb.K = K;
     pb.Kp = (K + BF_KC - 1) / BF_KC * BF_KC;
-    pb.Np = (N + 31) / 32 * 32;
+    pb.Np = (N + 255) / 256 * 256;
     pb.g = gen_map(*S, opS, ro_s, co_s, false);
     pb.nMB = (int)((M + BF_MT - 1) / BF_MT);
     pb.nNB = (int)((N + BF_NT - 1) / BF_NT);
@@ -287,7 +317,7 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     const size_t plane_elems = (size_t)safe_mul(pb.Kp, pb.Np);
     const size_t planes_bytes = ((size_t)n_terms * plane_elems * 2 + 255) / 256 * 256;
     const size_t part_bytes = (size_t)safe_mul(safe_mul(pb.nS, M), N) * 4;
-    char *ws = (char *)workspace(planes_bytes + part_bytes);
+    char *ws = (char *)scratch(SCRATCH_BF16, planes_bytes + part_bytes);
     pb.planes = (const uint16_t *)ws;
     pb.plane_stride = (int64_t)plane_elems;
     pb.part = (float *)(ws + planes_bytes);
@@ -296,12 +326,14 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         int64_t total = pb.Kp / 8 * pb.Np;
         int64_t blocks = (total + 255) / 256;
         if (blocks > 65536) blocks = 65536;
+        ProfScope prof("bf16_split_kernel", stream);
         hipLaunchKernelGGL(split_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, A, ars, acs, K, N, pb.Kp, pb.Np,
                            n_terms, (uint16_t *)pb.planes, pb.plane_stride);
         RBX_HIP(hipGetLastError());
     }
     {
         dim3 grid((unsigned)(pb.nMB * pb.nNB * pb.nS));
+        ProfScope prof("bf16_skge_kernel", stream);
         if (n_terms == 1) hipLaunchKernelGGL(bf16_skge_kernel<1>, grid, dim3(BF_NTHR), 0, stream, pb);
         else if (n_terms == 2) hipLaunchKernelGGL(bf16_skge_kernel<2>, grid, dim3(BF_NTHR), 0, stream, pb);
         else hipLaunchKernelGGL(bf16_skge_kernel<3>, grid, dim3(BF_NTHR), 0, stream, pb);
@@ -311,6 +343,7 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         int64_t total = M * N;
         int64_t blocks = (total + 255) / 256;
         if (blocks > 16384) blocks = 16384;
+        ProfScope prof("bf16_reduce_kernel", stream);
         hipLaunchKernelGGL(bf16_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, pb.part, pb.nS, M, N,
                            (double)alpha, (double)beta, B, brs, bcs);
         RBX_HIP(hipGetLastError());

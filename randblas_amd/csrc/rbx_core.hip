@@ -5,6 +5,7 @@
 #include <math.h>
 #include <string.h>
 #include <mutex>
+#include <vector>
 
 namespace rbx {
 
@@ -29,6 +30,63 @@ int num_cus() {
             cached = 256;
     }
     return cached;
+}
+
+// ---------------------------------------------------------------------------
+// Device scratch slots
+static std::mutex g_ws_mutex;
+static std::vector<std::pair<void *, size_t>> g_ws;  // index dev * SCRATCH_NSLOTS + slot
+
+void *scratch(int slot, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    int dev = 0;
+    RBX_HIP(hipGetDevice(&dev));
+    size_t idx = (size_t)dev * SCRATCH_NSLOTS + slot;
+    if (g_ws.size() <= idx) g_ws.resize(idx + 1, {nullptr, 0});
+    auto &w = g_ws[idx];
+    if (bytes == 0) bytes = 256;
+    if (w.second < bytes) {
+        if (w.first) {
+            RBX_HIP(hipDeviceSynchronize());
+            RBX_HIP(hipFree(w.first));
+            w.first = nullptr;
+            w.second = 0;
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) RBX_FAIL(RBX_OUT_OF_MEMORY, "scratch: hipMalloc of %zu bytes failed", bytes);
+        w.first = p;
+        w.second = bytes;
+    }
+    return w.first;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel timing records (rbx_profile_*)
+struct ProfRec {
+    const char *name;
+    hipEvent_t a, b;
+};
+static std::mutex g_prof_mutex;
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof;
+
+ProfScope::ProfScope(const char *n, hipStream_t s) : name(n), stream(s) {
+    if (!g_prof_on) return;
+    if (hipEventCreate(&start) != hipSuccess) { start = nullptr; return; }
+    if (hipEventRecord(start, stream) != hipSuccess) { (void)hipEventDestroy(start); start = nullptr; }
+}
+
+ProfScope::~ProfScope() {
+    if (start == nullptr) return;
+    hipEvent_t stop;
+    if (hipEventCreate(&stop) != hipSuccess) { (void)hipEventDestroy(start); return; }
+    if (hipEventRecord(stop, stream) != hipSuccess) {
+        (void)hipEventDestroy(start);
+        (void)hipEventDestroy(stop);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_prof_mutex);
+    g_prof.push_back({name, start, stop});
 }
 
 // ---------------------------------------------------------------------------
@@ -71,6 +129,26 @@ __global__ __launch_bounds__(256) void fill_dense_kernel(GenMap g, int64_t n_p, 
     }
 }
 
+// Write the generated matrix G (n_p x n_q) to buff[p*rs + q*cs].
+template <typename T>
+void fill_generated(const GenMap &g, int64_t n_p, int64_t n_q, T *buff, int64_t rs, int64_t cs, hipStream_t stream) {
+    if (n_p <= 0 || n_q <= 0) return;
+    int64_t n_b = g.b_along_rows ? n_p : n_q;
+    int64_t n_a = g.b_along_rows ? n_q : n_p;
+    int64_t cell_first = g.b_off / 4;
+    int64_t cells = (g.b_off + n_b - 1) / 4 - cell_first + 1;
+    int64_t total = safe_mul(cells, n_a);
+    int64_t blocks = (total + 255) / 256;
+    int64_t cap = (int64_t)num_cus() * 64;
+    if (blocks > cap) blocks = cap;
+    ProfScope prof("fill_dense_kernel", stream);
+    hipLaunchKernelGGL(fill_dense_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, g, n_p, n_q, buff, rs, cs,
+                       cell_first, cells, n_a);
+    RBX_HIP(hipGetLastError());
+}
+template void fill_generated<float>(const GenMap &, int64_t, int64_t, float *, int64_t, int64_t, hipStream_t);
+template void fill_generated<double>(const GenMap &, int64_t, int64_t, double *, int64_t, int64_t, hipStream_t);
+
 template <typename T>
 static void fill_dense_impl(char layout, const rbx_dense_dist *D, int64_t n_rows, int64_t n_cols,
                             int64_t ro_s, int64_t co_s, T *buff, const rbx_rng_state *seed,
@@ -100,6 +178,7 @@ static void fill_dense_impl(char layout, const rbx_dense_dist *D, int64_t n_rows
     int64_t blocks = (total + 255) / 256;
     int64_t cap = (int64_t)num_cus() * 64;
     if (blocks > cap) blocks = cap;
+    ProfScope prof("fill_dense_kernel", stream);
     hipLaunchKernelGGL(fill_dense_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, g, n_rows,
                        n_cols, buff, rs, cs, cell_first, cells, n_a);
     RBX_HIP(hipGetLastError());
@@ -140,6 +219,44 @@ const char *rbx_last_error_message(void) { return g_err; }
 
 const char *rbx_version(void) { return "randblas_amd 0.1 (gfx950)"; }
 
+rbx_status rbx_profile_enable(int on) {
+    RBX_API_BEGIN
+    g_prof_on = on != 0;
+    RBX_API_END
+}
+
+int rbx_profile_collect(char *names, int name_len, int *counts, double *total_ms, int max_entries) {
+    clear_error();
+    std::vector<ProfRec> recs;
+    {
+        std::lock_guard<std::mutex> lk(g_prof_mutex);
+        recs.swap(g_prof);
+    }
+    std::vector<std::string> nm;
+    std::vector<int> cnt;
+    std::vector<double> tot;
+    int rc = 0;
+    for (auto &r : recs) {
+        float ms = 0.f;
+        if (hipEventSynchronize(r.b) != hipSuccess || hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) rc = -1;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+        size_t i = 0;
+        while (i < nm.size() && nm[i] != r.name) ++i;
+        if (i == nm.size()) { nm.push_back(r.name); cnt.push_back(0); tot.push_back(0.0); }
+        cnt[i] += 1;
+        tot[i] += ms;
+    }
+    if (rc < 0) { set_error("rbx_profile_collect: event query failed"); return -1; }
+    int n = 0;
+    for (size_t i = 0; i < nm.size() && n < max_entries; ++i, ++n) {
+        if (names && name_len > 0) snprintf(names + (size_t)n * name_len, name_len, "%s", nm[i].c_str());
+        if (counts) counts[n] = cnt[i];
+        if (total_ms) total_ms[n] = tot[i];
+    }
+    return n;
+}
+
 rbx_status rbx_rng_state_from_key(uint64_t key, rbx_rng_state *out) {
     RBX_API_BEGIN
     RBX_REQUIRE(out != nullptr);
@@ -165,6 +282,7 @@ rbx_status rbx_philox4x32_10(const rbx_rng_state *state, int64_t n_counters, uin
     int64_t blocks = (n_counters + 255) / 256;
     int64_t cap = (int64_t)num_cus() * 64;
     if (blocks > cap) blocks = cap;
+    ProfScope prof("philox_stream_kernel", (hipStream_t)stream);
     hipLaunchKernelGGL(philox_stream_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        ctr_from_words(state->counter), state->key[0], state->key[1], n_counters,
                        (uint4 *)out);

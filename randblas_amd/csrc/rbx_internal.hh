@@ -140,4 +140,25 @@ inline GenMap gen_map(const rbx_dense_skop &S, char opS, int64_t ro_s, int64_t c
 
 int num_cus();
 
+// Library-owned device scratch, one grow-only buffer per (device, slot).  A
+// slot's contents live until the next call that uses the same slot on the
+// same device; calls that overlap on one device (different host threads or
+// streams) and use the same slot race.  Growing a slot synchronises the device.
+enum ScratchSlot { SCRATCH_BF16 = 0, SCRATCH_SPARSE_OP = 1, SCRATCH_SPARSE_CSR = 2, SCRATCH_DENSE_OP = 3,
+                   SCRATCH_NSLOTS = 4 };
+void *scratch(int slot, size_t bytes);
+
+// Write the generated matrix G (n_p x n_q) of a GenMap to buff[p*rs + q*cs].
+template <typename T>
+void fill_generated(const GenMap &g, int64_t n_p, int64_t n_q, T *buff, int64_t rs, int64_t cs, hipStream_t stream);
+
+// Event-bracketed launch record used by rbx_profile_* (a no-op unless enabled).
+struct ProfScope {
+    ProfScope(const char *name, hipStream_t stream);
+    ~ProfScope();
+    const char *name;
+    hipStream_t stream;
+    hipEvent_t start = nullptr;
+};
+
 }  // namespace rbx

@@ -9,8 +9,9 @@
 // is not materialised (buff == NULL) each K-slice of G is generated from its
 // Philox counters straight into LDS; G never reaches HBM.
 //
-// Tile: 64 x 64 outputs per 256-thread workgroup (4 waves in 2 x 2, each
-// 32 x 32 = 2 x 2 blocks of 16 x 16), K-slice 16, v_mfma_{f64,f32}_16x16x4.
+// Tile: 64 x 256 outputs per 256-thread workgroup when the operator is X
+// (256 x 64 when it is Y), 4 waves in 2 x 2, K-slice 16,
+// v_mfma_{f64,f32}_16x16x4.  Each generated operator entry feeds 256 outputs.
 #include "rbx_internal.hh"
 #include "rbx_gemm_common.hh"
 
@@ -37,20 +38,23 @@ struct Mfma16<float> {
     static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-constexpr int BM = 64, BN = 64, BK = 16, NTHR = 256;
-constexpr int LDX = BM + 1;  // padded LDS rows
-constexpr int LDY = BN + 1;
+constexpr int BK = 16, NTHR = 256;
 
-template <typename T, bool GEN_X, bool S_MEM>
+// Workgroup tile BM x BN (4 waves in 2 x 2, each (BM/2) x (BN/2) in 16 x 16
+// blocks).  The operator tile is BM x BK (GEN_X) or BK x BN (!GEN_X); the
+// other extent is made large (256) so each generated entry feeds 256 outputs.
+template <typename T, bool GEN_X, bool S_MEM, int BM, int BN>
 __global__ __launch_bounds__(NTHR) void skge_kernel(GemmProblem pb) {
+    constexpr int LDX = BM + 1, LDY = BN + 1;
+    constexpr int MI = BM / 32, NJ = BN / 32;  // 16x16 blocks per wave
     __shared__ T Xs[BK * LDX];
     __shared__ T Ys[BK * LDY];
     using M16 = Mfma16<T>;
-    typename M16::acc_t acc[2][2];
+    typename M16::acc_t acc[MI][NJ];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = typename M16::acc_t{0, 0, 0, 0};
+        for (int j = 0; j < NJ; ++j) acc[i][j] = typename M16::acc_t{0, 0, 0, 0};
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -78,15 +82,15 @@ __global__ __launch_bounds__(NTHR) void skge_kernel(GemmProblem pb) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + (lane >> 4);
-            T a[2], b[2];
+            T a[MI], b[NJ];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = Xs[kr * LDX + wm * 32 + i * 16 + (lane & 15)];
+            for (int i = 0; i < MI; ++i) a[i] = Xs[kr * LDX + wm * (BM / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Ys[kr * LDY + wn * 32 + j * 16 + (lane & 15)];
+            for (int j = 0; j < NJ; ++j) b[j] = Ys[kr * LDY + wn * (BN / 2) + j * 16 + (lane & 15)];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < MI; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = M16::mma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < NJ; ++j) acc[i][j] = M16::mma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
@@ -96,13 +100,13 @@ __global__ __launch_bounds__(NTHR) void skge_kernel(GemmProblem pb) {
     T *Cp = (T *)pb.C;
     const T alpha = (T)pb.alpha, beta = (T)pb.beta;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < NJ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int64_t gi = m0 + wm * 32 + i * 16 + M16::out_row(lane, r);
-                int64_t gj = n0 + wn * 32 + j * 16 + (lane & 15);
+                int64_t gi = m0 + wm * (BM / 2) + i * 16 + M16::out_row(lane, r);
+                int64_t gj = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
                 if (gi < pb.M && gj < pb.N) {
                     T *c = Cp + gi * pb.crs + gj * pb.ccs;
                     T v = alpha * acc[i][j][r];
@@ -114,13 +118,21 @@ __global__ __launch_bounds__(NTHR) void skge_kernel(GemmProblem pb) {
 
 template <typename T>
 static void launch_skge(const GemmProblem &pb, hipStream_t stream) {
-    dim3 grid((unsigned)((pb.M + BM - 1) / BM), (unsigned)((pb.N + BN - 1) / BN));
-    RBX_REQUIRE(grid.y <= 65535u);
     bool gx = pb.gen_is_x, mem = pb.sbuf != nullptr;
-    if (gx && !mem) hipLaunchKernelGGL((skge_kernel<T, true, false>), grid, dim3(NTHR), 0, stream, pb);
-    else if (gx && mem) hipLaunchKernelGGL((skge_kernel<T, true, true>), grid, dim3(NTHR), 0, stream, pb);
-    else if (!mem) hipLaunchKernelGGL((skge_kernel<T, false, false>), grid, dim3(NTHR), 0, stream, pb);
-    else hipLaunchKernelGGL((skge_kernel<T, false, true>), grid, dim3(NTHR), 0, stream, pb);
+    ProfScope prof(sizeof(T) == 8 ? "skge_kernel_f64" : "skge_kernel_f32", stream);
+    if (gx) {
+        constexpr int TM = 64, TN = 256;
+        dim3 grid((unsigned)((pb.M + TM - 1) / TM), (unsigned)((pb.N + TN - 1) / TN));
+        RBX_REQUIRE(grid.y <= 65535u);
+        if (!mem) hipLaunchKernelGGL((skge_kernel<T, true, false, TM, TN>), grid, dim3(NTHR), 0, stream, pb);
+        else hipLaunchKernelGGL((skge_kernel<T, true, true, TM, TN>), grid, dim3(NTHR), 0, stream, pb);
+    } else {
+        constexpr int TM = 256, TN = 64;
+        dim3 grid((unsigned)((pb.M + TM - 1) / TM), (unsigned)((pb.N + TN - 1) / TN));
+        RBX_REQUIRE(grid.y <= 65535u);
+        if (!mem) hipLaunchKernelGGL((skge_kernel<T, false, false, TM, TN>), grid, dim3(NTHR), 0, stream, pb);
+        else hipLaunchKernelGGL((skge_kernel<T, false, true, TM, TN>), grid, dim3(NTHR), 0, stream, pb);
+    }
     RBX_HIP(hipGetLastError());
 }
 
@@ -142,6 +154,7 @@ void scale_output(T *C, int64_t M, int64_t N, int64_t crs, int64_t ccs, double b
     if (M <= 0 || N <= 0 || beta == 1.0) return;
     int64_t blocks = (M * N + 255) / 256;
     if (blocks > 4096) blocks = 4096;
+    ProfScope prof("scale_kernel", stream);
     hipLaunchKernelGGL(scale_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, C, M, N, crs, ccs, beta);
     RBX_HIP(hipGetLastError());
 }

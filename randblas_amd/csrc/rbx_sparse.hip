@@ -1,0 +1,761 @@
+// rbx_sparse.hip -- sparse sketching operators (SASO / LASO) and the sparse
+// products of RandBLAS's hot path:
+//   fill_sparse_unpacked_nosub  (sparse_skops.hh:515-565)
+//   lskges / rskges             (skge.hh:465-626, spmm_dispatch.hh:52-219)
+//   rsksp3                      (sparse_data/sksp.hh:277-326, csr_spmm_impl.hh:51-155)
+//
+// Sampling.  SASO: vector i (of dim_minor) is a partial Fisher-Yates shuffle
+// of the identity on dim_major points using counters seed + i*vec_nnz + j
+// (sparse_skops.hh:71-103; the reference restores its work array after every
+// vector, so vectors are independent) -- one thread per vector keeps the few
+// displaced positions in registers.  LASO: vec_nnz iid samples per vector from
+// two Philox words each (util.hh:516-546), duplicates merged to
+// sqrt(count)*first_sign in first-occurrence order, then compacted.
+//
+// Products.  Each output entry of op(S)*op(B) is accumulated by one thread in
+// ascending operator column, which is the order of the reference's fallback
+// kernels (csc_spmm_impl.hh:74-96, 154-209), so +-1 operators give
+// bit-identical results.  The COO operator is transposed once into row-major
+// CSR (rocPRIM radix sort on the (row, column) key); the apply kernel stages a
+// chunk of op(B) in LDS and lets every thread walk its rows' sorted entries.
+#include "rbx_internal.hh"
+
+#include <math.h>
+#include <string.h>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+namespace rbx {
+
+// ---------------------------------------------------------------- geometry
+struct SparseGeom {
+    int64_t dim_major, dim_minor, full_nnz;
+};
+
+static SparseGeom sparse_geom(const rbx_sparse_dist &D) {
+    if (D.n_rows <= 0 || D.n_cols <= 0) RBX_FAIL(RBX_INVALID_ARG, "SparseDist: n_rows and n_cols must be positive");
+    if (D.vec_nnz <= 0) RBX_FAIL(RBX_INVALID_ARG, "SparseDist: vec_nnz must be positive");
+    if (D.major_axis != 'S' && D.major_axis != 'L') RBX_FAIL(RBX_INVALID_ARG, "SparseDist: major_axis must be 'S' or 'L'");
+    SparseGeom g;
+    int64_t mx = D.n_rows > D.n_cols ? D.n_rows : D.n_cols;
+    int64_t mn = D.n_rows < D.n_cols ? D.n_rows : D.n_cols;
+    g.dim_major = D.major_axis == 'S' ? mn : mx;
+    g.dim_minor = D.n_rows + D.n_cols - g.dim_major;
+    if (D.vec_nnz > g.dim_major) RBX_FAIL(RBX_INVALID_ARG, "SparseDist: requirement failed: vec_nnz <= dim_major");
+    g.full_nnz = safe_mul(D.vec_nnz, g.dim_minor);
+    return g;
+}
+
+// compute_next_state(SparseDist, ...) (sparse_skops.hh:266-283)
+static int64_t sparse_incr(const rbx_sparse_dist &D) {
+    if (D.major_axis == 'S') {
+        int64_t mx = D.n_rows > D.n_cols ? D.n_rows : D.n_cols;
+        return safe_mul(mx, D.vec_nnz);
+    }
+    int64_t mn = D.n_rows < D.n_cols ? D.n_rows : D.n_cols;
+    return safe_mul(mn, (D.vec_nnz + 1) / 2);
+}
+
+// ---------------------------------------------------------------- SASO
+template <typename T, int MAXK>
+__global__ __launch_bounds__(256) void saso_kernel(Ctr128 ctr, uint32_t k0, uint32_t k1, int64_t vec_nnz,
+                                                   int64_t dim_major, int64_t dim_minor, int64_t *idx_major,
+                                                   int64_t *idx_minor, T *vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= dim_minor) return;
+    // displaced positions (> current step) and the values they hold
+    int64_t pos[MAXK], val[MAXK];
+    int nm = 0;
+    const int64_t base = i * vec_nnz;
+    for (int64_t j = 0; j < vec_nnz; ++j) {
+        uint4 w = philox4x32_10(ctr_add(ctr, (uint64_t)(base + j)), k0, k1);
+        const int64_t ell = j + (int64_t)((int64_t)w.x % (dim_major - j));
+        int64_t a = j, b = ell;
+        int ia = -1, ib = -1;
+        for (int t = 0; t < nm; ++t) {
+            if (pos[t] == j) { a = val[t]; ia = t; }
+            if (pos[t] == ell) { b = val[t]; ib = t; }
+        }
+        int64_t out = a;
+        if (ell != j) {
+            out = b;
+            if (ib >= 0) val[ib] = a;
+            else if (ia >= 0) { pos[ia] = ell; val[ia] = a; ia = -1; }  // reuse j's slot: j is final now
+            else { pos[nm] = ell; val[nm] = a; ++nm; }
+        }
+        if (ia >= 0) {  // drop position j (never read again)
+            pos[ia] = pos[nm - 1];
+            val[ia] = val[nm - 1];
+            --nm;
+        }
+        idx_major[base + j] = out;
+        if (idx_minor) idx_minor[base + j] = i;
+        vals[base + j] = (w.y % 2u == 0u) ? (T)1 : (T)-1;
+    }
+}
+
+// ---------------------------------------------------------------- LASO
+template <typename T, int MAXK>
+__global__ __launch_bounds__(256) void laso_kernel(Ctr128 ctr, uint32_t k0, uint32_t k1, int64_t vec_nnz,
+                                                   int64_t dim_major, int64_t dim_minor, int64_t *tmp_idx, T *tmp_val,
+                                                   int64_t *counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= dim_minor) return;
+    const int64_t per = (vec_nnz + 1) / 2;
+    const double dN = (double)dim_major;
+    int64_t uidx[MAXK];
+    int cnt[MAXK];
+    float sgn[MAXK];
+    int nu = 0;
+    uint4 w = make_uint4(0, 0, 0, 0);
+    for (int64_t s = 0; s < vec_nnz; ++s) {
+        if ((s & 1) == 0) w = philox4x32_10(ctr_add(ctr, (uint64_t)(i * per + s / 2)), k0, k1);
+        const uint32_t wi = (s & 1) ? w.z : w.x;
+        const uint32_t ws = (s & 1) ? w.w : w.y;
+        const double u = __dmul_rn(__dadd_rn((double)uneg11(wi), 1.0), 0.5);
+        const int64_t idx = (int64_t)__dmul_rn(dN, u);
+        const float sg = (uneg11(ws) >= 0.0f) ? 1.0f : -1.0f;
+        int f = -1;
+        for (int t = 0; t < nu; ++t)
+            if (uidx[t] == idx) { f = t; break; }
+        if (f >= 0) cnt[f] += 1;
+        else { uidx[nu] = idx; cnt[nu] = 1; sgn[nu] = sg; ++nu; }
+    }
+    for (int t = 0; t < nu; ++t) {
+        tmp_idx[i * vec_nnz + t] = uidx[t];
+        T c = (T)cnt[t];
+        tmp_val[i * vec_nnz + t] = (cnt[t] == 1) ? (T)sgn[t] : (T)sqrt(c) * (T)sgn[t];
+    }
+    counts[i] = nu;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void laso_compact_kernel(const int64_t *tmp_idx, const T *tmp_val,
+                                                           const int64_t *counts, const int64_t *offsets,
+                                                           int64_t vec_nnz, int64_t dim_minor, int64_t *idx_long,
+                                                           int64_t *idx_short, T *vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= dim_minor) return;
+    const int64_t o = offsets[i];
+    for (int64_t t = 0; t < counts[i]; ++t) {
+        idx_long[o + t] = tmp_idx[i * vec_nnz + t];
+        idx_short[o + t] = i;
+        vals[o + t] = tmp_val[i * vec_nnz + t];
+    }
+}
+
+template <typename T>
+static void launch_saso(Ctr128 c, uint32_t k0, uint32_t k1, int64_t k, int64_t dmaj, int64_t dmin, int64_t *imaj,
+                        int64_t *imin, T *vals, hipStream_t st) {
+    dim3 grid((unsigned)((dmin + 255) / 256));
+    ProfScope prof("saso_kernel", st);
+    if (k <= 8) hipLaunchKernelGGL((saso_kernel<T, 8>), grid, dim3(256), 0, st, c, k0, k1, k, dmaj, dmin, imaj, imin, vals);
+    else if (k <= 32) hipLaunchKernelGGL((saso_kernel<T, 32>), grid, dim3(256), 0, st, c, k0, k1, k, dmaj, dmin, imaj, imin, vals);
+    else if (k <= 128) hipLaunchKernelGGL((saso_kernel<T, 128>), grid, dim3(256), 0, st, c, k0, k1, k, dmaj, dmin, imaj, imin, vals);
+    else RBX_FAIL(RBX_UNSUPPORTED, "fill_sparse: SASO vec_nnz > 128 is not supported on the device");
+    RBX_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void launch_laso(Ctr128 c, uint32_t k0, uint32_t k1, int64_t k, int64_t dmaj, int64_t dmin, int64_t *tidx,
+                        T *tval, int64_t *counts, hipStream_t st) {
+    dim3 grid((unsigned)((dmin + 255) / 256));
+    ProfScope prof("laso_kernel", st);
+    if (k <= 8) hipLaunchKernelGGL((laso_kernel<T, 8>), grid, dim3(256), 0, st, c, k0, k1, k, dmaj, dmin, tidx, tval, counts);
+    else if (k <= 32) hipLaunchKernelGGL((laso_kernel<T, 32>), grid, dim3(256), 0, st, c, k0, k1, k, dmaj, dmin, tidx, tval, counts);
+    else if (k <= 128) hipLaunchKernelGGL((laso_kernel<T, 128>), grid, dim3(256), 0, st, c, k0, k1, k, dmaj, dmin, tidx, tval, counts);
+    else RBX_FAIL(RBX_UNSUPPORTED, "fill_sparse: LASO vec_nnz > 128 is not supported on the device");
+    RBX_HIP(hipGetLastError());
+}
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// fill_sparse_unpacked_nosub; returns nnz (synchronises the stream for LASO).
+template <typename T>
+static int64_t fill_sparse_impl(const rbx_sparse_dist &D, T *vals, int64_t *rows, int64_t *cols,
+                                const rbx_rng_state &seed, rbx_rng_state *next_state, hipStream_t st) {
+    SparseGeom g = sparse_geom(D);
+    RBX_REQUIRE(vals != nullptr && rows != nullptr && cols != nullptr);
+    const Ctr128 c = ctr_from_words(seed.counter);
+    int64_t *idxs_short = (D.n_rows <= D.n_cols) ? rows : cols;
+    int64_t *idxs_long = (D.n_rows <= D.n_cols) ? cols : rows;
+    int64_t nnz;
+    if (D.major_axis == 'S') {
+        launch_saso<T>(c, seed.key[0], seed.key[1], D.vec_nnz, g.dim_major, g.dim_minor, idxs_short, idxs_long, vals, st);
+        nnz = g.full_nnz;
+    } else {
+        // temporaries: idx, val, counts, offsets, scan storage
+        size_t b_idx = align256((size_t)g.full_nnz * 8), b_val = align256((size_t)g.full_nnz * sizeof(T));
+        size_t b_cnt = align256((size_t)g.dim_minor * 8);
+        size_t scan_bytes = 0;
+        RBX_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, (int64_t *)nullptr, (int64_t *)nullptr, (int64_t)0,
+                                        (size_t)g.dim_minor, rocprim::plus<int64_t>(), st));
+        char *ws = (char *)scratch(SCRATCH_SPARSE_OP, b_idx + b_val + 2 * b_cnt + align256(scan_bytes) + 256);
+        int64_t *tidx = (int64_t *)ws;
+        T *tval = (T *)(ws + b_idx);
+        int64_t *counts = (int64_t *)(ws + b_idx + b_val);
+        int64_t *offs = (int64_t *)(ws + b_idx + b_val + b_cnt);
+        void *tmp = ws + b_idx + b_val + 2 * b_cnt;
+        launch_laso<T>(c, seed.key[0], seed.key[1], D.vec_nnz, g.dim_major, g.dim_minor, tidx, tval, counts, st);
+        RBX_HIP(rocprim::exclusive_scan(tmp, scan_bytes, counts, offs, (int64_t)0, (size_t)g.dim_minor,
+                                        rocprim::plus<int64_t>(), st));
+        dim3 grid((unsigned)((g.dim_minor + 255) / 256));
+        hipLaunchKernelGGL(laso_compact_kernel<T>, grid, dim3(256), 0, st, tidx, tval, counts, offs, D.vec_nnz,
+                           g.dim_minor, idxs_long, idxs_short, vals);
+        RBX_HIP(hipGetLastError());
+        int64_t last_off = 0, last_cnt = 0;
+        RBX_HIP(hipMemcpyAsync(&last_off, offs + g.dim_minor - 1, 8, hipMemcpyDeviceToHost, st));
+        RBX_HIP(hipMemcpyAsync(&last_cnt, counts + g.dim_minor - 1, 8, hipMemcpyDeviceToHost, st));
+        RBX_HIP(hipStreamSynchronize(st));
+        nnz = last_off + last_cnt;
+    }
+    if (next_state) {
+        *next_state = seed;
+        ctr_to_words(ctr_add(c, (uint64_t)sparse_incr(D)), next_state->counter);
+    }
+    return nnz;
+}
+
+// ---------------------------------------------------------------- COO -> row CSR
+// key = r * m + c for entries inside the d x m window (after the offsets),
+// key = d * m (sorts last) otherwise.
+__global__ __launch_bounds__(256) void coo_keys_kernel(const int64_t *rows, const int64_t *cols, int64_t nnz,
+                                                       int64_t ro, int64_t co, int64_t d, int64_t m, uint64_t *keys,
+                                                       int64_t *perm) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = rows[e] - ro, c = cols[e] - co;
+        const bool in = r >= 0 && r < d && c >= 0 && c < m;
+        keys[e] = in ? (uint64_t)r * (uint64_t)m + (uint64_t)c : (uint64_t)d * (uint64_t)m;
+        perm[e] = e;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void csr_from_sorted_kernel(const uint64_t *keys, const int64_t *perm, const T *vals,
+                                                              int64_t nnz, int64_t d, int64_t m, int64_t *row_ptr,
+                                                              int32_t *col, T *val) {
+    const uint64_t dm = (uint64_t)d * (uint64_t)m;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[p];
+        const int64_t r = k < dm ? (int64_t)(k / (uint64_t)m) : d;
+        const int64_t rp = p == 0 ? -1 : (keys[p - 1] < dm ? (int64_t)(keys[p - 1] / (uint64_t)m) : d);
+        for (int64_t q = rp + 1; q <= r && q <= d; ++q) row_ptr[q] = p;
+        if (p == nnz - 1)
+            for (int64_t q = r + 1; q <= d; ++q) row_ptr[q] = nnz;
+        if (k < dm) {
+            col[p] = (int32_t)(k % (uint64_t)m);
+            val[p] = vals[perm[p]];
+        }
+    }
+}
+
+__global__ void zero_rowptr_kernel(int64_t *row_ptr, int64_t d) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= d; q += (int64_t)gridDim.x * blockDim.x)
+        row_ptr[q] = 0;
+}
+
+static int bits_for(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (x >> b) != 0) ++b;
+    return b < 1 ? 1 : b;
+}
+
+// ---------------------------------------------------------------- apply
+// Each thread owns RPT rows of the output window [r0, r0 + 256*RPT) and JB
+// columns [j0, j0+JB); op(B) rows are staged through LDS in chunks of CK.
+// C[r,j] = RN(beta*C) (or 0), then C += prod for entries in ascending column,
+// prod = val * RN(alpha * b)   (column-major kernels, csc_spmm_impl.hh:74-96)
+//      = RN(alpha * val) * b   (row-major kernel,    csc_spmm_impl.hh:154-209)
+constexpr int AP_JB = 4, AP_CK = 1024, AP_NT = 256;
+
+template <typename T, int RPT>
+__global__ __launch_bounds__(AP_NT) void csr_apply_kernel(const int64_t *row_ptr, const int32_t *col, const T *val,
+                                                          int64_t d, int64_t n, int64_t m, T alpha, T beta,
+                                                          const T *B, int64_t brs, int64_t bcs, T *C, int64_t crs,
+                                                          int64_t ccs, int kib) {
+    __shared__ T Bs[AP_CK * AP_JB];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * (AP_NT * RPT);
+    const int64_t j0 = (int64_t)blockIdx.y * AP_JB;
+    int64_t ptr[RPT], end[RPT];
+    T acc[RPT][AP_JB];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int64_t r = r0 + tid + q * AP_NT;
+        ptr[q] = end[q] = 0;
+        if (r < d) {
+            ptr[q] = row_ptr[r];
+            end[q] = row_ptr[r + 1];
+        }
+#pragma unroll
+        for (int jj = 0; jj < AP_JB; ++jj) {
+            T c0 = (T)0;
+            if (r < d && j0 + jj < n && beta != (T)0) c0 = beta * C[r * crs + (j0 + jj) * ccs];
+            acc[q][jj] = c0;
+        }
+    }
+    for (int64_t c0 = 0; c0 < m; c0 += AP_CK) {
+        // stage op(B)[c0:c0+CK, j0:j0+JB] -> Bs[c][jj]
+        if (brs == 1) {
+            for (int idx = tid; idx < AP_CK * AP_JB; idx += AP_NT) {
+                const int cc = idx % AP_CK, jj = idx / AP_CK;
+                const int64_t c = c0 + cc, j = j0 + jj;
+                Bs[cc * AP_JB + jj] = (c < m && j < n) ? B[c + j * bcs] : (T)0;
+            }
+        } else {
+            for (int idx = tid; idx < AP_CK * AP_JB; idx += AP_NT) {
+                const int jj = idx % AP_JB, cc = idx / AP_JB;
+                const int64_t c = c0 + cc, j = j0 + jj;
+                Bs[cc * AP_JB + jj] = (c < m && j < n) ? B[c * brs + j * bcs] : (T)0;
+            }
+        }
+        __syncthreads();
+        const int64_t cend = c0 + AP_CK;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            while (ptr[q] < end[q]) {
+                const int64_t c = col[ptr[q]];
+                if (c >= cend) break;
+                const T v = val[ptr[q]];
+                const T *b = &Bs[(c - c0) * AP_JB];
+                if (kib) {
+                    const T s = alpha * v;
+#pragma unroll
+                    for (int jj = 0; jj < AP_JB; ++jj) acc[q][jj] = acc[q][jj] + s * b[jj];
+                } else {
+#pragma unroll
+                    for (int jj = 0; jj < AP_JB; ++jj) acc[q][jj] = acc[q][jj] + v * (alpha * b[jj]);
+                }
+                ++ptr[q];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int64_t r = r0 + tid + q * AP_NT;
+        if (r >= d) continue;
+#pragma unroll
+        for (int jj = 0; jj < AP_JB; ++jj)
+            if (j0 + jj < n) C[r * crs + (j0 + jj) * ccs] = acc[q][jj];
+    }
+}
+
+template <typename T>
+static void scale_only(T *C, int64_t d, int64_t n, int64_t crs, int64_t ccs, T beta, hipStream_t st);
+
+template <typename T>
+__global__ __launch_bounds__(256) void sp_scale_kernel(T *C, int64_t d, int64_t n, int64_t crs, int64_t ccs, T beta) {
+    const int64_t total = d * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % d, j = idx / d;
+        T *c = C + i * crs + j * ccs;
+        *c = (beta == (T)0) ? (T)0 : beta * (*c);
+    }
+}
+
+template <typename T>
+static void scale_only(T *C, int64_t d, int64_t n, int64_t crs, int64_t ccs, T beta, hipStream_t st) {
+    if (d <= 0 || n <= 0) return;
+    int64_t blocks = (d * n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(sp_scale_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, C, d, n, crs, ccs, beta);
+    RBX_HIP(hipGetLastError());
+}
+
+// left_spmm for a COO operator (spmm_dispatch.hh:52-179):
+// C = alpha * op(submat(A)) * op(B) + beta * C, A given by COO arrays.
+template <typename T>
+static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n, int64_t m, T alpha, int64_t A_rows,
+                          int64_t A_cols, int64_t nnz, const T *vals, const int64_t *rows, const int64_t *cols,
+                          int64_t ro_a, int64_t co_a, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
+                          hipStream_t st) {
+    check_layout(layout);
+    check_op(opA);
+    check_op(opB);
+    RBX_REQUIRE(d >= 0 && n >= 0 && m >= 0);
+    if (opA == 'T') {  // transposed view (spmm_dispatch.hh:72-90)
+        const int64_t *t = rows; rows = cols; cols = t;
+        int64_t x = A_rows; A_rows = A_cols; A_cols = x;
+        x = ro_a; ro_a = co_a; co_a = x;
+    }
+    RBX_REQUIRE(A_rows >= d);
+    RBX_REQUIRE(A_cols >= m);
+    int64_t rows_B = opB == 'N' ? m : n, cols_B = opB == 'N' ? n : m;
+    char layout_opB = opB == 'N' ? layout : (layout == 'C' ? 'R' : 'C');
+    if (layout == 'C') {
+        RBX_REQUIRE(ldb >= rows_B);
+        RBX_REQUIRE(ldc >= d);
+    } else {
+        RBX_REQUIRE(ldc >= n);
+        RBX_REQUIRE(ldb >= cols_B);
+    }
+    const int64_t crs = layout == 'C' ? 1 : ldc, ccs = layout == 'C' ? ldc : 1;
+    if (d == 0 || n == 0) return;
+    if (alpha == (T)0 || m == 0 || nnz <= 0) {
+        scale_only<T>(C, d, n, crs, ccs, beta, st);
+        return;
+    }
+    RBX_REQUIRE(m < (int64_t)INT32_MAX);
+    // op(B)[c, j] strides
+    const int64_t brs = layout_opB == 'C' ? 1 : ldb, bcs = layout_opB == 'C' ? ldb : 1;
+    const int kib = (layout_opB == 'R' && layout == 'R') ? 1 : 0;
+    safe_mul(d + 1, m);
+
+    // scratch: keys in/out, perm in/out, row_ptr, col, val, sort storage
+    size_t sort_bytes = 0;
+    const int end_bit = bits_for((uint64_t)d * (uint64_t)m);
+    RBX_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                      (int64_t *)nullptr, (int64_t *)nullptr, (size_t)nnz, 0, end_bit, st));
+    const size_t bk = align256((size_t)nnz * 8);
+    const size_t brp = align256((size_t)(d + 1) * 8), bcol = align256((size_t)nnz * 4), bval = align256((size_t)nnz * sizeof(T));
+    char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, 4 * bk + brp + bcol + bval + align256(sort_bytes));
+    uint64_t *keys_in = (uint64_t *)ws, *keys_out = (uint64_t *)(ws + bk);
+    int64_t *perm_in = (int64_t *)(ws + 2 * bk), *perm_out = (int64_t *)(ws + 3 * bk);
+    int64_t *row_ptr = (int64_t *)(ws + 4 * bk);
+    int32_t *ccol = (int32_t *)(ws + 4 * bk + brp);
+    T *cval = (T *)(ws + 4 * bk + brp + bcol);
+    void *sort_tmp = ws + 4 * bk + brp + bcol + bval;
+    {
+        ProfScope prof("coo_to_csr", st);
+        int64_t blocks = (nnz + 255) / 256;
+        if (blocks > 16384) blocks = 16384;
+        hipLaunchKernelGGL(coo_keys_kernel, dim3((unsigned)blocks), dim3(256), 0, st, rows, cols, nnz, ro_a, co_a, d, m,
+                           keys_in, perm_in);
+        RBX_HIP(hipGetLastError());
+        RBX_HIP(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_in, keys_out, perm_in, perm_out, (size_t)nnz, 0,
+                                          end_bit, st));
+        hipLaunchKernelGGL(zero_rowptr_kernel, dim3(64), dim3(256), 0, st, row_ptr, d);
+        hipLaunchKernelGGL(csr_from_sorted_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, keys_out, perm_out, vals,
+                           nnz, d, m, row_ptr, ccol, cval);
+        RBX_HIP(hipGetLastError());
+    }
+    {
+        ProfScope prof("csr_apply_kernel", st);
+        const int64_t nJ = (n + AP_JB - 1) / AP_JB;
+        RBX_REQUIRE(nJ <= 65535);
+        if (d <= 256) {
+            dim3 grid((unsigned)((d + 255) / 256), (unsigned)nJ);
+            hipLaunchKernelGGL((csr_apply_kernel<T, 1>), grid, dim3(AP_NT), 0, st, row_ptr, ccol, cval, d, n, m, alpha,
+                               beta, B, brs, bcs, C, crs, ccs, kib);
+        } else if (d <= 1024) {
+            dim3 grid((unsigned)((d + 1023) / 1024), (unsigned)nJ);
+            hipLaunchKernelGGL((csr_apply_kernel<T, 4>), grid, dim3(AP_NT), 0, st, row_ptr, ccol, cval, d, n, m, alpha,
+                               beta, B, brs, bcs, C, crs, ccs, kib);
+        } else {
+            dim3 grid((unsigned)((d + 2047) / 2048), (unsigned)nJ);
+            hipLaunchKernelGGL((csr_apply_kernel<T, 8>), grid, dim3(AP_NT), 0, st, row_ptr, ccol, cval, d, n, m, alpha,
+                               beta, B, brs, bcs, C, crs, ccs, kib);
+        }
+        RBX_HIP(hipGetLastError());
+    }
+}
+
+// Sampled operator arrays for the duration of a call (skge.hh:483-487).
+template <typename T>
+struct SkopView {
+    int64_t nnz;
+    const T *vals;
+    const int64_t *rows, *cols;
+};
+
+template <typename T>
+static SkopView<T> view_or_sample(const rbx_sparse_skop *S, hipStream_t st) {
+    RBX_REQUIRE(S != nullptr);
+    SparseGeom g = sparse_geom(S->dist);
+    if (S->nnz >= 0) {
+        RBX_REQUIRE(S->nnz == 0 || (S->vals && S->rows && S->cols));
+        RBX_REQUIRE(S->nnz <= g.full_nnz);
+        return {S->nnz, (const T *)S->vals, S->rows, S->cols};
+    }
+    const size_t bv = align256((size_t)g.full_nnz * sizeof(T)), bi = align256((size_t)g.full_nnz * 8);
+    // The sampled operator lives in the dense-operator slot: LASO sampling
+    // itself uses the sparse-operator slot for its temporaries.
+    char *ws = (char *)scratch(SCRATCH_DENSE_OP, bv + 2 * bi);
+    T *vals = (T *)ws;
+    int64_t *rows = (int64_t *)(ws + bv), *cols = (int64_t *)(ws + bv + bi);
+    int64_t nnz = fill_sparse_impl<T>(S->dist, vals, rows, cols, S->seed_state, nullptr, st);
+    return {nnz, vals, rows, cols};
+}
+
+template <typename T>
+static void lskges_impl(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, T alpha,
+                        const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, const T *A, int64_t lda, T beta, T *B,
+                        int64_t ldb, hipStream_t st) {
+    SkopView<T> v = view_or_sample<T>(S, st);
+    left_spmm_coo<T>(layout, opS, opA, d, n, m, alpha, S->dist.n_rows, S->dist.n_cols, v.nnz, v.vals, v.rows, v.cols,
+                     ro_s, co_s, A, lda, beta, B, ldb, st);
+}
+
+// rskges -> right_spmm -> left_spmm with flipped layout and op (spmm_dispatch.hh:181-219)
+template <typename T>
+static void rskges_impl(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, T alpha, const T *A,
+                        int64_t lda, const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, T beta, T *B, int64_t ldb,
+                        hipStream_t st) {
+    check_layout(layout);
+    check_op(opS);
+    SkopView<T> v = view_or_sample<T>(S, st);
+    char trans_opS = opS == 'N' ? 'T' : 'N';
+    char trans_layout = layout == 'C' ? 'R' : 'C';
+    left_spmm_coo<T>(trans_layout, trans_opS, opA, d, m, n, alpha, S->dist.n_rows, S->dist.n_cols, v.nnz, v.vals,
+                     v.rows, v.cols, ro_s, co_s, A, lda, beta, B, ldb, st);
+}
+
+// ---------------------------------------------------------------- CSR data x dense operator
+// left_spmm for CSR, opA = NoTrans (csr_spmm_impl.hh:51-155).  One wave per
+// output row; lanes walk output columns.
+//   jik: C[i,j] = RN(beta*C) + RN(alpha * sum_l RN(val_l * B[col_l, j]))
+//   ikb: C[i,j] = RN(beta*C), then C += RN(RN(alpha*val_l) * B[col_l, j])
+template <typename T>
+__global__ __launch_bounds__(256) void csr_rows_kernel(const int64_t *rowptr, const int64_t *colidx, const T *vals,
+                                                       int64_t mrows, int64_t ncols_out, T alpha, T beta, const T *B,
+                                                       int64_t brs, int64_t bcs, T *C, int64_t crs, int64_t ccs,
+                                                       int ikb) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= mrows) return;
+    const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
+    for (int64_t j = lane; j < ncols_out; j += 64) {
+        T *c = C + i * crs + j * ccs;
+        T acc = (beta == (T)0) ? (T)0 : beta * (*c);
+        if (alpha != (T)0) {
+            if (ikb) {
+                for (int64_t p = p0; p < p1; ++p) acc = acc + (alpha * vals[p]) * B[colidx[p] * brs + j * bcs];
+            } else {
+                T diff = (T)0;
+                for (int64_t p = p0; p < p1; ++p) diff = diff + vals[p] * B[colidx[p] * brs + j * bcs];
+                acc = acc + alpha * diff;
+            }
+        }
+        *c = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void csr_to_coo_rows_kernel(const int64_t *rowptr, int64_t nrows, int64_t *rows) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) rows[p] = i;
+}
+
+template <typename T>
+static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, T alpha,
+                        const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a, const rbx_dense_skop *S, int64_t ro_s,
+                        int64_t co_s, T beta, T *B, int64_t ldb, hipStream_t st) {
+    RBX_REQUIRE(A != nullptr && S != nullptr);
+    check_layout(layout);
+    check_op(opA);
+    check_op(opS);
+    RBX_REQUIRE(m >= 0 && d >= 0 && n >= 0);
+    DenseGeom geo = dense_geom(S->dist);
+    safe_mul(geo.dim_minor, geo.P);
+    int64_t rows_S = opS == 'N' ? n : d, cols_S = opS == 'N' ? d : n;
+    int64_t rows_A = opA == 'N' ? m : n, cols_A = opA == 'N' ? n : m;
+    RBX_REQUIRE(A->n_rows >= rows_A + ro_a);
+    RBX_REQUIRE(A->n_cols >= cols_A + co_a);
+    RBX_REQUIRE(S->dist.n_rows >= rows_S + ro_s);
+    RBX_REQUIRE(S->dist.n_cols >= cols_S + co_s);
+    if (layout == 'C') RBX_REQUIRE(ldb >= m);
+    else RBX_REQUIRE(ldb >= d);
+    // CSR (and its CSC transpose) require the whole matrix (spmm_dispatch.hh:103-110)
+    RBX_REQUIRE(A->n_rows == rows_A && A->n_cols == cols_A);
+    RBX_REQUIRE(ro_a == 0 && co_a == 0);
+    if (m == 0 || d == 0) return;
+    const int64_t crs = layout == 'C' ? 1 : ldb, ccs = layout == 'C' ? ldb : 1;
+    // The reference materialises submat(S) in its natural layout and flips opS
+    // when that differs from `layout` (sksp.hh:298-324); the product kernel it
+    // then picks depends on that layout, which fixes the summation form.
+    char opS_ref = (geo.natural != layout) ? (opS == 'N' ? 'T' : 'N') : opS;
+    char layout_opB = opS_ref == 'N' ? layout : (layout == 'C' ? 'R' : 'C');
+    // Materialise G = op(submat(S)) (n x d) row-major: lanes read rows of G.
+    rbx_dense_skop Sg = *S;
+    size_t gbytes = (size_t)safe_mul(n, d) * sizeof(T);
+    T *G = (T *)scratch(SCRATCH_DENSE_OP, gbytes);
+    if (S->buff != nullptr) {
+        RBX_FAIL(RBX_UNSUPPORTED, "rsksp3: materialised operators are not supported; pass buff == NULL");
+    }
+    GenMap gm = gen_map(Sg, opS, ro_s, co_s, false);
+    fill_generated<T>(gm, n, d, G, d, 1, st);
+    if (opA == 'N') {
+        if (alpha == (T)0 || n == 0 || A->nnz == 0) {
+            // beta scaling only (util::safe_scal); reuse the row kernel with alpha = 0
+        }
+        const int ikb = (layout_opB == 'R' && layout == 'R') ? 1 : 0;
+        ProfScope prof("csr_rows_kernel", st);
+        dim3 grid((unsigned)((m + 3) / 4));
+        hipLaunchKernelGGL(csr_rows_kernel<T>, grid, dim3(256), 0, st, A->rowptr, A->colidxs, (const T *)A->vals, m, d,
+                           alpha, beta, (const T *)G, (int64_t)d, (int64_t)1, B, crs, ccs, ikb);
+        RBX_HIP(hipGetLastError());
+        return;
+    }
+    // opA = Trans: the reference transposes the CSR matrix into CSC and runs the
+    // CSC kernels, whose per-entry order is ascending column -- the COO path.
+    int64_t *rows = (int64_t *)scratch(SCRATCH_SPARSE_OP, (size_t)(A->nnz > 0 ? A->nnz : 1) * 8);
+    if (A->nnz > 0) {
+        dim3 grid((unsigned)((A->n_rows + 255) / 256));
+        hipLaunchKernelGGL(csr_to_coo_rows_kernel, grid, dim3(256), 0, st, A->rowptr, A->n_rows, rows);
+        RBX_HIP(hipGetLastError());
+    }
+    // G as op(B) in the reference's orientation: B_ref = G with layout_opB rows
+    // k (inner) and columns j; we hand left_spmm a view with those strides.
+    char opG = 'N';
+    char layG = layout;  // G stored row-major n x d; express op(B) for `layout`
+    int64_t ldg;
+    if (layout == 'C') { opG = 'T'; ldg = d; }  // row-major G == column-major G^T
+    else { opG = 'N'; ldg = d; }
+    (void)layG;
+    left_spmm_coo<T>(layout, 'T', opG, m, d, n, alpha, A->n_rows, A->n_cols, A->nnz, (const T *)A->vals, rows,
+                     A->colidxs, 0, 0, G, ldg, beta, B, ldb, st);
+}
+
+template <typename T>
+static void fill_sparse_skop(rbx_sparse_skop *S, hipStream_t st) {
+    RBX_REQUIRE(S != nullptr);
+    SparseGeom g = sparse_geom(S->dist);
+    if (S->vals == nullptr && S->rows == nullptr && S->cols == nullptr) {
+        void *v = nullptr, *r = nullptr, *c = nullptr;
+        if (hipMalloc(&v, (size_t)g.full_nnz * sizeof(T)) != hipSuccess ||
+            hipMalloc(&r, (size_t)g.full_nnz * 8) != hipSuccess || hipMalloc(&c, (size_t)g.full_nnz * 8) != hipSuccess) {
+            if (v) (void)hipFree(v);
+            if (r) (void)hipFree(r);
+            if (c) (void)hipFree(c);
+            RBX_FAIL(RBX_OUT_OF_MEMORY, "fill_sparse: hipMalloc failed");
+        }
+        S->vals = v;
+        S->rows = (int64_t *)r;
+        S->cols = (int64_t *)c;
+        S->own_memory = 1;
+    }
+    RBX_REQUIRE(S->vals != nullptr && S->rows != nullptr && S->cols != nullptr);
+    S->nnz = fill_sparse_impl<T>(S->dist, (T *)S->vals, S->rows, S->cols, S->seed_state, nullptr, st);
+}
+
+}  // namespace rbx
+
+using namespace rbx;
+
+extern "C" {
+
+rbx_status rbx_sparse_dist_props(const rbx_sparse_dist *dist, int64_t *dim_major, int64_t *dim_minor,
+                                 int64_t *full_nnz, double *isometry_scale) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(dist != nullptr);
+    SparseGeom g = sparse_geom(*dist);
+    if (dim_major) *dim_major = g.dim_major;
+    if (dim_minor) *dim_minor = g.dim_minor;
+    if (full_nnz) *full_nnz = g.full_nnz;
+    // sparse::isometry_scale (sparse_skops.hh:108-114)
+    if (isometry_scale)
+        *isometry_scale = dist->major_axis == 'S'
+                              ? pow((double)dist->vec_nnz, -0.5)
+                              : sqrt((double)g.dim_major / ((double)dist->vec_nnz * (double)g.dim_minor));
+    RBX_API_END
+}
+
+rbx_status rbx_sparse_skop_init(const rbx_sparse_dist *dist, const rbx_rng_state *seed, rbx_sparse_skop *out) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(dist != nullptr && seed != nullptr && out != nullptr);
+    sparse_geom(*dist);
+    memset(out, 0, sizeof(*out));
+    out->dist = *dist;
+    out->seed_state = *seed;
+    out->next_state = *seed;
+    ctr_to_words(ctr_add(ctr_from_words(seed->counter), (uint64_t)sparse_incr(*dist)), out->next_state.counter);
+    out->nnz = -1;
+    RBX_API_END
+}
+
+rbx_status rbx_sfill_sparse_unpacked_nosub(const rbx_sparse_dist *D, int64_t *nnz, float *vals, int64_t *rows,
+                                           int64_t *cols, const rbx_rng_state *seed, rbx_rng_state *next_state,
+                                           void *stream) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(D != nullptr && seed != nullptr && nnz != nullptr);
+    *nnz = fill_sparse_impl<float>(*D, vals, rows, cols, *seed, next_state, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_dfill_sparse_unpacked_nosub(const rbx_sparse_dist *D, int64_t *nnz, double *vals, int64_t *rows,
+                                           int64_t *cols, const rbx_rng_state *seed, rbx_rng_state *next_state,
+                                           void *stream) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(D != nullptr && seed != nullptr && nnz != nullptr);
+    *nnz = fill_sparse_impl<double>(*D, vals, rows, cols, *seed, next_state, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_sfill_sparse_skop(rbx_sparse_skop *S, void *stream) {
+    RBX_API_BEGIN
+    fill_sparse_skop<float>(S, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_dfill_sparse_skop(rbx_sparse_skop *S, void *stream) {
+    RBX_API_BEGIN
+    fill_sparse_skop<double>(S, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_sparse_skop_free(rbx_sparse_skop *S) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(S != nullptr);
+    if (S->own_memory) {
+        if (S->vals) RBX_HIP(hipFree(S->vals));
+        if (S->rows) RBX_HIP(hipFree(S->rows));
+        if (S->cols) RBX_HIP(hipFree(S->cols));
+    }
+    S->vals = nullptr;
+    S->rows = S->cols = nullptr;
+    S->nnz = -1;
+    S->own_memory = 0;
+    RBX_API_END
+}
+
+rbx_status rbx_slskges(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, float alpha,
+                       const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, const float *A, int64_t lda, float beta,
+                       float *B, int64_t ldb, void *stream) {
+    RBX_API_BEGIN
+    lskges_impl<float>(layout, opS, opA, d, n, m, alpha, S, ro_s, co_s, A, lda, beta, B, ldb, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_dlskges(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, double alpha,
+                       const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, const double *A, int64_t lda, double beta,
+                       double *B, int64_t ldb, void *stream) {
+    RBX_API_BEGIN
+    lskges_impl<double>(layout, opS, opA, d, n, m, alpha, S, ro_s, co_s, A, lda, beta, B, ldb, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_srskges(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, float alpha, const float *A,
+                       int64_t lda, const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, float beta, float *B,
+                       int64_t ldb, void *stream) {
+    RBX_API_BEGIN
+    rskges_impl<float>(layout, opA, opS, m, d, n, alpha, A, lda, S, ro_s, co_s, beta, B, ldb, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_drskges(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, double alpha, const double *A,
+                       int64_t lda, const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, double beta, double *B,
+                       int64_t ldb, void *stream) {
+    RBX_API_BEGIN
+    rskges_impl<double>(layout, opA, opS, m, d, n, alpha, A, lda, S, ro_s, co_s, beta, B, ldb, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_srsksp3(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, float alpha,
+                       const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a, const rbx_dense_skop *S, int64_t ro_s,
+                       int64_t co_s, float beta, float *B, int64_t ldb, void *stream) {
+    RBX_API_BEGIN
+    rsksp3_impl<float>(layout, opA, opS, m, d, n, alpha, A, ro_a, co_a, S, ro_s, co_s, beta, B, ldb, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_drsksp3(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, double alpha,
+                       const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a, const rbx_dense_skop *S, int64_t ro_s,
+                       int64_t co_s, double beta, double *B, int64_t ldb, void *stream) {
+    RBX_API_BEGIN
+    rsksp3_impl<double>(layout, opA, opS, m, d, n, alpha, A, ro_a, co_a, S, ro_s, co_s, beta, B, ldb, (hipStream_t)stream);
+    RBX_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+"""GPU parity for sparse operators and sparse products against the oracle.
+
+Bars: SASO / LASO index arrays and values bit-exact (LASO vectors with
+duplicates are written in first-occurrence order by both sides); lskges /
+rskges bit-exact with +-1 operators (same per-entry summation order as the
+reference's fallback kernels), FP32 and FP64; rsksp3 bit-exact in the
+reference's dot-product form.  The SASO left sketch of BASELINE's sparse
+configuration (S 2000 x 100000, vec_nnz 8, key 7; A 100000 x 500) runs at
+full size.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def view(buf, r, c, layout, ld):
+    st = buf.itemsize
+    return np.lib.stride_tricks.as_strided(buf, (r, c), (st, ld * st) if layout == "C" else (ld * st, st))
+
+
+@pytest.mark.parametrize("shape,k,axis", [((2000, 100000), 8, "S"), ((300, 20), 5, "S"), ((64, 64), 64, "S"),
+                                          ((30, 500), 40, "L"), ((700, 40), 9, "L"), ((1, 1), 1, "S")])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fill_sparse_bit_exact(rb, orc, shape, k, axis, dtype):
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    D = rb.SparseDist(shape[0], shape[1], k, rb.Axis(axis))
+    full = D.full_nnz
+    vals = torch.empty(full, dtype=tdt, device=DEV)
+    rows = torch.empty(full, dtype=torch.int64, device=DEV)
+    cols = torch.empty(full, dtype=torch.int64, device=DEV)
+    nnz, nxt = rb.fill_sparse_unpacked_nosub(D, vals, rows, cols, rb.RNGState(7, (3, 0, 0, 0)))
+    ov, orr, oc, onnz, onxt = orc.fill_sparse(shape[0], shape[1], k, axis, [3, 0, 0, 0], [7, 0], dtype=dtype)
+    assert nnz == onnz
+    assert np.array_equal(host(rows)[:nnz], orr)
+    assert np.array_equal(host(cols)[:nnz], oc)
+    assert np.array_equal(host(vals)[:nnz], ov)
+    assert list(nxt.counter) == [int(x) for x in onxt]
+
+
+def _lsk(rb, orc, dtype, layout, opS, opA, Sshape, k, axis, d, n, m, ro, co, alpha, beta, prefill, seed=0):
+    rng = np.random.default_rng(seed)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    rA, cA = (m, n) if opA == "N" else (n, m)
+    lda = (rA if layout == "C" else cA) + 1
+    ldb = (d if layout == "C" else n) + 2
+    A = rng.uniform(-1, 1, lda * (cA if layout == "C" else rA)).astype(dtype)
+    B0 = rng.uniform(-1, 1, ldb * (n if layout == "C" else d)).astype(dtype)
+    S = rb.SparseSkOp(rb.SparseDist(Sshape[0], Sshape[1], k, rb.Axis(axis)), rb.RNGState(7), dtype=tdt)
+    if prefill:
+        rb.fill_sparse(S)
+    Bd = torch.from_numpy(B0.copy()).to(DEV)
+    rb.lskges(layout, opS, opA, d, n, m, alpha, S, ro, co, torch.from_numpy(A).to(DEV), lda, beta, Bd, ldb)
+    got = host(Bd)
+    vals, rows, cols, nnz, _ = orc.fill_sparse(Sshape[0], Sshape[1], k, axis, [0, 0, 0, 0], [7, 0], dtype=dtype)
+    ref = B0.copy()
+    orc.left_spmm_coo("d" if dtype == np.float64 else "s", layout, opS, opA, d, n, m, alpha, Sshape, vals, rows,
+                      cols, ro, co, A, lda, beta, ref, ldb)
+    return view(got, d, n, layout, ldb), view(ref, d, n, layout, ldb)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lskges_saso_config_full_size(rb, orc, dtype):
+    """S = SparseDist(2000, 100000, vec_nnz 8, Short), key 7; A 100000 x 500 ColMajor Uniform(-1,1)."""
+    got, ref = _lsk(rb, orc, dtype, "C", "N", "N", (2000, 100000), 8, "S", 2000, 500, 100000, 0, 0, 1.0, 0.0,
+                    prefill=False)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("opS", ["N", "T"])
+@pytest.mark.parametrize("opA", ["N", "T"])
+def test_lskges_variants(rb, orc, layout, opS, opA):
+    Sshape = (60, 900)  # submat(S) is (d x m) for NoTrans and (m x d) for Trans
+    d, m = (50, 880) if opS == "N" else (880, 50)
+    got, ref = _lsk(rb, orc, np.float64, layout, opS, opA, Sshape, 4, "S", d, 33, m, 3, 7, 1.0, -0.5, prefill=True)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("axis", ["S", "L"])
+def test_lskges_general_alpha(rb, orc, axis):
+    got, ref = _lsk(rb, orc, np.float64, "R", "N", "N", (40, 3000), 24, axis, 40, 20, 3000, 0, 0, 0.37, 1.5,
+                    prefill=False)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("opS", ["N", "T"])
+def test_rskges(rb, orc, layout, opS):
+    rng = np.random.default_rng(9)
+    m, d, n = 41, 30, 1200
+    Sshape = (n, d) if opS == "N" else (d, n)
+    lda = m if layout == "C" else n
+    ldb = m if layout == "C" else d
+    A = rng.uniform(-1, 1, m * n)
+    B0 = rng.uniform(-1, 1, m * d)
+    S = rb.SparseSkOp(rb.SparseDist(*Sshape, 6), rb.RNGState(13))
+    Bd = torch.from_numpy(B0.copy()).to(DEV)
+    rb.rskges(layout, "N", opS, m, d, n, 1.0, torch.from_numpy(A).to(DEV), lda, S, 0, 0, 2.0, Bd, ldb)
+    vals, rows, cols, _, _ = orc.fill_sparse(*Sshape, 6, "S", [0, 0, 0, 0], [13, 0])
+    ref = B0.copy()
+    # right_spmm -> left_spmm with flipped layout and op (spmm_dispatch.hh:181-219)
+    tl = "R" if layout == "C" else "C"
+    orc.left_spmm_coo("d", tl, "T" if opS == "N" else "N", "N", d, m, n, 1.0, Sshape, vals, rows, cols, 0, 0, A,
+                      lda, 2.0, ref, ldb)
+    assert np.array_equal(host(Bd), ref)
+
+
+def _csr(rng, nrows, ncols, mean_nnz):
+    deg = rng.poisson(mean_nnz, nrows)
+    rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.int64)
+    cols = np.concatenate([np.sort(rng.choice(ncols, size=min(k, ncols), replace=False)) for k in deg]).astype(np.int64)
+    rowptr = np.concatenate([[0], np.cumsum(np.minimum(deg, ncols))]).astype(np.int64)
+    vals = rng.standard_normal(len(cols))
+    return rowptr, cols, vals
+
+
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rsksp3_csr(rb, orc, layout, dtype):
+    rng = np.random.default_rng(17)
+    m, n, d = 500, 3000, 64
+    rowptr, colidx, vals = _csr(rng, m, n, 20)
+    vals = vals.astype(dtype)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    S = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(42))  # tall, natural ColMajor
+    Acsr = rb.CSRMatrix(m, n, torch.from_numpy(vals).to(DEV), torch.from_numpy(rowptr).to(DEV),
+                        torch.from_numpy(colidx).to(DEV))
+    ldb = m if layout == "C" else d
+    B = torch.zeros(m * d, dtype=tdt, device=DEV)
+    rb.rsksp3(layout, "N", "N", m, d, n, 1.0, Acsr, 0, 0, S, 0, 0, 0.0, B, ldb)
+    got = host(B)
+    # reference: materialise S in its natural layout, then left_spmm with opS
+    # flipped when layouts differ (sksp.hh:298-324)
+    Sm, _ = orc.fill_dense(n, d, [0, 0, 0, 0], [42, 0], dtype=dtype)  # n x d values
+    nat = orc.natural_layout(n, d, "L")
+    Sbuf = np.ascontiguousarray(Sm.reshape(-1, order="F" if nat == "C" else "C"))
+    lds = n if nat == "C" else d
+    opS_ref = "N" if nat == layout else "T"
+    ref = np.zeros(m * d, dtype)
+    orc.left_spmm_csr("d" if dtype == np.float64 else "s", layout, opS_ref, m, d, n, 1.0, vals, rowptr, colidx,
+                      Sbuf, lds, 0.0, ref, ldb)
+    assert np.array_equal(got, ref)
+
+
+def test_unsampled_right_sketch_returns(rb):
+    # rskges on an operator with nnz < 0 samples it and returns (the reference's
+    # skge.hh:616-620 falls through to a failing requirement instead)
+    S = rb.SparseSkOp(rb.SparseDist(100, 10, 3), rb.RNGState(2))
+    A = torch.randn(5 * 100, dtype=torch.float64, device=DEV)
+    B = torch.empty(5 * 10, dtype=torch.float64, device=DEV)
+    rb.rskges("C", "N", "N", 5, 10, 100, 1.0, A, 5, S, 0, 0, 0.0, B, 5)
+    assert torch.isfinite(B).all()
