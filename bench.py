@@ -34,8 +34,25 @@ PEAK_BF16 = 2500.0   # TFLOP/s, MI355X dense BF16 MFMA
 PEAK_FP64 = 78.6     # TFLOP/s, MI355X FP64 MFMA
 
 
+PROFILE_MD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                          "bf16_skge_kernel_262144x1024_d4096.md")
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` at this workload (T = 3), as measured by
+    rocprofv3 FETCH_SIZE / WRITE_SIZE and recorded in the profile summary."""
+    try:
+        with open(PROFILE_MD) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -49,6 +66,9 @@ def main():
     ap.add_argument("--err-rows", type=int, default=64, help="rows of B checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip fp64 / error / distortion legs")
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="first run one plain device copy of this many GiB (calibrates traffic counters)")
+    ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 sketch (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -71,6 +91,22 @@ def main():
     A.view(N, M).mul_(scales.view(N, 1))
     S = rb.DenseSkOp(rb.DenseDist(D, M), rb.RNGState(KEY_S))
     B = torch.empty(D * N, dtype=torch.float32, device=dev)
+    if args.copy_probe_gib > 0:
+        n_el = int(args.copy_probe_gib * (1 << 30)) // 4
+        src = torch.ones(n_el, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {n_el * 4} bytes read, {n_el * 4} bytes written")
+        del src, dst
+    if args.fp64_only:
+        A64 = A.double()
+        B64 = torch.empty(D * N, dtype=torch.float64, device=dev)
+        for _ in range(args.warmup + args.steps):
+            rb.lskge3("C", "N", "N", D, N, M, 1.0, S, 0, 0, A64, M, 0.0, B64, D)
+        torch.cuda.synchronize()
+        log("fp64-only done")
+        return
 
     def step():
         rb.lskge3_bf16split("C", "N", "N", D, N, M, 1.0, S, 0, 0, A, M, 0.0, B, D, n_terms=args.terms)
@@ -99,7 +135,8 @@ def main():
     roofline = {"bound": "mfma", "kernel": "bf16_skge_kernel", "achieved": round(achieved, 2),
                 "peak": PEAK_BF16, "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16, 4),
                 "executed_mfma_frac": round(achieved * args.terms / PEAK_BF16, 4),
-                "kernel_ms": round(k_ms, 4), "traffic": None,
+                "kernel_ms": round(k_ms, 4),
+                "traffic": profiled_traffic("bf16_skge_kernel") if args.terms == 3 else None,
                 "per_kernel_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}}
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "TFLOP/s", "n_gpus": 1,

@@ -287,6 +287,15 @@ rbx_status rbx_drsksp3(char layout, char opA, char opS, int64_t m, int64_t d, in
                        const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, double beta,
                        double *B, int64_t ldb, void *stream);
 
+/* ---------------------------------------------------------------- utilities
+ * util::require_symmetric (RandBLAS/util.hh:127-148), the check that
+ * sketch_symmetric runs first (sksy.hh:175,310): fails with RBX_INVALID_ARG
+ * and the reference's message for the first pair (i < j, row-major loop
+ * order) with |A(i,j) - A(j,i)| > (|A(i,j)| + |A(j,i)| + 1) * tol, computed in
+ * the matrix's precision.  tol < 0 skips the check.  Synchronises `stream`. */
+rbx_status rbx_srequire_symmetric(char layout, const float *A, int64_t n, int64_t lda, float tol, void *stream);
+rbx_status rbx_drequire_symmetric(char layout, const double *A, int64_t n, int64_t lda, double tol, void *stream);
+
 /* ---------------------------------------------------------------- diagnostics
  * (no reference counterpart) Kernel timing for benchmarks: while enabled,
  * every kernel the library launches is bracketed by HIP events on its stream.

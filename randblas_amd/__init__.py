@@ -142,6 +142,8 @@ _SIG = {
                     _I64, _P],
     "rbx_drsksp3": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _CSR, _I64, _I64, _DS, _I64, _I64, _D, _P,
                     _I64, _P],
+    "rbx_srequire_symmetric": [_CH, _P, _I64, _I64, _F, _P],
+    "rbx_drequire_symmetric": [_CH, _P, _I64, _I64, _D, _P],
     "rbx_profile_enable": [C.c_int],
     "rbx_profile_collect": [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int],
 }
@@ -349,7 +351,7 @@ def rskge3_adjoint(layout, opS, m, d, n, alpha, Bbar, ldbb, S: DenseSkOp, ro_s, 
 
 
 def lskge3_bf16split(layout, opS, opA, d, n, m, alpha, S: DenseSkOp, ro_s, co_s, A, lda, beta, B, ldb,
-                     n_terms: int = 2):
+                     n_terms: int = 3):
     """B = alpha * op(bf16(submat S)) * op(A) + beta * B on the BF16 matrix cores."""
     _call("rbx_slskge3_bf16split", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
           _ptr(A), lda, beta, _ptr(B), ldb, n_terms, _stream())
@@ -569,21 +571,9 @@ def raise_if(cond: bool, msg: str):
 
 
 def require_symmetric(layout, A, n, lda, tol):
-    """util::require_symmetric (util.hh:127-148), evaluated on the device."""
-    if tol < 0:
-        return
-    import torch
-    if _c(layout) == b"C":
-        M = A.as_strided((n, n), (1, lda))
-    else:
-        M = A.as_strided((n, n), (lda, 1))
-    viol = (M - M.T).abs()
-    rel = (M.abs() + M.T.abs() + 1) * tol
-    bad = torch.nonzero(torch.triu(viol > rel, diagonal=1))
-    if bad.numel():
-        i, j = (int(v) for v in bad[0])
-        raise Error(f"Symmetry check failed. |A({i},{j}) - A({j},{i})| was {float(viol[i, j]):e}, "
-                    f"which exceeds tolerance of {float(rel[i, j]):e}.")
+    """util::require_symmetric (util.hh:127-148), evaluated on the device by
+    rbx_[sd]require_symmetric; raises Error with the reference's message."""
+    _call(f"rbx_{_prefix(A)}require_symmetric", _c(layout), _ptr(A), n, lda, tol, _stream())
 
 
 def profile_enable(on: bool = True) -> None:

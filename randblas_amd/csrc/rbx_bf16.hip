@@ -61,23 +61,22 @@ __device__ __forceinline__ uint16_t entry_value<uint16_t>(float v, bool gauss) {
 // ---------------------------------------------------------------- split
 // Element (k, j) of plane t lives at
 //   t*plane_stride + (((j/32)*(Kp/16) + k/16)*64 + j%32 + 32*((k/8)%2))*8 + k%8.
-__device__ __forceinline__ int64_t packed_index(int64_t k, int64_t j, int64_t KS) {
-    return ((((j >> 5) * KS + (k >> 4)) * 64 + (j & 31) + 32 * ((k >> 3) & 1)) << 3) + (k & 7);
-}
-
+// One wave per (32-column block cb, 16-deep K step s).  Lane l handles column
+// 32*cb + l%32 and k = 16*s + 8*(l/32) .. +7, and writes lane l's 16-byte slot
+// of the block's 1 KB run in every plane, so the wave's stores are one
+// contiguous 1 KB per plane.
 __global__ __launch_bounds__(256) void split_kernel(const float *A, int64_t rs, int64_t cs, int64_t K, int64_t N,
                                                     int64_t Kp, int64_t Np, int terms, uint16_t *planes,
                                                     int64_t plane_stride) {
-    // one thread per (8 consecutive k, one column j); consecutive threads walk
-    // the unit-stride direction of op(A)
-    const int64_t kg = Kp / 8;
-    const int64_t total = kg * Np;
     const int64_t KS = Kp / 16;
-    const bool k_fast = rs == 1;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = k_fast ? idx / kg : idx % Np;
-        const int64_t k0 = (k_fast ? idx % kg : idx / Np) * 8;
+    const int64_t n_waves = (Np / 32) * KS;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t w = wave0; w < n_waves; w += wstride) {
+        const int64_t cb = w / KS, st = w - cb * KS;
+        const int64_t j = cb * 32 + (lane & 31);
+        const int64_t k0 = st * 16 + 8 * (lane >> 5);
         uint16_t out[3][8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -90,7 +89,7 @@ __global__ __launch_bounds__(256) void split_kernel(const float *A, int64_t rs, 
                 a = __fsub_rn(a, bf16_to_f32(h));  // exact
             }
         }
-        const int64_t dst = packed_index(k0, j, KS);
+        const int64_t dst = ((cb * KS + st) * 64 + lane) * 8;
         for (int t = 0; t < terms; ++t) {
             u32x4 v;
             v.x = (uint32_t)out[t][0] | ((uint32_t)out[t][1] << 16);
@@ -235,20 +234,38 @@ __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
             }
 }
 
-// B[i,j] = alpha * sum_s part[s][i*N + j] + beta * B[i*brs + j*bcs]
+// B[i,j] = alpha * sum_s part[s][i*N + j] + beta * B[i*brs + j*bcs], sums in
+// FP64 in split order.  32 x 32 tiles: partials are read along j; a
+// column-major B is written along i after a transpose through LDS.
 __global__ __launch_bounds__(256) void bf16_reduce_kernel(const float *part, int nS, int64_t M, int64_t N,
                                                           double alpha, double beta, float *B, int64_t brs,
                                                           int64_t bcs) {
+    __shared__ double tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const int64_t i0 = (int64_t)blockIdx.y * 32, j0 = (int64_t)blockIdx.x * 32;
     const int64_t total = M * N;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = idx / N, j = idx - i * N;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t i = i0 + ty + 8 * q, j = j0 + tx;
         double v = 0.0;
-        for (int s = 0; s < nS; ++s) v += (double)part[(int64_t)s * total + idx];
-        float *b = B + i * brs + j * bcs;
-        double out = alpha * v;
-        if (beta != 0.0) out += beta * (double)(*b);
-        *b = (float)out;
+        if (i < M && j < N)
+            for (int s = 0; s < nS; ++s) v += (double)part[(int64_t)s * total + i * N + j];
+        tile[ty + 8 * q][tx] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        // column-major: lanes walk i; row-major: lanes walk j
+        const bool cm = brs == 1;
+        const int64_t i = cm ? i0 + tx : i0 + ty + 8 * q;
+        const int64_t j = cm ? j0 + ty + 8 * q : j0 + tx;
+        if (i < M && j < N) {
+            const double v = cm ? tile[tx][ty + 8 * q] : tile[ty + 8 * q][tx];
+            float *b = B + i * brs + j * bcs;
+            double out = alpha * v;
+            if (beta != 0.0) out += beta * (double)(*b);
+            *b = (float)out;
+        }
     }
 }
 
@@ -323,7 +340,7 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     pb.part = (float *)(ws + planes_bytes);
 
     {
-        int64_t total = pb.Kp / 8 * pb.Np;
+        int64_t total = (pb.Np / 32) * (pb.Kp / 16) * 64;  // one wave per (block, K step)
         int64_t blocks = (total + 255) / 256;
         if (blocks > 65536) blocks = 65536;
         ProfScope prof("bf16_split_kernel", stream);
@@ -340,11 +357,10 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         RBX_HIP(hipGetLastError());
     }
     {
-        int64_t total = M * N;
-        int64_t blocks = (total + 255) / 256;
-        if (blocks > 16384) blocks = 16384;
+        dim3 grid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32));
+        RBX_REQUIRE(grid.y <= 65535u);
         ProfScope prof("bf16_reduce_kernel", stream);
-        hipLaunchKernelGGL(bf16_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, pb.part, pb.nS, M, N,
+        hipLaunchKernelGGL(bf16_reduce_kernel, grid, dim3(256), 0, stream, pb.part, pb.nS, M, N,
                            (double)alpha, (double)beta, B, brs, bcs);
         RBX_HIP(hipGetLastError());
     }

@@ -145,7 +145,7 @@ int num_cus();
 // same device; calls that overlap on one device (different host threads or
 // streams) and use the same slot race.  Growing a slot synchronises the device.
 enum ScratchSlot { SCRATCH_BF16 = 0, SCRATCH_SPARSE_OP = 1, SCRATCH_SPARSE_CSR = 2, SCRATCH_DENSE_OP = 3,
-                   SCRATCH_NSLOTS = 4 };
+                   SCRATCH_SMALL = 4, SCRATCH_NSLOTS = 5 };
 void *scratch(int slot, size_t bytes);
 
 // Write the generated matrix G (n_p x n_q) of a GenMap to buff[p*rs + q*cs].

@@ -1,0 +1,265 @@
+// test_cpp_api.cc -- drives the C++ layer (randblas_amd/cpp/RandBLAS.hh) the
+// way a RandBLAS user would.
+//
+//   test_cpp_api host          host-side semantics only (no GPU needed)
+//   test_cpp_api gpu OUTDIR    sketches on the GPU; raw results are written to
+//                              OUTDIR/<name>.bin for the Python test to check
+//                              against the CPU oracle
+//
+// Exit status 0 when every in-program check passed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "RandBLAS.hh"
+
+using blas::Layout;
+using blas::Op;
+using RandBLAS::Axis;
+using RandBLAS::DenseDist;
+using RandBLAS::DenseSkOp;
+using RandBLAS::RNGState;
+using RandBLAS::ScalarDist;
+using RandBLAS::SparseDist;
+using RandBLAS::SparseSkOp;
+
+static int n_checks = 0, n_fail = 0;
+#define EXPECT(cond)                                                               \
+    do {                                                                           \
+        ++n_checks;                                                                \
+        if (!(cond)) {                                                             \
+            ++n_fail;                                                              \
+            std::fprintf(stderr, "FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);   \
+        }                                                                          \
+    } while (0)
+
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                      \
+            std::exit(2);                                                          \
+        }                                                                          \
+    } while (0)
+
+template <typename F>
+static bool throws_error(F f, const char *needle = nullptr) {
+    try {
+        f();
+    } catch (const RandBLAS::Error &e) {
+        return needle == nullptr || std::strstr(e.what(), needle) != nullptr;
+    } catch (...) {
+        return false;
+    }
+    return false;
+}
+
+static void host_checks() {
+    // RNGState(uint64 k): counter 0, key {k mod 2^32, k >> 32} (base.hh:119)
+    RNGState<> s(0x100000002ull);
+    EXPECT(s.key[0] == 2 && s.key[1] == 1);
+    EXPECT(s.counter[0] == 0 && s.counter[1] == 0 && s.counter[2] == 0 && s.counter[3] == 0);
+    RNGState<> t;
+    EXPECT(t.key[0] == 0 && t.key[1] == 0);
+    // multi-word counter increments carry across words (test_r123.cc:706-797)
+    RNGState<> u;
+    u.counter.v[0] = 0xFFFFFFFFu;
+    u.counter.v[1] = 0xFFFFFFFFu;
+    u.counter.incr(1);
+    EXPECT(u.counter[0] == 0 && u.counter[1] == 0 && u.counter[2] == 1 && u.counter[3] == 0);
+    RNGState<> v;
+    v.counter.v[0] = 0xFFFFFFFEu;
+    v.counter.incr(0x100000005ull);  // 0xFFFFFFFE + 0x1_00000005 = 0x2_00000003
+    EXPECT(v.counter[0] == 3 && v.counter[1] == 2 && v.counter[2] == 0);
+    // same arithmetic as the library's own incr
+    for (uint64_t n : {1ull, 7ull, 0xFFFFFFFFull, 0x123456789ABCull, ~0ull}) {
+        RNGState<> a;
+        a.counter.v[0] = 0xFFFFFFF0u;
+        a.counter.v[1] = 0x80000000u;
+        rbx_rng_state c = a.c();
+        EXPECT(rbx_rng_state_incr(&c, n) == RBX_OK);
+        a.counter.incr(n);
+        EXPECT(RNGState<>::from_c(c) == a);
+    }
+
+    // DenseDist (dense_skops.hh:313-329) and natural layouts (:187-199)
+    DenseDist D(400, 2000);
+    EXPECT(D.dim_major == 2000 && D.dim_minor == 400);
+    EXPECT(D.natural_layout == Layout::RowMajor);
+    EXPECT(std::fabs(D.isometry_scale - std::pow(400.0, -0.5)) < 1e-15);
+    EXPECT(DenseDist(2000, 400).natural_layout == Layout::ColMajor);
+    DenseDist Ds(400, 2000, ScalarDist::Uniform, Axis::Short);
+    EXPECT(Ds.dim_major == 400 && Ds.natural_layout == Layout::ColMajor);
+    // next_state = seed + dim_minor * ceil(dim_major / 4) (dense_skops.hh:172-185)
+    DenseSkOp<double> S(D, RNGState<>(42));
+    EXPECT(S.next_state.counter[0] == 400u * 500u && S.next_state.key[0] == 42);
+    EXPECT(S.layout == Layout::RowMajor && S.buff == nullptr && S.own_memory);
+    EXPECT(throws_error([] { DenseDist bad(0, 5); }));
+
+    // SparseDist (sparse_skops.hh:208-245)
+    SparseDist SD(2000, 100000, 8);
+    EXPECT(SD.dim_major == 2000 && SD.dim_minor == 100000 && SD.full_nnz == 800000);
+    EXPECT(std::fabs(SD.isometry_scale - std::pow(8.0, -0.5)) < 1e-15);
+    SparseSkOp<float> SS(SD, RNGState<>(7));
+    EXPECT(SS.next_state.counter[0] == 800000u && SS.nnz == -1 && SS.own_memory);
+    EXPECT(throws_error([] { SparseDist bad(10, 20, 11); }, "vec_nnz"));
+    SparseDist SL(30, 500, 7, Axis::Long);
+    EXPECT(SL.dim_major == 500 && SL.dim_minor == 30);
+    EXPECT(RandBLAS::compute_next_state(SL, RNGState<>(1)).counter[0] == 30u * 4u);
+}
+
+template <typename T>
+static T *dev_alloc(size_t n) {
+    T *p = nullptr;
+    HIPCHK(hipMalloc((void **)&p, n * sizeof(T) + 16));
+    return p;
+}
+
+template <typename T>
+static void dump(const std::string &dir, const char *name, const T *dev, size_t n) {
+    std::vector<T> h(n);
+    HIPCHK(hipMemcpy(h.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost));
+    std::string path = dir + "/" + name + ".bin";
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+        std::perror(path.c_str());
+        std::exit(2);
+    }
+    std::fwrite(h.data(), sizeof(T), n, f);
+    std::fclose(f);
+}
+
+static void gpu_checks(const std::string &out) {
+    hipStream_t stream;
+    HIPCHK(hipStreamCreate(&stream));
+    RandBLAS::gpu::set_stream(stream);
+
+    // Dense left sketch: S 400 x 2000 Gaussian (key 42), A 2000 x 200 from
+    // DenseDist(2000, 200) key 1 (natural ColMajor), B = S * A.
+    const int64_t d = 400, m = 2000, n = 200;
+    double *A = dev_alloc<double>(m * n), *B = dev_alloc<double>(d * n), *Bt = dev_alloc<double>(n * d);
+    DenseDist DA(m, n);
+    RNGState<> nxt = RandBLAS::fill_dense(DA, A, RNGState<>(1));
+    EXPECT(nxt.counter[0] == 200u * 500u);
+    DenseSkOp<double> S(DenseDist(d, m), RNGState<>(42));
+    RandBLAS::sketch_general(Layout::ColMajor, Op::NoTrans, Op::NoTrans, d, n, m, 1.0, S, A, m, 0.0, B, d);
+    // right form: Bt (n x d) = A^T * S^T
+    RandBLAS::sketch_general(Layout::ColMajor, Op::Trans, Op::Trans, n, d, m, 1.0, A, m, S, 0.0, Bt, n);
+    HIPCHK(hipStreamSynchronize(stream));
+    dump(out, "dense_A", A, m * n);
+    dump(out, "dense_B", B, d * n);
+    dump(out, "dense_Bt", Bt, n * d);
+
+    // sketch_vector: y = S * A[:, 0]
+    double *y = dev_alloc<double>(d);
+    RandBLAS::sketch_vector(Op::NoTrans, 1.0, S, A, (int64_t)1, 0.0, y, (int64_t)1);
+    HIPCHK(hipStreamSynchronize(stream));
+    dump(out, "vector_y", y, d);
+
+    // adjoint: Abar = S^T * B (m x n)
+    double *Abar = dev_alloc<double>(m * n);
+    RandBLAS::amd::lskge3_adjoint(Layout::ColMajor, Op::NoTrans, d, n, m, 1.0, S, 0, 0, B, d, 0.0, Abar, m);
+    HIPCHK(hipStreamSynchronize(stream));
+    dump(out, "adjoint_Abar", Abar, m * n);
+
+    // argument checks surface as RandBLAS::Error
+    EXPECT(throws_error([&] {
+        RandBLAS::sketch_general(Layout::ColMajor, Op::NoTrans, Op::NoTrans, d, n, m, 1.0, S, 0, 0, A, m - 1, 0.0,
+                                 B, d);
+    }));
+    EXPECT(throws_error([&] {
+        RandBLAS::sketch_general(Layout::ColMajor, Op::NoTrans, Op::NoTrans, d - 1, n, m, 1.0, S, A, m, 0.0, B, d);
+    }));
+
+    // materialised operator: fill_dense(S) allocates and owns a device buffer
+    {
+        DenseSkOp<float> Sf(DenseDist(64, 100), RNGState<>(5));
+        RandBLAS::fill_dense(Sf);
+        EXPECT(Sf.buff != nullptr && Sf.own_memory);
+        HIPCHK(hipStreamSynchronize(stream));
+        dump(out, "dense_fill_f32", Sf.buff, 64 * 100);
+    }  // destructor frees Sf.buff
+
+    // sparse left sketch: SASO 200 x 5000, vec_nnz 8, key 7; A2 5000 x 50 Uniform (key 3)
+    const int64_t sd = 200, sm = 5000, sn = 50;
+    double *A2 = dev_alloc<double>(sm * sn), *B2 = dev_alloc<double>(sd * sn);
+    RandBLAS::fill_dense(DenseDist(sm, sn, ScalarDist::Uniform), A2, RNGState<>(3));
+    SparseSkOp<double> SS(SparseDist(sd, sm, 8), RNGState<>(7));
+    RandBLAS::sketch_general(Layout::ColMajor, Op::NoTrans, Op::NoTrans, sd, sn, sm, 1.0, SS, A2, sm, 0.0, B2, sd);
+    HIPCHK(hipStreamSynchronize(stream));
+    dump(out, "sparse_A", A2, sm * sn);
+    dump(out, "sparse_B", B2, sd * sn);
+    RandBLAS::fill_sparse(SS);
+    EXPECT(SS.nnz == 8 * sm && SS.vals != nullptr);
+    dump(out, "sparse_vals", SS.vals, SS.nnz);
+    dump(out, "sparse_rows", SS.rows, SS.nnz);
+    dump(out, "sparse_cols", SS.cols, SS.nnz);
+
+    // sketch_symmetric on an asymmetric matrix (test_sketch_symmetric.cc:390-420)
+    {
+        const double h[9] = {1.0, 0.0, 0.0, 5.0, 2.0, 0.0, 0.0, 0.0, 3.0};
+        double *A3 = dev_alloc<double>(9), *B3 = dev_alloc<double>(6);
+        HIPCHK(hipMemcpy(A3, h, sizeof(h), hipMemcpyHostToDevice));
+        DenseSkOp<double> S3(DenseDist(2, 3, ScalarDist::Uniform, Axis::Short), RNGState<>(42));
+        EXPECT(throws_error(
+            [&] { RandBLAS::sketch_symmetric(Layout::ColMajor, (int64_t)2, (int64_t)3, 1.0, S3, 0, 0, A3, 3, 0.0, B3, 2); },
+            "Symmetry check failed. |A(0,1) - A(1,0)| was 5.000000e+00"));
+        HIPCHK(hipFree(A3));
+        HIPCHK(hipFree(B3));
+    }
+
+    // rsksp3: CSR A (50 x 300, 5 nonzeros per row) times S (300 x 16 Gaussian, key 42)
+    {
+        const int64_t cm = 50, cn = 300, cd = 16, per = 5;
+        std::vector<int64_t> rowptr(cm + 1), colidx;
+        std::vector<double> vals;
+        for (int64_t i = 0; i < cm; ++i) {
+            rowptr[i] = (int64_t)colidx.size();
+            for (int64_t k = 0; k < per; ++k) {
+                colidx.push_back((i * 7 + k * 61) % cn);
+                vals.push_back(std::sin((double)(i * per + k)));
+            }
+            std::sort(colidx.begin() + rowptr[i], colidx.end());
+        }
+        rowptr[cm] = (int64_t)colidx.size();
+        const int64_t nnz = rowptr[cm];
+        double *v = dev_alloc<double>(nnz), *B4 = dev_alloc<double>(cm * cd);
+        int64_t *rp = dev_alloc<int64_t>(cm + 1), *ci = dev_alloc<int64_t>(nnz);
+        HIPCHK(hipMemcpy(v, vals.data(), nnz * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(rp, rowptr.data(), (cm + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ci, colidx.data(), nnz * sizeof(int64_t), hipMemcpyHostToDevice));
+        RandBLAS::sparse_data::CSRMatrix<double> Acsr(cm, cn, nnz, v, rp, ci);
+        DenseSkOp<double> S4(DenseDist(cn, cd), RNGState<>(42));
+        RandBLAS::sketch_sparse(Layout::ColMajor, Op::NoTrans, Op::NoTrans, cm, cd, cn, 1.0, Acsr, S4, 0, 0, 0.0, B4,
+                                cm);
+        HIPCHK(hipStreamSynchronize(stream));
+        dump(out, "csr_vals", v, nnz);
+        dump(out, "csr_rowptr", rp, cm + 1);
+        dump(out, "csr_colidx", ci, nnz);
+        dump(out, "csr_B", B4, cm * cd);
+    }
+
+    RandBLAS::gpu::set_stream(nullptr);
+    HIPCHK(hipStreamDestroy(stream));
+}
+
+int main(int argc, char **argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s host | gpu OUTDIR\n", argv[0]);
+        return 2;
+    }
+    const std::string mode = argv[1];
+    host_checks();
+    if (mode == "gpu") {
+        if (argc < 3) return 2;
+        gpu_checks(argv[2]);
+    }
+    std::printf("cpp api (%s): %d checks, %d failed\n", mode.c_str(), n_checks, n_fail);
+    return n_fail == 0 ? 0 : 1;
+}
