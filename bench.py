@@ -179,6 +179,9 @@ def main():
         del A64, B64
         torch.cuda.synchronize()
 
+        # ---- sparse data: B = A_csr * S (rsksp3) and its adjoint w.r.t. A's values ----
+        out["sparse_rsksp3"] = sparse_leg(rb, torch, dev)
+
     # ---- CPU: FP64 oracle rows (also the timed CPU baseline), errors, distortion ----
     # All reference linear algebra runs on the host (numpy/OpenBLAS, scipy).
     if not args.no_cpu:
@@ -212,6 +215,57 @@ def main():
             log(f"distortion bf16 {out['distortion_bf16']}, fp64 {out['distortion_fp64']}")
 
     print(json.dumps(out), flush=True)
+
+
+def sparse_leg(rb, torch, dev, reps=10):
+    """rsksp3 on a 200000 x 50000 CSR matrix with power-law column degrees
+    (exponent 2.1, ~20 stored entries per row, N(0,1) FP32 values) times a
+    generated 50000 x 256 Gaussian operator (key 42), column-major B; and the
+    gradient with respect to A's stored values (SDDMM).  Both are bound by
+    memory traffic: achieved GB/s of their algorithmic bytes against HBM."""
+    from randblas_amd.datagen import power_law_csr
+    m, n, d = 200000, 50000, 256
+    log("sparse leg: generating the CSR matrix")
+    rowptr, colidx, vals = power_law_csr(m, n, 20, 2.1, 17)
+    nnz = len(vals)
+    A = rb.CSRMatrix(m, n, torch.from_numpy(vals).to(dev), torch.from_numpy(rowptr).to(dev),
+                     torch.from_numpy(colidx).to(dev))
+    S = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(42))
+    B = torch.empty(m * d, dtype=torch.float32, device=dev)
+    Bbar = torch.empty(m * d, dtype=torch.float32, device=dev)
+    rb.fill_dense(rb.DenseDist(m, d), Bbar, rb.RNGState(3))
+    dv = torch.empty(nnz, dtype=torch.float32, device=dev)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        rb.profile_collect()
+        rb.profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rb.profile_enable(False)
+        prof = rb.profile_collect()
+        return (t1 - t0) * 1e3 / reps, {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}
+
+    fwd_ms, fwd_k = timed(lambda: rb.rsksp3("C", "N", "N", m, d, n, 1.0, A, 0, 0, S, 0, 0, 0.0, B, m))
+    adj_ms, adj_k = timed(lambda: rb.rsksp3_adjoint("C", "N", m, d, n, 1.0, A, S, 0, 0, Bbar, m, dv))
+    # algorithmic bytes: stored entries (4 B value + 8 B column index), row
+    # pointers, the dense operand read once, the output written once
+    fwd_bytes = nnz * 12 + (m + 1) * 8 + n * d * 4 + m * d * 4
+    adj_bytes = nnz * 8 + (m + 1) * 8 + n * d * 4 + m * d * 4 + nnz * 4
+    res = {"workload": "CSR 200000x50000 power-law (exp 2.1, ~20/row) FP32 * Gaussian 50000x256 (key 42)",
+           "nnz": nnz, "flop": 2 * nnz * d,
+           "fwd_ms": round(fwd_ms, 4), "fwd_GBps": round(fwd_bytes / (fwd_ms * 1e-3) / 1e9, 1),
+           "fwd_frac_hbm": round(fwd_bytes / (fwd_ms * 1e-3) / 8e12, 4), "fwd_kernels_ms": fwd_k,
+           "adj_ms": round(adj_ms, 4), "adj_GBps": round(adj_bytes / (adj_ms * 1e-3) / 1e9, 1),
+           "adj_frac_hbm": round(adj_bytes / (adj_ms * 1e-3) / 8e12, 4), "adj_kernels_ms": adj_k}
+    log(f"sparse: rsksp3 {fwd_ms:.3f} ms, adjoint {adj_ms:.3f} ms ({nnz} stored entries)")
+    del A, B, Bbar, dv
+    torch.cuda.synchronize()
+    return res
 
 
 def cpu_baseline(A, rows, np, torch):

@@ -274,10 +274,11 @@ rbx_status rbx_drskges(char layout, char opA, char opS, int64_t m, int64_t d, in
 
 /* ---------------------------------------------------------------- sparse data
  * rsksp3 (sparse_data/sksp.hh:277-326): B = alpha * op(A) * op(submat(S)) + beta * B
- * with A a CSR matrix (opA = 'N', ro_a = co_a = 0) and S a DenseSkOp.  A
- * generated operator is materialised once into library scratch (as the
- * reference does, sksp.hh:298-305) and the CSR product accumulates each output
- * entry in the reference's order (csr_spmm_impl.hh:51-119). */
+ * with A a whole CSR matrix (ro_a = co_a = 0) and S a DenseSkOp.  op(submat(S))
+ * is materialised once into library scratch (generated from its counters, as
+ * the reference does at sksp.hh:298-305, or gathered from S->buff) and the
+ * product accumulates each output entry in the reference's order
+ * (csr_spmm_impl.hh:51-119; opA = 'T' takes the CSC order). */
 rbx_status rbx_srsksp3(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
                        float alpha, const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a,
                        const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, float beta,
@@ -286,6 +287,20 @@ rbx_status rbx_drsksp3(char layout, char opA, char opS, int64_t m, int64_t d, in
                        double alpha, const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a,
                        const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, double beta,
                        double *B, int64_t ldb, void *stream);
+
+/* Adjoint of rsksp3 (opA = 'N') with respect to the stored values of A (no
+ * reference counterpart; it is the gradient of B = alpha * A * op(submat(S))
+ * with respect to A's nonzeros):
+ *   dvals[k] = alpha * sum_j Bbar[i_k, j] * op(submat(S))[c_k, j]
+ * for every stored entry k = (i_k, c_k) of A (a sampled dense-dense product).
+ * A is m x n, Bbar is m x d in `layout`, dvals has A->nnz entries.  S may be
+ * generated (buff == NULL) or materialised. */
+rbx_status rbx_srsksp3_adjoint(char layout, char opS, int64_t m, int64_t d, int64_t n, float alpha,
+                               const rbx_csr_matrix *A, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
+                               const float *Bbar, int64_t ldbb, float *dvals, void *stream);
+rbx_status rbx_drsksp3_adjoint(char layout, char opS, int64_t m, int64_t d, int64_t n, double alpha,
+                               const rbx_csr_matrix *A, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
+                               const double *Bbar, int64_t ldbb, double *dvals, void *stream);
 
 /* ---------------------------------------------------------------- utilities
  * util::require_symmetric (RandBLAS/util.hh:127-148), the check that

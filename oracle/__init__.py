@@ -75,9 +75,12 @@ def lib() -> C.CDLL:
             C.c_char, C.c_char, C.c_char, C.c_int64, C.c_int64, C.c_int64, C.c_double,
             C.c_void_p, _i64p, _i64p, C.c_void_p, C.c_int64, C.c_double, C.c_void_p,
             C.c_int64, C.c_int]
+        L.orc_sddmm_csr.argtypes = [
+            C.c_char, C.c_int64, C.c_int64, C.c_double, _i64p, _i64p, C.c_void_p, C.c_int64,
+            C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int]
         for f in ("orc_dense_next_state", "orc_fill_dense_unpacked", "orc_lskge3", "orc_rskge3",
                   "orc_sparse_next_state", "orc_fill_sparse", "orc_left_spmm_coo",
-                  "orc_left_spmm_csr"):
+                  "orc_left_spmm_csr", "orc_sddmm_csr"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -262,6 +265,17 @@ def left_spmm_csr(dtype, layout, opB, d, n, m, alpha, vals, rowptr, colidxs, B, 
                                  np.ascontiguousarray(colidxs, np.int64), _ptr(B), ldb, beta,
                                  _ptr(Cbuf), ldc, nthreads)
     _check(rc, "left_spmm_csr")
+
+
+def sddmm_csr(dtype, m, d, alpha, rowptr, colidxs, Bbar, brs, bcs, Sm, srs, scs, nthreads=0):
+    """dvals[k] = alpha * sum_j Bbar[i_k, j] * Sm[c_k, j] (returns a new array)."""
+    dt = np.float64 if dtype == "d" else np.float32
+    out = np.zeros(int(rowptr[-1]), dt)
+    rc = lib().orc_sddmm_csr(_b(dtype), m, d, alpha, np.ascontiguousarray(rowptr, np.int64),
+                             np.ascontiguousarray(colidxs, np.int64), _ptr(Bbar), brs, bcs, _ptr(Sm), srs,
+                             scs, _ptr(out), nthreads)
+    _check(rc, "sddmm_csr")
+    return out
 
 
 # ---------------------------------------------------------------- bfloat16 helpers

@@ -757,3 +757,36 @@ int orc_left_spmm_csr(char dtype, char layout, char opB, int64_t d, int64_t n, i
                         (const float *)B, ldb, (float)beta, (float *)C, ldc, nthreads);
     return 0;
 }
+
+/* Sampled dense-dense product: the gradient of B = alpha * A * Sm (A CSR m x n,
+ * Sm dense n x d) with respect to A's stored values,
+ *   dvals[k] = alpha * sum_j Bbar[i_k, j] * Sm[c_k, j],
+ * accumulated in double in ascending j.  No reference counterpart (the
+ * reference has no adjoints); it is the definition of that gradient.
+ * Bbar(i, j) at Bbar[i*brs + j*bcs], Sm(c, j) at Sm[c*srs + j*scs]. */
+int orc_sddmm_csr(char dtype, int64_t m, int64_t d, double alpha, const int64_t *rowptr,
+                  const int64_t *colidxs, const void *Bbar, int64_t brs, int64_t bcs,
+                  const void *Sm, int64_t srs, int64_t scs, void *dvals, int nthreads) {
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int64_t i = 0; i < m; ++i) {
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+            const int64_t c = colidxs[k];
+            double acc = 0.0;
+            for (int64_t j = 0; j < d; ++j) {
+                double b, sv;
+                if (dtype == 'd') {
+                    b = ((const double *)Bbar)[i * brs + j * bcs];
+                    sv = ((const double *)Sm)[c * srs + j * scs];
+                } else {
+                    b = ((const float *)Bbar)[i * brs + j * bcs];
+                    sv = ((const float *)Sm)[c * srs + j * scs];
+                }
+                acc += b * sv;
+            }
+            if (dtype == 'd') ((double *)dvals)[k] = alpha * acc;
+            else ((float *)dvals)[k] = (float)(alpha * acc);
+        }
+    }
+    return 0;
+}

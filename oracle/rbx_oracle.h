@@ -91,6 +91,12 @@ int orc_left_spmm_csr(char dtype, char layout, char opB, int64_t d, int64_t n, i
                       const int64_t *colidxs, const void *B, int64_t ldb,
                       double beta, void *C, int64_t ldc, int nthreads);
 
+/* Gradient of alpha * A * Sm with respect to A's stored values (SDDMM),
+ * double accumulation in ascending j. */
+int orc_sddmm_csr(char dtype, int64_t m, int64_t d, double alpha, const int64_t *rowptr,
+                  const int64_t *colidxs, const void *Bbar, int64_t brs, int64_t bcs,
+                  const void *Sm, int64_t srs, int64_t scs, void *dvals, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

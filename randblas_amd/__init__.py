@@ -142,6 +142,8 @@ _SIG = {
                     _I64, _P],
     "rbx_drsksp3": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _CSR, _I64, _I64, _DS, _I64, _I64, _D, _P,
                     _I64, _P],
+    "rbx_srsksp3_adjoint": [_CH, _CH, _I64, _I64, _I64, _F, _CSR, _DS, _I64, _I64, _P, _I64, _P, _P],
+    "rbx_drsksp3_adjoint": [_CH, _CH, _I64, _I64, _I64, _D, _CSR, _DS, _I64, _I64, _P, _I64, _P, _P],
     "rbx_srequire_symmetric": [_CH, _P, _I64, _I64, _F, _P],
     "rbx_drequire_symmetric": [_CH, _P, _I64, _I64, _D, _P],
     "rbx_profile_enable": [C.c_int],
@@ -477,6 +479,13 @@ def rsksp3(layout, opA, opS, m, d, n, alpha, A: CSRMatrix, ro_a, co_a, S: DenseS
 
 
 # ---------------------------------------------------------------- dispatching wrappers
+def rsksp3_adjoint(layout, opS, m, d, n, alpha, A: CSRMatrix, S: DenseSkOp, ro_s, co_s, Bbar, ldbb, dvals):
+    """Gradient of B = alpha * A * op(submat(S)) with respect to A's stored values:
+    dvals[k] = alpha * sum_j Bbar[i_k, j] * op(submat(S))[c_k, j]."""
+    _call(f"rbx_{_prefix(dvals)}rsksp3_adjoint", _c(layout), _c(opS), m, d, n, alpha, C.byref(A.c()),
+          C.byref(S.c()), ro_s, co_s, _ptr(Bbar), ldbb, _ptr(dvals), _stream())
+
+
 def sketch_general(layout, op1, op2, a1, a2, a3, alpha, *rest):
     """sketch_general overloads (skge.hh:756-1199), dispatched on which argument
     is the sketching operator, with or without (ro_s, co_s):

@@ -705,6 +705,24 @@ void rskge3_adjoint(blas::Layout layout, blas::Op opS, int64_t m, int64_t d, int
     detail::check(st, "rskge3_adjoint");
 }
 
+// Gradient of B = alpha * A * op(submat(S)) (A CSR, opA = NoTrans) with
+// respect to A's stored values: dvals[k] = alpha * sum_j Bbar[i_k, j] * op(submat(S))[c_k, j].
+template <typename T, typename RNG>
+void rsksp3_adjoint(blas::Layout layout, blas::Op opS, int64_t m, int64_t d, int64_t n, T alpha,
+                    const sparse_data::CSRMatrix<T, int64_t> &A, const DenseSkOp<T, RNG> &S, int64_t ro_s,
+                    int64_t co_s, const T *Bbar, int64_t ldbb, T *dvals) {
+    rbx_csr_matrix a = A.c();
+    rbx_dense_skop s = S.c();
+    rbx_status st;
+    if constexpr (std::is_same<T, float>::value)
+        st = rbx_srsksp3_adjoint(detail::ch(layout), detail::ch(opS), m, d, n, alpha, &a, &s, ro_s, co_s, Bbar, ldbb,
+                                 dvals, gpu::stream());
+    else
+        st = rbx_drsksp3_adjoint(detail::ch(layout), detail::ch(opS), m, d, n, alpha, &a, &s, ro_s, co_s, Bbar, ldbb,
+                                 dvals, gpu::stream());
+    detail::check(st, "rsksp3_adjoint");
+}
+
 // BF16-split left sketch of an FP32 matrix (n_terms bfloat16 terms of A).
 template <typename RNG>
 void lskge3_bf16split(blas::Layout layout, blas::Op opS, blas::Op opA, int64_t d, int64_t n, int64_t m, float alpha,

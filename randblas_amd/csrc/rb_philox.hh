@@ -168,6 +168,21 @@ __device__ __forceinline__ float4 gen4(bool gauss, Ctr128 c, uint32_t k0, uint32
     return make_float4(uneg11(w.x), uneg11(w.y), uneg11(w.z), uneg11(w.w));
 }
 
+// gen4 with the two Box-Muller pairs kept apart by scheduling fences, for
+// kernels whose register budget cannot hold both pairs' temporaries at once.
+__device__ __forceinline__ float4 gen4_lean(bool gauss, Ctr128 c, uint32_t k0, uint32_t k1) {
+    uint4 w = philox4x32_10(c, k0, k1);
+    if (gauss) {
+        __builtin_amdgcn_sched_barrier(0);
+        float2 a = boxmuller(w.x, w.y);
+        __builtin_amdgcn_sched_barrier(0);
+        float2 b = boxmuller(w.z, w.w);
+        __builtin_amdgcn_sched_barrier(0);
+        return make_float4(a.x, a.y, b.x, b.y);
+    }
+    return make_float4(uneg11(w.x), uneg11(w.y), uneg11(w.z), uneg11(w.w));
+}
+
 __device__ __forceinline__ float f4_get(const float4 &v, int t) {
     return t == 0 ? v.x : (t == 1 ? v.y : (t == 2 ? v.z : v.w));
 }

@@ -58,16 +58,86 @@ __device__ __forceinline__ uint16_t entry_value<uint16_t>(float v, bool gauss) {
     return bf16_rne(gauss ? v : __fmul_rn(v, 0x1.bb67aep+0f));
 }
 
+// ---------------------------------------------------------------- fast generator
+// Gaussian operator entries rounded to bfloat16 with the hardware
+// transcendentals (v_sin_f32, v_cos_f32, v_log_f32, v_sqrt_f32), exact by
+// construction: the portable Box-Muller path (rb_philox.hh) defines each
+// entry g = RN(s * r); the fast path computes g' = RN(s' * r') from hardware
+// factors whose errors are bounded over every possible input word,
+//   |s' - s|, |c' - c| <= BM_ES   (absolute, the sin / cos factor of w0)
+//   |r' - r| <= BM_ER * r         (relative, the radius factor of w1),
+// so |g' - g| <= D + ulp(g') with D = r' (|s'| BM_ER + BM_ES) / (1 - BM_ER).
+// When g' lies farther than that from every bf16 rounding midpoint (the
+// distance taken at half weight to cover a binade edge), RNE(g') = RNE(g).
+// Otherwise the lane recomputes the pair on the exact path.  The bounds are
+// twice the maxima measured over all 2^32 words.
+constexpr float BM_ES = 0x1p-21f;   // sin / cos factor, absolute
+constexpr float BM_ER = 0x1p-20f;   // radius factor, relative
+
+// true when RNE to bfloat16 of any value within D of g equals RNE(g)
+__device__ __forceinline__ bool bf16_round_safe(float g, float D) {
+    const uint32_t b = __float_as_uint(g);
+    const uint32_t e = b & 0x7f800000u;
+    const float ulp = __uint_as_float(e) * 0x1p-23f;
+    const float dist = (float)abs((int)(b & 0xffffu) - 0x8000);
+    return e != 0u && e != 0x7f800000u && (0.5f * dist - 2.0f) * ulp > D;
+}
+
+// Fast bf16 pair of one Box-Muller draw on words (w0, w1), packed low =
+// first; `ok` is cleared when either value is too close to a rounding
+// midpoint for the bound above.
+__device__ __forceinline__ uint32_t boxmuller_bf16_fast(uint32_t w0, uint32_t w1, bool &ok) {
+    const float u0 = uneg11(w0), u1 = u01(w1);
+    const float x = u0 * 0.5f;  // revolutions: sin(2 pi x) = sin(pi u0)
+    const float sf = __builtin_amdgcn_sinf(x), cf = __builtin_amdgcn_cosf(x);
+    const float rf = __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u1) * -0x1.62e430p+0f);
+    const float g0 = sf * rf, g1 = cf * rf;
+    const float k = rf * (1.0f + 0x1p-18f);
+    ok = ok && bf16_round_safe(g0, k * (fabsf(sf) * BM_ER + BM_ES)) &&
+         bf16_round_safe(g1, k * (fabsf(cf) * BM_ER + BM_ES));
+    return (uint32_t)bf16_rne(g0) | ((uint32_t)bf16_rne(g1) << 16);
+}
+
+// One Philox counter -> four operator entries as bf16 (two per 32-bit word).
+// A lane whose fast values are not provably exact recomputes all four on the
+// exact path (one copy of that code, rarely taken).
+template <bool GAUSS>
+__device__ __forceinline__ uint2 gen4_bf16(Ctr128 c, uint32_t k0, uint32_t k1) {
+    const uint4 w = philox4x32_10(c, k0, k1);
+    uint2 pk;
+    if (GAUSS) {
+        bool ok = true;
+        pk.x = boxmuller_bf16_fast(w.x, w.y, ok);
+        pk.y = boxmuller_bf16_fast(w.z, w.w, ok);
+        if (!ok) {
+            __builtin_amdgcn_sched_barrier(0);
+            const float2 a = boxmuller(w.x, w.y);
+            __builtin_amdgcn_sched_barrier(0);
+            const float2 b = boxmuller(w.z, w.w);
+            pk.x = (uint32_t)bf16_rne(a.x) | ((uint32_t)bf16_rne(a.y) << 16);
+            pk.y = (uint32_t)bf16_rne(b.x) | ((uint32_t)bf16_rne(b.y) << 16);
+        }
+    } else {
+        pk.x = (uint32_t)entry_value<uint16_t>(uneg11(w.x), false) |
+               ((uint32_t)entry_value<uint16_t>(uneg11(w.y), false) << 16);
+        pk.y = (uint32_t)entry_value<uint16_t>(uneg11(w.z), false) |
+               ((uint32_t)entry_value<uint16_t>(uneg11(w.w), false) << 16);
+    }
+    return pk;
+}
+
 // ---------------------------------------------------------------- split
-// Element (k, j) of plane t lives at
-//   t*plane_stride + (((j/32)*(Kp/16) + k/16)*64 + j%32 + 32*((k/8)%2))*8 + k%8.
+// Element (k, j) of term t (of T) lives at
+//   ((((j/32)*(Kp/16) + k/16)*T + t)*64 + j%32 + 32*((k/8)%2))*8 + k%8,
+// i.e. for each 32-column block and 16-deep K step the T terms' 1 KB runs are
+// adjacent: a kernel needs one pointer per column block, and the term is a
+// constant 1 KB offset.
 // One wave per (32-column block cb, 16-deep K step s).  Lane l handles column
 // 32*cb + l%32 and k = 16*s + 8*(l/32) .. +7, and writes lane l's 16-byte slot
-// of the block's 1 KB run in every plane, so the wave's stores are one
-// contiguous 1 KB per plane.
+// of the block's 1 KB run of every term, so the wave's stores are T
+// contiguous KB.
 __global__ __launch_bounds__(256) void split_kernel(const float *A, int64_t rs, int64_t cs, int64_t K, int64_t N,
-                                                    int64_t Kp, int64_t Np, int terms, uint16_t *planes,
-                                                    int64_t plane_stride) {
+                                                    int64_t Kp, int64_t Np, int terms, uint16_t *planes) {
     const int64_t KS = Kp / 16;
     const int64_t n_waves = (Np / 32) * KS;
     const int lane = threadIdx.x & 63;
@@ -89,14 +159,14 @@ __global__ __launch_bounds__(256) void split_kernel(const float *A, int64_t rs, 
                 a = __fsub_rn(a, bf16_to_f32(h));  // exact
             }
         }
-        const int64_t dst = ((cb * KS + st) * 64 + lane) * 8;
+        const int64_t dst = (cb * KS + st) * terms * 512 + lane * 8;
         for (int t = 0; t < terms; ++t) {
             u32x4 v;
             v.x = (uint32_t)out[t][0] | ((uint32_t)out[t][1] << 16);
             v.y = (uint32_t)out[t][2] | ((uint32_t)out[t][3] << 16);
             v.z = (uint32_t)out[t][4] | ((uint32_t)out[t][5] << 16);
             v.w = (uint32_t)out[t][6] | ((uint32_t)out[t][7] << 16);
-            *(u32x4 *)(planes + t * plane_stride + dst) = v;
+            *(u32x4 *)(planes + dst + t * 512) = v;
         }
     }
 }
@@ -104,43 +174,23 @@ __global__ __launch_bounds__(256) void split_kernel(const float *A, int64_t rs, 
 struct Bf16Problem {
     int64_t M, N, K, Kp, Np;
     GenMap g;
-    const uint16_t *planes;
-    int64_t plane_stride;
+    const uint16_t *planes;  // fragment-packed terms (see split_kernel)
     float *part;  // nS x M x N, row-major
     int64_t kw;   // K per split (multiple of BF_KC)
     int nMB, nNB, nS;
+    int mb0;      // first row block of the launch
 };
 
 // 32x32 output block, element r of lane l: row 8*(r/4) + 4*(l/32) + r%4, column l%32.
 __device__ __forceinline__ int out_row32(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
-// Operator chunk [m0, m0+64) x [k0, k0+32) -> LDS (row-major, BF_LDK).  When
-// the chunk is inside the operator and its major axis runs along K from a
-// counter boundary, each Philox counter fills four consecutive entries of a
-// row with one 8-byte store.
-__device__ __forceinline__ void gen_chunk(const GenMap &g, int64_t m0, int64_t k0, int64_t M, int64_t K,
-                                          uint16_t *lds, int tid) {
-    const bool fast = !g.b_along_rows && ((g.b_off + k0) & 3) == 0 && m0 + BF_MT <= M && k0 + BF_KC <= K;
-    if (!fast) {
-        gen_tile<uint16_t, BF_MT, BF_KC, BF_NTHR>(g, m0, k0, M, K, lds, BF_LDK, 1, tid);
-        return;
-    }
-    const bool gauss = g.gauss != 0;
-    const int64_t cell0 = (g.b_off + k0) >> 2;
-#pragma unroll
-    for (int it = 0; it < (BF_MT * BF_KC / 4) / BF_NTHR; ++it) {
-        const int idx = tid + it * BF_NTHR;
-        const int row = idx >> 3, cell = idx & 7;
-        const Ctr128 c = ctr_add(g.ctr, (uint64_t)((g.a_off + m0 + row) * g.P + cell0 + cell));
-        const float4 v = gen4(gauss, c, g.k0, g.k1);
-        uint2 pk;
-        pk.x = (uint32_t)entry_value<uint16_t>(v.x, gauss) | ((uint32_t)entry_value<uint16_t>(v.y, gauss) << 16);
-        pk.y = (uint32_t)entry_value<uint16_t>(v.z, gauss) | ((uint32_t)entry_value<uint16_t>(v.w, gauss) << 16);
-        *(uint2 *)(lds + row * BF_LDK + cell * 4) = pk;
-    }
-}
-
-template <int TERMS>
+// FAST: the launched row blocks are interior (m0 + 64 <= M), K is a multiple
+// of the 32-deep chunk (so every chunk is inside the operator), N is a
+// multiple of 256 (every wave owns 8 valid column blocks), and the operator's
+// rows run along K from a counter boundary (b_off % 4 == 0).  Then each thread
+// owns two fixed Philox cells per chunk whose counters advance by 8 per chunk,
+// and no bounds logic is compiled in.  Row blocks start at pb.mb0.
+template <int TERMS, bool FAST>
 __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
     __shared__ __attribute__((aligned(16))) uint16_t Ss[2][BF_MT * BF_LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -153,12 +203,13 @@ __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
     const int rest = t / pb.nMB;
     const int nb = rest % pb.nNB;
     const int sp = rest / pb.nNB;
-    const int64_t m0 = (int64_t)mb * BF_MT;
+    const int64_t m0 = (int64_t)(mb + pb.mb0) * BF_MT;
     const int64_t n0 = (int64_t)nb * BF_NT;
     const int64_t kbeg = (int64_t)sp * pb.kw;
     int64_t kend = kbeg + pb.kw;
     if (kend > pb.Kp) kend = pb.Kp;
     const int nchunks = (int)((kend - kbeg) / BF_KC);
+    const GenMap g = pb.g;
 
     f32x16 acc[2][8];
 #pragma unroll
@@ -168,14 +219,38 @@ __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    const int64_t wcol = n0 + wave * 256;  // first column of this wave
-    const bool active = wcol < pb.Np;     // Np is a multiple of 256: all 8 blocks or none
+    const int64_t wcol = n0 + wave * 256;        // first column of this wave
+    const bool active = FAST || wcol < pb.Np;    // Np is a multiple of 256: all 8 blocks or none
     const int64_t KS = pb.Kp / 16;
     // lane's B operand for block j, K step s, term t:
-    //   planes + t*plane_stride + (((wcol/32 + j)*KS + s)*64 + lane)*8
-    const uint16_t *bbase = pb.planes + ((((wcol >> 5) * KS) * 64 + lane) << 3);
-    const int64_t bstep_blk = KS * 512;  // elements between column blocks
+    //   planes + (((wcol/32 + j)*KS + s)*TERMS + t)*512 + lane*8
+    const uint16_t *bbase = pb.planes + (wcol >> 5) * KS * (TERMS * 512) + lane * 8;
+    const int64_t bstep_blk = KS * (TERMS * 512);  // elements between column blocks
     const int64_t s_first = kbeg / 16, s_last = kend / 16 - 1;
+
+    // FAST generator cells: thread owns cell (tid & 7) of operator rows
+    // tid/8 and tid/8 + 32 of every chunk
+    const int frow = tid >> 3, fcell = tid & 7;
+    const uint64_t foff = FAST ? (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell) : 0;
+    const uint64_t foff2 = foff + (uint64_t)(32 * g.P);
+    auto gen_chunk = [&](int c, uint16_t *lds) {
+        if constexpr (FAST) {
+            const bool gauss = g.gauss != 0;
+#pragma unroll
+            for (int it = 0; it < 2; ++it) {
+                const float4 v = gen4(gauss, ctr_add(g.ctr, (it ? foff2 : foff) + 8 * (uint64_t)c), g.k0, g.k1);
+                uint2 pk;
+                pk.x = (uint32_t)entry_value<uint16_t>(v.x, gauss) |
+                       ((uint32_t)entry_value<uint16_t>(v.y, gauss) << 16);
+                pk.y = (uint32_t)entry_value<uint16_t>(v.z, gauss) |
+                       ((uint32_t)entry_value<uint16_t>(v.w, gauss) << 16);
+                *(uint2 *)(lds + (frow + 32 * it) * BF_LDK + fcell * 4) = pk;
+            }
+        } else {
+            gen_tile<uint16_t, BF_MT, BF_KC, BF_NTHR>(g, m0, kbeg + (int64_t)c * BF_KC, pb.M, pb.K, lds, BF_LDK, 1,
+                                                      tid);
+        }
+    };
 
     bf16x8 bq[8][TERMS];
     if (active) {
@@ -184,15 +259,15 @@ __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
 #pragma unroll
             for (int tt = 0; tt < TERMS; ++tt)
                 bq[j][tt] = __builtin_bit_cast(
-                    bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + tt * pb.plane_stride + s_first * 512));
+                    bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + s_first * (TERMS * 512) + tt * 512));
     }
 
-    if (nchunks > 0) gen_chunk(pb.g, m0, kbeg, pb.M, pb.K, Ss[0], tid);
+    if (nchunks > 0) gen_chunk(0, Ss[0]);
     __syncthreads();
 
     for (int c = 0; c < nchunks; ++c) {
         const int64_t kc = kbeg + (int64_t)c * BF_KC;
-        if (c + 1 < nchunks) gen_chunk(pb.g, m0, kc + BF_KC, pb.M, pb.K, Ss[(c + 1) & 1], tid);
+        if (c + 1 < nchunks) gen_chunk(c + 1, Ss[(c + 1) & 1]);
         const uint16_t *S = Ss[c & 1];
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
@@ -212,7 +287,11 @@ __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
 #pragma unroll
                     for (int tt = 0; tt < TERMS; ++tt)
                         bq[j][tt] = __builtin_bit_cast(
-                            bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + tt * pb.plane_stride + s_next * 512));
+                            bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + s_next * (TERMS * 512) + tt * 512));
+                    // keep the reload of block j here, a whole step ahead of
+                    // its use: without the barrier the scheduler sinks it
+                    // next to the use and every block waits out the latency
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
         }
@@ -230,7 +309,123 @@ __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
             for (int r = 0; r < 16; ++r) {
                 const int64_t gi = m0 + i * 32 + out_row32(lane, r);
                 const int64_t gj = wcol + j * 32 + (lane & 31);
-                if (gi < pb.M && gj < pb.N) P[gi * pb.N + gj] = acc[i][j][r];
+                if (FAST || (gi < pb.M && gj < pb.N)) P[gi * pb.N + gj] = acc[i][j][r];
+            }
+}
+
+// Interior tiles, two waves per SIMD.  A 512-thread workgroup owns the same
+// 64 x 1024 tile as above; each of its 8 waves keeps a 64 x 128 FP32
+// accumulator (2 x 4 blocks, 128 registers), so two waves fit on a SIMD.
+// Every thread generates one Philox cell of the next chunk per chunk.  Waves
+// 0-3 generate before their MFMAs and waves 4-7 after them, so the two waves
+// sharing a SIMD (w and w + 4) are in opposite phases: one wave's generator
+// VALU work issues while the other's MFMAs occupy the matrix core.  The same
+// shape conditions as the FAST form of bf16_skge_kernel apply.
+constexpr int BF_FTHR = 512, BF_FWN = 128;
+
+template <int TERMS, bool GAUSS>
+__global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem pb) {
+    __shared__ __attribute__((aligned(16))) uint16_t Ss[2][BF_MT * BF_LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    // XCD-aware tile order: consecutive tiles (same K-split) land on one XCD
+    const int W = pb.nMB * pb.nNB * pb.nS;
+    const int L = blockIdx.x;
+    const int t = (W % 8 == 0) ? (L % 8) * (W / 8) + L / 8 : L;
+    const int mb = t % pb.nMB;
+    const int rest = t / pb.nMB;
+    const int nb = rest % pb.nNB;
+    const int sp = rest / pb.nNB;
+    const int64_t m0 = (int64_t)(mb + pb.mb0) * BF_MT;
+    const int64_t n0 = (int64_t)nb * BF_NT;
+    const int64_t kbeg = (int64_t)sp * pb.kw;
+    int64_t kend = kbeg + pb.kw;
+    if (kend > pb.Kp) kend = pb.Kp;
+    const int nchunks = (int)((kend - kbeg) / BF_KC);
+    const GenMap g = pb.g;
+
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int64_t wcol = n0 + wave * BF_FWN;
+    const int64_t KS = pb.Kp / 16;
+    // lane's B operand for block j, K step s, term t:
+    //   planes + (((wcol/32 + j)*KS + s)*TERMS + t)*512 + lane*8
+    const uint16_t *bbase = pb.planes + (wcol >> 5) * KS * (TERMS * 512) + lane * 8;
+    const int64_t bstep_blk = KS * (TERMS * 512);
+    const int64_t s_first = kbeg / 16, s_last = kend / 16 - 1;
+
+    // generator: thread owns cell (tid & 7) of operator row tid / 8
+    const int frow = tid >> 3, fcell = tid & 7;
+    const uint64_t foff = (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
+    auto gen_chunk = [&](int c, uint16_t *lds) {
+        const float4 v = gen4_lean(GAUSS, ctr_add(g.ctr, foff + 8 * (uint64_t)c), g.k0, g.k1);
+        uint2 pk;
+        pk.x = (uint32_t)entry_value<uint16_t>(v.x, GAUSS) | ((uint32_t)entry_value<uint16_t>(v.y, GAUSS) << 16);
+        pk.y = (uint32_t)entry_value<uint16_t>(v.z, GAUSS) | ((uint32_t)entry_value<uint16_t>(v.w, GAUSS) << 16);
+        *(uint2 *)(lds + frow * BF_LDK + fcell * 4) = pk;
+    };
+
+    bf16x8 bq[4][TERMS];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int tt = 0; tt < TERMS; ++tt)
+            bq[j][tt] = __builtin_bit_cast(
+                bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + s_first * (TERMS * 512) + tt * 512));
+
+    // one 16-deep K step: block j's B operands for the next step are reloaded
+    // right after its MFMAs, a whole step ahead of their use
+    auto step = [&](const uint16_t *S, int ks, int64_t s_next) {
+        const int koff = ks * 16 + 8 * (lane >> 5);
+        const bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (lane & 31) * BF_LDK + koff));
+        const bf16x8 a1 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (32 + (lane & 31)) * BF_LDK + koff));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int tt = 0; tt < TERMS; ++tt) {
+                acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bq[j][tt], acc[0][j], 0, 0, 0);
+                acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bq[j][tt], acc[1][j], 0, 0, 0);
+            }
+#pragma unroll
+            for (int tt = 0; tt < TERMS; ++tt)
+                bq[j][tt] = __builtin_bit_cast(
+                    bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + s_next * (TERMS * 512) + tt * 512));
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+
+    const bool late = (wave & 4) != 0;
+    gen_chunk(0, Ss[0]);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const uint16_t *S = Ss[c & 1];
+        uint16_t *Sn = Ss[(c + 1) & 1];
+        const int64_t s0 = s_first + 2 * c;
+        const bool more = c + 1 < nchunks;
+        if (more && !late) gen_chunk(c + 1, Sn);
+        step(S, 0, s0 + 1);
+        step(S, 1, s0 + 2 > s_last ? s_last : s0 + 2);
+        if (more && late) gen_chunk(c + 1, Sn);
+        __syncthreads();
+    }
+
+    // partial tile -> scratch (row-major M x N per split; lanes walk columns)
+    float *P = pb.part + (int64_t)sp * pb.M * pb.N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = m0 + i * 32 + out_row32(lane, r);
+                const int64_t gj = wcol + j * 32 + (lane & 31);
+                P[gi * pb.N + gj] = acc[i][j][r];
             }
 }
 
@@ -336,7 +531,6 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     const size_t part_bytes = (size_t)safe_mul(safe_mul(pb.nS, M), N) * 4;
     char *ws = (char *)scratch(SCRATCH_BF16, planes_bytes + part_bytes);
     pb.planes = (const uint16_t *)ws;
-    pb.plane_stride = (int64_t)plane_elems;
     pb.part = (float *)(ws + planes_bytes);
 
     {
@@ -345,16 +539,40 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         if (blocks > 65536) blocks = 65536;
         ProfScope prof("bf16_split_kernel", stream);
         hipLaunchKernelGGL(split_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, A, ars, acs, K, N, pb.Kp, pb.Np,
-                           n_terms, (uint16_t *)pb.planes, pb.plane_stride);
+                           n_terms, (uint16_t *)pb.planes);
         RBX_HIP(hipGetLastError());
     }
     {
-        dim3 grid((unsigned)(pb.nMB * pb.nNB * pb.nS));
         ProfScope prof("bf16_skge_kernel", stream);
-        if (n_terms == 1) hipLaunchKernelGGL(bf16_skge_kernel<1>, grid, dim3(BF_NTHR), 0, stream, pb);
-        else if (n_terms == 2) hipLaunchKernelGGL(bf16_skge_kernel<2>, grid, dim3(BF_NTHR), 0, stream, pb);
-        else hipLaunchKernelGGL(bf16_skge_kernel<3>, grid, dim3(BF_NTHR), 0, stream, pb);
-        RBX_HIP(hipGetLastError());
+        // interior row blocks on the FAST kernel when the shape allows it,
+        // the rest (or everything) on the bounds-checked kernel
+        const bool fast_ok = K % BF_KC == 0 && N % 256 == 0 && !pb.g.b_along_rows && (pb.g.b_off & 3) == 0;
+        const int nfast = fast_ok ? (int)(M / BF_MT) : 0;
+        const int nMB_all = pb.nMB;
+        if (nfast > 0) {
+            Bf16Problem q = pb;
+            q.nMB = nfast;
+            q.mb0 = 0;
+            dim3 grid((unsigned)(q.nMB * q.nNB * q.nS));
+            const bool gs = q.g.gauss != 0;
+            if (n_terms == 1 && gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<1, true>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (n_terms == 1) hipLaunchKernelGGL((bf16_skge_pair_kernel<1, false>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (n_terms == 2 && gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<2, true>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (n_terms == 2) hipLaunchKernelGGL((bf16_skge_pair_kernel<2, false>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<3, true>), grid, dim3(BF_FTHR), 0, stream, q);
+            else hipLaunchKernelGGL((bf16_skge_pair_kernel<3, false>), grid, dim3(BF_FTHR), 0, stream, q);
+            RBX_HIP(hipGetLastError());
+        }
+        if (nMB_all > nfast) {
+            Bf16Problem q = pb;
+            q.nMB = nMB_all - nfast;
+            q.mb0 = nfast;
+            dim3 grid((unsigned)(q.nMB * q.nNB * q.nS));
+            if (n_terms == 1) hipLaunchKernelGGL((bf16_skge_kernel<1, false>), grid, dim3(BF_NTHR), 0, stream, q);
+            else if (n_terms == 2) hipLaunchKernelGGL((bf16_skge_kernel<2, false>), grid, dim3(BF_NTHR), 0, stream, q);
+            else hipLaunchKernelGGL((bf16_skge_kernel<3, false>), grid, dim3(BF_NTHR), 0, stream, q);
+            RBX_HIP(hipGetLastError());
+        }
     }
     {
         dim3 grid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32));

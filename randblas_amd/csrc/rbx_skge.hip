@@ -12,6 +12,9 @@
 // Tile: 64 x 256 outputs per 256-thread workgroup when the operator is X
 // (256 x 64 when it is Y), 4 waves in 2 x 2, K-slice 16,
 // v_mfma_{f64,f32}_16x16x4.  Each generated operator entry feeds 256 outputs.
+// K-slices are double-buffered in LDS: while the MFMAs of slice s run, the
+// data tile of slice s+1 is in flight from memory into registers, and the
+// operator tile of slice s+1 is generated into the other buffer.
 #include "rbx_internal.hh"
 #include "rbx_gemm_common.hh"
 
@@ -40,15 +43,60 @@ struct Mfma16<float> {
 
 constexpr int BK = 16, NTHR = 256;
 
-// Workgroup tile BM x BN (4 waves in 2 x 2, each (BM/2) x (BN/2) in 16 x 16
-// blocks).  The operator tile is BM x BK (GEN_X) or BK x BN (!GEN_X); the
-// other extent is made large (256) so each generated entry feeds 256 outputs.
-template <typename T, bool GEN_X, bool S_MEM, int BM, int BN>
-__global__ __launch_bounds__(NTHR) void skge_kernel(GemmProblem pb) {
+// Thread-to-element map of a PR x QC tile loaded from a strided matrix:
+// consecutive threads walk the unit-stride direction.  Element e of this
+// thread is (pp0 + e*dpp, qq0 + e*dqq).
+template <int PR, int QC>
+struct TileMap {
+    static constexpr int NE = PR * QC / NTHR;
+    static_assert(NE >= 1 && NTHR % PR == 0 && NTHR % QC == 0, "tile does not fit the thread map");
+    int pp0, qq0, dpp, dqq;
+    __device__ TileMap(bool p_unit, int tid) {
+        if (p_unit) {
+            pp0 = tid % PR; qq0 = tid / PR; dpp = 0; dqq = NTHR / PR;
+        } else {
+            qq0 = tid % QC; pp0 = tid / QC; dqq = 0; dpp = NTHR / QC;
+        }
+    }
+};
+
+// Aligned fast path of the operator tile: every Philox cell of the tile is
+// whole and inside the operator.  Orientation 1 (major axis along q): cell cc
+// of tile row pp holds entries (pp, 4cc .. 4cc+3).  Orientation 2 (major axis
+// along p): cell cc of tile column qq holds entries (4cc .. 4cc+3, qq).  The
+// thread's cells in the first K-slice are at counter offsets off0 + e*doff;
+// each further K-slice adds `step`.
+template <int PR, int QC>
+struct GenCells {
+    static constexpr int NG1 = PR * (QC / 4) / NTHR, NG2 = QC * (PR / 4) / NTHR;
+    static_assert(NG1 >= 1 && NG2 >= 1 && NG1 == NG2, "operator tile does not fit the generator map");
+    static constexpr int NG = NG1;
+    int a, cc;       // tile row (orientation 1) or column (orientation 2) of cell 0; cell index
+    int da;          // row / column step between the thread's cells
+    uint64_t off0, doff, step;
+};
+
+// FAST: every tile of the launch is interior (M, N multiples of the tile on
+// the launched blocks, K a multiple of BK) and the generated operator's cells
+// are aligned (b_off % 4 == 0), so no bounds logic is compiled in.  Blocks are
+// offset by (bx0, by0) so that a FAST launch over the interior and generic
+// launches over the ragged edge strips can split one product.
+// GM >= 0 fixes the generated operator's family and orientation at compile
+// time (bit 0: Gaussian, bit 1: major axis along the tile's p axis), so the
+// FAST loop body is one basic block; GM < 0 reads them from the problem.
+template <typename T, bool GEN_X, bool S_MEM, int BM, int BN, bool FAST, int GM>
+__global__ __launch_bounds__(NTHR, 1) void skge_kernel(GemmProblem pb, int bx0, int by0) {
     constexpr int LDX = BM + 1, LDY = BN + 1;
     constexpr int MI = BM / 32, NJ = BN / 32;  // 16x16 blocks per wave
-    __shared__ T Xs[BK * LDX];
-    __shared__ T Ys[BK * LDY];
+    // operator tile OR x OC and data tile DR x DC (rows p, columns q)
+    constexpr int OR = GEN_X ? BM : BK, OC = GEN_X ? BK : BN;
+    constexpr int DR = GEN_X ? BK : BM, DC = GEN_X ? BN : BK;
+    // LDS strides (p, q) of the operator and data tiles: X is stored [k][i]
+    // (ldp 1, ldq LDX), Y is stored [k][j] (ldp LDY, ldq 1)
+    constexpr int OLP = GEN_X ? 1 : LDY, OLQ = GEN_X ? LDX : 1;
+    constexpr int DLP = GEN_X ? LDY : 1, DLQ = GEN_X ? 1 : LDX;
+    __shared__ T Xs[2][BK * LDX];
+    __shared__ T Ys[2][BK * LDY];
     using M16 = Mfma16<T>;
     typename M16::acc_t acc[MI][NJ];
 #pragma unroll
@@ -59,41 +107,154 @@ __global__ __launch_bounds__(NTHR) void skge_kernel(GemmProblem pb) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    const int64_t m0 = (int64_t)blockIdx.x * BM;
-    const int64_t n0 = (int64_t)blockIdx.y * BN;
-    const T *D = (const T *)pb.D;
+    const int64_t m0 = (int64_t)(blockIdx.x + bx0) * BM;
+    const int64_t n0 = (int64_t)(blockIdx.y + by0) * BN;
+    const int64_t nk = (pb.K + BK - 1) / BK;
+    const GenMap g = pb.g;
+    const bool g_gauss = GM >= 0 ? (GM & 1) != 0 : g.gauss != 0;
+    const bool g_rows = GM >= 0 ? (GM & 2) != 0 : g.b_along_rows != 0;
 
-    for (int64_t k0 = 0; k0 < pb.K; k0 += BK) {
-        // X tile: rows i in [m0, m0+BM), cols k in [k0, k0+BK) -> Xs[k][i]
-        if (GEN_X) {
-            if (S_MEM)
-                load_tile<T, BM, BK, NTHR>((const T *)pb.sbuf + pb.soff, pb.srs, pb.scs, m0, k0, pb.M, pb.K, Xs, 1, LDX, tid);
-            else
-                gen_tile<T, BM, BK, NTHR>(pb.g, m0, k0, pb.M, pb.K, Xs, 1, LDX, tid);
-            load_tile<T, BK, BN, NTHR>(D, pb.drs, pb.dcs, k0, n0, pb.K, pb.N, Ys, LDY, 1, tid);
+    // ---- data tile: D(p, q) at D[p*drs + q*dcs]; the K axis is p when the
+    // data is Y (GEN_X) and q when it is X
+    const T *D = (const T *)pb.D;
+    const int64_t dp_lim = GEN_X ? pb.K : pb.M, dq_lim = GEN_X ? pb.N : pb.K;
+    const int64_t dp0 = GEN_X ? 0 : m0, dq0 = GEN_X ? n0 : 0;
+    const TileMap<DR, DC> dmap(pb.drs == 1, tid);
+    const int64_t d_estride = (int64_t)dmap.dpp * pb.drs + (int64_t)dmap.dqq * pb.dcs;
+    const int64_t d_kstride = GEN_X ? BK * pb.drs : BK * pb.dcs;
+    const bool d_fixed_in = FAST || (GEN_X ? (n0 + BN <= pb.N) : (m0 + BM <= pb.M));
+    const T *dptr = D + (dp0 + dmap.pp0) * pb.drs + (dq0 + dmap.qq0) * pb.dcs;
+    T dreg[TileMap<DR, DC>::NE];
+
+    // ---- materialised operator tile (S_MEM): G(p, q) at sbuf[soff + p*srs + q*scs]
+    const T *Sb = (const T *)pb.sbuf + pb.soff;
+    const int64_t op_lim = GEN_X ? pb.M : pb.K, oq_lim = GEN_X ? pb.K : pb.N;
+    const int64_t op0 = GEN_X ? m0 : 0, oq0 = GEN_X ? 0 : n0;
+    const bool o_fixed_in = FAST || (GEN_X ? (m0 + BM <= pb.M) : (n0 + BN <= pb.N));
+    T oreg[S_MEM ? TileMap<OR, OC>::NE : 1];
+
+    // ---- generated operator tile, aligned fast path
+    GenCells<OR, OC> gc;
+    const bool gen_fast = FAST || (!S_MEM && (g.b_off & 3) == 0 && o_fixed_in);
+    if (!S_MEM) {
+        const bool o1 = !g_rows;
+        const int cells_q = OC / 4, cells_p = OR / 4;
+        if (o1) {
+            gc.cc = tid % cells_q; gc.a = tid / cells_q; gc.da = NTHR / cells_q;
+            gc.off0 = (uint64_t)((g.a_off + op0 + gc.a) * g.P + ((g.b_off + oq0) >> 2) + gc.cc);
+            gc.doff = (uint64_t)(gc.da * g.P);
+            gc.step = GEN_X ? (uint64_t)(BK / 4) : (uint64_t)(BK * g.P);
         } else {
-            load_tile<T, BM, BK, NTHR>(D, pb.drs, pb.dcs, m0, k0, pb.M, pb.K, Xs, 1, LDX, tid);
-            if (S_MEM)
-                load_tile<T, BK, BN, NTHR>((const T *)pb.sbuf + pb.soff, pb.srs, pb.scs, k0, n0, pb.K, pb.N, Ys, LDY, 1, tid);
-            else
-                gen_tile<T, BK, BN, NTHR>(pb.g, k0, n0, pb.K, pb.N, Ys, LDY, 1, tid);
+            gc.cc = tid % cells_p; gc.a = tid / cells_p; gc.da = NTHR / cells_p;
+            gc.off0 = (uint64_t)((g.a_off + oq0 + gc.a) * g.P + ((g.b_off + op0) >> 2) + gc.cc);
+            gc.doff = (uint64_t)(gc.da * g.P);
+            gc.step = GEN_X ? (uint64_t)(BK * g.P) : (uint64_t)(BK / 4);
         }
-        __syncthreads();
+    }
+
+    auto load_data = [&](int64_t s) {
+        const int64_t k0 = s * BK;
+        if (FAST || (d_fixed_in && k0 + BK <= pb.K)) {
+            const T *p = dptr + s * d_kstride;
+#pragma unroll
+            for (int e = 0; e < TileMap<DR, DC>::NE; ++e) dreg[e] = p[e * d_estride];
+        } else if constexpr (!FAST) {
+#pragma unroll
+            for (int e = 0; e < TileMap<DR, DC>::NE; ++e) {
+                const int64_t p = dp0 + (GEN_X ? k0 : 0) + dmap.pp0 + e * dmap.dpp;
+                const int64_t q = dq0 + (GEN_X ? 0 : k0) + dmap.qq0 + e * dmap.dqq;
+                dreg[e] = (p < dp_lim && q < dq_lim) ? D[p * pb.drs + q * pb.dcs] : (T)0;
+            }
+        }
+    };
+    auto store_data = [&](int buf) {
+        T *lds = GEN_X ? Ys[buf] : Xs[buf];
+#pragma unroll
+        for (int e = 0; e < TileMap<DR, DC>::NE; ++e)
+            lds[(dmap.pp0 + e * dmap.dpp) * DLP + (dmap.qq0 + e * dmap.dqq) * DLQ] = dreg[e];
+    };
+    // materialised operator: same register staging with the operator's strides
+    const TileMap<OR, OC> omap(pb.srs == 1, tid);
+    auto load_op = [&](int64_t s) {
+        const int64_t k0 = s * BK;
+        const bool in = FAST || (o_fixed_in && k0 + BK <= pb.K);
+#pragma unroll
+        for (int e = 0; e < (S_MEM ? TileMap<OR, OC>::NE : 0); ++e) {
+            const int64_t p = op0 + (GEN_X ? 0 : k0) + omap.pp0 + e * omap.dpp;
+            const int64_t q = oq0 + (GEN_X ? k0 : 0) + omap.qq0 + e * omap.dqq;
+            oreg[e] = (in || (p < op_lim && q < oq_lim)) ? Sb[p * pb.srs + q * pb.scs] : (T)0;
+        }
+    };
+    auto store_op = [&](int buf) {
+        T *lds = GEN_X ? Xs[buf] : Ys[buf];
+#pragma unroll
+        for (int e = 0; e < (S_MEM ? TileMap<OR, OC>::NE : 0); ++e)
+            lds[(omap.pp0 + e * omap.dpp) * OLP + (omap.qq0 + e * omap.dqq) * OLQ] = oreg[e];
+    };
+    auto gen_op = [&](int64_t s, int buf) {
+        T *lds = GEN_X ? Xs[buf] : Ys[buf];
+        const int64_t k0 = s * BK;
+        if (FAST || (gen_fast && k0 + BK <= pb.K)) {
+            const bool gauss = g_gauss;
+#pragma unroll
+            for (int e = 0; e < GenCells<OR, OC>::NG; ++e) {
+                const float4 v = gen4(gauss, ctr_add(g.ctr, gc.off0 + e * gc.doff + s * gc.step), g.k0, g.k1);
+                const int a = gc.a + e * gc.da;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int b = 4 * gc.cc + t;
+                    const T val = entry_value<T>(f4_get(v, t), gauss);
+                    if (!g_rows) lds[a * OLP + b * OLQ] = val;
+                    else lds[b * OLP + a * OLQ] = val;
+                }
+            }
+        } else if constexpr (!FAST) {
+            if (GEN_X) gen_tile<T, OR, OC, NTHR>(g, m0, k0, pb.M, pb.K, lds, OLP, OLQ, tid);
+            else gen_tile<T, OR, OC, NTHR>(g, k0, n0, pb.K, pb.N, lds, OLP, OLQ, tid);
+        }
+    };
+
+    // prologue: K-slice 0 into buffer 0
+    load_data(0);
+    if (S_MEM) load_op(0);
+    else gen_op(0, 0);
+    store_data(0);
+    if (S_MEM) store_op(0);
+    __syncthreads();
+
+    auto mma = [&](int cur) {
+        const T *xs = Xs[cur];
+        const T *ys = Ys[cur];
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + (lane >> 4);
             T a[MI], b[NJ];
 #pragma unroll
-            for (int i = 0; i < MI; ++i) a[i] = Xs[kr * LDX + wm * (BM / 2) + i * 16 + (lane & 15)];
+            for (int i = 0; i < MI; ++i) a[i] = xs[kr * LDX + wm * (BM / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) b[j] = Ys[kr * LDY + wn * (BN / 2) + j * 16 + (lane & 15)];
+            for (int j = 0; j < NJ; ++j) b[j] = ys[kr * LDY + wn * (BN / 2) + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < MI; ++i)
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) acc[i][j] = M16::mma(a[i], b[j], acc[i][j]);
         }
+    };
+    // One K-slice: the next slice's global loads are in flight during this
+    // slice's MFMAs; the next operator tile is generated into the other
+    // buffer, which was last read before the previous barrier.
+    auto slice = [&](int64_t s) {
+        const int cur = (int)(s & 1);
+        load_data(s + 1);
+        if (S_MEM) load_op(s + 1);
+        mma(cur);
+        if (!S_MEM) gen_op(s + 1, cur ^ 1);
+        store_data(cur ^ 1);
+        if (S_MEM) store_op(cur ^ 1);
         __syncthreads();
-    }
+    };
+    // the last slice is peeled so that the loop body has no branches
+    for (int64_t s = 0; s + 1 < nk; ++s) slice(s);
+    mma((int)((nk - 1) & 1));
 
     // epilogue: element r of the lane's accumulator is row M16::out_row(lane, r),
     // column lane%16 of each 16x16 block (the f64 and f32 forms differ)
@@ -107,7 +268,7 @@ __global__ __launch_bounds__(NTHR) void skge_kernel(GemmProblem pb) {
             for (int r = 0; r < 4; ++r) {
                 int64_t gi = m0 + wm * (BM / 2) + i * 16 + M16::out_row(lane, r);
                 int64_t gj = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
-                if (gi < pb.M && gj < pb.N) {
+                if (FAST || (gi < pb.M && gj < pb.N)) {
                     T *c = Cp + gi * pb.crs + gj * pb.ccs;
                     T v = alpha * acc[i][j][r];
                     if (pb.beta != 0.0) v = v + beta * (*c);
@@ -116,24 +277,46 @@ __global__ __launch_bounds__(NTHR) void skge_kernel(GemmProblem pb) {
             }
 }
 
+template <typename T, bool GEN_X, bool S_MEM, int TM, int TN>
+static void launch_tiles(const GemmProblem &pb, hipStream_t stream) {
+    const int64_t gx = (pb.M + TM - 1) / TM, gy = (pb.N + TN - 1) / TN;
+    RBX_REQUIRE(gx < (1ll << 31) && gy <= 65535);
+    // interior blocks take the FAST kernel when K and the generator allow it
+    const bool fast_ok = pb.K % BK == 0 && (S_MEM || (pb.g.b_off & 3) == 0);
+    int64_t fx = fast_ok ? pb.M / TM : 0, fy = fast_ok ? pb.N / TN : 0;
+    if (fx > 0 && fy > 0) {
+        const dim3 grid((unsigned)fx, (unsigned)fy);
+        const int gm = S_MEM ? -1 : (pb.g.gauss ? 1 : 0) | (pb.g.b_along_rows ? 2 : 0);
+        if (gm == 0) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 0>), grid, dim3(NTHR), 0, stream, pb, 0, 0);
+        else if (gm == 1) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 1>), grid, dim3(NTHR), 0, stream, pb, 0, 0);
+        else if (gm == 2) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 2>), grid, dim3(NTHR), 0, stream, pb, 0, 0);
+        else if (gm == 3) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 3>), grid, dim3(NTHR), 0, stream, pb, 0, 0);
+        else hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, -1>), grid, dim3(NTHR), 0, stream, pb, 0, 0);
+    } else {
+        fx = fy = 0;
+    }
+    // ragged edges (or everything) with the bounds-checked kernel:
+    // blocks [fx, gx) x [0, gy) and [0, fx) x [fy, gy)
+    if (gx > fx)
+        hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, false, -1>), dim3((unsigned)(gx - fx), (unsigned)gy),
+                           dim3(NTHR), 0, stream, pb, (int)fx, 0);
+    if (fx > 0 && gy > fy)
+        hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, false, -1>), dim3((unsigned)fx, (unsigned)(gy - fy)),
+                           dim3(NTHR), 0, stream, pb, 0, (int)fy);
+    RBX_HIP(hipGetLastError());
+}
+
 template <typename T>
 static void launch_skge(const GemmProblem &pb, hipStream_t stream) {
-    bool gx = pb.gen_is_x, mem = pb.sbuf != nullptr;
+    const bool mem = pb.sbuf != nullptr;
     ProfScope prof(sizeof(T) == 8 ? "skge_kernel_f64" : "skge_kernel_f32", stream);
-    if (gx) {
-        constexpr int TM = 64, TN = 256;
-        dim3 grid((unsigned)((pb.M + TM - 1) / TM), (unsigned)((pb.N + TN - 1) / TN));
-        RBX_REQUIRE(grid.y <= 65535u);
-        if (!mem) hipLaunchKernelGGL((skge_kernel<T, true, false, TM, TN>), grid, dim3(NTHR), 0, stream, pb);
-        else hipLaunchKernelGGL((skge_kernel<T, true, true, TM, TN>), grid, dim3(NTHR), 0, stream, pb);
+    if (pb.gen_is_x) {
+        if (!mem) launch_tiles<T, true, false, 64, 256>(pb, stream);
+        else launch_tiles<T, true, true, 64, 256>(pb, stream);
     } else {
-        constexpr int TM = 256, TN = 64;
-        dim3 grid((unsigned)((pb.M + TM - 1) / TM), (unsigned)((pb.N + TN - 1) / TN));
-        RBX_REQUIRE(grid.y <= 65535u);
-        if (!mem) hipLaunchKernelGGL((skge_kernel<T, false, false, TM, TN>), grid, dim3(NTHR), 0, stream, pb);
-        else hipLaunchKernelGGL((skge_kernel<T, false, true, TM, TN>), grid, dim3(NTHR), 0, stream, pb);
+        if (!mem) launch_tiles<T, false, false, 256, 64>(pb, stream);
+        else launch_tiles<T, false, true, 256, 64>(pb, stream);
     }
-    RBX_HIP(hipGetLastError());
 }
 
 // C = beta * C on an M x N strided matrix (util::safe_scal semantics).

@@ -538,6 +538,45 @@ __global__ __launch_bounds__(256) void csr_to_coo_rows_kernel(const int64_t *row
     for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) rows[p] = i;
 }
 
+// op(submat(S)) of a materialised operator (S.buff in S.layout) gathered into
+// a row-major n_p x n_q matrix.
+template <typename T>
+__global__ __launch_bounds__(256) void gather_op_kernel(const T *src, int64_t rs, int64_t cs, int64_t n_p,
+                                                        int64_t n_q, T *dst) {
+    const int64_t total = n_p * n_q;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = idx / n_q, q = idx - p * n_q;
+        dst[idx] = src[p * rs + q * cs];
+    }
+}
+
+// G = op(submat(S)) as a row-major n_p x n_q matrix in library scratch: the
+// operator is generated from its counters, or gathered from S.buff.
+template <typename T>
+static T *materialize_op(const rbx_dense_skop *S, char opS, int64_t ro_s, int64_t co_s, int64_t n_p, int64_t n_q,
+                         hipStream_t st) {
+    const size_t bytes = (size_t)safe_mul(n_p > 0 ? n_p : 1, n_q > 0 ? n_q : 1) * sizeof(T);
+    T *G = (T *)scratch(SCRATCH_DENSE_OP, bytes);
+    if (n_p == 0 || n_q == 0) return G;
+    if (S->buff == nullptr) {
+        GenMap gm = gen_map(*S, opS, ro_s, co_s, false);
+        fill_generated<T>(gm, n_p, n_q, G, n_q, 1, st);
+        return G;
+    }
+    // S.buff holds the full operator in S.layout; G[p, q] = S[ro_s + p, co_s + q]
+    // (opS = 'N') or S[ro_s + q, co_s + p] (opS = 'T')
+    if (S->layout != 'C' && S->layout != 'R') RBX_FAIL(RBX_INVALID_ARG, "DenseSkOp.layout must be 'C' or 'R'");
+    const int64_t frs = S->layout == 'C' ? 1 : S->dist.n_cols, fcs = S->layout == 'C' ? S->dist.n_rows : 1;
+    const T *base = (const T *)S->buff + ro_s * frs + co_s * fcs;
+    const int64_t rs = opS == 'N' ? frs : fcs, cs = opS == 'N' ? fcs : frs;
+    int64_t blocks = (n_p * n_q + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(gather_op_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, base, rs, cs, n_p, n_q, G);
+    RBX_HIP(hipGetLastError());
+    return G;
+}
+
 template <typename T>
 static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, T alpha,
                         const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a, const rbx_dense_skop *S, int64_t ro_s,
@@ -565,17 +604,12 @@ static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, i
     // The reference materialises submat(S) in its natural layout and flips opS
     // when that differs from `layout` (sksp.hh:298-324); the product kernel it
     // then picks depends on that layout, which fixes the summation form.
-    char opS_ref = (geo.natural != layout) ? (opS == 'N' ? 'T' : 'N') : opS;
+    // (a materialised operator is used in its own layout, sksp.hh:318-321)
+    const char s_layout = S->buff != nullptr ? S->layout : geo.natural;
+    char opS_ref = (s_layout != layout) ? (opS == 'N' ? 'T' : 'N') : opS;
     char layout_opB = opS_ref == 'N' ? layout : (layout == 'C' ? 'R' : 'C');
     // Materialise G = op(submat(S)) (n x d) row-major: lanes read rows of G.
-    rbx_dense_skop Sg = *S;
-    size_t gbytes = (size_t)safe_mul(n, d) * sizeof(T);
-    T *G = (T *)scratch(SCRATCH_DENSE_OP, gbytes);
-    if (S->buff != nullptr) {
-        RBX_FAIL(RBX_UNSUPPORTED, "rsksp3: materialised operators are not supported; pass buff == NULL");
-    }
-    GenMap gm = gen_map(Sg, opS, ro_s, co_s, false);
-    fill_generated<T>(gm, n, d, G, d, 1, st);
+    T *G = materialize_op<T>(S, opS, ro_s, co_s, n, d, st);
     if (opA == 'N') {
         if (alpha == (T)0 || n == 0 || A->nnz == 0) {
             // beta scaling only (util::safe_scal); reuse the row kernel with alpha = 0
@@ -606,6 +640,105 @@ static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, i
     (void)layG;
     left_spmm_coo<T>(layout, 'T', opG, m, d, n, alpha, A->n_rows, A->n_cols, A->nnz, (const T *)A->vals, rows,
                      A->colidxs, 0, 0, G, ldg, beta, B, ldb, st);
+}
+
+// Adjoint of rsksp3 (opA = 'N') with respect to the stored values of A, the
+// sampled dense-dense product
+//   dvals[k] = alpha * sum_j Bbar[i_k, j] * G[c_k, j],  G = op(submat(S)) (n x d)
+// for every stored entry k = (i_k, c_k) of A.  A workgroup owns SD_RB rows of
+// A; Bbar is staged through LDS in SD_DC-column chunks (coalesced for either
+// layout), each wave walks the stored entries of its rows, and a row of G is
+// one coalesced read.  Partial sums over the chunks accumulate in dvals in a
+// fixed order.
+constexpr int SD_RB = 32;
+template <typename T>
+struct SddmmChunk {
+    static constexpr int DC = sizeof(T) == 8 ? 128 : 256;  // 32 x DC tile: 32 KB
+};
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sddmm_csr_kernel(const int64_t *rowptr, const int64_t *colidx, int64_t m,
+                                                        int64_t d, T alpha, const T *Bb, int64_t brs, int64_t bcs,
+                                                        const T *G, T *dvals) {
+    constexpr int DC = SddmmChunk<T>::DC, NQ = DC / 64;
+    __shared__ T tile[SD_RB][DC + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t i0 = (int64_t)blockIdx.x * SD_RB;
+    for (int64_t j0 = 0; j0 < d; j0 += DC) {
+        const int64_t dc = d - j0 < DC ? d - j0 : DC;
+        const bool last = j0 + DC >= d;
+        __syncthreads();
+        // Bbar[i0 .. i0+SD_RB, j0 .. j0+dc) -> tile, threads walking the
+        // unit-stride direction
+        for (int idx = tid; idx < SD_RB * DC; idx += 256) {
+            int r, c;
+            if (brs == 1) { r = idx % SD_RB; c = idx / SD_RB; }
+            else { c = idx % DC; r = idx / DC; }
+            const int64_t i = i0 + r;
+            tile[r][c] = (i < m && c < dc) ? Bb[i * brs + (j0 + c) * bcs] : (T)0;
+        }
+        __syncthreads();
+        for (int r = wave; r < SD_RB; r += 4) {
+            const int64_t i = i0 + r;
+            if (i >= m) break;
+            T gv[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) gv[q] = tile[r][lane + 64 * q];
+            const int64_t kb = rowptr[i], ke = rowptr[i + 1];
+            for (int64_t k = kb; k < ke; ++k) {
+                const T *grow = G + colidx[k] * d + j0;
+                T p = (T)0;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const int j = lane + 64 * q;
+                    if (j < dc) p = fma(gv[q], grow[j], p);
+                }
+                p = wave_sum(p);
+                if (lane == 0) {
+                    const T tot = j0 == 0 ? p : dvals[k] + p;
+                    dvals[k] = last ? alpha * tot : tot;
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+static void rsksp3_adjoint_impl(char layout, char opS, int64_t m, int64_t d, int64_t n, T alpha,
+                                const rbx_csr_matrix *A, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
+                                const T *Bbar, int64_t ldbb, T *dvals, hipStream_t st) {
+    RBX_REQUIRE(A != nullptr && S != nullptr);
+    check_layout(layout);
+    check_op(opS);
+    RBX_REQUIRE(m >= 0 && d >= 0 && n >= 0);
+    DenseGeom geo = dense_geom(S->dist);
+    safe_mul(geo.dim_minor, geo.P);
+    const int64_t rows_S = opS == 'N' ? n : d, cols_S = opS == 'N' ? d : n;
+    RBX_REQUIRE(A->n_rows == m && A->n_cols == n);
+    RBX_REQUIRE(S->dist.n_rows >= rows_S + ro_s);
+    RBX_REQUIRE(S->dist.n_cols >= cols_S + co_s);
+    if (layout == 'C') RBX_REQUIRE(ldbb >= m);
+    else RBX_REQUIRE(ldbb >= d);
+    if (A->nnz == 0 || m == 0) return;
+    if (d == 0 || n == 0 || alpha == (T)0) {
+        scale_only<T>(dvals, A->nnz, 1, 1, A->nnz, (T)0, st);  // zeros
+        return;
+    }
+    const T *G = materialize_op<T>(S, opS, ro_s, co_s, n, d, st);
+    const int64_t brs = layout == 'C' ? 1 : ldbb, bcs = layout == 'C' ? ldbb : 1;
+    const int64_t blocks = (m + SD_RB - 1) / SD_RB;
+    RBX_REQUIRE(blocks < (1ll << 31));
+    ProfScope prof("sddmm_csr_kernel", st);
+    hipLaunchKernelGGL(sddmm_csr_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, A->rowptr, A->colidxs, m, d,
+                       alpha, Bbar, brs, bcs, G, dvals);
+    RBX_HIP(hipGetLastError());
 }
 
 template <typename T>
@@ -747,6 +880,22 @@ rbx_status rbx_srsksp3(char layout, char opA, char opS, int64_t m, int64_t d, in
                        int64_t co_s, float beta, float *B, int64_t ldb, void *stream) {
     RBX_API_BEGIN
     rsksp3_impl<float>(layout, opA, opS, m, d, n, alpha, A, ro_a, co_a, S, ro_s, co_s, beta, B, ldb, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_srsksp3_adjoint(char layout, char opS, int64_t m, int64_t d, int64_t n, float alpha,
+                               const rbx_csr_matrix *A, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
+                               const float *Bbar, int64_t ldbb, float *dvals, void *stream) {
+    RBX_API_BEGIN
+    rsksp3_adjoint_impl<float>(layout, opS, m, d, n, alpha, A, S, ro_s, co_s, Bbar, ldbb, dvals, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_drsksp3_adjoint(char layout, char opS, int64_t m, int64_t d, int64_t n, double alpha,
+                               const rbx_csr_matrix *A, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
+                               const double *Bbar, int64_t ldbb, double *dvals, void *stream) {
+    RBX_API_BEGIN
+    rsksp3_adjoint_impl<double>(layout, opS, m, d, n, alpha, A, S, ro_s, co_s, Bbar, ldbb, dvals, (hipStream_t)stream);
     RBX_API_END
 }
 

@@ -158,3 +158,101 @@ def test_unsampled_right_sketch_returns(rb):
     B = torch.empty(5 * 10, dtype=torch.float64, device=DEV)
     rb.rskges("C", "N", "N", 5, 10, 100, 1.0, A, 5, S, 0, 0, 0.0, B, 5)
     assert torch.isfinite(B).all()
+
+
+def _op_submat(orc, Sshape, key, opS, ro, co, rows, cols, dtype):
+    """op(submat(S)) (rows x cols) of a Gaussian DenseSkOp, from the oracle."""
+    Sm, _ = orc.fill_dense(Sshape[0], Sshape[1], [0, 0, 0, 0], [key, 0], dtype=dtype)
+    if opS == "N":
+        return np.ascontiguousarray(Sm[ro:ro + rows, co:co + cols])
+    return np.ascontiguousarray(Sm[ro:ro + cols, co:co + rows].T)
+
+
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rsksp3_materialised_operator(rb, orc, layout, dtype):
+    """rsksp3 with S.buff set uses the buffer in S.layout (sksp.hh:318-321)."""
+    rng = np.random.default_rng(21)
+    m, n, d = 300, 2000, 48
+    rowptr, colidx, vals = _csr(rng, m, n, 15)
+    vals = vals.astype(dtype)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    S = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(42))
+    rb.fill_dense(S, dtype=tdt)
+    Acsr = rb.CSRMatrix(m, n, torch.from_numpy(vals).to(DEV), torch.from_numpy(rowptr).to(DEV),
+                        torch.from_numpy(colidx).to(DEV))
+    ldb = m if layout == "C" else d
+    B = torch.zeros(m * d, dtype=tdt, device=DEV)
+    rb.rsksp3(layout, "N", "N", m, d, n, 1.0, Acsr, 0, 0, S, 0, 0, 0.0, B, ldb)
+    Sm, _ = orc.fill_dense(n, d, [0, 0, 0, 0], [42, 0], dtype=dtype)
+    nat = S.layout
+    Sbuf = np.ascontiguousarray(Sm.reshape(-1, order="F" if nat == "C" else "C"))
+    opS_ref = "N" if nat == layout else "T"
+    ref = np.zeros(m * d, dtype)
+    orc.left_spmm_csr("d" if dtype == np.float64 else "s", layout, opS_ref, m, d, n, 1.0, vals, rowptr, colidx,
+                      Sbuf, n if nat == "C" else d, 0.0, ref, ldb)
+    assert np.array_equal(host(B), ref)
+
+
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("opS", ["N", "T"])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-5)])
+@pytest.mark.parametrize("materialised", [False, True])
+def test_rsksp3_adjoint(rb, orc, layout, opS, dtype, tol, materialised):
+    from conftest import rel_fro
+    rng = np.random.default_rng(23)
+    m, n, d = 250, 1500, 300      # d > one LDS chunk for both precisions
+    ro, co = 3, 5
+    rowptr, colidx, vals = _csr(rng, m, n, 12)
+    vals = vals.astype(dtype)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    Sshape = (n + 10, d + 10) if opS == "N" else (d + 10, n + 10)
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape), rb.RNGState(8))
+    if materialised:
+        rb.fill_dense(S, dtype=tdt)
+    Acsr = rb.CSRMatrix(m, n, torch.from_numpy(vals).to(DEV), torch.from_numpy(rowptr).to(DEV),
+                        torch.from_numpy(colidx).to(DEV))
+    ldbb = m if layout == "C" else d
+    Bbar = rng.standard_normal(m * d).astype(dtype)
+    dvals = torch.zeros(len(vals), dtype=tdt, device=DEV)
+    rb.rsksp3_adjoint(layout, opS, m, d, n, 0.5, Acsr, S, ro, co, torch.from_numpy(Bbar).to(DEV), ldbb, dvals)
+    G = _op_submat(orc, Sshape, 8, opS, ro, co, n, d, dtype)  # n x d
+    brs, bcs = (1, ldbb) if layout == "C" else (ldbb, 1)
+    ref = orc.sddmm_csr("d", m, d, 0.5, rowptr, colidx, Bbar.astype(np.float64), brs, bcs,
+                        G.astype(np.float64).reshape(-1), d, 1)
+    assert rel_fro(host(dvals), ref) <= tol
+    # adjoint identity against the forward product: <vals, dvals> = 0.5 * <A op(S), Bbar>
+    if dtype == np.float64:
+        B = torch.zeros(m * d, dtype=tdt, device=DEV)
+        rb.rsksp3(layout, "N", opS, m, d, n, 1.0, Acsr, 0, 0, S, ro, co, 0.0, B, ldbb)
+        lhs = float(np.dot(vals, host(dvals)))
+        rhs = 0.5 * float(np.dot(host(B), Bbar))
+        assert abs(lhs - rhs) <= 1e-12 * max(abs(rhs), 1.0)
+
+
+def test_rsksp3_power_law_full_size(rb, orc):
+    """A 200000 x 50000 CSR, power-law column degrees (exponent 2.1, ~20 stored
+    entries per row), N(0,1) FP32 values; S = DenseDist(50000, 256) key 42.
+    B = A * S bit-exact against the oracle (csr_spmm_impl.hh order); the
+    adjoint with respect to A's values within 1e-5 of the FP64 oracle."""
+    from conftest import rel_fro
+    from randblas_amd.datagen import power_law_csr
+    m, n, d = 200000, 50000, 256
+    rowptr, colidx, vals = power_law_csr(m, n, 20, 2.1, 17)
+    assert 3.0e6 < len(vals) < 4.2e6
+    S = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(42))
+    Acsr = rb.CSRMatrix(m, n, torch.from_numpy(vals).to(DEV), torch.from_numpy(rowptr).to(DEV),
+                        torch.from_numpy(colidx).to(DEV))
+    B = torch.zeros(m * d, dtype=torch.float32, device=DEV)
+    rb.rsksp3("C", "N", "N", m, d, n, 1.0, Acsr, 0, 0, S, 0, 0, 0.0, B, m)
+    Sm, _ = orc.fill_dense(n, d, [0, 0, 0, 0], [42, 0], dtype=np.float32)  # natural ColMajor
+    ref = np.zeros(m * d, np.float32)
+    orc.left_spmm_csr("s", "C", "N", m, d, n, 1.0, vals, rowptr, colidx,
+                      np.ascontiguousarray(Sm.reshape(-1, order="F")), n, 0.0, ref, m)
+    assert np.array_equal(host(B), ref)
+    Bbar = np.random.default_rng(3).standard_normal(m * d).astype(np.float32)
+    dvals = torch.zeros(len(vals), dtype=torch.float32, device=DEV)
+    rb.rsksp3_adjoint("C", "N", m, d, n, 1.0, Acsr, S, 0, 0, torch.from_numpy(Bbar).to(DEV), m, dvals)
+    refd = orc.sddmm_csr("d", m, d, 1.0, rowptr, colidx, Bbar.astype(np.float64), 1, m,
+                         Sm.astype(np.float64).reshape(-1), d, 1)
+    assert rel_fro(host(dvals), refd) <= 1e-5

@@ -138,3 +138,23 @@ def test_left_spmm_csr_matches_dense(orc):
     C = np.zeros(d * n)
     orc.left_spmm_csr("d", "C", "N", d, n, m, 1.5, vals, rowptr, colidx, B, m, 0.0, C, d)
     np.testing.assert_allclose(C.reshape(n, d).T, 1.5 * A @ B.reshape(n, m).T, rtol=1e-13, atol=1e-13)
+
+
+def test_sddmm_oracle_pins(orc):
+    """The oracle's SDDMM equals the dense formula alpha*(Bbar @ Sm^T) sampled at
+    A's nonzeros, and satisfies the adjoint identity
+    sum_k vals_k * dvals_k = alpha * <A @ Sm, Bbar>."""
+    rng = np.random.default_rng(2)
+    m, n, d = 40, 70, 9
+    A = (rng.random((m, n)) < 0.15) * rng.standard_normal((m, n))
+    rowptr = np.concatenate([[0], np.cumsum((A != 0).sum(1))]).astype(np.int64)
+    col = np.nonzero(A)[1].astype(np.int64)
+    vals = A[A != 0]
+    Bb = rng.standard_normal((m, d))
+    Sm = rng.standard_normal((n, d))
+    # Bbar column-major, Sm row-major
+    BbF = np.asfortranarray(Bb)
+    out = orc.sddmm_csr("d", m, d, 0.75, rowptr, col, BbF.reshape(-1, order="F"), 1, m,
+                        np.ascontiguousarray(Sm).reshape(-1), d, 1)
+    np.testing.assert_allclose(out, 0.75 * (Bb @ Sm.T)[np.nonzero(A)], rtol=1e-14, atol=1e-14)
+    assert abs(np.dot(vals, out) - 0.75 * np.sum((A @ Sm) * Bb)) <= 1e-12 * np.abs(out).sum()
