@@ -70,9 +70,11 @@ __device__ __forceinline__ uint16_t entry_value<uint16_t>(float v, bool gauss) {
 // When g' lies farther than that from every bf16 rounding midpoint (the
 // distance taken at half weight to cover a binade edge), RNE(g') = RNE(g).
 // Otherwise the lane recomputes the pair on the exact path.  The bounds are
-// twice the maxima measured over all 2^32 words.
-constexpr float BM_ES = 0x1p-21f;   // sin / cos factor, absolute
-constexpr float BM_ER = 0x1p-20f;   // radius factor, relative
+// at least twice the maxima over all 2^32 words measured on gfx950
+// (max |s' - s| = 2.53e-7, max |c' - c| = 2.38e-7, max |r' - r| / r =
+// 1.68e-7, and r' = 0 exactly when r = 0).
+constexpr float BM_ES = 0x1p-20f;   // sin / cos factor, absolute (9.5e-7)
+constexpr float BM_ER = 0x1p-21f;   // radius factor, relative (4.8e-7)
 
 // true when RNE to bfloat16 of any value within D of g equals RNE(g)
 __device__ __forceinline__ bool bf16_round_safe(float g, float D) {
@@ -364,11 +366,7 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
     const int frow = tid >> 3, fcell = tid & 7;
     const uint64_t foff = (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
     auto gen_chunk = [&](int c, uint16_t *lds) {
-        const float4 v = gen4_lean(GAUSS, ctr_add(g.ctr, foff + 8 * (uint64_t)c), g.k0, g.k1);
-        uint2 pk;
-        pk.x = (uint32_t)entry_value<uint16_t>(v.x, GAUSS) | ((uint32_t)entry_value<uint16_t>(v.y, GAUSS) << 16);
-        pk.y = (uint32_t)entry_value<uint16_t>(v.z, GAUSS) | ((uint32_t)entry_value<uint16_t>(v.w, GAUSS) << 16);
-        *(uint2 *)(lds + frow * BF_LDK + fcell * 4) = pk;
+        *(uint2 *)(lds + frow * BF_LDK + fcell * 4) = gen4_bf16<GAUSS>(ctr_add(g.ctr, foff + 8 * (uint64_t)c), g.k0, g.k1);
     };
 
     bf16x8 bq[4][TERMS];
@@ -385,18 +383,18 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
         const int koff = ks * 16 + 8 * (lane >> 5);
         const bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (lane & 31) * BF_LDK + koff));
         const bf16x8 a1 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (32 + (lane & 31)) * BF_LDK + koff));
+        // term-outer: consecutive MFMAs on one accumulator are 8 apart, and
+        // each accumulator still adds its terms in order hi, lo, lo2
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int tt = 0; tt < TERMS; ++tt) {
 #pragma unroll
-            for (int tt = 0; tt < TERMS; ++tt) {
+            for (int j = 0; j < 4; ++j) {
                 acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bq[j][tt], acc[0][j], 0, 0, 0);
                 acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bq[j][tt], acc[1][j], 0, 0, 0);
-            }
-#pragma unroll
-            for (int tt = 0; tt < TERMS; ++tt)
                 bq[j][tt] = __builtin_bit_cast(
                     bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + s_next * (TERMS * 512) + tt * 512));
-            __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     };
 

@@ -46,16 +46,16 @@ constexpr int BK = 16, NTHR = 256;
 // Thread-to-element map of a PR x QC tile loaded from a strided matrix:
 // consecutive threads walk the unit-stride direction.  Element e of this
 // thread is (pp0 + e*dpp, qq0 + e*dqq).
-template <int PR, int QC>
+template <int PR, int QC, int NT>
 struct TileMap {
-    static constexpr int NE = PR * QC / NTHR;
-    static_assert(NE >= 1 && NTHR % PR == 0 && NTHR % QC == 0, "tile does not fit the thread map");
+    static constexpr int NE = PR * QC / NT;
+    static_assert(NE >= 1 && NT % PR == 0 && NT % QC == 0, "tile does not fit the thread map");
     int pp0, qq0, dpp, dqq;
     __device__ TileMap(bool p_unit, int tid) {
         if (p_unit) {
-            pp0 = tid % PR; qq0 = tid / PR; dpp = 0; dqq = NTHR / PR;
+            pp0 = tid % PR; qq0 = tid / PR; dpp = 0; dqq = NT / PR;
         } else {
-            qq0 = tid % QC; pp0 = tid / QC; dqq = 0; dpp = NTHR / QC;
+            qq0 = tid % QC; pp0 = tid / QC; dqq = 0; dpp = NT / QC;
         }
     }
 };
@@ -65,12 +65,14 @@ struct TileMap {
 // of tile row pp holds entries (pp, 4cc .. 4cc+3).  Orientation 2 (major axis
 // along p): cell cc of tile column qq holds entries (4cc .. 4cc+3, qq).  The
 // thread's cells in the first K-slice are at counter offsets off0 + e*doff;
-// each further K-slice adds `step`.
-template <int PR, int QC>
+// each further K-slice adds `step`.  With fewer cells than threads, threads
+// [0, NCELLS) generate one cell each and the others none.
+template <int PR, int QC, int NT>
 struct GenCells {
-    static constexpr int NG1 = PR * (QC / 4) / NTHR, NG2 = QC * (PR / 4) / NTHR;
-    static_assert(NG1 >= 1 && NG2 >= 1 && NG1 == NG2, "operator tile does not fit the generator map");
-    static constexpr int NG = NG1;
+    static constexpr int NCELLS = PR * QC / 4;
+    static_assert(NCELLS % NT == 0 || NT % NCELLS == 0, "operator tile does not fit the generator map");
+    static constexpr int NG = NCELLS >= NT ? NCELLS / NT : 1;
+    static constexpr int NTG = NCELLS >= NT ? NT : NCELLS;  // generating threads
     int a, cc;       // tile row (orientation 1) or column (orientation 2) of cell 0; cell index
     int da;          // row / column step between the thread's cells
     uint64_t off0, doff, step;
@@ -84,10 +86,16 @@ struct GenCells {
 // GM >= 0 fixes the generated operator's family and orientation at compile
 // time (bit 0: Gaussian, bit 1: major axis along the tile's p axis), so the
 // FAST loop body is one basic block; GM < 0 reads them from the problem.
-template <typename T, bool GEN_X, bool S_MEM, int BM, int BN, bool FAST, int GM>
-__global__ __launch_bounds__(NTHR, 1) void skge_kernel(GemmProblem pb, int bx0, int by0) {
+// NW waves (4 or 8) in a 2 x NW/2 grid over the tile.  With 8 waves two
+// share each SIMD (the accumulators halve); the operator tile's cells go to
+// the first waves only, so on every SIMD one wave's generator work overlaps
+// the other wave's matrix work.
+template <typename T, bool GEN_X, bool S_MEM, int BM, int BN, bool FAST, int GM, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void skge_kernel(GemmProblem pb, int bx0, int by0) {
+    constexpr int NT = 64 * NW;
     constexpr int LDX = BM + 1, LDY = BN + 1;
-    constexpr int MI = BM / 32, NJ = BN / 32;  // 16x16 blocks per wave
+    constexpr int WN = BN / (NW / 2);          // columns per wave
+    constexpr int MI = BM / 32, NJ = WN / 16;  // 16x16 blocks per wave
     // operator tile OR x OC and data tile DR x DC (rows p, columns q)
     constexpr int OR = GEN_X ? BM : BK, OC = GEN_X ? BK : BN;
     constexpr int DR = GEN_X ? BK : BM, DC = GEN_X ? BN : BK;
@@ -119,33 +127,33 @@ __global__ __launch_bounds__(NTHR, 1) void skge_kernel(GemmProblem pb, int bx0, 
     const T *D = (const T *)pb.D;
     const int64_t dp_lim = GEN_X ? pb.K : pb.M, dq_lim = GEN_X ? pb.N : pb.K;
     const int64_t dp0 = GEN_X ? 0 : m0, dq0 = GEN_X ? n0 : 0;
-    const TileMap<DR, DC> dmap(pb.drs == 1, tid);
+    const TileMap<DR, DC, NT> dmap(pb.drs == 1, tid);
     const int64_t d_estride = (int64_t)dmap.dpp * pb.drs + (int64_t)dmap.dqq * pb.dcs;
     const int64_t d_kstride = GEN_X ? BK * pb.drs : BK * pb.dcs;
     const bool d_fixed_in = FAST || (GEN_X ? (n0 + BN <= pb.N) : (m0 + BM <= pb.M));
     const T *dptr = D + (dp0 + dmap.pp0) * pb.drs + (dq0 + dmap.qq0) * pb.dcs;
-    T dreg[TileMap<DR, DC>::NE];
+    T dreg[TileMap<DR, DC, NT>::NE];
 
     // ---- materialised operator tile (S_MEM): G(p, q) at sbuf[soff + p*srs + q*scs]
     const T *Sb = (const T *)pb.sbuf + pb.soff;
     const int64_t op_lim = GEN_X ? pb.M : pb.K, oq_lim = GEN_X ? pb.K : pb.N;
     const int64_t op0 = GEN_X ? m0 : 0, oq0 = GEN_X ? 0 : n0;
     const bool o_fixed_in = FAST || (GEN_X ? (m0 + BM <= pb.M) : (n0 + BN <= pb.N));
-    T oreg[S_MEM ? TileMap<OR, OC>::NE : 1];
+    T oreg[S_MEM ? TileMap<OR, OC, NT>::NE : 1];
 
     // ---- generated operator tile, aligned fast path
-    GenCells<OR, OC> gc;
+    GenCells<OR, OC, NT> gc;
     const bool gen_fast = FAST || (!S_MEM && (g.b_off & 3) == 0 && o_fixed_in);
     if (!S_MEM) {
         const bool o1 = !g_rows;
         const int cells_q = OC / 4, cells_p = OR / 4;
         if (o1) {
-            gc.cc = tid % cells_q; gc.a = tid / cells_q; gc.da = NTHR / cells_q;
+            gc.cc = tid % cells_q; gc.a = tid / cells_q; gc.da = NT / cells_q;
             gc.off0 = (uint64_t)((g.a_off + op0 + gc.a) * g.P + ((g.b_off + oq0) >> 2) + gc.cc);
             gc.doff = (uint64_t)(gc.da * g.P);
             gc.step = GEN_X ? (uint64_t)(BK / 4) : (uint64_t)(BK * g.P);
         } else {
-            gc.cc = tid % cells_p; gc.a = tid / cells_p; gc.da = NTHR / cells_p;
+            gc.cc = tid % cells_p; gc.a = tid / cells_p; gc.da = NT / cells_p;
             gc.off0 = (uint64_t)((g.a_off + oq0 + gc.a) * g.P + ((g.b_off + op0) >> 2) + gc.cc);
             gc.doff = (uint64_t)(gc.da * g.P);
             gc.step = GEN_X ? (uint64_t)(BK * g.P) : (uint64_t)(BK / 4);
@@ -157,10 +165,10 @@ __global__ __launch_bounds__(NTHR, 1) void skge_kernel(GemmProblem pb, int bx0, 
         if (FAST || (d_fixed_in && k0 + BK <= pb.K)) {
             const T *p = dptr + s * d_kstride;
 #pragma unroll
-            for (int e = 0; e < TileMap<DR, DC>::NE; ++e) dreg[e] = p[e * d_estride];
+            for (int e = 0; e < TileMap<DR, DC, NT>::NE; ++e) dreg[e] = p[e * d_estride];
         } else if constexpr (!FAST) {
 #pragma unroll
-            for (int e = 0; e < TileMap<DR, DC>::NE; ++e) {
+            for (int e = 0; e < TileMap<DR, DC, NT>::NE; ++e) {
                 const int64_t p = dp0 + (GEN_X ? k0 : 0) + dmap.pp0 + e * dmap.dpp;
                 const int64_t q = dq0 + (GEN_X ? 0 : k0) + dmap.qq0 + e * dmap.dqq;
                 dreg[e] = (p < dp_lim && q < dq_lim) ? D[p * pb.drs + q * pb.dcs] : (T)0;
@@ -170,16 +178,16 @@ __global__ __launch_bounds__(NTHR, 1) void skge_kernel(GemmProblem pb, int bx0, 
     auto store_data = [&](int buf) {
         T *lds = GEN_X ? Ys[buf] : Xs[buf];
 #pragma unroll
-        for (int e = 0; e < TileMap<DR, DC>::NE; ++e)
+        for (int e = 0; e < TileMap<DR, DC, NT>::NE; ++e)
             lds[(dmap.pp0 + e * dmap.dpp) * DLP + (dmap.qq0 + e * dmap.dqq) * DLQ] = dreg[e];
     };
     // materialised operator: same register staging with the operator's strides
-    const TileMap<OR, OC> omap(pb.srs == 1, tid);
+    const TileMap<OR, OC, NT> omap(pb.srs == 1, tid);
     auto load_op = [&](int64_t s) {
         const int64_t k0 = s * BK;
         const bool in = FAST || (o_fixed_in && k0 + BK <= pb.K);
 #pragma unroll
-        for (int e = 0; e < (S_MEM ? TileMap<OR, OC>::NE : 0); ++e) {
+        for (int e = 0; e < (S_MEM ? TileMap<OR, OC, NT>::NE : 0); ++e) {
             const int64_t p = op0 + (GEN_X ? 0 : k0) + omap.pp0 + e * omap.dpp;
             const int64_t q = oq0 + (GEN_X ? k0 : 0) + omap.qq0 + e * omap.dqq;
             oreg[e] = (in || (p < op_lim && q < oq_lim)) ? Sb[p * pb.srs + q * pb.scs] : (T)0;
@@ -188,7 +196,7 @@ __global__ __launch_bounds__(NTHR, 1) void skge_kernel(GemmProblem pb, int bx0, 
     auto store_op = [&](int buf) {
         T *lds = GEN_X ? Xs[buf] : Ys[buf];
 #pragma unroll
-        for (int e = 0; e < (S_MEM ? TileMap<OR, OC>::NE : 0); ++e)
+        for (int e = 0; e < (S_MEM ? TileMap<OR, OC, NT>::NE : 0); ++e)
             lds[(omap.pp0 + e * omap.dpp) * OLP + (omap.qq0 + e * omap.dqq) * OLQ] = oreg[e];
     };
     auto gen_op = [&](int64_t s, int buf) {
@@ -196,8 +204,9 @@ __global__ __launch_bounds__(NTHR, 1) void skge_kernel(GemmProblem pb, int bx0, 
         const int64_t k0 = s * BK;
         if (FAST || (gen_fast && k0 + BK <= pb.K)) {
             const bool gauss = g_gauss;
+            if (GenCells<OR, OC, NT>::NTG < NT && tid >= GenCells<OR, OC, NT>::NTG) return;
 #pragma unroll
-            for (int e = 0; e < GenCells<OR, OC>::NG; ++e) {
+            for (int e = 0; e < GenCells<OR, OC, NT>::NG; ++e) {
                 const float4 v = gen4(gauss, ctr_add(g.ctr, gc.off0 + e * gc.doff + s * gc.step), g.k0, g.k1);
                 const int a = gc.a + e * gc.da;
 #pragma unroll
@@ -209,8 +218,8 @@ __global__ __launch_bounds__(NTHR, 1) void skge_kernel(GemmProblem pb, int bx0, 
                 }
             }
         } else if constexpr (!FAST) {
-            if (GEN_X) gen_tile<T, OR, OC, NTHR>(g, m0, k0, pb.M, pb.K, lds, OLP, OLQ, tid);
-            else gen_tile<T, OR, OC, NTHR>(g, k0, n0, pb.K, pb.N, lds, OLP, OLQ, tid);
+            if (GEN_X) gen_tile<T, OR, OC, NT>(g, m0, k0, pb.M, pb.K, lds, OLP, OLQ, tid);
+            else gen_tile<T, OR, OC, NT>(g, k0, n0, pb.K, pb.N, lds, OLP, OLQ, tid);
         }
     };
 
@@ -232,7 +241,7 @@ __global__ __launch_bounds__(NTHR, 1) void skge_kernel(GemmProblem pb, int bx0, 
 #pragma unroll
             for (int i = 0; i < MI; ++i) a[i] = xs[kr * LDX + wm * (BM / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) b[j] = ys[kr * LDY + wn * (BN / 2) + j * 16 + (lane & 15)];
+            for (int j = 0; j < NJ; ++j) b[j] = ys[kr * LDY + wn * WN + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -267,7 +276,7 @@ __global__ __launch_bounds__(NTHR, 1) void skge_kernel(GemmProblem pb, int bx0, 
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 int64_t gi = m0 + wm * (BM / 2) + i * 16 + M16::out_row(lane, r);
-                int64_t gj = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
+                int64_t gj = n0 + wn * WN + j * 16 + (lane & 15);
                 if (FAST || (gi < pb.M && gj < pb.N)) {
                     T *c = Cp + gi * pb.crs + gj * pb.ccs;
                     T v = alpha * acc[i][j][r];
@@ -287,21 +296,22 @@ static void launch_tiles(const GemmProblem &pb, hipStream_t stream) {
     if (fx > 0 && fy > 0) {
         const dim3 grid((unsigned)fx, (unsigned)fy);
         const int gm = S_MEM ? -1 : (pb.g.gauss ? 1 : 0) | (pb.g.b_along_rows ? 2 : 0);
-        if (gm == 0) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 0>), grid, dim3(NTHR), 0, stream, pb, 0, 0);
-        else if (gm == 1) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 1>), grid, dim3(NTHR), 0, stream, pb, 0, 0);
-        else if (gm == 2) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 2>), grid, dim3(NTHR), 0, stream, pb, 0, 0);
-        else if (gm == 3) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 3>), grid, dim3(NTHR), 0, stream, pb, 0, 0);
-        else hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, -1>), grid, dim3(NTHR), 0, stream, pb, 0, 0);
+        constexpr int NF = 8, TF = 64 * NF;  // two waves per SIMD on interior tiles
+        if (gm == 0) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 0, NF>), grid, dim3(TF), 0, stream, pb, 0, 0);
+        else if (gm == 1) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 1, NF>), grid, dim3(TF), 0, stream, pb, 0, 0);
+        else if (gm == 2) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 2, NF>), grid, dim3(TF), 0, stream, pb, 0, 0);
+        else if (gm == 3) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 3, NF>), grid, dim3(TF), 0, stream, pb, 0, 0);
+        else hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, -1, NF>), grid, dim3(TF), 0, stream, pb, 0, 0);
     } else {
         fx = fy = 0;
     }
     // ragged edges (or everything) with the bounds-checked kernel:
     // blocks [fx, gx) x [0, gy) and [0, fx) x [fy, gy)
     if (gx > fx)
-        hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, false, -1>), dim3((unsigned)(gx - fx), (unsigned)gy),
+        hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, false, -1, 4>), dim3((unsigned)(gx - fx), (unsigned)gy),
                            dim3(NTHR), 0, stream, pb, (int)fx, 0);
     if (fx > 0 && gy > fy)
-        hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, false, -1>), dim3((unsigned)fx, (unsigned)(gy - fy)),
+        hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, false, -1, 4>), dim3((unsigned)fx, (unsigned)(gy - fy)),
                            dim3(NTHR), 0, stream, pb, 0, (int)fy);
     RBX_HIP(hipGetLastError());
 }

@@ -45,6 +45,23 @@ def test_identity_gives_rounded_operator(rb, orc, terms):
     assert np.array_equal(got, orc.bf16_round(ref))
 
 
+@pytest.mark.parametrize("family", ["G", "U"])
+def test_identity_interior_tiles_exact(rb, orc, family):
+    """Interior shapes (d multiple of 64, n of 256, m of 32) take the two-wave
+    kernel and, for Gaussians, the hardware-transcendental generator with its
+    exact fallback: all 8.4M entries of bf16(S) must equal the oracle's."""
+    d, m = 2048, 4096
+    S = rb.DenseSkOp(rb.DenseDist(d, m, rb.ScalarDist(family)), rb.RNGState(77))
+    I = torch.eye(m, dtype=torch.float32, device=DEV).reshape(-1)
+    B = torch.empty(d * m, dtype=torch.float32, device=DEV)
+    rb.lskge3_bf16split("C", "N", "N", d, m, m, 1.0, S, 0, 0, I, m, 0.0, B, d, n_terms=3)
+    got = host(B).reshape(m, d).T
+    ref, _ = orc.fill_dense(d, m, [0, 0, 0, 0], [77, 0], family=family, dtype=np.float32)
+    want = orc.bf16_round(ref)
+    bad = np.count_nonzero(got != want)
+    assert bad == 0, f"{bad} entries differ"
+
+
 def _case(rb, orc, layout, opS, opA, d, n, m, terms, alpha=1.0, beta=0.0, ro=0, co=0, key=42, seed=0):
     rng = np.random.default_rng(seed)
     Sshape = (d + ro, m + co) if opS == "N" else (m + ro, d + co)
