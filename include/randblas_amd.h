@@ -109,6 +109,28 @@ typedef struct {
     int64_t *colidxs;
 } rbx_csr_matrix;
 
+/* CSCMatrix<T, int64_t> (RandBLAS/sparse_data/csc_matrix.hh:74-244), zero-based,
+ * device arrays. */
+typedef struct {
+    int64_t n_rows;
+    int64_t n_cols;
+    int64_t nnz;
+    void *vals;
+    int64_t *rowidxs;
+    int64_t *colptr;
+} rbx_csc_matrix;
+
+/* COOMatrix<T, int64_t> (RandBLAS/sparse_data/coo_matrix.hh:83-308), zero-based,
+ * device arrays in any order (entries need not be sorted). */
+typedef struct {
+    int64_t n_rows;
+    int64_t n_cols;
+    int64_t nnz;
+    void *vals;
+    int64_t *rows;
+    int64_t *cols;
+} rbx_coo_matrix;
+
 /* ---------------------------------------------------------------- errors */
 const char *rbx_last_error_message(void);
 /* Version string of this library. */
@@ -287,6 +309,52 @@ rbx_status rbx_drsksp3(char layout, char opA, char opS, int64_t m, int64_t d, in
                        double alpha, const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a,
                        const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, double beta,
                        double *B, int64_t ldb, void *stream);
+
+/* lsksp3 (sparse_data/sksp.hh:132-180): B = alpha * op(submat(S)) * op(A) + beta * B
+ * with S a DenseSkOp (op(submat(S)) d x m at (ro_s, co_s)) and A a whole CSR
+ * matrix (op(A) m x n, ro_a = co_a = 0); each output entry is accumulated in
+ * the order of the reference's reduction through right_spmm
+ * (spmm_dispatch.hh:182-219). */
+rbx_status rbx_slsksp3(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, float alpha,
+                       const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, const rbx_csr_matrix *A, int64_t ro_a,
+                       int64_t co_a, float beta, float *B, int64_t ldb, void *stream);
+rbx_status rbx_dlsksp3(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, double alpha,
+                       const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, const rbx_csr_matrix *A, int64_t ro_a,
+                       int64_t co_a, double beta, double *B, int64_t ldb, void *stream);
+
+/* rsksp3 / lsksp3 for CSC and COO data matrices (the reference's templates
+ * over SparseMatrix, sksp.hh:132-180, 277-326).  A CSC matrix is used whole
+ * (ro_a = co_a = 0); a COO matrix may be a submatrix at (ro_a, co_a), as in
+ * spmm_dispatch.hh:95-98.  Summation order per output entry is the
+ * reference's for that format (spmm_dispatch.hh:139-176). */
+rbx_status rbx_srsksp3_csc(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                           float alpha, const rbx_csc_matrix *A, int64_t ro_a, int64_t co_a,
+                           const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, float beta,
+                           float *B, int64_t ldb, void *stream);
+rbx_status rbx_drsksp3_csc(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                           double alpha, const rbx_csc_matrix *A, int64_t ro_a, int64_t co_a,
+                           const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, double beta,
+                           double *B, int64_t ldb, void *stream);
+rbx_status rbx_srsksp3_coo(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                           float alpha, const rbx_coo_matrix *A, int64_t ro_a, int64_t co_a,
+                           const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, float beta,
+                           float *B, int64_t ldb, void *stream);
+rbx_status rbx_drsksp3_coo(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                           double alpha, const rbx_coo_matrix *A, int64_t ro_a, int64_t co_a,
+                           const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, double beta,
+                           double *B, int64_t ldb, void *stream);
+rbx_status rbx_slsksp3_csc(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, float alpha,
+                           const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, const rbx_csc_matrix *A,
+                           int64_t ro_a, int64_t co_a, float beta, float *B, int64_t ldb, void *stream);
+rbx_status rbx_dlsksp3_csc(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, double alpha,
+                           const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, const rbx_csc_matrix *A,
+                           int64_t ro_a, int64_t co_a, double beta, double *B, int64_t ldb, void *stream);
+rbx_status rbx_slsksp3_coo(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, float alpha,
+                           const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, const rbx_coo_matrix *A,
+                           int64_t ro_a, int64_t co_a, float beta, float *B, int64_t ldb, void *stream);
+rbx_status rbx_dlsksp3_coo(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, double alpha,
+                           const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, const rbx_coo_matrix *A,
+                           int64_t ro_a, int64_t co_a, double beta, double *B, int64_t ldb, void *stream);
 
 /* Adjoint of rsksp3 (opA = 'N') with respect to the stored values of A (no
  * reference counterpart; it is the gradient of B = alpha * A * op(submat(S))

@@ -98,12 +98,23 @@ class c_csr_matrix(C.Structure):
                 ("vals", C.c_void_p), ("rowptr", C.c_void_p), ("colidxs", C.c_void_p)]
 
 
+class c_csc_matrix(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("n_cols", C.c_int64), ("nnz", C.c_int64),
+                ("vals", C.c_void_p), ("rowidxs", C.c_void_p), ("colptr", C.c_void_p)]
+
+
+class c_coo_matrix(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("n_cols", C.c_int64), ("nnz", C.c_int64),
+                ("vals", C.c_void_p), ("rows", C.c_void_p), ("cols", C.c_void_p)]
+
+
 RBX_OK, RBX_INVALID_ARG, RBX_OVERFLOW, RBX_HIP_ERROR, RBX_UNSUPPORTED, RBX_OUT_OF_MEMORY = range(6)
 
 # symbol -> argtypes (restype is always rbx_status unless listed in _RESTYPE)
 _I64, _F, _D, _P, _CH = C.c_int64, C.c_float, C.c_double, C.c_void_p, C.c_char
 _RS, _DD, _DS = C.POINTER(c_rng_state), C.POINTER(c_dense_dist), C.POINTER(c_dense_skop)
 _SD, _SS, _CSR = C.POINTER(c_sparse_dist), C.POINTER(c_sparse_skop), C.POINTER(c_csr_matrix)
+_CSC, _COO = C.POINTER(c_csc_matrix), C.POINTER(c_coo_matrix)
 _SIG = {
     "rbx_last_error_message": [],
     "rbx_version": [],
@@ -142,6 +153,12 @@ _SIG = {
                     _I64, _P],
     "rbx_drsksp3": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _CSR, _I64, _I64, _DS, _I64, _I64, _D, _P,
                     _I64, _P],
+    "rbx_slsksp3": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _DS, _I64, _I64, _CSR, _I64, _I64, _F, _P, _I64, _P],
+    "rbx_dlsksp3": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _DS, _I64, _I64, _CSR, _I64, _I64, _D, _P, _I64, _P],
+    **{f"rbx_{p}rsksp3{sfx}": [_CH, _CH, _CH, _I64, _I64, _I64, sc, mat, _I64, _I64, _DS, _I64, _I64, sc, _P, _I64, _P]
+       for p, sc in (("s", _F), ("d", _D)) for sfx, mat in (("_csc", _CSC), ("_coo", _COO))},
+    **{f"rbx_{p}lsksp3{sfx}": [_CH, _CH, _CH, _I64, _I64, _I64, sc, _DS, _I64, _I64, mat, _I64, _I64, sc, _P, _I64, _P]
+       for p, sc in (("s", _F), ("d", _D)) for sfx, mat in (("_csc", _CSC), ("_coo", _COO))},
     "rbx_srsksp3_adjoint": [_CH, _CH, _I64, _I64, _I64, _F, _CSR, _DS, _I64, _I64, _P, _I64, _P, _P],
     "rbx_drsksp3_adjoint": [_CH, _CH, _I64, _I64, _I64, _D, _CSR, _DS, _I64, _I64, _P, _I64, _P, _P],
     "rbx_srequire_symmetric": [_CH, _P, _I64, _I64, _F, _P],
@@ -472,13 +489,71 @@ class CSRMatrix:
                             self.rowptr.data_ptr(), self.colidxs.data_ptr())
 
 
-def rsksp3(layout, opA, opS, m, d, n, alpha, A: CSRMatrix, ro_a, co_a, S: DenseSkOp, ro_s, co_s, beta, B, ldb):
-    """rsksp3 (sparse_data/sksp.hh:277-326)."""
-    _call(f"rbx_{_prefix(B)}rsksp3", _c(layout), _c(opA), _c(opS), m, d, n, alpha, C.byref(A.c()), ro_a, co_a,
-          C.byref(S.c()), ro_s, co_s, beta, _ptr(B), ldb, _stream())
+@dataclass
+class CSCMatrix:
+    """CSCMatrix<T, int64_t> (sparse_data/csc_matrix.hh:74-244) over GPU tensors."""
+    n_rows: int
+    n_cols: int
+    vals: object
+    rowidxs: object
+    colptr: object
+
+    def c(self) -> c_csc_matrix:
+        return c_csc_matrix(self.n_rows, self.n_cols, int(self.vals.numel()), self.vals.data_ptr(),
+                            self.rowidxs.data_ptr(), self.colptr.data_ptr())
+
+
+@dataclass
+class COOMatrix:
+    """COOMatrix<T, int64_t> (sparse_data/coo_matrix.hh:83-308) over GPU tensors;
+    entries in any order."""
+    n_rows: int
+    n_cols: int
+    vals: object
+    rows: object
+    cols: object
+
+    def c(self) -> c_coo_matrix:
+        return c_coo_matrix(self.n_rows, self.n_cols, int(self.vals.numel()), self.vals.data_ptr(),
+                            self.rows.data_ptr(), self.cols.data_ptr())
+
+
+def _sp_suffix(A) -> str:
+    if isinstance(A, CSRMatrix):
+        return ""
+    if isinstance(A, CSCMatrix):
+        return "_csc"
+    if isinstance(A, COOMatrix):
+        return "_coo"
+    raise TypeError(f"expected CSRMatrix, CSCMatrix or COOMatrix, got {type(A).__name__}")
+
+
+def rsksp3(layout, opA, opS, m, d, n, alpha, A, ro_a, co_a, S: DenseSkOp, ro_s, co_s, beta, B, ldb):
+    """rsksp3 (sparse_data/sksp.hh:277-326) for a CSRMatrix, CSCMatrix or COOMatrix A."""
+    _call(f"rbx_{_prefix(B)}rsksp3{_sp_suffix(A)}", _c(layout), _c(opA), _c(opS), m, d, n, alpha, C.byref(A.c()),
+          ro_a, co_a, C.byref(S.c()), ro_s, co_s, beta, _ptr(B), ldb, _stream())
 
 
 # ---------------------------------------------------------------- dispatching wrappers
+def lsksp3(layout, opS, opA, d, n, m, alpha, S: DenseSkOp, ro_s, co_s, A, ro_a, co_a, beta, B, ldb):
+    """lsksp3 (sparse_data/sksp.hh:132-180): B = alpha * op(submat(S)) * op(A) + beta * B,
+    A a CSRMatrix, CSCMatrix or COOMatrix."""
+    _call(f"rbx_{_prefix(B)}lsksp3{_sp_suffix(A)}", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()),
+          ro_s, co_s, C.byref(A.c()), ro_a, co_a, beta, _ptr(B), ldb, _stream())
+
+
+def sketch_sparse(layout, op1, op2, a1, a2, a3, alpha, X, *rest):
+    """sketch_sparse (sparse_data/sksp.hh:418-539):
+
+      sketch_sparse(layout, opS, opA, d, n, m, alpha, S, ro_s, co_s, A, beta, B, ldb)   -> lsksp3
+      sketch_sparse(layout, opA, opS, m, d, n, alpha, A, S, ro_s, co_s, beta, B, ldb)   -> rsksp3"""
+    if isinstance(X, DenseSkOp):
+        ro_s, co_s, A, beta, B, ldb = rest
+        return lsksp3(layout, op1, op2, a1, a2, a3, alpha, X, ro_s, co_s, A, 0, 0, beta, B, ldb)
+    S, ro_s, co_s, beta, B, ldb = rest
+    return rsksp3(layout, op1, op2, a1, a2, a3, alpha, X, 0, 0, S, ro_s, co_s, beta, B, ldb)
+
+
 def rsksp3_adjoint(layout, opS, m, d, n, alpha, A: CSRMatrix, S: DenseSkOp, ro_s, co_s, Bbar, ldbb, dvals):
     """Gradient of B = alpha * A * op(submat(S)) with respect to A's stored values:
     dvals[k] = alpha * sum_j Bbar[i_k, j] * op(submat(S))[c_k, j]."""

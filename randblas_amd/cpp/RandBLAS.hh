@@ -644,29 +644,118 @@ struct CSRMatrix {
     rbx_csr_matrix c() const { return rbx_csr_matrix{n_rows, n_cols, nnz, (void *)vals, rowptr, colidxs}; }
 };
 
-// rsksp3 (sparse_data/sksp.hh:277-326): B = alpha * op(submat(A)) * op(submat(S)) + beta * B.
-template <typename T, typename RNG>
+// CSCMatrix (sparse_data/csc_matrix.hh:74-244), zero-based, device arrays, borrowed.
+template <typename T, typename sint_t = int64_t>
+struct CSCMatrix {
+    static_assert(std::is_same<sint_t, int64_t>::value, "int64_t indices");
+    using scalar_t = T;
+    using index_t = sint_t;
+    const int64_t n_rows;
+    const int64_t n_cols;
+    const bool own_memory;
+    int64_t nnz;
+    IndexBase index_base;
+    T *vals;
+    sint_t *rowidxs;
+    sint_t *colptr;
+
+    CSCMatrix(int64_t n_rows, int64_t n_cols, int64_t nnz, T *vals, sint_t *rowidxs, sint_t *colptr,
+              IndexBase index_base = IndexBase::Zero)
+        : n_rows(n_rows), n_cols(n_cols), own_memory(false), nnz(nnz), index_base(index_base), vals(vals),
+          rowidxs(rowidxs), colptr(colptr) {
+        if (index_base != IndexBase::Zero) throw Unsupported("CSCMatrix: only zero-based indices are supported");
+    }
+    rbx_csc_matrix c() const { return rbx_csc_matrix{n_rows, n_cols, nnz, (void *)vals, rowidxs, colptr}; }
+};
+
+// COOMatrix (sparse_data/coo_matrix.hh:83-308), zero-based, device arrays,
+// borrowed; the entries may be in any order (the library sorts a copy).
+template <typename T, typename sint_t = int64_t>
+struct COOMatrix {
+    static_assert(std::is_same<sint_t, int64_t>::value, "int64_t indices");
+    using scalar_t = T;
+    using index_t = sint_t;
+    const int64_t n_rows;
+    const int64_t n_cols;
+    const bool own_memory;
+    int64_t nnz;
+    IndexBase index_base;
+    T *vals;
+    sint_t *rows;
+    sint_t *cols;
+
+    COOMatrix(int64_t n_rows, int64_t n_cols, int64_t nnz, T *vals, sint_t *rows, sint_t *cols,
+              IndexBase index_base = IndexBase::Zero)
+        : n_rows(n_rows), n_cols(n_cols), own_memory(false), nnz(nnz), index_base(index_base), vals(vals),
+          rows(rows), cols(cols) {
+        if (index_base != IndexBase::Zero) throw Unsupported("COOMatrix: only zero-based indices are supported");
+    }
+    rbx_coo_matrix c() const { return rbx_coo_matrix{n_rows, n_cols, nnz, (void *)vals, rows, cols}; }
+};
+
+namespace formats {
+// One C entry point per (precision, format) pair.
+#define RBX_SKSP3_OVERLOAD(T, PFX, MAT, SFX)                                                                           \
+    inline rbx_status rsksp3_c(char l, char oa, char os, int64_t m, int64_t d, int64_t n, T alpha, const MAT *a,      \
+                               int64_t ro_a, int64_t co_a, const rbx_dense_skop *s, int64_t ro_s, int64_t co_s,        \
+                               T beta, T *B, int64_t ldb, void *st) {                                                 \
+        return rbx_##PFX##rsksp3##SFX(l, oa, os, m, d, n, alpha, a, ro_a, co_a, s, ro_s, co_s, beta, B, ldb, st);     \
+    }                                                                                                                 \
+    inline rbx_status lsksp3_c(char l, char os, char oa, int64_t d, int64_t n, int64_t m, T alpha,                   \
+                               const rbx_dense_skop *s, int64_t ro_s, int64_t co_s, const MAT *a, int64_t ro_a,        \
+                               int64_t co_a, T beta, T *B, int64_t ldb, void *st) {                                   \
+        return rbx_##PFX##lsksp3##SFX(l, os, oa, d, n, m, alpha, s, ro_s, co_s, a, ro_a, co_a, beta, B, ldb, st);     \
+    }
+RBX_SKSP3_OVERLOAD(float, s, rbx_csr_matrix, )
+RBX_SKSP3_OVERLOAD(double, d, rbx_csr_matrix, )
+RBX_SKSP3_OVERLOAD(float, s, rbx_csc_matrix, _csc)
+RBX_SKSP3_OVERLOAD(double, d, rbx_csc_matrix, _csc)
+RBX_SKSP3_OVERLOAD(float, s, rbx_coo_matrix, _coo)
+RBX_SKSP3_OVERLOAD(double, d, rbx_coo_matrix, _coo)
+#undef RBX_SKSP3_OVERLOAD
+}  // namespace formats
+
+// rsksp3 (sparse_data/sksp.hh:277-326): B = alpha * op(submat(A)) * op(submat(S)) + beta * B,
+// A a CSRMatrix, CSCMatrix or COOMatrix.
+template <typename T, typename SpMat, typename RNG>
 void rsksp3(blas::Layout layout, blas::Op opA, blas::Op opS, int64_t m, int64_t d, int64_t n, T alpha,
-            const CSRMatrix<T, int64_t> &A, int64_t ro_a, int64_t co_a, const DenseSkOp<T, RNG> &S, int64_t ro_s,
-            int64_t co_s, T beta, T *B, int64_t ldb) {
-    rbx_csr_matrix a = A.c();
+            const SpMat &A, int64_t ro_a, int64_t co_a, const DenseSkOp<T, RNG> &S, int64_t ro_s, int64_t co_s,
+            T beta, T *B, int64_t ldb) {
+    static_assert(std::is_same<typename SpMat::scalar_t, T>::value, "A and B must have the same scalar type");
+    auto a = A.c();
     rbx_dense_skop s = S.c();
-    rbx_status st;
-    if constexpr (std::is_same<T, float>::value)
-        st = rbx_srsksp3(detail::ch(layout), detail::ch(opA), detail::ch(opS), m, d, n, alpha, &a, ro_a, co_a, &s,
-                         ro_s, co_s, beta, B, ldb, gpu::stream());
-    else
-        st = rbx_drsksp3(detail::ch(layout), detail::ch(opA), detail::ch(opS), m, d, n, alpha, &a, ro_a, co_a, &s,
-                         ro_s, co_s, beta, B, ldb, gpu::stream());
-    detail::check(st, "rsksp3");
+    detail::check(formats::rsksp3_c(detail::ch(layout), detail::ch(opA), detail::ch(opS), m, d, n, alpha, &a, ro_a,
+                                   co_a, &s, ro_s, co_s, beta, B, ldb, gpu::stream()),
+                  "rsksp3");
+}
+
+// lsksp3 (sparse_data/sksp.hh:132-180): B = alpha * op(submat(S)) * op(submat(A)) + beta * B.
+template <typename T, typename RNG, typename SpMat>
+void lsksp3(blas::Layout layout, blas::Op opS, blas::Op opA, int64_t d, int64_t n, int64_t m, T alpha,
+            const DenseSkOp<T, RNG> &S, int64_t ro_s, int64_t co_s, const SpMat &A, int64_t ro_a, int64_t co_a,
+            T beta, T *B, int64_t ldb) {
+    static_assert(std::is_same<typename SpMat::scalar_t, T>::value, "A and B must have the same scalar type");
+    auto a = A.c();
+    rbx_dense_skop s = S.c();
+    detail::check(formats::lsksp3_c(detail::ch(layout), detail::ch(opS), detail::ch(opA), d, n, m, alpha, &s, ro_s,
+                                   co_s, &a, ro_a, co_a, beta, B, ldb, gpu::stream()),
+                  "lsksp3");
 }
 }  // namespace sparse_data
 
+// sketch_sparse, left-sketch form (sparse_data/sksp.hh:418-439).
+template <typename T, typename RNG, typename SpMat>
+auto sketch_sparse(blas::Layout layout, blas::Op opS, blas::Op opA, int64_t d, int64_t n, int64_t m, T alpha,
+                   const DenseSkOp<T, RNG> &S, int64_t ro_s, int64_t co_s, const SpMat &A, T beta, T *B,
+                   int64_t ldb) -> decltype((void)A.c(), void()) {
+    sparse_data::lsksp3(layout, opS, opA, d, n, m, alpha, S, ro_s, co_s, A, 0, 0, beta, B, ldb);
+}
+
 // sketch_sparse, right-sketch form (sparse_data/sksp.hh:520-539).
-template <typename T, typename RNG>
-void sketch_sparse(blas::Layout layout, blas::Op opA, blas::Op opS, int64_t m, int64_t d, int64_t n, T alpha,
-                   const sparse_data::CSRMatrix<T, int64_t> &A, const DenseSkOp<T, RNG> &S, int64_t ro_s,
-                   int64_t co_s, T beta, T *B, int64_t ldb) {
+template <typename T, typename SpMat, typename RNG>
+auto sketch_sparse(blas::Layout layout, blas::Op opA, blas::Op opS, int64_t m, int64_t d, int64_t n, T alpha,
+                   const SpMat &A, const DenseSkOp<T, RNG> &S, int64_t ro_s, int64_t co_s, T beta, T *B,
+                   int64_t ldb) -> decltype((void)A.c(), void()) {
     sparse_data::rsksp3(layout, opA, opS, m, d, n, alpha, A, 0, 0, S, ro_s, co_s, beta, B, ldb);
 }
 

@@ -66,23 +66,25 @@ __device__ __forceinline__ uint16_t entry_value<uint16_t>(float v, bool gauss) {
 // factors whose errors are bounded over every possible input word,
 //   |s' - s|, |c' - c| <= BM_ES   (absolute, the sin / cos factor of w0)
 //   |r' - r| <= BM_ER * r         (relative, the radius factor of w1),
-// so |g' - g| <= D + ulp(g') with D = r' (|s'| BM_ER + BM_ES) / (1 - BM_ER).
-// When g' lies farther than that from every bf16 rounding midpoint (the
-// distance taken at half weight to cover a binade edge), RNE(g') = RNE(g).
-// Otherwise the lane recomputes the pair on the exact path.  The bounds are
-// at least twice the maxima over all 2^32 words measured on gfx950
-// (max |s' - s| = 2.53e-7, max |c' - c| = 2.38e-7, max |r' - r| / r =
-// 1.68e-7, and r' = 0 exactly when r = 0).
-constexpr float BM_ES = 0x1p-20f;   // sin / cos factor, absolute (9.5e-7)
-constexpr float BM_ER = 0x1p-21f;   // radius factor, relative (4.8e-7)
+// so |g' - g| <= D + 1.5 ulp(g') with D = r' (|s'| BM_ER + BM_ES) / (1 - BM_ER)
+// (two half-ulp product roundings; ulp(g) <= 2 ulp(g')).  RNE to bfloat16
+// cannot differ between g and g' when no rounding midpoint lies within that
+// distance of g'.  The nearest midpoint is |low16(g') - 0x8000| ulps away
+// (a midpoint across a binade edge is at least 0x4000 ulps away, far beyond
+// any D accepted here).  Otherwise the lane recomputes on the exact path.
+// The bounds are twice the maxima measured on gfx950 over all 2^32 words:
+// max |s' - s| = 2.53e-7, max |c' - c| = 2.38e-7, max |r' - r| / r =
+// 1.68e-7, and r' = 0 exactly when r = 0.
+constexpr float BM_ES = 5.07e-7f;   // sin / cos factor, absolute
+constexpr float BM_ER = 3.36e-7f;   // radius factor, relative
 
-// true when RNE to bfloat16 of any value within D of g equals RNE(g)
+// true when RNE to bfloat16 of every value within D + 1.5 ulp of g equals RNE(g)
 __device__ __forceinline__ bool bf16_round_safe(float g, float D) {
     const uint32_t b = __float_as_uint(g);
     const uint32_t e = b & 0x7f800000u;
     const float ulp = __uint_as_float(e) * 0x1p-23f;
     const float dist = (float)abs((int)(b & 0xffffu) - 0x8000);
-    return e != 0u && e != 0x7f800000u && (0.5f * dist - 2.0f) * ulp > D;
+    return ((int)(e != 0u) & (int)(e != 0x7f800000u) & (int)((dist - 2.0f) * ulp > D)) != 0;
 }
 
 // Fast bf16 pair of one Box-Muller draw on words (w0, w1), packed low =
@@ -95,8 +97,10 @@ __device__ __forceinline__ uint32_t boxmuller_bf16_fast(uint32_t w0, uint32_t w1
     const float rf = __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u1) * -0x1.62e430p+0f);
     const float g0 = sf * rf, g1 = cf * rf;
     const float k = rf * (1.0f + 0x1p-18f);
-    ok = ok && bf16_round_safe(g0, k * (fabsf(sf) * BM_ER + BM_ES)) &&
-         bf16_round_safe(g1, k * (fabsf(cf) * BM_ER + BM_ES));
+    // both checks evaluated (no short-circuit branch)
+    const int safe0 = bf16_round_safe(g0, k * (fabsf(sf) * BM_ER + BM_ES));
+    const int safe1 = bf16_round_safe(g1, k * (fabsf(cf) * BM_ER + BM_ES));
+    ok = ((int)ok & safe0 & safe1) != 0;
     return (uint32_t)bf16_rne(g0) | ((uint32_t)bf16_rne(g1) << 16);
 }
 
@@ -162,6 +166,57 @@ __global__ __launch_bounds__(256) void split_kernel(const float *A, int64_t rs, 
             }
         }
         const int64_t dst = (cb * KS + st) * terms * 512 + lane * 8;
+        for (int t = 0; t < terms; ++t) {
+            u32x4 v;
+            v.x = (uint32_t)out[t][0] | ((uint32_t)out[t][1] << 16);
+            v.y = (uint32_t)out[t][2] | ((uint32_t)out[t][3] << 16);
+            v.z = (uint32_t)out[t][4] | ((uint32_t)out[t][5] << 16);
+            v.w = (uint32_t)out[t][6] | ((uint32_t)out[t][7] << 16);
+            *(u32x4 *)(planes + dst + t * 512) = v;
+        }
+    }
+}
+
+// Unit-stride K (column-major A, or row-major A^T): one workgroup per
+// (32-column block, 256-deep K range).  Each column's 256 consecutive values
+// are one contiguous 1 KB read; the tile is transposed through LDS and then
+// written as the block's 16 steps of fragment runs (the layout above).
+constexpr int SPL_KT = 256, SPL_LD = 260;
+__global__ __launch_bounds__(256) void split_kunit_kernel(const float *A, int64_t cs, int64_t K, int64_t N,
+                                                          int64_t Kp, int terms, uint16_t *planes) {
+    __shared__ __attribute__((aligned(16))) float tile[32][SPL_LD];
+    const int tid = threadIdx.x;
+    const int64_t KS = Kp / 16;
+    const int64_t cb = blockIdx.y;
+    const int64_t kt0 = (int64_t)blockIdx.x * SPL_KT;
+    // load: iteration c reads column 32*cb + c, k in [kt0, kt0 + 256)
+    const int64_t k = kt0 + tid;
+#pragma unroll 4
+    for (int c = 0; c < 32; ++c) {
+        const int64_t j = cb * 32 + c;
+        tile[c][tid] = (k < K && j < N) ? A[k + j * cs] : 0.0f;
+    }
+    __syncthreads();
+    // split: lane task (step s, lane l) of this K range, 4 per thread
+    const int64_t steps = (KS - kt0 / 16) < 16 ? (KS - kt0 / 16) : 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int task = tid + 256 * i;
+        const int st = task >> 6, l = task & 63;
+        if (st >= steps) break;
+        const float *src = &tile[l & 31][16 * st + 8 * (l >> 5)];
+        uint16_t out[3][8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float a = src[e];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const uint16_t h = bf16_rne(a);
+                out[t][e] = h;
+                a = __fsub_rn(a, bf16_to_f32(h));  // exact
+            }
+        }
+        const int64_t dst = (cb * KS + kt0 / 16 + st) * terms * 512 + l * 8;
         for (int t = 0; t < terms; ++t) {
             u32x4 v;
             v.x = (uint32_t)out[t][0] | ((uint32_t)out[t][1] << 16);
@@ -315,19 +370,28 @@ __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
             }
 }
 
-// Interior tiles, two waves per SIMD.  A 512-thread workgroup owns the same
-// 64 x 1024 tile as above; each of its 8 waves keeps a 64 x 128 FP32
-// accumulator (2 x 4 blocks, 128 registers), so two waves fit on a SIMD.
-// Every thread generates one Philox cell of the next chunk per chunk.  Waves
-// 0-3 generate before their MFMAs and waves 4-7 after them, so the two waves
-// sharing a SIMD (w and w + 4) are in opposite phases: one wave's generator
-// VALU work issues while the other's MFMAs occupy the matrix core.  The same
-// shape conditions as the FAST form of bf16_skge_kernel apply.
-constexpr int BF_FTHR = 512, BF_FWN = 128;
+// Interior tiles, two waves per SIMD.  A 512-thread workgroup owns a
+// (32 RB) x (256 CB) output tile; each of its 8 waves owns all 32 RB rows and
+// 32 CB columns of it (RB x CB blocks of 32 x 32, 16 RB CB accumulator
+// registers, so two waves fit on a SIMD).  The B operands a wave streams per
+// 16-deep step are CB x TERMS runs of 1 KB, so the global-load volume per
+// MFMA falls as RB grows; the operator rows are generated once per
+// workgroup, so generation per output column grows as CB shrinks.  Every
+// thread generates RB/2 Philox cells of the next 32-deep chunk per chunk;
+// waves 0-3 generate before their MFMAs and waves 4-7 after them, so the two
+// waves sharing a SIMD (w and w + 4) are in opposite phases.  The same shape
+// conditions as the FAST form of bf16_skge_kernel apply, with the tile
+// (32 RB) x (256 CB).
+constexpr int BF_FTHR = 512;
+#ifndef RBX_BF16_TALL_RB
+#define RBX_BF16_TALL_RB 4
+#endif
+constexpr int BF_RB = RBX_BF16_TALL_RB, BF_CB = 8 / BF_RB;  // blocks per wave: RB x CB = 8
 
-template <int TERMS, bool GAUSS>
+template <int TERMS, bool GAUSS, int RB, int CB>
 __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem pb) {
-    __shared__ __attribute__((aligned(16))) uint16_t Ss[2][BF_MT * BF_LDK];
+    constexpr int MT = 32 * RB, NT = 256 * CB, NCELL = MT * 8 / BF_FTHR;  // cells per thread per chunk
+    __shared__ __attribute__((aligned(16))) uint16_t Ss[2][MT * BF_LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     // XCD-aware tile order: consecutive tiles (same K-split) land on one XCD
@@ -338,23 +402,23 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
     const int rest = t / pb.nMB;
     const int nb = rest % pb.nNB;
     const int sp = rest / pb.nNB;
-    const int64_t m0 = (int64_t)(mb + pb.mb0) * BF_MT;
-    const int64_t n0 = (int64_t)nb * BF_NT;
+    const int64_t m0 = (int64_t)(mb + pb.mb0) * MT;
+    const int64_t n0 = (int64_t)nb * NT;
     const int64_t kbeg = (int64_t)sp * pb.kw;
     int64_t kend = kbeg + pb.kw;
     if (kend > pb.Kp) kend = pb.Kp;
     const int nchunks = (int)((kend - kbeg) / BF_KC);
     const GenMap g = pb.g;
 
-    f32x16 acc[2][4];
+    f32x16 acc[RB][CB];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < CB; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    const int64_t wcol = n0 + wave * BF_FWN;
+    const int64_t wcol = n0 + wave * (32 * CB);
     const int64_t KS = pb.Kp / 16;
     // lane's B operand for block j, K step s, term t:
     //   planes + (((wcol/32 + j)*KS + s)*TERMS + t)*512 + lane*8
@@ -362,16 +426,21 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
     const int64_t bstep_blk = KS * (TERMS * 512);
     const int64_t s_first = kbeg / 16, s_last = kend / 16 - 1;
 
-    // generator: thread owns cell (tid & 7) of operator row tid / 8
+    // generator: thread owns cell (tid & 7) of operator rows tid / 8 + 64 i
     const int frow = tid >> 3, fcell = tid & 7;
     const uint64_t foff = (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
     auto gen_chunk = [&](int c, uint16_t *lds) {
-        *(uint2 *)(lds + frow * BF_LDK + fcell * 4) = gen4_bf16<GAUSS>(ctr_add(g.ctr, foff + 8 * (uint64_t)c), g.k0, g.k1);
+#pragma unroll
+        for (int i = 0; i < NCELL; ++i) {
+            *(uint2 *)(lds + (frow + 64 * i) * BF_LDK + fcell * 4) =
+                gen4_bf16<GAUSS>(ctr_add(g.ctr, foff + (uint64_t)(64 * i) * g.P + 8 * (uint64_t)c), g.k0, g.k1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
     };
 
-    bf16x8 bq[4][TERMS];
+    bf16x8 bq[CB][TERMS];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < CB; ++j)
 #pragma unroll
         for (int tt = 0; tt < TERMS; ++tt)
             bq[j][tt] = __builtin_bit_cast(
@@ -381,16 +450,19 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
     // right after its MFMAs, a whole step ahead of their use
     auto step = [&](const uint16_t *S, int ks, int64_t s_next) {
         const int koff = ks * 16 + 8 * (lane >> 5);
-        const bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (lane & 31) * BF_LDK + koff));
-        const bf16x8 a1 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (32 + (lane & 31)) * BF_LDK + koff));
-        // term-outer: consecutive MFMAs on one accumulator are 8 apart, and
-        // each accumulator still adds its terms in order hi, lo, lo2
+        bf16x8 a[RB];
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+            a[i] = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (32 * i + (lane & 31)) * BF_LDK + koff));
+        // term-outer: consecutive MFMAs on one accumulator are RB x CB apart,
+        // and each accumulator still adds its terms in order hi, lo, lo2
 #pragma unroll
         for (int tt = 0; tt < TERMS; ++tt) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bq[j][tt], acc[0][j], 0, 0, 0);
-                acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bq[j][tt], acc[1][j], 0, 0, 0);
+            for (int j = 0; j < CB; ++j) {
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bq[j][tt], acc[i][j], 0, 0, 0);
                 bq[j][tt] = __builtin_bit_cast(
                     bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + s_next * (TERMS * 512) + tt * 512));
                 __builtin_amdgcn_sched_barrier(0);
@@ -416,9 +488,9 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
     // partial tile -> scratch (row-major M x N per split; lanes walk columns)
     float *P = pb.part + (int64_t)sp * pb.M * pb.N;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < CB; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int64_t gi = m0 + i * 32 + out_row32(lane, r);
@@ -531,7 +603,15 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     pb.planes = (const uint16_t *)ws;
     pb.part = (float *)(ws + planes_bytes);
 
-    {
+    if (ars == 1) {
+        // unit-stride K: coalesced column reads, transpose through LDS
+        const int64_t kt = (pb.Kp + SPL_KT - 1) / SPL_KT;
+        RBX_REQUIRE(kt < (1ll << 31) && pb.Np / 32 <= 65535);
+        ProfScope prof("bf16_split_kernel", stream);
+        hipLaunchKernelGGL(split_kunit_kernel, dim3((unsigned)kt, (unsigned)(pb.Np / 32)), dim3(256), 0, stream, A, acs,
+                           K, N, pb.Kp, n_terms, (uint16_t *)pb.planes);
+        RBX_HIP(hipGetLastError());
+    } else {
         int64_t total = (pb.Np / 32) * (pb.Kp / 16) * 64;  // one wave per (block, K step)
         int64_t blocks = (total + 255) / 256;
         if (blocks > 65536) blocks = 65536;
@@ -542,29 +622,32 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     }
     {
         ProfScope prof("bf16_skge_kernel", stream);
-        // interior row blocks on the FAST kernel when the shape allows it,
-        // the rest (or everything) on the bounds-checked kernel
-        const bool fast_ok = K % BF_KC == 0 && N % 256 == 0 && !pb.g.b_along_rows && (pb.g.b_off & 3) == 0;
-        const int nfast = fast_ok ? (int)(M / BF_MT) : 0;
-        const int nMB_all = pb.nMB;
+        // interior row blocks on the two-wave kernel when the shape allows
+        // it, the rest (or everything) on the bounds-checked kernel
+        constexpr int TMT = 32 * BF_RB, TNT = 256 * BF_CB;
+        const bool fast_ok = K % BF_KC == 0 && N % TNT == 0 && !pb.g.b_along_rows && (pb.g.b_off & 3) == 0;
+        const int nfast = fast_ok ? (int)(M / TMT) : 0;   // row blocks of TMT rows
+        const int nMB_all = pb.nMB;                        // row blocks of BF_MT rows
+        const int nfast64 = nfast * (TMT / BF_MT);
         if (nfast > 0) {
             Bf16Problem q = pb;
             q.nMB = nfast;
+            q.nNB = (int)(N / TNT);
             q.mb0 = 0;
             dim3 grid((unsigned)(q.nMB * q.nNB * q.nS));
             const bool gs = q.g.gauss != 0;
-            if (n_terms == 1 && gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<1, true>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (n_terms == 1) hipLaunchKernelGGL((bf16_skge_pair_kernel<1, false>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (n_terms == 2 && gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<2, true>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (n_terms == 2) hipLaunchKernelGGL((bf16_skge_pair_kernel<2, false>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<3, true>), grid, dim3(BF_FTHR), 0, stream, q);
-            else hipLaunchKernelGGL((bf16_skge_pair_kernel<3, false>), grid, dim3(BF_FTHR), 0, stream, q);
+            if (n_terms == 1 && gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<1, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (n_terms == 1) hipLaunchKernelGGL((bf16_skge_pair_kernel<1, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (n_terms == 2 && gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<2, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (n_terms == 2) hipLaunchKernelGGL((bf16_skge_pair_kernel<2, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<3, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+            else hipLaunchKernelGGL((bf16_skge_pair_kernel<3, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
             RBX_HIP(hipGetLastError());
         }
-        if (nMB_all > nfast) {
+        if (nMB_all > nfast64) {
             Bf16Problem q = pb;
-            q.nMB = nMB_all - nfast;
-            q.mb0 = nfast;
+            q.nMB = nMB_all - nfast64;
+            q.mb0 = nfast64;
             dim3 grid((unsigned)(q.nMB * q.nNB * q.nS));
             if (n_terms == 1) hipLaunchKernelGGL((bf16_skge_kernel<1, false>), grid, dim3(BF_NTHR), 0, stream, q);
             else if (n_terms == 2) hipLaunchKernelGGL((bf16_skge_kernel<2, false>), grid, dim3(BF_NTHR), 0, stream, q);

@@ -370,7 +370,7 @@ template <typename T>
 static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n, int64_t m, T alpha, int64_t A_rows,
                           int64_t A_cols, int64_t nnz, const T *vals, const int64_t *rows, const int64_t *cols,
                           int64_t ro_a, int64_t co_a, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
-                          hipStream_t st) {
+                          hipStream_t st, int kib_force = -1) {
     check_layout(layout);
     check_op(opA);
     check_op(opB);
@@ -400,7 +400,10 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
     RBX_REQUIRE(m < (int64_t)INT32_MAX);
     // op(B)[c, j] strides
     const int64_t brs = layout_opB == 'C' ? 1 : ldb, bcs = layout_opB == 'C' ? ldb : 1;
-    const int kib = (layout_opB == 'R' && layout == 'R') ? 1 : 0;
+    // summation form of the reference's kernel (coo_spmm_impl.hh:94-101); a
+    // caller whose B is a re-laid-out copy of the reference's B passes the form
+    // the reference's own layout selects
+    const int kib = kib_force >= 0 ? kib_force : ((layout_opB == 'R' && layout == 'R') ? 1 : 0);
     safe_mul(d + 1, m);
 
     // scratch: keys in/out, perm in/out, row_ptr, col, val, sort storage
@@ -577,28 +580,70 @@ static T *materialize_op(const rbx_dense_skop *S, char opS, int64_t ro_s, int64_
     return G;
 }
 
+// A sparse data matrix in one of the reference's three formats
+// (sparse_data/csr_matrix.hh, csc_matrix.hh, coo_matrix.hh), zero-based.
+struct SpData {
+    char fmt;  // 'R' CSR, 'C' CSC, 'O' COO
+    int64_t n_rows, n_cols, nnz;
+    const void *vals;
+    const int64_t *ptr;   // CSR rowptr / CSC colptr
+    const int64_t *idx;   // CSR colidxs / CSC rowidxs
+    const int64_t *rows;  // COO
+    const int64_t *cols;  // COO
+};
+
+static SpData sp_of(const rbx_csr_matrix *A) {
+    RBX_REQUIRE(A != nullptr);
+    return SpData{'R', A->n_rows, A->n_cols, A->nnz, A->vals, A->rowptr, A->colidxs, nullptr, nullptr};
+}
+static SpData sp_of(const rbx_csc_matrix *A) {
+    RBX_REQUIRE(A != nullptr);
+    return SpData{'C', A->n_rows, A->n_cols, A->nnz, A->vals, A->colptr, A->rowidxs, nullptr, nullptr};
+}
+static SpData sp_of(const rbx_coo_matrix *A) {
+    RBX_REQUIRE(A != nullptr);
+    return SpData{'O', A->n_rows, A->n_cols, A->nnz, A->vals, nullptr, nullptr, A->rows, A->cols};
+}
+
+// Major index of every stored entry of a compressed matrix (CSR row / CSC
+// column), expanded from its pointer array into library scratch.
+static const int64_t *expand_ptr(const int64_t *ptr, int64_t nmaj, int64_t nnz, hipStream_t st) {
+    int64_t *out = (int64_t *)scratch(SCRATCH_SPARSE_OP, (size_t)(nnz > 0 ? nnz : 1) * 8);
+    if (nnz > 0 && nmaj > 0) {
+        dim3 grid((unsigned)((nmaj + 255) / 256));
+        hipLaunchKernelGGL(csr_to_coo_rows_kernel, grid, dim3(256), 0, st, ptr, nmaj, out);
+        RBX_HIP(hipGetLastError());
+    }
+    return out;
+}
+
+// rsksp3 (sparse_data/sksp.hh:277-326) for a CSR, CSC or COO data matrix.
 template <typename T>
-static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, T alpha,
-                        const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a, const rbx_dense_skop *S, int64_t ro_s,
-                        int64_t co_s, T beta, T *B, int64_t ldb, hipStream_t st) {
-    RBX_REQUIRE(A != nullptr && S != nullptr);
+static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, T alpha, const SpData &A,
+                        int64_t ro_a, int64_t co_a, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, T beta, T *B,
+                        int64_t ldb, hipStream_t st) {
+    RBX_REQUIRE(S != nullptr);
     check_layout(layout);
     check_op(opA);
     check_op(opS);
     RBX_REQUIRE(m >= 0 && d >= 0 && n >= 0);
+    RBX_REQUIRE(ro_a >= 0 && co_a >= 0 && A.nnz >= 0);
     DenseGeom geo = dense_geom(S->dist);
     safe_mul(geo.dim_minor, geo.P);
     int64_t rows_S = opS == 'N' ? n : d, cols_S = opS == 'N' ? d : n;
     int64_t rows_A = opA == 'N' ? m : n, cols_A = opA == 'N' ? n : m;
-    RBX_REQUIRE(A->n_rows >= rows_A + ro_a);
-    RBX_REQUIRE(A->n_cols >= cols_A + co_a);
+    RBX_REQUIRE(A.n_rows >= rows_A + ro_a);
+    RBX_REQUIRE(A.n_cols >= cols_A + co_a);
     RBX_REQUIRE(S->dist.n_rows >= rows_S + ro_s);
     RBX_REQUIRE(S->dist.n_cols >= cols_S + co_s);
     if (layout == 'C') RBX_REQUIRE(ldb >= m);
     else RBX_REQUIRE(ldb >= d);
-    // CSR (and its CSC transpose) require the whole matrix (spmm_dispatch.hh:103-110)
-    RBX_REQUIRE(A->n_rows == rows_A && A->n_cols == cols_A);
-    RBX_REQUIRE(ro_a == 0 && co_a == 0);
+    // CSR and CSC take the whole matrix; COO accepts a submatrix
+    // (spmm_dispatch.hh:95-103)
+    if (A.fmt != 'O') {
+        RBX_REQUIRE(A.n_rows == rows_A && A.n_cols == cols_A);
+        RBX_REQUIRE(ro_a == 0 && co_a == 0);
+    }
     if (m == 0 || d == 0) return;
     const int64_t crs = layout == 'C' ? 1 : ldb, ccs = layout == 'C' ? ldb : 1;
     // The reference materialises submat(S) in its natural layout and flips opS
@@ -606,40 +651,56 @@ static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, i
     // then picks depends on that layout, which fixes the summation form.
     // (a materialised operator is used in its own layout, sksp.hh:318-321)
     const char s_layout = S->buff != nullptr ? S->layout : geo.natural;
-    char opS_ref = (s_layout != layout) ? (opS == 'N' ? 'T' : 'N') : opS;
-    char layout_opB = opS_ref == 'N' ? layout : (layout == 'C' ? 'R' : 'C');
-    // Materialise G = op(submat(S)) (n x d) row-major: lanes read rows of G.
+    const char opS_ref = (s_layout != layout) ? (opS == 'N' ? 'T' : 'N') : opS;
+    // row-major op(B) with a row-major C selects the rank-one-update kernels
+    // (spmm_dispatch.hh:147-176)
+    const int kib = (layout == 'R' && opS_ref == 'N') ? 1 : 0;
+    // G = op(submat(S)), n x d, row-major in scratch
     T *G = materialize_op<T>(S, opS, ro_s, co_s, n, d, st);
-    if (opA == 'N') {
-        if (alpha == (T)0 || n == 0 || A->nnz == 0) {
-            // beta scaling only (util::safe_scal); reuse the row kernel with alpha = 0
-        }
-        const int ikb = (layout_opB == 'R' && layout == 'R') ? 1 : 0;
+    // Which kernel family the reference reaches (spmm_dispatch.hh:72-90,
+    // 139-176): CSR with op(A) = N, or CSC with op(A) = T (its transpose is the
+    // CSR matrix sharing its arrays), runs row by row in stored order; every
+    // other case runs column by column, i.e. ascending column of op(A).
+    const bool row_kernel = (A.fmt == 'R' && opA == 'N') || (A.fmt == 'C' && opA == 'T');
+    if (row_kernel) {
         ProfScope prof("csr_rows_kernel", st);
         dim3 grid((unsigned)((m + 3) / 4));
-        hipLaunchKernelGGL(csr_rows_kernel<T>, grid, dim3(256), 0, st, A->rowptr, A->colidxs, (const T *)A->vals, m, d,
-                           alpha, beta, (const T *)G, (int64_t)d, (int64_t)1, B, crs, ccs, ikb);
+        hipLaunchKernelGGL(csr_rows_kernel<T>, grid, dim3(256), 0, st, A.ptr, A.idx, (const T *)A.vals, m, d, alpha,
+                           beta, (const T *)G, (int64_t)d, (int64_t)1, B, crs, ccs, kib);
         RBX_HIP(hipGetLastError());
         return;
     }
-    // opA = Trans: the reference transposes the CSR matrix into CSC and runs the
-    // CSC kernels, whose per-entry order is ascending column -- the COO path.
-    int64_t *rows = (int64_t *)scratch(SCRATCH_SPARSE_OP, (size_t)(A->nnz > 0 ? A->nnz : 1) * 8);
-    if (A->nnz > 0) {
-        dim3 grid((unsigned)((A->n_rows + 255) / 256));
-        hipLaunchKernelGGL(csr_to_coo_rows_kernel, grid, dim3(256), 0, st, A->rowptr, A->n_rows, rows);
-        RBX_HIP(hipGetLastError());
+    const int64_t *rows = A.rows, *cols = A.cols;
+    if (A.fmt == 'R') {
+        rows = expand_ptr(A.ptr, A.n_rows, A.nnz, st);
+        cols = A.idx;
+    } else if (A.fmt == 'C') {
+        rows = A.idx;
+        cols = expand_ptr(A.ptr, A.n_cols, A.nnz, st);
     }
-    // G as op(B) in the reference's orientation: B_ref = G with layout_opB rows
-    // k (inner) and columns j; we hand left_spmm a view with those strides.
-    char opG = 'N';
-    char layG = layout;  // G stored row-major n x d; express op(B) for `layout`
-    int64_t ldg;
-    if (layout == 'C') { opG = 'T'; ldg = d; }  // row-major G == column-major G^T
-    else { opG = 'N'; ldg = d; }
-    (void)layG;
-    left_spmm_coo<T>(layout, 'T', opG, m, d, n, alpha, A->n_rows, A->n_cols, A->nnz, (const T *)A->vals, rows,
-                     A->colidxs, 0, 0, G, ldg, beta, B, ldb, st);
+    // G is row-major n x d: op(B) = G for a row-major C, G^T of the
+    // column-major d x n buffer otherwise
+    const char opG = layout == 'C' ? 'T' : 'N';
+    left_spmm_coo<T>(layout, opA, opG, m, d, n, alpha, A.n_rows, A.n_cols, A.nnz, (const T *)A.vals, rows, cols, ro_a,
+                     co_a, G, d, beta, B, ldb, st, kib);
+}
+
+// lsksp3 (sparse_data/sksp.hh:132-180): B = alpha * op(submat(S)) * op(A) + beta * B.
+// The reference reduces it through right_spmm (spmm_dispatch.hh:182-219) to
+// left_spmm on the transposed problem with the opposite layout and op(A)
+// flipped; rsksp3's path takes that same left_spmm call when given the
+// flipped layout, the flipped op(A) and the flipped op(S) (which makes its
+// operator-layout rule match sksp.hh:176-177).
+template <typename T>
+static void lsksp3_impl(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, T alpha,
+                        const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, const SpData &A, int64_t ro_a,
+                        int64_t co_a, T beta, T *B, int64_t ldb, hipStream_t st) {
+    check_layout(layout);
+    check_op(opS);
+    check_op(opA);
+    const char tl = layout == 'C' ? 'R' : 'C';
+    rsksp3_impl<T>(tl, opA == 'N' ? 'T' : 'N', opS == 'N' ? 'T' : 'N', n, d, m, alpha, A, ro_a, co_a, S, ro_s, co_s,
+                   beta, B, ldb, st);
 }
 
 // Adjoint of rsksp3 (opA = 'N') with respect to the stored values of A, the
@@ -875,13 +936,31 @@ rbx_status rbx_drskges(char layout, char opA, char opS, int64_t m, int64_t d, in
     RBX_API_END
 }
 
-rbx_status rbx_srsksp3(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, float alpha,
-                       const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a, const rbx_dense_skop *S, int64_t ro_s,
-                       int64_t co_s, float beta, float *B, int64_t ldb, void *stream) {
-    RBX_API_BEGIN
-    rsksp3_impl<float>(layout, opA, opS, m, d, n, alpha, A, ro_a, co_a, S, ro_s, co_s, beta, B, ldb, (hipStream_t)stream);
-    RBX_API_END
-}
+// rsksp3 / lsksp3 entry points for the three data-matrix formats
+#define RBX_SKSP3_ENTRY(PFX, T, SFX, MAT)                                                                              \
+    rbx_status rbx_##PFX##rsksp3##SFX(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, T alpha,     \
+                                      const MAT *A, int64_t ro_a, int64_t co_a, const rbx_dense_skop *S,            \
+                                      int64_t ro_s, int64_t co_s, T beta, T *B, int64_t ldb, void *stream) {        \
+        RBX_API_BEGIN                                                                                               \
+        rsksp3_impl<T>(layout, opA, opS, m, d, n, alpha, sp_of(A), ro_a, co_a, S, ro_s, co_s, beta, B, ldb,         \
+                       (hipStream_t)stream);                                                                        \
+        RBX_API_END                                                                                                 \
+    }                                                                                                               \
+    rbx_status rbx_##PFX##lsksp3##SFX(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, T alpha,     \
+                                      const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, const MAT *A,            \
+                                      int64_t ro_a, int64_t co_a, T beta, T *B, int64_t ldb, void *stream) {        \
+        RBX_API_BEGIN                                                                                               \
+        lsksp3_impl<T>(layout, opS, opA, d, n, m, alpha, S, ro_s, co_s, sp_of(A), ro_a, co_a, beta, B, ldb,         \
+                       (hipStream_t)stream);                                                                        \
+        RBX_API_END                                                                                                 \
+    }
+
+RBX_SKSP3_ENTRY(s, float, , rbx_csr_matrix)
+RBX_SKSP3_ENTRY(d, double, , rbx_csr_matrix)
+RBX_SKSP3_ENTRY(s, float, _csc, rbx_csc_matrix)
+RBX_SKSP3_ENTRY(d, double, _csc, rbx_csc_matrix)
+RBX_SKSP3_ENTRY(s, float, _coo, rbx_coo_matrix)
+RBX_SKSP3_ENTRY(d, double, _coo, rbx_coo_matrix)
 
 rbx_status rbx_srsksp3_adjoint(char layout, char opS, int64_t m, int64_t d, int64_t n, float alpha,
                                const rbx_csr_matrix *A, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
@@ -899,12 +978,5 @@ rbx_status rbx_drsksp3_adjoint(char layout, char opS, int64_t m, int64_t d, int6
     RBX_API_END
 }
 
-rbx_status rbx_drsksp3(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, double alpha,
-                       const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a, const rbx_dense_skop *S, int64_t ro_s,
-                       int64_t co_s, double beta, double *B, int64_t ldb, void *stream) {
-    RBX_API_BEGIN
-    rsksp3_impl<double>(layout, opA, opS, m, d, n, alpha, A, ro_a, co_a, S, ro_s, co_s, beta, B, ldb, (hipStream_t)stream);
-    RBX_API_END
-}
 
 }  // extern "C"

@@ -243,6 +243,52 @@ static void gpu_checks(const std::string &out) {
         dump(out, "csr_rowptr", rp, cm + 1);
         dump(out, "csr_colidx", ci, nnz);
         dump(out, "csr_B", B4, cm * cd);
+
+        // The same matrix as CSC and as COO (entries reversed): with alpha = 1,
+        // beta = 0 and ascending columns every format sums the same products in
+        // the same order, so the results agree bit for bit.
+        std::vector<int64_t> colptr(cn + 1, 0), rowidx(nnz), crow, ccol;
+        std::vector<double> cvals(nnz), ovals;
+        for (int64_t p = 0; p < nnz; ++p) colptr[colidx[p] + 1]++;
+        for (int64_t c = 0; c < cn; ++c) colptr[c + 1] += colptr[c];
+        std::vector<int64_t> fill(colptr.begin(), colptr.end() - 1);
+        for (int64_t i = 0; i < cm; ++i)
+            for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+                rowidx[fill[colidx[p]]] = i;
+                cvals[fill[colidx[p]]++] = vals[p];
+            }
+        for (int64_t i = cm - 1; i >= 0; --i)
+            for (int64_t p = rowptr[i + 1] - 1; p >= rowptr[i]; --p) {
+                crow.push_back(i);
+                ccol.push_back(colidx[p]);
+                ovals.push_back(vals[p]);
+            }
+        double *cv = dev_alloc<double>(nnz), *ov = dev_alloc<double>(nnz), *B5 = dev_alloc<double>(cm * cd),
+               *B6 = dev_alloc<double>(cm * cd);
+        int64_t *cp = dev_alloc<int64_t>(cn + 1), *cr = dev_alloc<int64_t>(nnz), *orow = dev_alloc<int64_t>(nnz),
+                *ocol = dev_alloc<int64_t>(nnz);
+        HIPCHK(hipMemcpy(cv, cvals.data(), nnz * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ov, ovals.data(), nnz * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(cp, colptr.data(), (cn + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(cr, rowidx.data(), nnz * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(orow, crow.data(), nnz * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ocol, ccol.data(), nnz * sizeof(int64_t), hipMemcpyHostToDevice));
+        RandBLAS::sparse_data::CSCMatrix<double> Acsc(cm, cn, nnz, cv, cr, cp);
+        RandBLAS::sparse_data::COOMatrix<double> Acoo(cm, cn, nnz, ov, orow, ocol);
+        RandBLAS::sketch_sparse(Layout::ColMajor, Op::NoTrans, Op::NoTrans, cm, cd, cn, 1.0, Acsc, S4, 0, 0, 0.0, B5,
+                                cm);
+        RandBLAS::sparse_data::rsksp3(Layout::ColMajor, Op::NoTrans, Op::NoTrans, cm, cd, cn, 1.0, Acoo, 0, 0, S4, 0,
+                                      0, 0.0, B6, cm);
+        HIPCHK(hipStreamSynchronize(stream));
+        std::vector<double> h4(cm * cd), h5(cm * cd), h6(cm * cd);
+        HIPCHK(hipMemcpy(h4.data(), B4, cm * cd * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h5.data(), B5, cm * cd * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h6.data(), B6, cm * cd * sizeof(double), hipMemcpyDeviceToHost));
+        EXPECT(std::memcmp(h4.data(), h5.data(), h4.size() * sizeof(double)) == 0);
+        EXPECT(std::memcmp(h4.data(), h6.data(), h4.size() * sizeof(double)) == 0);
+        for (void *p : {(void *)cv, (void *)ov, (void *)B5, (void *)B6, (void *)cp, (void *)cr, (void *)orow,
+                        (void *)ocol})
+            HIPCHK(hipFree(p));
     }
 
     RandBLAS::gpu::set_stream(nullptr);

@@ -256,3 +256,172 @@ def test_rsksp3_power_law_full_size(rb, orc):
     refd = orc.sddmm_csr("d", m, d, 1.0, rowptr, colidx, Bbar.astype(np.float64), 1, m,
                          Sm.astype(np.float64).reshape(-1), d, 1)
     assert rel_fro(host(dvals), refd) <= 1e-5
+
+
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("opS", ["N", "T"])
+@pytest.mark.parametrize("opA", ["N", "T"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lsksp3(rb, orc, layout, opS, opA, dtype):
+    """lsksp3 bit-exact against the reference's reduction (sksp.hh:132-180):
+    right_spmm -> left_spmm(opposite layout, op(A) flipped) on the operator
+    submatrix in its natural layout (op(S) flipped when that differs)."""
+    rng = np.random.default_rng(29)
+    d, m, n = 40, 900, 70
+    ro, co = 2, 3
+    rowsA, colsA = (m, n) if opA == "N" else (n, m)
+    rowptr, colidx, vals = _csr(rng, rowsA, colsA, 9)
+    vals = vals.astype(dtype)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    rS, cS = (d, m) if opS == "N" else (m, d)           # submat(S) before op
+    Sshape = (rS + ro + 5, cS + co + 4)
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape), rb.RNGState(31))
+    Acsr = rb.CSRMatrix(rowsA, colsA, torch.from_numpy(vals).to(DEV), torch.from_numpy(rowptr).to(DEV),
+                        torch.from_numpy(colidx).to(DEV))
+    ldb = d if layout == "C" else n
+    B0 = rng.standard_normal(d * n).astype(dtype)
+    Bd = torch.from_numpy(B0.copy()).to(DEV)
+    rb.sketch_sparse(layout, opS, opA, d, n, m, 0.75, S, ro, co, Acsr, -0.5, Bd, ldb)
+    # reference chain
+    full, _ = orc.fill_dense(*Sshape, [0, 0, 0, 0], [31, 0], dtype=dtype)
+    nat = orc.natural_layout(*Sshape, "L")
+    sub = np.ascontiguousarray(full[ro:ro + rS, co:co + cS])
+    Ssub = np.ascontiguousarray(sub.reshape(-1, order="F" if nat == "C" else "C"))
+    lds = rS if nat == "C" else cS
+    opS_ref = opS if nat == layout else ("T" if opS == "N" else "N")
+    tl = "R" if layout == "C" else "C"
+    ref = B0.copy()
+    dt = "d" if dtype == np.float64 else "s"
+    if opA == "T":   # left_spmm(tl, 'N', ...) on the CSR matrix
+        orc.left_spmm_csr(dt, tl, opS_ref, n, d, m, 0.75, vals, rowptr, colidx, Ssub, lds, -0.5, ref, ldb)
+    else:            # left_spmm(tl, 'T', ...): the CSC order of the transposed CSR matrix
+        rows = np.repeat(np.arange(rowsA, dtype=np.int64), np.diff(rowptr))
+        orc.left_spmm_coo(dt, tl, "T", opS_ref, n, d, m, 0.75, (rowsA, colsA), vals, rows, colidx, 0, 0, Ssub, lds,
+                          -0.5, ref, ldb)
+    assert np.array_equal(host(Bd), ref)
+
+
+def _expand(ptr):
+    return np.repeat(np.arange(len(ptr) - 1, dtype=np.int64), np.diff(ptr))
+
+
+def _ref_left_spmm(orc, fmt, dt, layout, opA, opB, d, n, m, alpha, shape, parts, ro_a, co_a, Bm, ldb, beta, C, ldc):
+    """The reference's left_spmm dispatch (spmm_dispatch.hh:72-176) for one data
+    format, in oracle calls: CSR with op(A) = N and CSC with op(A) = T (whose
+    transpose is that CSR matrix) run the CSR row kernels; every other case runs
+    the CSC column kernels, i.e. the COO oracle."""
+    vals = parts["vals"]
+    if fmt == "csr":
+        if opA == "N":
+            orc.left_spmm_csr(dt, layout, opB, d, n, m, alpha, vals, parts["rowptr"], parts["colidx"], Bm, ldb, beta,
+                              C, ldc)
+        else:
+            orc.left_spmm_coo(dt, layout, "T", opB, d, n, m, alpha, shape, vals, _expand(parts["rowptr"]),
+                              parts["colidx"], 0, 0, Bm, ldb, beta, C, ldc)
+    elif fmt == "csc":
+        if opA == "T":
+            orc.left_spmm_csr(dt, layout, opB, d, n, m, alpha, vals, parts["colptr"], parts["rowidx"], Bm, ldb, beta,
+                              C, ldc)
+        else:
+            orc.left_spmm_coo(dt, layout, "N", opB, d, n, m, alpha, shape, vals, parts["rowidx"],
+                              _expand(parts["colptr"]), 0, 0, Bm, ldb, beta, C, ldc)
+    else:
+        orc.left_spmm_coo(dt, layout, opA, opB, d, n, m, alpha, shape, vals, parts["rows"], parts["cols"], ro_a, co_a,
+                          Bm, ldb, beta, C, ldc)
+
+
+def _sparse_in_format(rb, rng, fmt, nrows, ncols, mean_nnz, dtype, pad=(0, 0)):
+    """A random sparse matrix as (rb matrix, host parts, stored shape).  COO
+    matrices get pad extra rows/columns of entries around the window (offsets
+    (pad[0], pad[1])) and a shuffled entry order; CSC has ascending rows per
+    column."""
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+    if fmt == "coo":
+        R, Cc = nrows + 2 * pad[0], ncols + 2 * pad[1]
+        rowptr, colidx, vals = _csr(rng, R, Cc, mean_nnz)
+        rows = _expand(rowptr)
+        perm = rng.permutation(len(vals))
+        parts = dict(vals=vals[perm].astype(dtype), rows=rows[perm], cols=colidx[perm])
+        return rb.COOMatrix(R, Cc, t(parts["vals"]), t(parts["rows"]), t(parts["cols"])), parts, (R, Cc)
+    rowptr, colidx, vals = _csr(rng, nrows, ncols, mean_nnz)
+    vals = vals.astype(dtype)
+    if fmt == "csr":
+        parts = dict(vals=vals, rowptr=rowptr, colidx=colidx)
+        return rb.CSRMatrix(nrows, ncols, t(vals), t(rowptr), t(colidx)), parts, (nrows, ncols)
+    rows = _expand(rowptr)
+    order = np.lexsort((rows, colidx))       # by column, then row
+    colptr = np.concatenate([[0], np.cumsum(np.bincount(colidx, minlength=ncols))]).astype(np.int64)
+    parts = dict(vals=vals[order], rowidx=rows[order], colptr=colptr)
+    return (rb.CSCMatrix(nrows, ncols, t(parts["vals"]), t(parts["rowidx"]), t(colptr)), parts, (nrows, ncols))
+
+
+@pytest.mark.parametrize("fmt", ["csr", "csc", "coo"])
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("opA", ["N", "T"])
+@pytest.mark.parametrize("opS", ["N", "T"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_rsksp3_formats(rb, orc, fmt, layout, opA, opS, dtype):
+    """rsksp3 for CSR / CSC / COO data bit-exact against the reference's
+    dispatch (sksp.hh:277-326 -> spmm_dispatch.hh:72-176), alpha and beta
+    general; COO as a submatrix with entries outside the window."""
+    rng = np.random.default_rng(41)
+    m, n, d = 260, 610, 36
+    ro_s, co_s = 3, 2
+    pad = (4, 6) if fmt == "coo" else (0, 0)
+    rowsA, colsA = (m, n) if opA == "N" else (n, m)
+    A, parts, shape = _sparse_in_format(rb, rng, fmt, rowsA, colsA, 7, dtype, pad)
+    rS, cS = (n, d) if opS == "N" else (d, n)
+    Sshape = (rS + ro_s + 2, cS + co_s + 5)
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape), rb.RNGState(13))
+    ldb = m + 3 if layout == "C" else d + 2
+    B0 = rng.standard_normal(ldb * (d if layout == "C" else m)).astype(dtype)
+    Bd = torch.from_numpy(B0.copy()).to(DEV)
+    rb.rsksp3(layout, opA, opS, m, d, n, 0.75, A, pad[0], pad[1], S, ro_s, co_s, -0.5, Bd, ldb)
+    full, _ = orc.fill_dense(*Sshape, [0, 0, 0, 0], [13, 0], dtype=dtype)
+    nat = orc.natural_layout(*Sshape, "L")
+    sub = np.ascontiguousarray(full[ro_s:ro_s + rS, co_s:co_s + cS])
+    Ssub = np.ascontiguousarray(sub.reshape(-1, order="F" if nat == "C" else "C"))
+    lds = rS if nat == "C" else cS
+    opS_ref = opS if nat == layout else ("T" if opS == "N" else "N")
+    ref = B0.copy()
+    dt = "d" if dtype == np.float64 else "s"
+    _ref_left_spmm(orc, fmt, dt, layout, opA, opS_ref, m, d, n, 0.75, shape, parts, pad[0], pad[1], Ssub, lds, -0.5,
+                   ref, ldb)
+    assert np.array_equal(host(Bd), ref)
+
+
+@pytest.mark.parametrize("fmt", ["csc", "coo"])
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("opA", ["N", "T"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lsksp3_formats(rb, orc, fmt, layout, opA, dtype):
+    """lsksp3 for CSC / COO data bit-exact against the reference's reduction
+    (sksp.hh:132-180 -> right_spmm -> left_spmm on the opposite layout with
+    op(A) flipped)."""
+    rng = np.random.default_rng(43)
+    d, m, n = 30, 520, 90
+    opS = "N"
+    ro_s, co_s = 1, 4
+    pad = (2, 5) if fmt == "coo" else (0, 0)
+    rowsA, colsA = (m, n) if opA == "N" else (n, m)
+    A, parts, shape = _sparse_in_format(rb, rng, fmt, rowsA, colsA, 6, dtype, pad)
+    rS, cS = d, m
+    Sshape = (rS + ro_s + 3, cS + co_s + 1)
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape), rb.RNGState(17))
+    ldb = d if layout == "C" else n
+    B0 = rng.standard_normal(d * n).astype(dtype)
+    Bd = torch.from_numpy(B0.copy()).to(DEV)
+    rb.lsksp3(layout, opS, opA, d, n, m, 1.25, S, ro_s, co_s, A, pad[0], pad[1], 0.5, Bd, ldb)
+    full, _ = orc.fill_dense(*Sshape, [0, 0, 0, 0], [17, 0], dtype=dtype)
+    nat = orc.natural_layout(*Sshape, "L")
+    sub = np.ascontiguousarray(full[ro_s:ro_s + rS, co_s:co_s + cS])
+    Ssub = np.ascontiguousarray(sub.reshape(-1, order="F" if nat == "C" else "C"))
+    lds = rS if nat == "C" else cS
+    opS_ref = opS if nat == layout else "T"
+    tl = "R" if layout == "C" else "C"
+    ref = B0.copy()
+    dt = "d" if dtype == np.float64 else "s"
+    _ref_left_spmm(orc, fmt, dt, tl, "T" if opA == "N" else "N", opS_ref, n, d, m, 1.25, shape, parts, pad[0],
+                   pad[1], Ssub, lds, 0.5, ref, ldb)
+    assert np.array_equal(host(Bd), ref)
