@@ -105,29 +105,40 @@ __device__ __forceinline__ uint32_t boxmuller_bf16_fast(uint32_t w0, uint32_t w1
 }
 
 // One Philox counter -> four operator entries as bf16 (two per 32-bit word).
-// A lane whose fast values are not provably exact recomputes all four on the
-// exact path (one copy of that code, rarely taken).
+// Branch-free part: the four entries of words w, and (Gaussian) whether each
+// pair's fast values are provably the exact path's.
 template <bool GAUSS>
-__device__ __forceinline__ uint2 gen4_bf16(Ctr128 c, uint32_t k0, uint32_t k1) {
-    const uint4 w = philox4x32_10(c, k0, k1);
+__device__ __forceinline__ uint2 gen4_bf16_fast(const uint4 &w, bool &ok0, bool &ok1) {
     uint2 pk;
+    ok0 = ok1 = true;
     if (GAUSS) {
-        bool ok = true;
-        pk.x = boxmuller_bf16_fast(w.x, w.y, ok);
-        pk.y = boxmuller_bf16_fast(w.z, w.w, ok);
-        if (!ok) {
-            __builtin_amdgcn_sched_barrier(0);
-            const float2 a = boxmuller(w.x, w.y);
-            __builtin_amdgcn_sched_barrier(0);
-            const float2 b = boxmuller(w.z, w.w);
-            pk.x = (uint32_t)bf16_rne(a.x) | ((uint32_t)bf16_rne(a.y) << 16);
-            pk.y = (uint32_t)bf16_rne(b.x) | ((uint32_t)bf16_rne(b.y) << 16);
-        }
+        pk.x = boxmuller_bf16_fast(w.x, w.y, ok0);
+        pk.y = boxmuller_bf16_fast(w.z, w.w, ok1);
     } else {
         pk.x = (uint32_t)entry_value<uint16_t>(uneg11(w.x), false) |
                ((uint32_t)entry_value<uint16_t>(uneg11(w.y), false) << 16);
         pk.y = (uint32_t)entry_value<uint16_t>(uneg11(w.z), false) |
                ((uint32_t)entry_value<uint16_t>(uneg11(w.w), false) << 16);
+    }
+    return pk;
+}
+
+// A lane whose fast values are not provably exact recomputes all four on the
+// exact (portable) path: one copy of that code, taken by a wave when any of
+// its lanes needs it.  (Separate checks per pair need more live registers
+// than the kernels have to spare.)
+template <bool GAUSS>
+__device__ __forceinline__ uint2 gen4_bf16(Ctr128 c, uint32_t k0, uint32_t k1) {
+    const uint4 w = philox4x32_10(c, k0, k1);
+    bool ok0, ok1;
+    uint2 pk = gen4_bf16_fast<GAUSS>(w, ok0, ok1);
+    if (GAUSS && !(ok0 && ok1)) {
+        __builtin_amdgcn_sched_barrier(0);
+        const float2 a = boxmuller(w.x, w.y);
+        __builtin_amdgcn_sched_barrier(0);
+        const float2 b = boxmuller(w.z, w.w);
+        pk.x = (uint32_t)bf16_rne(a.x) | ((uint32_t)bf16_rne(a.y) << 16);
+        pk.y = (uint32_t)bf16_rne(b.x) | ((uint32_t)bf16_rne(b.y) << 16);
     }
     return pk;
 }
@@ -182,6 +193,7 @@ __global__ __launch_bounds__(256) void split_kernel(const float *A, int64_t rs, 
 // are one contiguous 1 KB read; the tile is transposed through LDS and then
 // written as the block's 16 steps of fragment runs (the layout above).
 constexpr int SPL_KT = 256, SPL_LD = 260;
+template <bool VEC>
 __global__ __launch_bounds__(256) void split_kunit_kernel(const float *A, int64_t cs, int64_t K, int64_t N,
                                                           int64_t Kp, int terms, uint16_t *planes) {
     __shared__ __attribute__((aligned(16))) float tile[32][SPL_LD];
@@ -189,13 +201,27 @@ __global__ __launch_bounds__(256) void split_kunit_kernel(const float *A, int64_
     const int64_t KS = Kp / 16;
     const int64_t cb = blockIdx.y;
     const int64_t kt0 = (int64_t)blockIdx.x * SPL_KT;
-    // load: iteration c reads column 32*cb + c, k in [kt0, kt0 + 256)
-    const int64_t k = kt0 + tid;
-#pragma unroll 4
-    for (int c = 0; c < 32; ++c) {
-        const int64_t j = cb * 32 + c;
-        tile[c][tid] = (k < K && j < N) ? A[k + j * cs] : 0.0f;
+    // load: a wave reads 1 KB runs of one column, k in [kt0, kt0 + 256); all
+    // eight of a thread's loads are issued before the first LDS store.
+    // VEC: 16-byte loads (A and the column stride are 4-float aligned)
+    const int q = tid & 63, c0 = tid >> 6;
+    const int64_t k = kt0 + 4 * q;
+    float4 v[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        const int64_t j = cb * 32 + c0 + 4 * p;
+        const float *src = A + k + j * cs;
+        if (VEC && j < N && k + 3 < K) {
+            v[p] = *(const float4 *)src;
+        } else {
+            v[p].x = (j < N && k < K) ? src[0] : 0.0f;
+            v[p].y = (j < N && k + 1 < K) ? src[1] : 0.0f;
+            v[p].z = (j < N && k + 2 < K) ? src[2] : 0.0f;
+            v[p].w = (j < N && k + 3 < K) ? src[3] : 0.0f;
+        }
     }
+#pragma unroll
+    for (int p = 0; p < 8; ++p) *(float4 *)&tile[c0 + 4 * p][4 * q] = v[p];
     __syncthreads();
     // split: lane task (step s, lane l) of this K range, 4 per thread
     const int64_t steps = (KS - kt0 / 16) < 16 ? (KS - kt0 / 16) : 16;
@@ -384,7 +410,7 @@ __global__ __launch_bounds__(BF_NTHR, 1) void bf16_skge_kernel(Bf16Problem pb) {
 // (32 RB) x (256 CB).
 constexpr int BF_FTHR = 512;
 #ifndef RBX_BF16_TALL_RB
-#define RBX_BF16_TALL_RB 4
+#define RBX_BF16_TALL_RB 2
 #endif
 constexpr int BF_RB = RBX_BF16_TALL_RB, BF_CB = 8 / BF_RB;  // blocks per wave: RB x CB = 8
 
@@ -438,11 +464,13 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
         }
     };
 
+    // first B operands, loaded term-major: the order the loop reloads them
+    // in, so the wait before each MFMA group covers only that group's load
     bf16x8 bq[CB][TERMS];
 #pragma unroll
-    for (int j = 0; j < CB; ++j)
+    for (int tt = 0; tt < TERMS; ++tt)
 #pragma unroll
-        for (int tt = 0; tt < TERMS; ++tt)
+        for (int j = 0; j < CB; ++j)
             bq[j][tt] = __builtin_bit_cast(
                 bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + s_first * (TERMS * 512) + tt * 512));
 
@@ -513,8 +541,19 @@ __global__ __launch_bounds__(256) void bf16_reduce_kernel(const float *part, int
     for (int q = 0; q < 4; ++q) {
         const int64_t i = i0 + ty + 8 * q, j = j0 + tx;
         double v = 0.0;
-        if (i < M && j < N)
-            for (int s = 0; s < nS; ++s) v += (double)part[(int64_t)s * total + i * N + j];
+        if (i < M && j < N) {
+            // loads of eight splits are issued together; the sum stays in split order
+            const float *p = part + i * N + j;
+            int s = 0;
+            for (; s + 8 <= nS; s += 8) {
+                float x[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) x[u] = p[(int64_t)(s + u) * total];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v += (double)x[u];
+            }
+            for (; s < nS; ++s) v += (double)p[(int64_t)s * total];
+        }
         tile[ty + 8 * q][tx] = v;
     }
     __syncthreads();
@@ -608,8 +647,13 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         const int64_t kt = (pb.Kp + SPL_KT - 1) / SPL_KT;
         RBX_REQUIRE(kt < (1ll << 31) && pb.Np / 32 <= 65535);
         ProfScope prof("bf16_split_kernel", stream);
-        hipLaunchKernelGGL(split_kunit_kernel, dim3((unsigned)kt, (unsigned)(pb.Np / 32)), dim3(256), 0, stream, A, acs,
-                           K, N, pb.Kp, n_terms, (uint16_t *)pb.planes);
+        const dim3 grid((unsigned)kt, (unsigned)(pb.Np / 32));
+        if (acs % 4 == 0 && ((uintptr_t)A & 15) == 0)
+            hipLaunchKernelGGL(split_kunit_kernel<true>, grid, dim3(256), 0, stream, A, acs, K, N, pb.Kp, n_terms,
+                               (uint16_t *)pb.planes);
+        else
+            hipLaunchKernelGGL(split_kunit_kernel<false>, grid, dim3(256), 0, stream, A, acs, K, N, pb.Kp, n_terms,
+                               (uint16_t *)pb.planes);
         RBX_HIP(hipGetLastError());
     } else {
         int64_t total = (pb.Np / 32) * (pb.Kp / 16) * 64;  // one wave per (block, K step)
@@ -636,12 +680,14 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
             q.mb0 = 0;
             dim3 grid((unsigned)(q.nMB * q.nNB * q.nS));
             const bool gs = q.g.gauss != 0;
-            if (n_terms == 1 && gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<1, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (n_terms == 1) hipLaunchKernelGGL((bf16_skge_pair_kernel<1, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (n_terms == 2 && gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<2, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (n_terms == 2) hipLaunchKernelGGL((bf16_skge_pair_kernel<2, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<3, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-            else hipLaunchKernelGGL((bf16_skge_pair_kernel<3, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+#define RBX_BF16_MAIN bf16_skge_pair_kernel
+            if (n_terms == 1 && gs) hipLaunchKernelGGL((RBX_BF16_MAIN<1, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (n_terms == 1) hipLaunchKernelGGL((RBX_BF16_MAIN<1, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (n_terms == 2 && gs) hipLaunchKernelGGL((RBX_BF16_MAIN<2, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (n_terms == 2) hipLaunchKernelGGL((RBX_BF16_MAIN<2, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+            else if (gs) hipLaunchKernelGGL((RBX_BF16_MAIN<3, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+            else hipLaunchKernelGGL((RBX_BF16_MAIN<3, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
+#undef RBX_BF16_MAIN
             RBX_HIP(hipGetLastError());
         }
         if (nMB_all > nfast64) {
