@@ -535,6 +535,141 @@ __global__ __launch_bounds__(256) void csr_rows_kernel(const int64_t *rowptr, co
     }
 }
 
+// The same products, vectorised: a workgroup owns 32 consecutive rows and a
+// chunk of 64 * VEC output columns; each lane holds VEC consecutive columns,
+// so one 16-byte gather per stored entry covers the chunk.  A row's column
+// indices and values are read 64 at a time with one coalesced load each and
+// broadcast from registers, so the gathers of consecutive entries are
+// independent.  Per output entry the arithmetic and its order are those of
+// csr_rows_kernel.  Column-major outputs go through an LDS tile and are
+// written (and, with beta != 0, read) 32 rows = 128 B at a time.
+constexpr int CRV_ROWS = 32;
+template <typename T>
+struct VecOf;
+template <>
+struct VecOf<float> {
+    using type = float4;
+    static constexpr int n = 4;
+};
+template <>
+struct VecOf<double> {
+    using type = double2;
+    static constexpr int n = 2;
+};
+
+template <typename T>
+__device__ __forceinline__ T vget(const typename VecOf<T>::type &v, int e) {
+    return reinterpret_cast<const T *>(&v)[e];
+}
+
+template <typename T, bool COLMAJ>
+__global__ __launch_bounds__(256) void csr_rows_vec_kernel(const int64_t *rowptr, const int64_t *colidx,
+                                                           const T *vals, int64_t mrows, int64_t ncols_out, T alpha,
+                                                           T beta, const T *G, int64_t ldg, T *C, int64_t ldc,
+                                                           int ikb) {
+    using V = typename VecOf<T>::type;
+    constexpr int VEC = VecOf<T>::n, CW = 64 * VEC;
+    __shared__ T tile[COLMAJ ? CRV_ROWS : 1][COLMAJ ? CW + 1 : 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * CRV_ROWS;
+    const int64_t col0 = (int64_t)blockIdx.y * CW;
+    const int64_t cl = col0 + (int64_t)lane * VEC;  // this lane's first column
+    const bool lane_on = cl < ncols_out;            // ncols_out % VEC == 0
+    for (int rr = wave; rr < CRV_ROWS; rr += 4) {
+        const int64_t i = r0 + rr;
+        if (i >= mrows) break;
+        T acc[VEC];
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[e] = (T)0;
+        if (ikb && beta != (T)0 && lane_on) {  // row-major output only
+            const V c = *(const V *)(C + i * ldc + cl);
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) acc[e] = beta * vget<T>(c, e);
+        }
+        if (alpha != (T)0) {
+            const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
+            for (int64_t pb = p0; pb < p1; pb += 64) {
+                const int cnt = (int)(p1 - pb < 64 ? p1 - pb : 64);
+                const int64_t myc = lane < cnt ? colidx[pb + lane] : 0;
+                const T myv = lane < cnt ? vals[pb + lane] : (T)0;
+                int q = 0;
+                for (; q + 4 <= cnt; q += 4) {
+                    V g[4];
+                    T v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int64_t c = __shfl(myc, q + u);
+                        v[u] = __shfl(myv, q + u);
+                        if (lane_on) g[u] = *(const V *)(G + c * ldg + cl);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (ikb) {
+                            const T sc = alpha * v[u];
+#pragma unroll
+                            for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + sc * vget<T>(g[u], e);
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + v[u] * vget<T>(g[u], e);
+                        }
+                    }
+                }
+                for (; q < cnt; ++q) {
+                    const int64_t c = __shfl(myc, q);
+                    const T v = __shfl(myv, q);
+                    V g;
+                    if (lane_on) g = *(const V *)(G + c * ldg + cl);
+                    if (ikb) {
+                        const T sc = alpha * v;
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + sc * vget<T>(g, e);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + v * vget<T>(g, e);
+                    }
+                }
+            }
+        }
+        if (COLMAJ) {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) tile[rr][lane * VEC + e] = acc[e];
+        } else if (lane_on) {
+            // row-major output: ikb already holds the finished value
+            T out[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) out[e] = acc[e];
+            if (!ikb) {
+                V c;
+                if (beta != (T)0) c = *(const V *)(C + i * ldc + cl);
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    const T b0 = beta == (T)0 ? (T)0 : beta * vget<T>(c, e);
+                    out[e] = alpha != (T)0 ? b0 + alpha * acc[e] : b0;
+                }
+            }
+            V o;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) reinterpret_cast<T *>(&o)[e] = out[e];
+            *(V *)(C + i * ldc + cl) = o;
+        }
+    }
+    if (COLMAJ) {
+        // jik form (a column-major output never takes the ikb form):
+        // C = RN(beta * C) + RN(alpha * sum), 32 rows of a column per pass
+        __syncthreads();
+        const int rr = tid & 31;
+        const int64_t i = r0 + rr;
+        for (int cc = tid >> 5; cc < CW; cc += 8) {
+            const int64_t j = col0 + cc;
+            if (i < mrows && j < ncols_out) {
+                T *c = C + i + j * ldc;
+                const T b0 = beta == (T)0 ? (T)0 : beta * (*c);
+                *c = alpha != (T)0 ? b0 + alpha * tile[rr][cc] : b0;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void csr_to_coo_rows_kernel(const int64_t *rowptr, int64_t nrows, int64_t *rows) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrows) return;
@@ -664,9 +799,24 @@ static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, i
     const bool row_kernel = (A.fmt == 'R' && opA == 'N') || (A.fmt == 'C' && opA == 'T');
     if (row_kernel) {
         ProfScope prof("csr_rows_kernel", st);
-        dim3 grid((unsigned)((m + 3) / 4));
-        hipLaunchKernelGGL(csr_rows_kernel<T>, grid, dim3(256), 0, st, A.ptr, A.idx, (const T *)A.vals, m, d, alpha,
-                           beta, (const T *)G, (int64_t)d, (int64_t)1, B, crs, ccs, kib);
+        constexpr int VEC = VecOf<T>::n;
+        // vector path: whole VEC-column groups, 16-byte aligned rows of G and,
+        // for a row-major B, of B (G is d columns wide from a 256-byte base)
+        const bool vec_ok = d % VEC == 0 && (layout == 'C' || (ldb % VEC == 0 && ((uintptr_t)B % 16) == 0));
+        const int64_t nchunk = (d + 64 * VEC - 1) / (64 * VEC);
+        if (vec_ok && nchunk <= 65535) {
+            dim3 grid((unsigned)((m + CRV_ROWS - 1) / CRV_ROWS), (unsigned)nchunk);
+            if (layout == 'C')
+                hipLaunchKernelGGL((csr_rows_vec_kernel<T, true>), grid, dim3(256), 0, st, A.ptr, A.idx,
+                                   (const T *)A.vals, m, d, alpha, beta, (const T *)G, d, B, ldb, kib);
+            else
+                hipLaunchKernelGGL((csr_rows_vec_kernel<T, false>), grid, dim3(256), 0, st, A.ptr, A.idx,
+                                   (const T *)A.vals, m, d, alpha, beta, (const T *)G, d, B, ldb, kib);
+        } else {
+            dim3 grid((unsigned)((m + 3) / 4));
+            hipLaunchKernelGGL(csr_rows_kernel<T>, grid, dim3(256), 0, st, A.ptr, A.idx, (const T *)A.vals, m, d,
+                               alpha, beta, (const T *)G, (int64_t)d, (int64_t)1, B, crs, ccs, kib);
+        }
         RBX_HIP(hipGetLastError());
         return;
     }
@@ -771,6 +921,59 @@ __global__ __launch_bounds__(256) void sddmm_csr_kernel(const int64_t *rowptr, c
     }
 }
 
+// Vectorised form (d a multiple of the vector width): a group of 16 lanes
+// owns one stored entry, so a wave works on four entries of a row at once;
+// each lane gathers 16-byte pieces of op(S)'s row and the 16 partial dot
+// products are summed with four butterfly steps.
+template <typename T>
+__global__ __launch_bounds__(256) void sddmm_vec_kernel(const int64_t *rowptr, const int64_t *colidx, int64_t m,
+                                                        int64_t d, T alpha, const T *Bb, int64_t brs, int64_t bcs,
+                                                        const T *G, T *dvals) {
+    using V = typename VecOf<T>::type;
+    constexpr int VEC = VecOf<T>::n, DC = SddmmChunk<T>::DC, GS = 16 * VEC;
+    __shared__ T tile[SD_RB][DC + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int grp = lane >> 4, gl = lane & 15;
+    const int64_t i0 = (int64_t)blockIdx.x * SD_RB;
+    for (int64_t j0 = 0; j0 < d; j0 += DC) {
+        const int64_t dc = d - j0 < DC ? d - j0 : DC;
+        const bool last = j0 + DC >= d;
+        __syncthreads();
+        for (int idx = tid; idx < SD_RB * DC; idx += 256) {
+            int r, c;
+            if (brs == 1) { r = idx % SD_RB; c = idx / SD_RB; }
+            else { c = idx % DC; r = idx / DC; }
+            const int64_t i = i0 + r;
+            tile[r][c] = (i < m && c < dc) ? Bb[i * brs + (j0 + c) * bcs] : (T)0;
+        }
+        __syncthreads();
+        for (int r = wave; r < SD_RB; r += 4) {
+            const int64_t i = i0 + r;
+            if (i >= m) break;
+            const int64_t kb = rowptr[i], ke = rowptr[i + 1];
+            for (int64_t k0 = kb; k0 < ke; k0 += 4) {
+                const int64_t k = k0 + grp;
+                const bool on = k < ke;
+                const T *grow = G + (on ? colidx[k] : 0) * d + j0;
+                T p = (T)0;
+                if (on) {
+                    for (int c = gl * VEC; c < dc; c += GS) {
+                        const V gv = *(const V *)(grow + c);
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) p = fma(tile[r][c + e], vget<T>(gv, e), p);
+                    }
+                }
+#pragma unroll
+                for (int o = 8; o >= 1; o >>= 1) p += __shfl_xor(p, o);
+                if (on && gl == 0) {
+                    const T tot = j0 == 0 ? p : dvals[k] + p;
+                    dvals[k] = last ? alpha * tot : tot;
+                }
+            }
+        }
+    }
+}
+
 template <typename T>
 static void rsksp3_adjoint_impl(char layout, char opS, int64_t m, int64_t d, int64_t n, T alpha,
                                 const rbx_csr_matrix *A, const rbx_dense_skop *S, int64_t ro_s, int64_t co_s,
@@ -797,8 +1000,12 @@ static void rsksp3_adjoint_impl(char layout, char opS, int64_t m, int64_t d, int
     const int64_t blocks = (m + SD_RB - 1) / SD_RB;
     RBX_REQUIRE(blocks < (1ll << 31));
     ProfScope prof("sddmm_csr_kernel", st);
-    hipLaunchKernelGGL(sddmm_csr_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, A->rowptr, A->colidxs, m, d,
-                       alpha, Bbar, brs, bcs, G, dvals);
+    if (d % VecOf<T>::n == 0)
+        hipLaunchKernelGGL(sddmm_vec_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, A->rowptr, A->colidxs, m, d,
+                           alpha, Bbar, brs, bcs, (const T *)G, dvals);
+    else
+        hipLaunchKernelGGL(sddmm_csr_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, A->rowptr, A->colidxs, m, d,
+                           alpha, Bbar, brs, bcs, (const T *)G, dvals);
     RBX_HIP(hipGetLastError());
 }
 

@@ -198,10 +198,11 @@ def test_rsksp3_materialised_operator(rb, orc, layout, dtype):
 @pytest.mark.parametrize("opS", ["N", "T"])
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-5)])
 @pytest.mark.parametrize("materialised", [False, True])
-def test_rsksp3_adjoint(rb, orc, layout, opS, dtype, tol, materialised):
+@pytest.mark.parametrize("d", [300, 301])   # 301: not a multiple of the vector width
+def test_rsksp3_adjoint(rb, orc, layout, opS, dtype, tol, materialised, d):
     from conftest import rel_fro
     rng = np.random.default_rng(23)
-    m, n, d = 250, 1500, 300      # d > one LDS chunk for both precisions
+    m, n = 250, 1500              # d > one LDS chunk for both precisions
     ro, co = 3, 5
     rowptr, colidx, vals = _csr(rng, m, n, 12)
     vals = vals.astype(dtype)
@@ -424,4 +425,33 @@ def test_lsksp3_formats(rb, orc, fmt, layout, opA, dtype):
     dt = "d" if dtype == np.float64 else "s"
     _ref_left_spmm(orc, fmt, dt, tl, "T" if opA == "N" else "N", opS_ref, n, d, m, 1.25, shape, parts, pad[0],
                    pad[1], Ssub, lds, 0.5, ref, ldb)
+    assert np.array_equal(host(Bd), ref)
+
+
+@pytest.mark.parametrize("fmt", ["csr", "csc"])
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("d", [37, 300])
+def test_rsksp3_row_kernel_widths(rb, orc, fmt, layout, dtype, d):
+    """The row-kernel cases (CSR with op(A) = N, CSC with op(A) = T) at an odd
+    width (scalar kernel) and at a width spanning two vector chunks in FP64,
+    bit-exact, alpha and beta general, ldb padded."""
+    rng = np.random.default_rng(47)
+    m, n = 170, 900
+    opA = "N" if fmt == "csr" else "T"
+    rowsA, colsA = (m, n) if opA == "N" else (n, m)
+    A, parts, shape = _sparse_in_format(rb, rng, fmt, rowsA, colsA, 9, dtype)
+    S = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(19))
+    ldb = m + 1 if layout == "C" else d + 3
+    B0 = rng.standard_normal(ldb * (d if layout == "C" else m)).astype(dtype)
+    Bd = torch.from_numpy(B0.copy()).to(DEV)
+    rb.rsksp3(layout, opA, "N", m, d, n, -1.5, A, 0, 0, S, 0, 0, 0.25, Bd, ldb)
+    full, _ = orc.fill_dense(n, d, [0, 0, 0, 0], [19, 0], dtype=dtype)
+    nat = orc.natural_layout(n, d, "L")
+    Sbuf = np.ascontiguousarray(full.reshape(-1, order="F" if nat == "C" else "C"))
+    lds = n if nat == "C" else d
+    opS_ref = "N" if nat == layout else "T"
+    ref = B0.copy()
+    _ref_left_spmm(orc, fmt, "d" if dtype == np.float64 else "s", layout, opA, opS_ref, m, d, n, -1.5, shape, parts,
+                   0, 0, Sbuf, lds, 0.25, ref, ldb)
     assert np.array_equal(host(Bd), ref)
