@@ -181,6 +181,7 @@ def main():
 
         # ---- sparse data: B = A_csr * S (rsksp3) and its adjoint w.r.t. A's values ----
         out["sparse_rsksp3"] = sparse_leg(rb, torch, dev)
+        out["saso_lskges"] = saso_leg(rb, torch, dev)
 
     # ---- CPU: FP64 oracle rows (also the timed CPU baseline), errors, distortion ----
     # All reference linear algebra runs on the host (numpy/OpenBLAS, scipy).
@@ -265,6 +266,43 @@ def sparse_leg(rb, torch, dev, reps=10):
     log(f"sparse: rsksp3 {fwd_ms:.3f} ms, adjoint {adj_ms:.3f} ms ({nnz} stored entries)")
     del A, B, Bbar, dv
     torch.cuda.synchronize()
+    return res
+
+
+def saso_leg(rb, torch, dev, reps=10):
+    """lskges with a SASO operator: S = SparseDist(2000, 100000, vec_nnz 8,
+    Short), key 7, sampled once (fill_sparse); A 100000 x 500 column-major
+    Uniform(-1, 1) (key 3); B = S * A, 2000 x 500, in FP64 and FP32.  Bound by
+    HBM: algorithmic bytes = A read once + B written once (+ the operator's
+    800000 stored entries, 24 B each, read once)."""
+    d, m, n = 2000, 100000, 500
+    S = rb.SparseSkOp(rb.SparseDist(d, m, 8, rb.Axis.Short), rb.RNGState(7))
+    res = {"workload": "SASO 2000x100000 vec_nnz 8 (key 7) * A 100000x500 ColMajor Uniform(-1,1) (key 3)"}
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        A = torch.empty(m * n, dtype=dt, device=dev)
+        rb.fill_dense(rb.DenseDist(m, n, rb.ScalarDist.Uniform), A, rb.RNGState(3))
+        B = torch.empty(d * n, dtype=dt, device=dev)
+        rb.fill_sparse(S)
+        rb.lskges("C", "N", "N", d, n, m, 1.0, S, 0, 0, A, m, 0.0, B, d)
+        torch.cuda.synchronize()
+        rb.profile_collect()
+        rb.profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            rb.lskges("C", "N", "N", d, n, m, 1.0, S, 0, 0, A, m, 0.0, B, d)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rb.profile_enable(False)
+        prof = rb.profile_collect()
+        ms = (t1 - t0) * 1e3 / reps
+        esz = A.element_size()
+        nbytes = m * n * esz + d * n * esz + 8 * m * 24
+        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac_hbm": round(nbytes / (ms * 1e-3) / 8e12, 4),
+                     "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}}
+        log(f"saso lskges {name}: {ms:.3f} ms")
+        del A, B
+        torch.cuda.synchronize()
     return res
 
 

@@ -143,9 +143,13 @@ int num_cus();
 // Library-owned device scratch, one grow-only buffer per (device, slot).  A
 // slot's contents live until the next call that uses the same slot on the
 // same device; calls that overlap on one device (different host threads or
-// streams) and use the same slot race.  Growing a slot synchronises the device.
+// streams) and use the same slot race.  Growing a slot synchronises the device
+// and frees its old buffer, so one call never holds two pointers into a slot:
+// each slot has one user per call path (SCRATCH_DENSE_OP: an operator gathered
+// or generated for the sparse-data products; SCRATCH_SPARSE_G: the row-major
+// copy of the dense operand inside left_spmm_coo).
 enum ScratchSlot { SCRATCH_BF16 = 0, SCRATCH_SPARSE_OP = 1, SCRATCH_SPARSE_CSR = 2, SCRATCH_DENSE_OP = 3,
-                   SCRATCH_SMALL = 4, SCRATCH_NSLOTS = 5 };
+                   SCRATCH_SMALL = 4, SCRATCH_SPARSE_G = 5, SCRATCH_NSLOTS = 6 };
 void *scratch(int slot, size_t bytes);
 
 // Write the generated matrix G (n_p x n_q) of a GenMap to buff[p*rs + q*cs].

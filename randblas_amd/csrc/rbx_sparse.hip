@@ -219,19 +219,22 @@ static int64_t fill_sparse_impl(const rbx_sparse_dist &D, T *vals, int64_t *rows
 // ---------------------------------------------------------------- COO -> row CSR
 // key = r * m + c for entries inside the d x m window (after the offsets),
 // key = d * m (sorts last) otherwise.
+// Sort keys r * m + c of the entries inside the window (d * m for the
+// others) and their positions; KT / PT are 32-bit when d * m and nnz fit.
+template <typename KT, typename PT>
 __global__ __launch_bounds__(256) void coo_keys_kernel(const int64_t *rows, const int64_t *cols, int64_t nnz,
-                                                       int64_t ro, int64_t co, int64_t d, int64_t m, uint64_t *keys,
-                                                       int64_t *perm) {
+                                                       int64_t ro, int64_t co, int64_t d, int64_t m, KT *keys,
+                                                       PT *perm) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = rows[e] - ro, c = cols[e] - co;
         const bool in = r >= 0 && r < d && c >= 0 && c < m;
-        keys[e] = in ? (uint64_t)r * (uint64_t)m + (uint64_t)c : (uint64_t)d * (uint64_t)m;
-        perm[e] = e;
+        keys[e] = (KT)(in ? (uint64_t)r * (uint64_t)m + (uint64_t)c : (uint64_t)d * (uint64_t)m);
+        perm[e] = (PT)e;
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void csr_from_sorted_kernel(const uint64_t *keys, const int64_t *perm, const T *vals,
+template <typename T, typename KT, typename PT>
+__global__ __launch_bounds__(256) void csr_from_sorted_kernel(const KT *keys, const PT *perm, const T *vals,
                                                               int64_t nnz, int64_t d, int64_t m, int64_t *row_ptr,
                                                               int32_t *col, T *val) {
     const uint64_t dm = (uint64_t)d * (uint64_t)m;
@@ -261,86 +264,6 @@ static int bits_for(uint64_t x) {
 }
 
 // ---------------------------------------------------------------- apply
-// Each thread owns RPT rows of the output window [r0, r0 + 256*RPT) and JB
-// columns [j0, j0+JB); op(B) rows are staged through LDS in chunks of CK.
-// C[r,j] = RN(beta*C) (or 0), then C += prod for entries in ascending column,
-// prod = val * RN(alpha * b)   (column-major kernels, csc_spmm_impl.hh:74-96)
-//      = RN(alpha * val) * b   (row-major kernel,    csc_spmm_impl.hh:154-209)
-constexpr int AP_JB = 4, AP_CK = 1024, AP_NT = 256;
-
-template <typename T, int RPT>
-__global__ __launch_bounds__(AP_NT) void csr_apply_kernel(const int64_t *row_ptr, const int32_t *col, const T *val,
-                                                          int64_t d, int64_t n, int64_t m, T alpha, T beta,
-                                                          const T *B, int64_t brs, int64_t bcs, T *C, int64_t crs,
-                                                          int64_t ccs, int kib) {
-    __shared__ T Bs[AP_CK * AP_JB];
-    const int tid = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.x * (AP_NT * RPT);
-    const int64_t j0 = (int64_t)blockIdx.y * AP_JB;
-    int64_t ptr[RPT], end[RPT];
-    T acc[RPT][AP_JB];
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int64_t r = r0 + tid + q * AP_NT;
-        ptr[q] = end[q] = 0;
-        if (r < d) {
-            ptr[q] = row_ptr[r];
-            end[q] = row_ptr[r + 1];
-        }
-#pragma unroll
-        for (int jj = 0; jj < AP_JB; ++jj) {
-            T c0 = (T)0;
-            if (r < d && j0 + jj < n && beta != (T)0) c0 = beta * C[r * crs + (j0 + jj) * ccs];
-            acc[q][jj] = c0;
-        }
-    }
-    for (int64_t c0 = 0; c0 < m; c0 += AP_CK) {
-        // stage op(B)[c0:c0+CK, j0:j0+JB] -> Bs[c][jj]
-        if (brs == 1) {
-            for (int idx = tid; idx < AP_CK * AP_JB; idx += AP_NT) {
-                const int cc = idx % AP_CK, jj = idx / AP_CK;
-                const int64_t c = c0 + cc, j = j0 + jj;
-                Bs[cc * AP_JB + jj] = (c < m && j < n) ? B[c + j * bcs] : (T)0;
-            }
-        } else {
-            for (int idx = tid; idx < AP_CK * AP_JB; idx += AP_NT) {
-                const int jj = idx % AP_JB, cc = idx / AP_JB;
-                const int64_t c = c0 + cc, j = j0 + jj;
-                Bs[cc * AP_JB + jj] = (c < m && j < n) ? B[c * brs + j * bcs] : (T)0;
-            }
-        }
-        __syncthreads();
-        const int64_t cend = c0 + AP_CK;
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            while (ptr[q] < end[q]) {
-                const int64_t c = col[ptr[q]];
-                if (c >= cend) break;
-                const T v = val[ptr[q]];
-                const T *b = &Bs[(c - c0) * AP_JB];
-                if (kib) {
-                    const T s = alpha * v;
-#pragma unroll
-                    for (int jj = 0; jj < AP_JB; ++jj) acc[q][jj] = acc[q][jj] + s * b[jj];
-                } else {
-#pragma unroll
-                    for (int jj = 0; jj < AP_JB; ++jj) acc[q][jj] = acc[q][jj] + v * (alpha * b[jj]);
-                }
-                ++ptr[q];
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int64_t r = r0 + tid + q * AP_NT;
-        if (r >= d) continue;
-#pragma unroll
-        for (int jj = 0; jj < AP_JB; ++jj)
-            if (j0 + jj < n) C[r * crs + (j0 + jj) * ccs] = acc[q][jj];
-    }
-}
-
 template <typename T>
 static void scale_only(T *C, int64_t d, int64_t n, int64_t crs, int64_t ccs, T beta, hipStream_t st);
 
@@ -361,6 +284,244 @@ static void scale_only(T *C, int64_t d, int64_t n, int64_t crs, int64_t ccs, T b
     int64_t blocks = (d * n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(sp_scale_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, C, d, n, crs, ccs, beta);
+    RBX_HIP(hipGetLastError());
+}
+
+// Sparse rows times a dense row-major G (C = op(A_sparse) * G), vectorised:
+// a workgroup owns 32 consecutive rows of C and a chunk of 64 * VEC columns;
+// each lane holds VEC consecutive columns, so one 16-byte gather per stored
+// entry covers the chunk.  A row's column
+// indices and values are read 64 at a time with one coalesced load each and
+// broadcast from registers, so the gathers of consecutive entries are
+// independent.  Per output entry the entries are applied in stored order in
+// one of the reference's three summation forms (below).  Column-major
+// outputs go through an LDS tile and are written (and, with beta != 0, read)
+// 32 rows = 128 B at a time.
+constexpr int CRV_ROWS = 32;
+template <typename T>
+struct VecOf;
+template <>
+struct VecOf<float> {
+    using type = float4;
+    static constexpr int n = 4;
+};
+template <>
+struct VecOf<double> {
+    using type = double2;
+    static constexpr int n = 2;
+};
+
+template <typename T>
+__device__ __forceinline__ T vget(const typename VecOf<T>::type &v, int e) {
+    return reinterpret_cast<const T *>(&v)[e];
+}
+
+// Summation forms, per output entry c with stored entries (v, b) in order:
+//   CRV_JIK  c = RN(beta*c) + RN(alpha * sum RN(v*b))      (CSR dot products)
+//   CRV_KIB  c = RN(beta*c); c = RN(c + RN(RN(alpha*v)*b))  (rank-1 updates)
+//   CRV_JKI  c = RN(beta*c); c = RN(c + RN(v*RN(alpha*b)))  (CSC column kernel)
+// (beta = 0 reads nothing and starts from 0; alpha = 0 leaves RN(beta*c).)
+enum { CRV_JIK = 0, CRV_KIB = 1, CRV_JKI = 2 };
+
+// NW waves per workgroup (4, or 16 when the rows are few and long).
+template <typename T, typename IDX, bool COLMAJ, int NW>
+__global__ __launch_bounds__(64 * NW) void csr_rows_vec_kernel(const int64_t *rowptr, const IDX *colidx, const T *vals,
+                                                           int64_t mrows, int64_t ncols_out, T alpha, T beta,
+                                                           const T *G, int64_t ldg, T *C, int64_t ldc, int form,
+                                                           int cvec) {
+    using V = typename VecOf<T>::type;
+    constexpr int VEC = VecOf<T>::n, CW = 64 * VEC;
+    __shared__ T tile[COLMAJ ? CRV_ROWS : 1][COLMAJ ? CW + 1 : 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * CRV_ROWS;
+    const int64_t col0 = (int64_t)blockIdx.y * CW;
+    const int64_t cl = col0 + (int64_t)lane * VEC;  // this lane's first column
+    // a column-major C block is read through LDS (beta * C), 32 rows of a
+    // column at a time
+    if (COLMAJ && beta != (T)0) {
+        // beta * C for the block, read 32 rows of a column at a time
+        const int rr = tid & 31;
+        for (int cc = tid >> 5; cc < CW; cc += 2 * NW) {
+            const int64_t i = r0 + rr, j = col0 + cc;
+            tile[rr][cc] = (i < mrows && j < ncols_out) ? beta * C[i + j * ldc] : (T)0;
+        }
+        __syncthreads();
+    }
+    for (int rr = wave; rr < CRV_ROWS; rr += NW) {
+        const int64_t i = r0 + rr;
+        if (i >= mrows) break;
+        T acc[VEC];
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            T c0 = (T)0;
+            if (form != CRV_JIK && beta != (T)0) {
+                if (COLMAJ) c0 = tile[rr][lane * VEC + e];
+                else if (cl + e < ncols_out) c0 = beta * C[i * ldc + cl + e];
+            }
+            acc[e] = c0;
+        }
+        if (alpha != (T)0) {
+            const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
+            for (int64_t pb = p0; pb < p1; pb += 64) {
+                const int cnt = (int)(p1 - pb < 64 ? p1 - pb : 64);
+                const int64_t myc = lane < cnt ? (int64_t)colidx[pb + lane] : 0;
+                const T myv = lane < cnt ? vals[pb + lane] : (T)0;
+                auto apply = [&](T v, const V &g) {
+                    if (form == CRV_KIB) {
+                        const T sc = alpha * v;
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + sc * vget<T>(g, e);
+                    } else if (form == CRV_JKI) {
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + v * (alpha * vget<T>(g, e));
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + v * vget<T>(g, e);
+                    }
+                };
+                int q = 0;
+                for (; q + 8 <= cnt; q += 8) {
+                    V g[8];
+                    T v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int64_t c = __shfl(myc, q + u);
+                        v[u] = __shfl(myv, q + u);
+                        if (cl < ncols_out) g[u] = *(const V *)(G + c * ldg + cl);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) apply(v[u], g[u]);
+                }
+                for (; q < cnt; ++q) {
+                    const int64_t c = __shfl(myc, q);
+                    const T v = __shfl(myv, q);
+                    V g;
+                    if (cl < ncols_out) g = *(const V *)(G + c * ldg + cl);
+                    apply(v, g);
+                }
+            }
+        }
+        // finish the JIK form: RN(beta*c) + RN(alpha*sum)
+        if (form == CRV_JIK) {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                T b0 = (T)0;
+                if (beta != (T)0) {
+                    if (COLMAJ) b0 = tile[rr][lane * VEC + e];
+                    else if (cl + e < ncols_out) b0 = beta * C[i * ldc + cl + e];
+                }
+                acc[e] = alpha != (T)0 ? b0 + alpha * acc[e] : b0;
+            }
+        }
+        if (COLMAJ) {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) tile[rr][lane * VEC + e] = acc[e];
+        } else if (cvec && cl + VEC <= ncols_out) {
+            V o;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) reinterpret_cast<T *>(&o)[e] = acc[e];
+            *(V *)(C + i * ldc + cl) = o;
+        } else {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e)
+                if (cl + e < ncols_out) C[i * ldc + cl + e] = acc[e];
+        }
+    }
+    if (COLMAJ) {
+        // 32 rows of a column per pass: whole 128 B segments
+        __syncthreads();
+        const int rr = tid & 31;
+        const int64_t i = r0 + rr;
+        for (int cc = tid >> 5; cc < CW; cc += 2 * NW) {
+            const int64_t j = col0 + cc;
+            if (i < mrows && j < ncols_out) C[i + j * ldc] = tile[rr][cc];
+        }
+    }
+}
+
+// op(B) (m x n, strides brs / bcs) -> row-major copy with leading dimension
+// ldo, through 32 x 32 LDS tiles (coalesced on both sides).
+template <typename T>
+__global__ __launch_bounds__(256) void to_rowmajor_kernel(const T *B, int64_t brs, int64_t bcs, int64_t m, int64_t n,
+                                                          T *out, int64_t ldo) {
+    __shared__ T t[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int64_t c0 = (int64_t)blockIdx.x * 32, j0 = (int64_t)blockIdx.y * 32;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int a = ty + 8 * q;
+        // column-major source: lanes walk c; row-major: lanes walk j
+        const int64_t c = brs == 1 ? c0 + tx : c0 + a, j = brs == 1 ? j0 + a : j0 + tx;
+        const T v = (c < m && j < n) ? B[c * brs + j * bcs] : (T)0;
+        if (brs == 1) t[tx][a] = v;
+        else t[a][tx] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int a = ty + 8 * q;
+        const int64_t c = c0 + a, j = j0 + tx;
+        if (c < m && j < ldo) out[c * ldo + j] = j < n ? t[a][tx] : (T)0;
+    }
+}
+
+// COO entries inside the window [ro, ro + d) x [co, co + m) -> CSR (row
+// pointers, 32-bit columns, values) sorted by (row, column), stable for
+// duplicates, in library scratch.
+template <typename T, typename KT, typename PT>
+static void coo_to_csr(const int64_t *rows, const int64_t *cols, const T *vals, int64_t nnz, int64_t ro, int64_t co,
+                       int64_t d, int64_t m, int64_t **row_ptr_out, int32_t **col_out, T **val_out, hipStream_t st) {
+    size_t sort_bytes = 0;
+    const int end_bit = bits_for((uint64_t)d * (uint64_t)m);
+    RBX_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (KT *)nullptr, (KT *)nullptr, (PT *)nullptr, (PT *)nullptr,
+                                      (size_t)nnz, 0, end_bit, st));
+    const size_t bk = align256((size_t)nnz * sizeof(KT)), bp = align256((size_t)nnz * sizeof(PT));
+    const size_t brp = align256((size_t)(d + 1) * 8), bcol = align256((size_t)nnz * 4),
+                 bval = align256((size_t)nnz * sizeof(T));
+    char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, 2 * bk + 2 * bp + brp + bcol + bval + align256(sort_bytes));
+    KT *keys_in = (KT *)ws, *keys_out = (KT *)(ws + bk);
+    PT *perm_in = (PT *)(ws + 2 * bk), *perm_out = (PT *)(ws + 2 * bk + bp);
+    char *rest = ws + 2 * bk + 2 * bp;
+    int64_t *row_ptr = (int64_t *)rest;
+    int32_t *ccol = (int32_t *)(rest + brp);
+    T *cval = (T *)(rest + brp + bcol);
+    void *sort_tmp = rest + brp + bcol + bval;
+    int64_t blocks = (nnz + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL((coo_keys_kernel<KT, PT>), dim3((unsigned)blocks), dim3(256), 0, st, rows, cols, nnz, ro, co, d,
+                       m, keys_in, perm_in);
+    RBX_HIP(hipGetLastError());
+    RBX_HIP(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_in, keys_out, perm_in, perm_out, (size_t)nnz, 0,
+                                      end_bit, st));
+    hipLaunchKernelGGL(zero_rowptr_kernel, dim3(64), dim3(256), 0, st, row_ptr, d);
+    hipLaunchKernelGGL((csr_from_sorted_kernel<T, KT, PT>), dim3((unsigned)blocks), dim3(256), 0, st, keys_out,
+                       perm_out, vals, nnz, d, m, row_ptr, ccol, cval);
+    RBX_HIP(hipGetLastError());
+    *row_ptr_out = row_ptr;
+    *col_out = ccol;
+    *val_out = cval;
+}
+
+// Launch csr_rows_vec_kernel over mrows x ncols_out: 16 waves per workgroup
+// when 4-wave workgroups would leave the GPU short of waves.
+template <typename T, typename IDX>
+static void launch_rows_vec(bool colmaj, const int64_t *rowptr, const IDX *colidx, const T *vals, int64_t mrows,
+                            int64_t ncols_out, T alpha, T beta, const T *G, int64_t ldg, T *C, int64_t ldc, int form,
+                            int cvec, hipStream_t st) {
+    constexpr int VEC = VecOf<T>::n;
+    const int64_t nchunk = (ncols_out + 64 * VEC - 1) / (64 * VEC);
+    const int64_t nrb = (mrows + CRV_ROWS - 1) / CRV_ROWS;
+    RBX_REQUIRE(nchunk <= 65535 && nrb < (1ll << 31));
+    const dim3 grid((unsigned)nrb, (unsigned)nchunk);
+    const bool wide = nrb * nchunk < 4 * (int64_t)num_cus();
+#define RBX_ROWS_VEC(CM, NW)                                                                                  \
+    hipLaunchKernelGGL((csr_rows_vec_kernel<T, IDX, CM, NW>), grid, dim3(64 * NW), 0, st, rowptr, colidx, vals, \
+                       mrows, ncols_out, alpha, beta, G, ldg, C, ldc, form, cvec)
+    if (colmaj && wide) RBX_ROWS_VEC(true, 16);
+    else if (colmaj) RBX_ROWS_VEC(true, 4);
+    else if (wide) RBX_ROWS_VEC(false, 16);
+    else RBX_ROWS_VEC(false, 4);
+#undef RBX_ROWS_VEC
     RBX_HIP(hipGetLastError());
 }
 
@@ -406,51 +567,37 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
     const int kib = kib_force >= 0 ? kib_force : ((layout_opB == 'R' && layout == 'R') ? 1 : 0);
     safe_mul(d + 1, m);
 
-    // scratch: keys in/out, perm in/out, row_ptr, col, val, sort storage
-    size_t sort_bytes = 0;
-    const int end_bit = bits_for((uint64_t)d * (uint64_t)m);
-    RBX_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
-                                      (int64_t *)nullptr, (int64_t *)nullptr, (size_t)nnz, 0, end_bit, st));
-    const size_t bk = align256((size_t)nnz * 8);
-    const size_t brp = align256((size_t)(d + 1) * 8), bcol = align256((size_t)nnz * 4), bval = align256((size_t)nnz * sizeof(T));
-    char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, 4 * bk + brp + bcol + bval + align256(sort_bytes));
-    uint64_t *keys_in = (uint64_t *)ws, *keys_out = (uint64_t *)(ws + bk);
-    int64_t *perm_in = (int64_t *)(ws + 2 * bk), *perm_out = (int64_t *)(ws + 3 * bk);
-    int64_t *row_ptr = (int64_t *)(ws + 4 * bk);
-    int32_t *ccol = (int32_t *)(ws + 4 * bk + brp);
-    T *cval = (T *)(ws + 4 * bk + brp + bcol);
-    void *sort_tmp = ws + 4 * bk + brp + bcol + bval;
+    // the window's entries as a CSR matrix sorted by (row, column), in scratch
+    int64_t *row_ptr;
+    int32_t *ccol;
+    T *cval;
     {
         ProfScope prof("coo_to_csr", st);
-        int64_t blocks = (nnz + 255) / 256;
-        if (blocks > 16384) blocks = 16384;
-        hipLaunchKernelGGL(coo_keys_kernel, dim3((unsigned)blocks), dim3(256), 0, st, rows, cols, nnz, ro_a, co_a, d, m,
-                           keys_in, perm_in);
-        RBX_HIP(hipGetLastError());
-        RBX_HIP(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_in, keys_out, perm_in, perm_out, (size_t)nnz, 0,
-                                          end_bit, st));
-        hipLaunchKernelGGL(zero_rowptr_kernel, dim3(64), dim3(256), 0, st, row_ptr, d);
-        hipLaunchKernelGGL(csr_from_sorted_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, keys_out, perm_out, vals,
-                           nnz, d, m, row_ptr, ccol, cval);
-        RBX_HIP(hipGetLastError());
+        if ((uint64_t)d * (uint64_t)m < (1ull << 32) && nnz < (int64_t)INT32_MAX)
+            coo_to_csr<T, uint32_t, int32_t>(rows, cols, vals, nnz, ro_a, co_a, d, m, &row_ptr, &ccol, &cval, st);
+        else
+            coo_to_csr<T, uint64_t, int64_t>(rows, cols, vals, nnz, ro_a, co_a, d, m, &row_ptr, &ccol, &cval, st);
     }
     {
         ProfScope prof("csr_apply_kernel", st);
-        const int64_t nJ = (n + AP_JB - 1) / AP_JB;
-        RBX_REQUIRE(nJ <= 65535);
-        if (d <= 256) {
-            dim3 grid((unsigned)((d + 255) / 256), (unsigned)nJ);
-            hipLaunchKernelGGL((csr_apply_kernel<T, 1>), grid, dim3(AP_NT), 0, st, row_ptr, ccol, cval, d, n, m, alpha,
-                               beta, B, brs, bcs, C, crs, ccs, kib);
-        } else if (d <= 1024) {
-            dim3 grid((unsigned)((d + 1023) / 1024), (unsigned)nJ);
-            hipLaunchKernelGGL((csr_apply_kernel<T, 4>), grid, dim3(AP_NT), 0, st, row_ptr, ccol, cval, d, n, m, alpha,
-                               beta, B, brs, bcs, C, crs, ccs, kib);
-        } else {
-            dim3 grid((unsigned)((d + 2047) / 2048), (unsigned)nJ);
-            hipLaunchKernelGGL((csr_apply_kernel<T, 8>), grid, dim3(AP_NT), 0, st, row_ptr, ccol, cval, d, n, m, alpha,
-                               beta, B, brs, bcs, C, crs, ccs, kib);
+        // op(B) as an aligned row-major matrix: used in place when it is one,
+        // else copied (transposed) into scratch
+        constexpr int VEC = VecOf<T>::n;
+        const T *Gp = B;
+        int64_t ldg = ldb;
+        if (!(layout_opB == 'R' && ldb % VEC == 0 && ((uintptr_t)B % 16) == 0)) {
+            ldg = (n + VEC - 1) / VEC * VEC;
+            T *Gs = (T *)scratch(SCRATCH_SPARSE_G, (size_t)safe_mul(m, ldg) * sizeof(T));
+            const int64_t gx = (m + 31) / 32, gy = (ldg + 31) / 32;
+            RBX_REQUIRE(gx < (1ll << 31) && gy <= 65535);
+            hipLaunchKernelGGL(to_rowmajor_kernel<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, B, brs, bcs,
+                               m, n, Gs, ldg);
+            RBX_HIP(hipGetLastError());
+            Gp = Gs;
         }
+        const int cvec = layout == 'R' && ldc % VEC == 0 && ((uintptr_t)C % 16) == 0;
+        launch_rows_vec<T, int32_t>(layout == 'C', row_ptr, ccol, cval, d, n, alpha, beta, Gp, ldg, C, ldc,
+                                    kib ? CRV_KIB : CRV_JKI, cvec, st);
         RBX_HIP(hipGetLastError());
     }
 }
@@ -532,141 +679,6 @@ __global__ __launch_bounds__(256) void csr_rows_kernel(const int64_t *rowptr, co
             }
         }
         *c = acc;
-    }
-}
-
-// The same products, vectorised: a workgroup owns 32 consecutive rows and a
-// chunk of 64 * VEC output columns; each lane holds VEC consecutive columns,
-// so one 16-byte gather per stored entry covers the chunk.  A row's column
-// indices and values are read 64 at a time with one coalesced load each and
-// broadcast from registers, so the gathers of consecutive entries are
-// independent.  Per output entry the arithmetic and its order are those of
-// csr_rows_kernel.  Column-major outputs go through an LDS tile and are
-// written (and, with beta != 0, read) 32 rows = 128 B at a time.
-constexpr int CRV_ROWS = 32;
-template <typename T>
-struct VecOf;
-template <>
-struct VecOf<float> {
-    using type = float4;
-    static constexpr int n = 4;
-};
-template <>
-struct VecOf<double> {
-    using type = double2;
-    static constexpr int n = 2;
-};
-
-template <typename T>
-__device__ __forceinline__ T vget(const typename VecOf<T>::type &v, int e) {
-    return reinterpret_cast<const T *>(&v)[e];
-}
-
-template <typename T, bool COLMAJ>
-__global__ __launch_bounds__(256) void csr_rows_vec_kernel(const int64_t *rowptr, const int64_t *colidx,
-                                                           const T *vals, int64_t mrows, int64_t ncols_out, T alpha,
-                                                           T beta, const T *G, int64_t ldg, T *C, int64_t ldc,
-                                                           int ikb) {
-    using V = typename VecOf<T>::type;
-    constexpr int VEC = VecOf<T>::n, CW = 64 * VEC;
-    __shared__ T tile[COLMAJ ? CRV_ROWS : 1][COLMAJ ? CW + 1 : 1];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t r0 = (int64_t)blockIdx.x * CRV_ROWS;
-    const int64_t col0 = (int64_t)blockIdx.y * CW;
-    const int64_t cl = col0 + (int64_t)lane * VEC;  // this lane's first column
-    const bool lane_on = cl < ncols_out;            // ncols_out % VEC == 0
-    for (int rr = wave; rr < CRV_ROWS; rr += 4) {
-        const int64_t i = r0 + rr;
-        if (i >= mrows) break;
-        T acc[VEC];
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) acc[e] = (T)0;
-        if (ikb && beta != (T)0 && lane_on) {  // row-major output only
-            const V c = *(const V *)(C + i * ldc + cl);
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) acc[e] = beta * vget<T>(c, e);
-        }
-        if (alpha != (T)0) {
-            const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
-            for (int64_t pb = p0; pb < p1; pb += 64) {
-                const int cnt = (int)(p1 - pb < 64 ? p1 - pb : 64);
-                const int64_t myc = lane < cnt ? colidx[pb + lane] : 0;
-                const T myv = lane < cnt ? vals[pb + lane] : (T)0;
-                int q = 0;
-                for (; q + 4 <= cnt; q += 4) {
-                    V g[4];
-                    T v[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int64_t c = __shfl(myc, q + u);
-                        v[u] = __shfl(myv, q + u);
-                        if (lane_on) g[u] = *(const V *)(G + c * ldg + cl);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (ikb) {
-                            const T sc = alpha * v[u];
-#pragma unroll
-                            for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + sc * vget<T>(g[u], e);
-                        } else {
-#pragma unroll
-                            for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + v[u] * vget<T>(g[u], e);
-                        }
-                    }
-                }
-                for (; q < cnt; ++q) {
-                    const int64_t c = __shfl(myc, q);
-                    const T v = __shfl(myv, q);
-                    V g;
-                    if (lane_on) g = *(const V *)(G + c * ldg + cl);
-                    if (ikb) {
-                        const T sc = alpha * v;
-#pragma unroll
-                        for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + sc * vget<T>(g, e);
-                    } else {
-#pragma unroll
-                        for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + v * vget<T>(g, e);
-                    }
-                }
-            }
-        }
-        if (COLMAJ) {
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) tile[rr][lane * VEC + e] = acc[e];
-        } else if (lane_on) {
-            // row-major output: ikb already holds the finished value
-            T out[VEC];
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) out[e] = acc[e];
-            if (!ikb) {
-                V c;
-                if (beta != (T)0) c = *(const V *)(C + i * ldc + cl);
-#pragma unroll
-                for (int e = 0; e < VEC; ++e) {
-                    const T b0 = beta == (T)0 ? (T)0 : beta * vget<T>(c, e);
-                    out[e] = alpha != (T)0 ? b0 + alpha * acc[e] : b0;
-                }
-            }
-            V o;
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) reinterpret_cast<T *>(&o)[e] = out[e];
-            *(V *)(C + i * ldc + cl) = o;
-        }
-    }
-    if (COLMAJ) {
-        // jik form (a column-major output never takes the ikb form):
-        // C = RN(beta * C) + RN(alpha * sum), 32 rows of a column per pass
-        __syncthreads();
-        const int rr = tid & 31;
-        const int64_t i = r0 + rr;
-        for (int cc = tid >> 5; cc < CW; cc += 8) {
-            const int64_t j = col0 + cc;
-            if (i < mrows && j < ncols_out) {
-                T *c = C + i + j * ldc;
-                const T b0 = beta == (T)0 ? (T)0 : beta * (*c);
-                *c = alpha != (T)0 ? b0 + alpha * tile[rr][cc] : b0;
-            }
-        }
     }
 }
 
@@ -800,18 +812,12 @@ static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, i
     if (row_kernel) {
         ProfScope prof("csr_rows_kernel", st);
         constexpr int VEC = VecOf<T>::n;
-        // vector path: whole VEC-column groups, 16-byte aligned rows of G and,
-        // for a row-major B, of B (G is d columns wide from a 256-byte base)
-        const bool vec_ok = d % VEC == 0 && (layout == 'C' || (ldb % VEC == 0 && ((uintptr_t)B % 16) == 0));
-        const int64_t nchunk = (d + 64 * VEC - 1) / (64 * VEC);
-        if (vec_ok && nchunk <= 65535) {
-            dim3 grid((unsigned)((m + CRV_ROWS - 1) / CRV_ROWS), (unsigned)nchunk);
-            if (layout == 'C')
-                hipLaunchKernelGGL((csr_rows_vec_kernel<T, true>), grid, dim3(256), 0, st, A.ptr, A.idx,
-                                   (const T *)A.vals, m, d, alpha, beta, (const T *)G, d, B, ldb, kib);
-            else
-                hipLaunchKernelGGL((csr_rows_vec_kernel<T, false>), grid, dim3(256), 0, st, A.ptr, A.idx,
-                                   (const T *)A.vals, m, d, alpha, beta, (const T *)G, d, B, ldb, kib);
+        // vector path: 16-byte aligned rows of G (d columns wide from a
+        // 256-byte base); a row-major B takes vector stores when aligned too
+        const int cvec = layout == 'R' && ldb % VEC == 0 && ((uintptr_t)B % 16) == 0;
+        if (d % VEC == 0) {
+            launch_rows_vec<T, int64_t>(layout == 'C', A.ptr, A.idx, (const T *)A.vals, m, d, alpha, beta,
+                                        (const T *)G, d, B, ldb, kib ? CRV_KIB : CRV_JIK, cvec, st);
         } else {
             dim3 grid((unsigned)((m + 3) / 4));
             hipLaunchKernelGGL(csr_rows_kernel<T>, grid, dim3(256), 0, st, A.ptr, A.idx, (const T *)A.vals, m, d,
