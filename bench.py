@@ -182,6 +182,8 @@ def main():
         # ---- sparse data: B = A_csr * S (rsksp3) and its adjoint w.r.t. A's values ----
         out["sparse_rsksp3"] = sparse_leg(rb, torch, dev)
         out["saso_lskges"] = saso_leg(rb, torch, dev)
+        out["generator"] = generator_leg(rb, torch, dev)
+        out["sketch_and_solve"] = sketch_solve_leg(rb, torch, dev)
 
     # ---- CPU: FP64 oracle rows (also the timed CPU baseline), errors, distortion ----
     # All reference linear algebra runs on the host (numpy/OpenBLAS, scipy).
@@ -302,6 +304,70 @@ def saso_leg(rb, torch, dev, reps=10):
                      "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}}
         log(f"saso lskges {name}: {ms:.3f} ms")
         del A, B
+        torch.cuda.synchronize()
+    return res
+
+
+def _time(rb, torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    rb.profile_collect()
+    rb.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rb.profile_enable(False)
+    prof = rb.profile_collect()
+    return (t1 - t0) * 1e3 / reps, {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}
+
+
+def generator_leg(rb, torch, dev, reps=5):
+    """The generator alone: fill_dense of the headline operator, DenseDist(4096,
+    262144) Gaussian, key 42, into FP32 and FP64 buffers (4 / 8 GiB).  Bound by
+    the generator's VALU work, priced against the HBM write rate."""
+    res = {"workload": "fill_dense DenseDist(4096, 262144) Gaussian, key 42"}
+    n_el = D * M
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        buf = torch.empty(n_el, dtype=dt, device=dev)
+        ms, k = _time(rb, torch, lambda: rb.fill_dense(rb.DenseDist(D, M), buf, rb.RNGState(KEY_S)), reps)
+        nbytes = n_el * buf.element_size()
+        res[name] = {"ms": round(ms, 4), "gaussians_per_s": round(n_el / (ms * 1e-3), 1),
+                     "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1), "frac_hbm": round(nbytes / (ms * 1e-3) / 8e12, 4),
+                     "kernels_ms": k}
+        log(f"generator {name}: {ms:.3f} ms")
+        del buf
+        torch.cuda.synchronize()
+    return res
+
+
+def sketch_solve_leg(rb, torch, dev, reps=5):
+    """The sketch-and-solve product and its adjoint: S = DenseDist(1024, 65536)
+    (key 42) times [A b], A 65536 x 256 N(0,1) (key 11), FP32 and FP64; the
+    adjoint S^T G for G 1024 x 257 regenerates the operator.  2*1024*65536*257
+    flop each, against the FP32 / FP64 MFMA peaks (157.3 / 78.6 TFLOP/s)."""
+    d, m, n = 1024, 65536, 257
+    S = rb.DenseSkOp(rb.DenseDist(d, m), rb.RNGState(42))
+    fl = 2.0 * d * m * n
+    res = {"workload": "S 1024x65536 (key 42) * [A b] 65536x257 and S^T * G, G 1024x257", "flop": fl}
+    for name, dt, peak in (("fp32", torch.float32, 157.3), ("fp64", torch.float64, PEAK_FP64)):
+        Ab = torch.empty(m * n, dtype=dt, device=dev)
+        rb.fill_dense(rb.DenseDist(m, n), Ab, rb.RNGState(11))
+        SA = torch.empty(d * n, dtype=dt, device=dev)
+        G = torch.empty(d * n, dtype=dt, device=dev)
+        rb.fill_dense(rb.DenseDist(d, n), G, rb.RNGState(13))
+        Abar = torch.empty(m * n, dtype=dt, device=dev)
+        fwd_ms, fk = _time(rb, torch, lambda: rb.lskge3("C", "N", "N", d, n, m, 1.0, S, 0, 0, Ab, m, 0.0, SA, d), reps)
+        adj_ms, ak = _time(rb, torch, lambda: rb.lskge3_adjoint("C", "N", d, n, m, 1.0, S, 0, 0, G, d, 0.0, Abar, m),
+                           reps)
+        res[name] = {"fwd_ms": round(fwd_ms, 4), "fwd_tflops": round(fl / (fwd_ms * 1e-3) / 1e12, 2),
+                     "fwd_frac_peak": round(fl / (fwd_ms * 1e-3) / 1e12 / peak, 4),
+                     "adj_ms": round(adj_ms, 4), "adj_tflops": round(fl / (adj_ms * 1e-3) / 1e12, 2),
+                     "adj_frac_peak": round(fl / (adj_ms * 1e-3) / 1e12 / peak, 4),
+                     "fwd_kernels_ms": fk, "adj_kernels_ms": ak}
+        log(f"sketch-and-solve {name}: fwd {fwd_ms:.3f} ms, adjoint {adj_ms:.3f} ms")
+        del Ab, SA, G, Abar
         torch.cuda.synchronize()
     return res
 

@@ -413,6 +413,10 @@ constexpr int BF_FTHR = 512;
 #define RBX_BF16_TALL_RB 2
 #endif
 constexpr int BF_RB = RBX_BF16_TALL_RB, BF_CB = 8 / BF_RB;  // blocks per wave: RB x CB = 8
+// Wave priority while a wave issues its MFMAs (0: unchanged).
+#ifndef RBX_BF16_PRIO
+#define RBX_BF16_PRIO 1
+#endif
 
 template <int TERMS, bool GAUSS, int RB, int CB>
 __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem pb) {
@@ -477,6 +481,9 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
     // one 16-deep K step: block j's B operands for the next step are reloaded
     // right after its MFMAs, a whole step ahead of their use
     auto step = [&](const uint16_t *S, int ks, int64_t s_next) {
+#if RBX_BF16_PRIO
+        __builtin_amdgcn_s_setprio(RBX_BF16_PRIO);
+#endif
         const int koff = ks * 16 + 8 * (lane >> 5);
         bf16x8 a[RB];
 #pragma unroll
@@ -496,6 +503,9 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
+#if RBX_BF16_PRIO
+        __builtin_amdgcn_s_setprio(0);
+#endif
     };
 
     const bool late = (wave & 4) != 0;

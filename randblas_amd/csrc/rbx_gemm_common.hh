@@ -17,7 +17,42 @@ struct GemmProblem {
     int gen_is_x;         // 1: X = operator, Y = D; 0: X = D, Y = operator
     void *C;
     int64_t crs, ccs;
+    // split K: nsplit > 1 runs K ranges of kw (the last one shorter) as
+    // blockIdx.z, each writing its raw product to part + z * M * N
+    // (column-major M x N); a reduction then forms alpha * sum + beta * C
+    int nsplit = 1;
+    int64_t kw = 0;
+    void *part = nullptr;
 };
+
+// The sub-problem of K range z: the dense operand and the operator are
+// advanced along K, the output is the split's partial (alpha 1, beta 0).
+template <typename T>
+__device__ __forceinline__ GemmProblem split_view(const GemmProblem &in, int z) {
+    if (in.nsplit <= 1) return in;
+    GemmProblem q = in;
+    const int64_t k0 = (int64_t)z * in.kw;
+    q.K = in.K - k0 < in.kw ? in.K - k0 : in.kw;
+    if (in.gen_is_x) {
+        // X = operator (M x K, K = its column index q), Y = D (K x N, K = row)
+        q.D = (const T *)in.D + k0 * in.drs;
+        if (in.sbuf) q.soff = in.soff + k0 * in.scs;
+        else if (in.g.b_along_rows) q.g.a_off = in.g.a_off + k0;
+        else q.g.b_off = in.g.b_off + k0;
+    } else {
+        // X = D (M x K, K = column), Y = operator (K x N, K = its row index p)
+        q.D = (const T *)in.D + k0 * in.dcs;
+        if (in.sbuf) q.soff = in.soff + k0 * in.srs;
+        else if (in.g.b_along_rows) q.g.b_off = in.g.b_off + k0;
+        else q.g.a_off = in.g.a_off + k0;
+    }
+    q.C = (T *)in.part + (int64_t)z * in.M * in.N;
+    q.crs = 1;
+    q.ccs = in.M;
+    q.alpha = 1.0;
+    q.beta = 0.0;
+    return q;
+}
 
 // Point the problem at S->buff when the operator is materialised
 // (lskge3/rskge3 with S.buff != nullptr: offset_and_ldim, skge.hh:195-198).

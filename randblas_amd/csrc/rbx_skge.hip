@@ -91,7 +91,8 @@ struct GenCells {
 // the first waves only, so on every SIMD one wave's generator work overlaps
 // the other wave's matrix work.
 template <typename T, bool GEN_X, bool S_MEM, int BM, int BN, bool FAST, int GM, int NW>
-__global__ __launch_bounds__(64 * NW, 1) void skge_kernel(GemmProblem pb, int bx0, int by0) {
+__global__ __launch_bounds__(64 * NW, 1) void skge_kernel(GemmProblem pb_in, int bx0, int by0) {
+    const GemmProblem pb = split_view<T>(pb_in, blockIdx.z);
     constexpr int NT = 64 * NW;
     constexpr int LDX = BM + 1, LDY = BN + 1;
     constexpr int WN = BN / (NW / 2);          // columns per wave
@@ -291,10 +292,13 @@ static void launch_tiles(const GemmProblem &pb, hipStream_t stream) {
     const int64_t gx = (pb.M + TM - 1) / TM, gy = (pb.N + TN - 1) / TN;
     RBX_REQUIRE(gx < (1ll << 31) && gy <= 65535);
     // interior blocks take the FAST kernel when K and the generator allow it
-    const bool fast_ok = pb.K % BK == 0 && (S_MEM || (pb.g.b_off & 3) == 0);
+    // (with split K every range starts at a multiple of kw, itself a multiple
+    // of BK, so the alignment of the first range holds for all of them)
+    const bool fast_ok = pb.K % BK == 0 && (pb.nsplit <= 1 || pb.kw % BK == 0) &&
+                         (S_MEM || (pb.g.b_off & 3) == 0);
     int64_t fx = fast_ok ? pb.M / TM : 0, fy = fast_ok ? pb.N / TN : 0;
     if (fx > 0 && fy > 0) {
-        const dim3 grid((unsigned)fx, (unsigned)fy);
+        const dim3 grid((unsigned)fx, (unsigned)fy, (unsigned)pb.nsplit);
         const int gm = S_MEM ? -1 : (pb.g.gauss ? 1 : 0) | (pb.g.b_along_rows ? 2 : 0);
         constexpr int NF = 8, TF = 64 * NF;  // two waves per SIMD on interior tiles
         if (gm == 0) hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, true, 0, NF>), grid, dim3(TF), 0, stream, pb, 0, 0);
@@ -305,27 +309,97 @@ static void launch_tiles(const GemmProblem &pb, hipStream_t stream) {
     } else {
         fx = fy = 0;
     }
-    // ragged edges (or everything) with the bounds-checked kernel:
-    // blocks [fx, gx) x [0, gy) and [0, fx) x [fy, gy)
-    if (gx > fx)
-        hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, false, -1, 4>), dim3((unsigned)(gx - fx), (unsigned)gy),
-                           dim3(NTHR), 0, stream, pb, (int)fx, 0);
-    if (fx > 0 && gy > fy)
-        hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, false, -1, 4>), dim3((unsigned)fx, (unsigned)(gy - fy)),
-                           dim3(NTHR), 0, stream, pb, 0, (int)fy);
+    // ragged edges (or everything) with the bounds-checked kernel: rows
+    // [fx*TM, M) x all columns, then rows [0, fx*TM) x columns [fy*TN, N).
+    // The long side of a tile (TN = 256 for a generated left operand, TM =
+    // 256 for a generated right one) shrinks to 64 on an edge strip, so a
+    // narrow remainder (n = 257, or a sketched vector) does not pay for a
+    // whole 256-wide tile.
+    constexpr int E = 64;
+    static_assert(TM % E == 0 && TN % E == 0, "tiles are multiples of 64");
+    const unsigned nz = (unsigned)pb.nsplit;
+    if (gx > fx) {
+        if (GEN_X && pb.N < TN) {
+            hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, E, false, -1, 4>),
+                               dim3((unsigned)(gx - fx), (unsigned)((pb.N + E - 1) / E), nz), dim3(NTHR), 0, stream,
+                               pb, (int)fx, 0);
+        } else if (!GEN_X && TM > E) {
+            const int64_t r0 = fx * (TM / E);
+            hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, E, TN, false, -1, 4>),
+                               dim3((unsigned)((pb.M + E - 1) / E - r0), (unsigned)gy, nz), dim3(NTHR), 0, stream, pb,
+                               (int)r0, 0);
+        } else {
+            hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, false, -1, 4>),
+                               dim3((unsigned)(gx - fx), (unsigned)gy, nz), dim3(NTHR), 0, stream, pb, (int)fx, 0);
+        }
+    }
+    if (fx > 0 && gy > fy) {
+        if (TN > E) {
+            const int64_t c0 = fy * (TN / E);
+            hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, E, false, -1, 4>),
+                               dim3((unsigned)fx, (unsigned)((pb.N + E - 1) / E - c0), nz), dim3(NTHR), 0, stream, pb,
+                               0, (int)c0);
+        } else {
+            hipLaunchKernelGGL((skge_kernel<T, GEN_X, S_MEM, TM, TN, false, -1, 4>),
+                               dim3((unsigned)fx, (unsigned)(gy - fy), nz), dim3(NTHR), 0, stream, pb, 0, (int)fy);
+        }
+    }
     RBX_HIP(hipGetLastError());
 }
 
+// C = alpha * sum_z part[z] + beta * C, the sum in FP64 in split order.
 template <typename T>
-static void launch_skge(const GemmProblem &pb, hipStream_t stream) {
-    const bool mem = pb.sbuf != nullptr;
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const T *part, int nsplit, int64_t M, int64_t N,
+                                                            double alpha, double beta, T *C, int64_t crs,
+                                                            int64_t ccs) {
+    const int64_t total = M * N;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % M, j = idx / M;
+        double v = 0.0;
+        for (int z = 0; z < nsplit; ++z) v += (double)part[(int64_t)z * total + idx];
+        T *c = C + i * crs + j * ccs;
+        double out = alpha * v;
+        if (beta != 0.0) out += beta * (double)(*c);
+        *c = (T)out;
+    }
+}
+
+template <typename T>
+static void launch_skge(const GemmProblem &pb_in, hipStream_t stream) {
+    const bool mem = pb_in.sbuf != nullptr;
     ProfScope prof(sizeof(T) == 8 ? "skge_kernel_f64" : "skge_kernel_f32", stream);
+    GemmProblem pb = pb_in;
+    // Split K when the output tiles alone do not cover the CUs: ranges of at
+    // least 2048, enough of them for about two workgroups per CU.
+    const int64_t TMo = pb.gen_is_x ? 64 : 256, TNo = pb.gen_is_x ? 256 : 64;
+    const int64_t tiles = ((pb.M + TMo - 1) / TMo) * ((pb.N + TNo - 1) / TNo);
+    const int64_t want = 2 * (int64_t)num_cus();
+    if (tiles < num_cus() && pb.K >= 4096) {
+        int64_t ns = (want + tiles - 1) / tiles;
+        if (ns > pb.K / 2048) ns = pb.K / 2048;
+        if (ns > 64) ns = 64;
+        const int64_t kw = ((pb.K + ns - 1) / ns + BK - 1) / BK * BK;
+        ns = (pb.K + kw - 1) / kw;
+        if (ns > 1 && (uint64_t)ns * pb.M * pb.N * sizeof(T) <= (1ull << 30)) {
+            pb.nsplit = (int)ns;
+            pb.kw = kw;
+            pb.part = scratch(SCRATCH_SPLITK, (size_t)ns * pb.M * pb.N * sizeof(T));
+        }
+    }
     if (pb.gen_is_x) {
         if (!mem) launch_tiles<T, true, false, 64, 256>(pb, stream);
         else launch_tiles<T, true, true, 64, 256>(pb, stream);
     } else {
         if (!mem) launch_tiles<T, false, false, 256, 64>(pb, stream);
         else launch_tiles<T, false, true, 256, 64>(pb, stream);
+    }
+    if (pb.nsplit > 1) {
+        int64_t blocks = (pb.M * pb.N + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, (const T *)pb.part,
+                           pb.nsplit, pb.M, pb.N, pb.alpha, pb.beta, (T *)pb.C, pb.crs, pb.ccs);
+        RBX_HIP(hipGetLastError());
     }
 }
 

@@ -255,3 +255,41 @@ def test_right_adjoint_dot_product(rb):
     lhs = float((AS * G).sum())
     rhs = float((A * GSt).sum())
     assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1)
+
+
+# ---------------------------------------------------------------- split K
+# Few output tiles and a long K: the fused sketch runs K ranges as separate
+# workgroups and sums their partial products in FP64 in range order.
+@pytest.mark.parametrize("layout,opS,opA", [("C", "N", "N"), ("R", "T", "N"), ("C", "T", "T"), ("R", "N", "T")])
+@pytest.mark.parametrize("m", [20000, 20003])          # 20003: K not a multiple of 16 (bounds-checked kernel)
+@pytest.mark.parametrize("materialise", [False, True])
+def test_lskge3_split_k(rb, orc, layout, opS, opA, m, materialise):
+    got, ref = run_lskge3(rb, orc, np.float64, layout, opS, opA, 70, 33, m, 0.75, -0.5, ro=3, co=6, pad=(1, 2),
+                          materialise=materialise)
+    assert rel_fro(got, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("axis,fam", [("S", "G"), ("L", "U")])
+def test_lskge3_split_k_fp32_axes(rb, orc, axis, fam):
+    got, ref = run_lskge3(rb, orc, np.float32, "C", "N", "N", 64, 257, 16384, 1.0, 0.0, axis=axis, fam=fam,
+                          Sshape=(64, 16384))
+    assert rel_fro(got, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("opS", ["N", "T"])
+def test_rskge3_split_k(rb, orc, layout, opS):
+    rng = np.random.default_rng(5)
+    m, d, n = 45, 31, 12000
+    ro, co = 4, 2
+    Sshape = (n + ro, d + co) if opS == "N" else (d + ro, n + co)
+    lda = m + 1 if layout == "C" else n + 3
+    ldb = (m if layout == "C" else d) + 2
+    A = rng.standard_normal(lda * (n if layout == "C" else m))
+    B0 = rng.standard_normal(ldb * (d if layout == "C" else m))
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape), rb.RNGState(23))
+    Bd = dev(B0)
+    rb.rskge3(layout, "N", opS, m, d, n, 1.5, dev(A), lda, S, ro, co, 0.25, Bd, ldb)
+    ref = B0.copy()
+    orc.rskge3("d", layout, "N", opS, m, d, n, 1.5, A, lda, Sshape, [0, 0, 0, 0], [23, 0], ro, co, 0.25, ref, ldb)
+    assert rel_fro(strided(host(Bd), m, d, layout, ldb), strided(ref, m, d, layout, ldb)) <= 1e-13
