@@ -379,6 +379,43 @@ rbx_status rbx_drsksp3_adjoint(char layout, char opS, int64_t m, int64_t d, int6
 rbx_status rbx_srequire_symmetric(char layout, const float *A, int64_t n, int64_t lda, float tol, void *stream);
 rbx_status rbx_drequire_symmetric(char layout, const double *A, int64_t n, int64_t lda, double tol, void *stream);
 
+/* ---------------------------------------------------------------- index sampling
+ * sample_indices_iid_uniform(n, k, samples, state) (RandBLAS/util.hh:552-560):
+ * k iid indices in [0, n), samples[i] = (int64) (n * (uneg11(w) + 1) / 2) in
+ * double for word w = i mod 4 of counter state + floor(i / 4).  *next_state
+ * (may be NULL) receives the state the reference returns: the seed advanced
+ * by ceil(k / 4) counters.  Device array `samples` of k entries. */
+rbx_status rbx_sample_indices_iid_uniform(int64_t n, int64_t k, int64_t *samples, const rbx_rng_state *state,
+                                          rbx_rng_state *next_state, void *stream);
+/* The same with Rademacher signs (sample_indices_iid_uniform<T, sint_t, true>,
+ * util.hh:516-550, the sampler of LASO operators): sample i uses words
+ * 2(i mod 2) and 2(i mod 2) + 1 of counter state + floor(i / 2);
+ * rademachers[i] = +1 if uneg11 of the second word is >= 0, else -1;
+ * the state advances by ceil(k / 2). */
+rbx_status rbx_ssample_indices_iid_uniform_signed(int64_t n, int64_t k, int64_t *samples, float *rademachers,
+                                                  const rbx_rng_state *state, rbx_rng_state *next_state,
+                                                  void *stream);
+rbx_status rbx_dsample_indices_iid_uniform_signed(int64_t n, int64_t k, int64_t *samples, double *rademachers,
+                                                  const rbx_rng_state *state, rbx_rng_state *next_state,
+                                                  void *stream);
+/* sample_indices_iid(n, cdf, k, samples, state) (util.hh:493-514): samples[i]
+ * is the first index j with cdf[j] >= u (std::lower_bound), u = ((T)
+ * uneg11(w) + 1) / 2 in T for word w = i mod 4 of counter state + floor(i / 4);
+ * the state advances by ceil(k / 4).  `cdf` is a device array of n entries. */
+rbx_status rbx_ssample_indices_iid(int64_t n, const float *cdf, int64_t k, int64_t *samples,
+                                   const rbx_rng_state *state, rbx_rng_state *next_state, void *stream);
+rbx_status rbx_dsample_indices_iid(int64_t n, const double *cdf, int64_t k, int64_t *samples,
+                                   const rbx_rng_state *state, rbx_rng_state *next_state, void *stream);
+/* weights_to_cdf(n, w, error_if_below) (util.hh:459-472), in place on the
+ * device, summed in the reference's sequential order (one thread, so the
+ * partial sums round as the reference's do); the reference's default
+ * error_if_below is -sqrt(epsilon of T).  Fails with RBX_INVALID_ARG, leaving
+ * w as the reference leaves it when it throws, if a weight is below
+ * error_if_below or the weights sum to less than sqrt(n) * epsilon.
+ * Synchronises the stream. */
+rbx_status rbx_sweights_to_cdf(int64_t n, float *w, float error_if_below, void *stream);
+rbx_status rbx_dweights_to_cdf(int64_t n, double *w, double error_if_below, void *stream);
+
 /* ---------------------------------------------------------------- diagnostics
  * (no reference counterpart) Kernel timing for benchmarks: while enabled,
  * every kernel the library launches is bracketed by HIP events on its stream.

@@ -37,6 +37,11 @@ def lib() -> C.CDLL:
             build()
         L = C.CDLL(LIB_PATH)
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_sample_indices_iid_uniform.argtypes = [C.c_int64, C.c_int64, _i64p, C.c_void_p, _u32p, _u32p, _u32p]
+        L.orc_sample_indices_iid.argtypes = [C.c_char, C.c_int64, C.c_void_p, C.c_int64, _i64p, _u32p, _u32p,
+                                             _u32p]
+        L.orc_weights_to_cdf.argtypes = [C.c_char, C.c_int64, C.c_void_p, C.c_double]
+        L.orc_weights_to_cdf.restype = C.c_int64
         L.orc_philox_stream.argtypes = [_u32p, _u32p, C.c_int64, _u32p]
         L.orc_ctr_incr.argtypes = [_u32p, C.c_uint64]
         L.orc_uneg11.argtypes = [C.c_uint32]
@@ -276,6 +281,36 @@ def sddmm_csr(dtype, m, d, alpha, rowptr, colidxs, Bbar, brs, bcs, Sm, srs, scs,
                              scs, _ptr(out), nthreads)
     _check(rc, "sddmm_csr")
     return out
+
+
+# ---------------------------------------------------------------- index sampling
+def sample_indices_iid_uniform(n, k, counter, key, signed=False):
+    """(samples int64[k], rademachers float64[k] or None, next counter)."""
+    s = np.zeros(max(k, 1), np.int64)
+    r = np.zeros(max(k, 1), np.float64) if signed else None
+    nxt = np.zeros(4, np.uint32)
+    lib().orc_sample_indices_iid_uniform(n, k, s, r.ctypes.data if signed else None, _ctr(counter), _key(key), nxt)
+    return s[:k], (r[:k] if signed else None), nxt
+
+
+def sample_indices_iid(n, cdf, k, counter, key):
+    """(samples int64[k], next counter); cdf float32 or float64."""
+    c = np.ascontiguousarray(cdf)
+    dt = "d" if c.dtype == np.float64 else "s"
+    s = np.zeros(max(k, 1), np.int64)
+    nxt = np.zeros(4, np.uint32)
+    lib().orc_sample_indices_iid(_b(dt), n, c.ctypes.data, k, s, _ctr(counter), _key(key), nxt)
+    return s[:k], nxt
+
+
+def weights_to_cdf(w, error_if_below=None):
+    """(cdf array, status): status 0, 1 + i (w[i] below error_if_below) or -1."""
+    a = np.ascontiguousarray(w).copy()
+    dt = "d" if a.dtype == np.float64 else "s"
+    if error_if_below is None:
+        error_if_below = -float(np.sqrt(np.finfo(a.dtype).eps))
+    st = lib().orc_weights_to_cdf(_b(dt), a.size, a.ctypes.data, error_if_below)
+    return a, int(st)
 
 
 # ---------------------------------------------------------------- bfloat16 helpers

@@ -25,6 +25,7 @@
  */
 #define _GNU_SOURCE
 #include "rbx_oracle.h"
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -789,4 +790,101 @@ int orc_sddmm_csr(char dtype, int64_t m, int64_t d, double alpha, const int64_t 
         }
     }
     return 0;
+}
+
+/* ---- index sampling (util.hh:459-560), sequential as the reference ---- */
+
+/* sample_indices_iid_uniform (util.hh:516-560): one rv array of 4 uneg11
+ * values per counter, regenerated when exhausted; with signs the array length
+ * is rounded down to even (2 here: no change) and each sample takes two
+ * values.  rademachers == NULL selects the unsigned form (util.hh:552-560).
+ * Writes the returned state's counter to next_ctr. */
+void orc_sample_indices_iid_uniform(int64_t n, int64_t k, int64_t *samples, double *rademachers,
+                                    const uint32_t ctr[4], const uint32_t key[2], uint32_t next_ctr[4]) {
+    uint32_t c[4], w[4];
+    memcpy(c, ctr, 16);
+    orc_philox4x32_10(c, key, w);
+    const int len_c = 4;
+    int rv_index = 0;
+    const double dN = (double)n;
+    for (int64_t i = 0; i < k; ++i) {
+        const double u = ((double)orc_uneg11(w[rv_index]) + 1.0) / 2.0;   /* uneg11_to_u01<double> */
+        samples[i] = (int64_t)((double)(int64_t)dN * u);                 /* (sint_t) dN * u, util.hh:533 */
+        rv_index += 1;
+        if (rademachers) {
+            rademachers[i] = (orc_uneg11(w[rv_index]) >= 0.0f) ? 1.0 : -1.0;
+            rv_index += 1;
+        }
+        if (rv_index == len_c) {
+            orc_ctr_incr(c, 1);
+            orc_philox4x32_10(c, key, w);
+            rv_index = 0;
+        }
+    }
+    if (rv_index > 0) orc_ctr_incr(c, 1);
+    memcpy(next_ctr, c, 16);
+}
+
+/* sample_indices_iid (util.hh:493-514): u = uneg11_to_u01<T>(rv) in T, then
+ * std::lower_bound(cdf, cdf + n, u) - cdf. */
+#define DEF_IID_CDF(T, NAME)                                                                   \
+static void NAME(int64_t n, const T *cdf, int64_t k, int64_t *samples, const uint32_t ctr[4], \
+                 const uint32_t key[2], uint32_t next_ctr[4]) {                                \
+    uint32_t c[4], w[4];                                                                       \
+    memcpy(c, ctr, 16);                                                                        \
+    orc_philox4x32_10(c, key, w);                                                              \
+    int rv_index = 0;                                                                          \
+    for (int64_t i = 0; i < k; ++i) {                                                          \
+        volatile T s = (T)orc_uneg11(w[rv_index]) + (T)1.0;                                    \
+        const T u = s / (T)2.0;                                                                \
+        int64_t lo = 0, len = n;                                                               \
+        while (len > 0) {                                                                      \
+            const int64_t half = len / 2, mid = lo + half;                                     \
+            if (cdf[mid] < u) { lo = mid + 1; len -= half + 1; }                               \
+            else len = half;                                                                   \
+        }                                                                                      \
+        samples[i] = lo;                                                                       \
+        rv_index += 1;                                                                         \
+        if (rv_index == 4) {                                                                   \
+            orc_ctr_incr(c, 1);                                                                \
+            orc_philox4x32_10(c, key, w);                                                      \
+            rv_index = 0;                                                                      \
+        }                                                                                      \
+    }                                                                                          \
+    if (rv_index > 0) orc_ctr_incr(c, 1);                                                      \
+    memcpy(next_ctr, c, 16);                                                                   \
+}
+DEF_IID_CDF(double, iid_cdf_d)
+DEF_IID_CDF(float, iid_cdf_s)
+
+void orc_sample_indices_iid(char dtype, int64_t n, const void *cdf, int64_t k, int64_t *samples,
+                            const uint32_t ctr[4], const uint32_t key[2], uint32_t next_ctr[4]) {
+    if (dtype == 'd') iid_cdf_d(n, (const double *)cdf, k, samples, ctr, key, next_ctr);
+    else iid_cdf_s(n, (const float *)cdf, k, samples, ctr, key, next_ctr);
+}
+
+/* weights_to_cdf (util.hh:459-472).  Returns 0, or 1 + i when w[i] is below
+ * error_if_below (w[0..i) then hold the partial sums), or -1 when the total is
+ * below sqrt(n) * epsilon (w holds the unscaled partial sums). */
+#define DEF_W2CDF(T, NAME, EPS)                                                                \
+static int64_t NAME(int64_t n, T *w, T error_if_below) {                                       \
+    volatile T sum = (T)0.0;                                                                   \
+    for (int64_t i = 0; i < n; ++i) {                                                          \
+        T val = w[i];                                                                          \
+        if (!(val >= error_if_below)) return 1 + i;                                            \
+        val = val > (T)0.0 ? val : (T)0.0;                                                     \
+        sum = sum + val;                                                                       \
+        w[i] = sum;                                                                            \
+    }                                                                                          \
+    if (!(sum >= (T)sqrt((double)n) * (EPS))) return -1;                                       \
+    const T inv = (T)1.0 / sum;                                                                \
+    for (int64_t i = 0; i < n; ++i) w[i] = w[i] * inv;                                         \
+    return 0;                                                                                  \
+}
+DEF_W2CDF(double, w2cdf_d, DBL_EPSILON)
+DEF_W2CDF(float, w2cdf_s, FLT_EPSILON)
+
+int64_t orc_weights_to_cdf(char dtype, int64_t n, void *w, double error_if_below) {
+    if (dtype == 'd') return w2cdf_d(n, (double *)w, error_if_below);
+    return w2cdf_s(n, (float *)w, (float)error_if_below);
 }

@@ -97,6 +97,13 @@ int orc_sddmm_csr(char dtype, int64_t m, int64_t d, double alpha, const int64_t 
                   const int64_t *colidxs, const void *Bbar, int64_t brs, int64_t bcs,
                   const void *Sm, int64_t srs, int64_t scs, void *dvals, int nthreads);
 
+/* index sampling (util.hh:459-560) */
+void orc_sample_indices_iid_uniform(int64_t n, int64_t k, int64_t *samples, double *rademachers,
+                                    const uint32_t ctr[4], const uint32_t key[2], uint32_t next_ctr[4]);
+void orc_sample_indices_iid(char dtype, int64_t n, const void *cdf, int64_t k, int64_t *samples,
+                            const uint32_t ctr[4], const uint32_t key[2], uint32_t next_ctr[4]);
+int64_t orc_weights_to_cdf(char dtype, int64_t n, void *w, double error_if_below);
+
 #ifdef __cplusplus
 }
 #endif

@@ -162,6 +162,13 @@ _SIG = {
     "rbx_srsksp3_adjoint": [_CH, _CH, _I64, _I64, _I64, _F, _CSR, _DS, _I64, _I64, _P, _I64, _P, _P],
     "rbx_drsksp3_adjoint": [_CH, _CH, _I64, _I64, _I64, _D, _CSR, _DS, _I64, _I64, _P, _I64, _P, _P],
     "rbx_srequire_symmetric": [_CH, _P, _I64, _I64, _F, _P],
+    "rbx_sample_indices_iid_uniform": [_I64, _I64, _P, _RS, _RS, _P],
+    "rbx_ssample_indices_iid_uniform_signed": [_I64, _I64, _P, _P, _RS, _RS, _P],
+    "rbx_dsample_indices_iid_uniform_signed": [_I64, _I64, _P, _P, _RS, _RS, _P],
+    "rbx_ssample_indices_iid": [_I64, _P, _I64, _P, _RS, _RS, _P],
+    "rbx_dsample_indices_iid": [_I64, _P, _I64, _P, _RS, _RS, _P],
+    "rbx_sweights_to_cdf": [_I64, _P, _F, _P],
+    "rbx_dweights_to_cdf": [_I64, _P, _D, _P],
     "rbx_drequire_symmetric": [_CH, _P, _I64, _I64, _D, _P],
     "rbx_profile_enable": [C.c_int],
     "rbx_profile_collect": [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int],
@@ -658,6 +665,37 @@ def require_symmetric(layout, A, n, lda, tol):
     """util::require_symmetric (util.hh:127-148), evaluated on the device by
     rbx_[sd]require_symmetric; raises Error with the reference's message."""
     _call(f"rbx_{_prefix(A)}require_symmetric", _c(layout), _ptr(A), n, lda, tol, _stream())
+
+
+def sample_indices_iid_uniform(n, k, samples, state: RNGState, rademachers=None) -> RNGState:
+    """sample_indices_iid_uniform (util.hh:516-560): k iid indices in [0, n) into
+    the int64 device tensor `samples`, and (when given) Rademacher signs into
+    `rademachers`; returns the advanced state."""
+    nxt = c_rng_state()
+    if rademachers is None:
+        _call("rbx_sample_indices_iid_uniform", n, k, _ptr(samples), C.byref(state.c()), C.byref(nxt), _stream())
+    else:
+        _call(f"rbx_{_prefix(rademachers)}sample_indices_iid_uniform_signed", n, k, _ptr(samples), _ptr(rademachers),
+              C.byref(state.c()), C.byref(nxt), _stream())
+    return RNGState.from_c(nxt)
+
+
+def sample_indices_iid(n, cdf, k, samples, state: RNGState) -> RNGState:
+    """sample_indices_iid (util.hh:493-514): k iid draws from the CDF `cdf`
+    (a float32 / float64 device tensor of n entries); returns the advanced state."""
+    nxt = c_rng_state()
+    _call(f"rbx_{_prefix(cdf)}sample_indices_iid", n, _ptr(cdf), k, _ptr(samples), C.byref(state.c()),
+          C.byref(nxt), _stream())
+    return RNGState.from_c(nxt)
+
+
+def weights_to_cdf(n, w, error_if_below=None) -> None:
+    """weights_to_cdf (util.hh:459-472) in place on a float32 / float64 device
+    tensor; error_if_below defaults to -sqrt(epsilon) of w's type."""
+    if error_if_below is None:
+        import numpy as _np
+        error_if_below = -float(_np.sqrt(_np.finfo(_np.float32 if _prefix(w) == "s" else _np.float64).eps))
+    _call(f"rbx_{_prefix(w)}weights_to_cdf", n, _ptr(w), error_if_below, _stream())
 
 
 def profile_enable(on: bool = True) -> None:

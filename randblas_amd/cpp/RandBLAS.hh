@@ -18,9 +18,11 @@
 #ifndef RANDBLAS_AMD_CPP_HH
 #define RANDBLAS_AMD_CPP_HH
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <exception>
+#include <limits>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -587,6 +589,54 @@ void require_symmetric(blas::Layout layout, const T *A, int64_t n, int64_t lda, 
     detail::check(st, "require_symmetric");
 }
 }  // namespace util
+
+// ---------------------------------------------------------------- index sampling
+// weights_to_cdf (util.hh:459-472), in place on the device; synchronises.
+template <typename T>
+void weights_to_cdf(int64_t n, T *w, T error_if_below = -std::sqrt(std::numeric_limits<T>::epsilon())) {
+    rbx_status st;
+    if constexpr (std::is_same<T, float>::value) st = rbx_sweights_to_cdf(n, w, error_if_below, gpu::stream());
+    else st = rbx_dweights_to_cdf(n, w, error_if_below, gpu::stream());
+    detail::check(st, "weights_to_cdf");
+}
+
+// sample_indices_iid (util.hh:493-514): k iid draws from a device CDF.
+template <typename T, typename sint_t, typename state_t = RNGState<>>
+state_t sample_indices_iid(int64_t n, const T *cdf, int64_t k, sint_t *samples, const state_t &state) {
+    static_assert(std::is_same<sint_t, int64_t>::value, "int64_t indices");
+    rbx_rng_state s = state.c(), next;
+    rbx_status st;
+    if constexpr (std::is_same<T, float>::value)
+        st = rbx_ssample_indices_iid(n, cdf, k, samples, &s, &next, gpu::stream());
+    else
+        st = rbx_dsample_indices_iid(n, cdf, k, samples, &s, &next, gpu::stream());
+    detail::check(st, "sample_indices_iid");
+    return state_t::from_c(next);
+}
+
+// sample_indices_iid_uniform with Rademacher signs (util.hh:516-550).
+template <typename T, typename sint_t, typename state_t = RNGState<>>
+state_t sample_indices_iid_uniform(int64_t n, int64_t k, sint_t *samples, T *rademachers, const state_t &state) {
+    static_assert(std::is_same<sint_t, int64_t>::value, "int64_t indices");
+    rbx_rng_state s = state.c(), next;
+    rbx_status st;
+    if constexpr (std::is_same<T, float>::value)
+        st = rbx_ssample_indices_iid_uniform_signed(n, k, samples, rademachers, &s, &next, gpu::stream());
+    else
+        st = rbx_dsample_indices_iid_uniform_signed(n, k, samples, rademachers, &s, &next, gpu::stream());
+    detail::check(st, "sample_indices_iid_uniform");
+    return state_t::from_c(next);
+}
+
+// sample_indices_iid_uniform (util.hh:552-560).
+template <typename sint_t = int64_t, typename state_t = RNGState<>>
+state_t sample_indices_iid_uniform(int64_t n, int64_t k, sint_t *samples, const state_t &state) {
+    static_assert(std::is_same<sint_t, int64_t>::value, "int64_t indices");
+    rbx_rng_state s = state.c(), next;
+    detail::check(rbx_sample_indices_iid_uniform(n, k, samples, &s, &next, gpu::stream()),
+                  "sample_indices_iid_uniform");
+    return state_t::from_c(next);
+}
 
 // sketch_symmetric (sksy.hh:158-530): A is n x n, symmetric, stored as a
 // general matrix; checked first with require_symmetric(sym_check_tol).

@@ -291,6 +291,30 @@ static void gpu_checks(const std::string &out) {
             HIPCHK(hipFree(p));
     }
 
+    // index sampling (util.hh:459-560): uniform, signed, and from a CDF
+    {
+        const int64_t k = 1001, nw = 300;
+        int64_t *samp = dev_alloc<int64_t>(k), *samp2 = dev_alloc<int64_t>(k), *samp3 = dev_alloc<int64_t>(k);
+        double *rad = dev_alloc<double>(k), *w = dev_alloc<double>(nw);
+        RNGState<> seed(42);
+        RNGState<> n1 = RandBLAS::sample_indices_iid_uniform(100, k, samp, seed);
+        RNGState<> n2 = RandBLAS::sample_indices_iid_uniform(2000, k, samp2, rad, seed);
+        std::vector<double> hw(nw);
+        for (int64_t i = 0; i < nw; ++i) hw[i] = 1.0 / std::pow((double)i + 1.0, 1.5);
+        HIPCHK(hipMemcpy(w, hw.data(), nw * sizeof(double), hipMemcpyHostToDevice));
+        RandBLAS::weights_to_cdf(nw, w);
+        RNGState<> n3 = RandBLAS::sample_indices_iid(nw, (const double *)w, k, samp3, RNGState<>(7));
+        HIPCHK(hipStreamSynchronize(stream));
+        EXPECT(n1.counter[0] == (uint32_t)((k + 3) / 4) && n2.counter[0] == (uint32_t)((k + 1) / 2) &&
+               n3.counter[0] == (uint32_t)((k + 3) / 4) && n1.key[0] == 42u);
+        EXPECT(throws_error([&] { RandBLAS::weights_to_cdf(nw, w, 2.0); }, "weights_to_cdf"));
+        dump(out, "iid_uniform", samp, k);
+        dump(out, "iid_signed", samp2, k);
+        dump(out, "iid_signs", rad, k);
+        dump(out, "iid_cdf_samples", samp3, k);
+        for (void *p : {(void *)samp, (void *)samp2, (void *)samp3, (void *)rad, (void *)w}) HIPCHK(hipFree(p));
+    }
+
     RandBLAS::gpu::set_stream(nullptr);
     HIPCHK(hipStreamDestroy(stream));
 }

@@ -85,3 +85,13 @@ def test_cpp_layer_on_gpu(rb, orc, tmp_path):
     orc.left_spmm_csr("d", "C", "N", cm, cd, cn, 1.0, cv, rp, ci, np.ascontiguousarray(S4.reshape(-1, order="F")),
                       cn, 0.0, ref4, cm)
     assert np.array_equal(load("csr_B", np.float64), ref4)
+
+    # index sampling through the C++ layer
+    k = 1001
+    s1, _, _ = orc.sample_indices_iid_uniform(100, k, [0, 0, 0, 0], [42, 0])
+    assert np.array_equal(load("iid_uniform", np.int64), s1)
+    s2, r2, _ = orc.sample_indices_iid_uniform(2000, k, [0, 0, 0, 0], [42, 0], signed=True)
+    assert np.array_equal(load("iid_signed", np.int64), s2) and np.array_equal(load("iid_signs", np.float64), r2)
+    cdf, st = orc.weights_to_cdf(1.0 / (np.arange(300) + 1.0) ** 1.5)
+    s3, _ = orc.sample_indices_iid(300, cdf, k, [0, 0, 0, 0], [7, 0])
+    assert st == 0 and np.array_equal(load("iid_cdf_samples", np.int64), s3)

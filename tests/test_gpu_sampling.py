@@ -1,0 +1,79 @@
+"""GPU parity for index sampling and weights_to_cdf (util.hh:459-560):
+bit-exact against the oracle's sequential restatement (indices, signs, CDF
+values, returned states), including counters that carry across 32-bit
+words, and the reference's error behaviour."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("n,k,ctr", [(100, 37, (0, 0, 0, 0)), (7, 1, (5, 0, 0, 0)), (2 ** 40, 1000, (0xFFFFFFF0, 3, 0, 0)),
+                                     (123457, 1 << 20, (9, 9, 9, 9)), (5, 0, (0, 0, 0, 0))])
+def test_uniform_bit_exact(rb, orc, n, k, ctr):
+    st = rb.RNGState(key=42, counter=ctr)
+    s = torch.full((max(k, 1),), -1, dtype=torch.int64, device=DEV)
+    nxt = rb.sample_indices_iid_uniform(n, k, s, st)
+    ref, _, rnxt = orc.sample_indices_iid_uniform(n, k, list(ctr), [42, 0])
+    assert np.array_equal(host(s)[:k], ref)
+    assert list(nxt.counter) == [int(x) for x in rnxt] and nxt.key == 42
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("k", [1, 2, 3, 1001, 65536])
+def test_signed_bit_exact(rb, orc, dtype, k):
+    st = rb.RNGState(key=3, counter=(0xFFFFFFFF, 0, 0, 0))
+    s = torch.empty(k, dtype=torch.int64, device=DEV)
+    r = torch.empty(k, dtype=dtype, device=DEV)
+    nxt = rb.sample_indices_iid_uniform(2000, k, s, st, rademachers=r)
+    ref, rr, rnxt = orc.sample_indices_iid_uniform(2000, k, [0xFFFFFFFF, 0, 0, 0], [3, 0], signed=True)
+    assert np.array_equal(host(s), ref) and np.array_equal(host(r).astype(np.float64), rr)
+    assert list(nxt.counter) == [int(x) for x in rnxt]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_weights_to_cdf_and_sampling_bit_exact(rb, orc, dtype):
+    rng = np.random.default_rng(3)
+    n = 5000
+    w = (rng.random(n) / (np.arange(n) + 1.0) ** 0.8).astype(dtype)
+    w[::11] = 0
+    wd = torch.from_numpy(w.copy()).to(DEV)
+    rb.weights_to_cdf(n, wd)
+    ref, st = orc.weights_to_cdf(w)
+    assert st == 0
+    cdf = host(wd)
+    assert np.array_equal(cdf, ref)
+    k = 300001
+    s = torch.empty(k, dtype=torch.int64, device=DEV)
+    nxt = rb.sample_indices_iid(n, wd, k, s, rb.RNGState(key=77, counter=(1, 2, 3, 4)))
+    rs, rnxt = orc.sample_indices_iid(n, ref, k, [1, 2, 3, 4], [77, 0])
+    assert np.array_equal(host(s), rs)
+    assert list(nxt.counter) == [int(x) for x in rnxt]
+
+
+def test_degenerate_distribution_on_device(rb):
+    # test_discrete.cc:153-165: a delta at 17 with a tiny negative weight clipped
+    w = torch.zeros(100, dtype=torch.float32, device=DEV)
+    w[17] = 99.0
+    w[3] = -float(np.finfo(np.float32).eps) / 10
+    rb.weights_to_cdf(100, w)
+    s = torch.empty(10000, dtype=torch.int64, device=DEV)
+    rb.sample_indices_iid(100, w, 10000, s, rb.RNGState(4))
+    assert bool((s == 17).all())
+
+
+def test_weights_to_cdf_errors_on_device(rb, orc):
+    w = torch.tensor([1.0, 2.0, -1e-3, 4.0], dtype=torch.float64, device=DEV)
+    with pytest.raises(rb.Error, match="w\\[2\\]"):
+        rb.weights_to_cdf(4, w)
+    # the reference has overwritten the entries before the failing one
+    assert host(w).tolist()[:2] == [1.0, 3.0]
+    with pytest.raises(rb.Error, match="sum"):
+        rb.weights_to_cdf(5, torch.zeros(5, dtype=torch.float32, device=DEV))
