@@ -379,6 +379,30 @@ rbx_status rbx_drsksp3_adjoint(char layout, char opS, int64_t m, int64_t d, int6
 rbx_status rbx_srequire_symmetric(char layout, const float *A, int64_t n, int64_t lda, float tol, void *stream);
 rbx_status rbx_drequire_symmetric(char layout, const double *A, int64_t n, int64_t lda, double tol, void *stream);
 
+/* ---------------------------------------------------------------- format conversions
+ * csr_to_coo / csc_to_coo (sparse_data/conversions.hh:49-75): the COO arrays
+ * (caller-allocated, nnz entries) receive the stored entries in the compressed
+ * order; coo->nnz is set.  coo_to_csr / coo_to_csc (conversions.hh:79-126):
+ * the entries sorted by (row, column) / (column, row) into the caller's
+ * index and value arrays (nnz entries) and pointer array (n_rows + 1 /
+ * n_cols + 1); the sort is stable, so equal (row, column) pairs keep their
+ * stored order.  coo_sort(order, coo) (coo_arrays_apply_sort,
+ * sparse_data/base.hh:169-220) sorts a COO matrix in place, order 'R' (CSR)
+ * or 'C' (CSC).  coo_determine_sort (base.hh:142-167) sets *order to 'C' if
+ * the entries are in CSC order, else 'R' if in CSR order, else 'N', and
+ * synchronises the stream.  Zero-based indices. */
+rbx_status rbx_scsr_to_coo(const rbx_csr_matrix *csr, rbx_coo_matrix *coo, void *stream);
+rbx_status rbx_dcsr_to_coo(const rbx_csr_matrix *csr, rbx_coo_matrix *coo, void *stream);
+rbx_status rbx_scsc_to_coo(const rbx_csc_matrix *csc, rbx_coo_matrix *coo, void *stream);
+rbx_status rbx_dcsc_to_coo(const rbx_csc_matrix *csc, rbx_coo_matrix *coo, void *stream);
+rbx_status rbx_scoo_to_csr(const rbx_coo_matrix *coo, rbx_csr_matrix *csr, void *stream);
+rbx_status rbx_dcoo_to_csr(const rbx_coo_matrix *coo, rbx_csr_matrix *csr, void *stream);
+rbx_status rbx_scoo_to_csc(const rbx_coo_matrix *coo, rbx_csc_matrix *csc, void *stream);
+rbx_status rbx_dcoo_to_csc(const rbx_coo_matrix *coo, rbx_csc_matrix *csc, void *stream);
+rbx_status rbx_scoo_sort(char order, rbx_coo_matrix *coo, void *stream);
+rbx_status rbx_dcoo_sort(char order, rbx_coo_matrix *coo, void *stream);
+rbx_status rbx_coo_determine_sort(const rbx_coo_matrix *coo, char *order, void *stream);
+
 /* ---------------------------------------------------------------- index sampling
  * sample_indices_iid_uniform(n, k, samples, state) (RandBLAS/util.hh:552-560):
  * k iid indices in [0, n), samples[i] = (int64) (n * (uneg11(w) + 1) / 2) in

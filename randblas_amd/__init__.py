@@ -162,6 +162,13 @@ _SIG = {
     "rbx_srsksp3_adjoint": [_CH, _CH, _I64, _I64, _I64, _F, _CSR, _DS, _I64, _I64, _P, _I64, _P, _P],
     "rbx_drsksp3_adjoint": [_CH, _CH, _I64, _I64, _I64, _D, _CSR, _DS, _I64, _I64, _P, _I64, _P, _P],
     "rbx_srequire_symmetric": [_CH, _P, _I64, _I64, _F, _P],
+    **{f"rbx_{p}{a}_to_{b}": [sa, sb, _P]
+       for p in ("s", "d")
+       for a, b, sa, sb in (("csr", "coo", _CSR, _COO), ("csc", "coo", _CSC, _COO), ("coo", "csr", _COO, _CSR),
+                            ("coo", "csc", _COO, _CSC))},
+    "rbx_scoo_sort": [_CH, _COO, _P],
+    "rbx_dcoo_sort": [_CH, _COO, _P],
+    "rbx_coo_determine_sort": [_COO, C.POINTER(C.c_char), _P],
     "rbx_sample_indices_iid_uniform": [_I64, _I64, _P, _RS, _RS, _P],
     "rbx_ssample_indices_iid_uniform_signed": [_I64, _I64, _P, _P, _RS, _RS, _P],
     "rbx_dsample_indices_iid_uniform_signed": [_I64, _I64, _P, _P, _RS, _RS, _P],
@@ -665,6 +672,44 @@ def require_symmetric(layout, A, n, lda, tol):
     """util::require_symmetric (util.hh:127-148), evaluated on the device by
     rbx_[sd]require_symmetric; raises Error with the reference's message."""
     _call(f"rbx_{_prefix(A)}require_symmetric", _c(layout), _ptr(A), n, lda, tol, _stream())
+
+
+def _conv(name, src, dst):
+    a, b = src.c(), dst.c()
+    _call(f"rbx_{_prefix(src.vals)}{name}", C.byref(a), C.byref(b), _stream())
+
+
+def csr_to_coo(csr: "CSRMatrix", coo: "COOMatrix") -> None:
+    """csr_to_coo (sparse_data/conversions.hh:63-75) into caller-allocated COO arrays."""
+    _conv("csr_to_coo", csr, coo)
+
+
+def csc_to_coo(csc: "CSCMatrix", coo: "COOMatrix") -> None:
+    """csc_to_coo (sparse_data/conversions.hh:49-61) into caller-allocated COO arrays."""
+    _conv("csc_to_coo", csc, coo)
+
+
+def coo_to_csr(coo: "COOMatrix", csr: "CSRMatrix") -> None:
+    """coo_to_csr (sparse_data/conversions.hh:101-126): stable (row, column) order."""
+    _conv("coo_to_csr", coo, csr)
+
+
+def coo_to_csc(coo: "COOMatrix", csc: "CSCMatrix") -> None:
+    """coo_to_csc (sparse_data/conversions.hh:79-99): stable (column, row) order."""
+    _conv("coo_to_csc", coo, csc)
+
+
+def coo_sort(coo: "COOMatrix", order: str) -> None:
+    """COOMatrix::sort_arrays (coo_matrix.hh:222-226) in place: order "R" (CSR) or "C" (CSC)."""
+    a = coo.c()
+    _call(f"rbx_{_prefix(coo.vals)}coo_sort", _c(order), C.byref(a), _stream())
+
+
+def coo_determine_sort(coo: "COOMatrix") -> str:
+    """coo_arrays_determine_sort (sparse_data/base.hh:142-167): "C", "R" or "N"."""
+    out = C.create_string_buffer(1)
+    _call("rbx_coo_determine_sort", C.byref(coo.c()), out, _stream())
+    return out.raw.decode()
 
 
 def sample_indices_iid_uniform(n, k, samples, state: RNGState, rademachers=None) -> RNGState:

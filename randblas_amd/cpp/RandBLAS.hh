@@ -718,6 +718,9 @@ struct CSCMatrix {
     rbx_csc_matrix c() const { return rbx_csc_matrix{n_rows, n_cols, nnz, (void *)vals, rowidxs, colptr}; }
 };
 
+// NonzeroSort (sparse_data/base.hh:103-107).
+enum class NonzeroSort : char { CSC = 'C', CSR = 'R', None = 'N' };
+
 // COOMatrix (sparse_data/coo_matrix.hh:83-308), zero-based, device arrays,
 // borrowed; the entries may be in any order (the library sorts a copy).
 template <typename T, typename sint_t = int64_t>
@@ -741,7 +744,73 @@ struct COOMatrix {
         if (index_base != IndexBase::Zero) throw Unsupported("COOMatrix: only zero-based indices are supported");
     }
     rbx_coo_matrix c() const { return rbx_coo_matrix{n_rows, n_cols, nnz, (void *)vals, rows, cols}; }
+
+    // sort_arrays (coo_matrix.hh:222-226): stable, in place on the device.
+    void sort_arrays(NonzeroSort s) {
+        if (s == NonzeroSort::None) return;
+        rbx_coo_matrix a = c();
+        rbx_status st;
+        if constexpr (std::is_same<T, float>::value) st = rbx_scoo_sort((char)s, &a, gpu::stream());
+        else st = rbx_dcoo_sort((char)s, &a, gpu::stream());
+        if (st != RBX_OK) throw Error(std::string("sort_arrays: ") + rbx_last_error_message());
+    }
+    // coo_arrays_determine_sort (sparse_data/base.hh:142-167); synchronises.
+    NonzeroSort determine_sort() const {
+        rbx_coo_matrix a = c();
+        char o = 'N';
+        if (rbx_coo_determine_sort(&a, &o, gpu::stream()) != RBX_OK)
+            throw Error(std::string("determine_sort: ") + rbx_last_error_message());
+        return static_cast<NonzeroSort>(o);
+    }
 };
+
+// Format conversions (sparse_data/conversions.hh:49-126) into caller-allocated
+// device arrays of the destination (nnz entries; n_rows + 1 / n_cols + 1
+// pointers).  COO -> compressed sorts stably.
+namespace formats {
+#define RBX_CONV_OVERLOAD(T, PFX)                                                        \
+    inline rbx_status conv(const rbx_csr_matrix *a, rbx_coo_matrix *b, T *) {            \
+        return rbx_##PFX##csr_to_coo(a, b, gpu::stream());                               \
+    }                                                                                    \
+    inline rbx_status conv(const rbx_csc_matrix *a, rbx_coo_matrix *b, T *) {            \
+        return rbx_##PFX##csc_to_coo(a, b, gpu::stream());                               \
+    }                                                                                    \
+    inline rbx_status conv(const rbx_coo_matrix *a, rbx_csr_matrix *b, T *) {            \
+        return rbx_##PFX##coo_to_csr(a, b, gpu::stream());                               \
+    }                                                                                    \
+    inline rbx_status conv(const rbx_coo_matrix *a, rbx_csc_matrix *b, T *) {            \
+        return rbx_##PFX##coo_to_csc(a, b, gpu::stream());                               \
+    }
+RBX_CONV_OVERLOAD(float, s)
+RBX_CONV_OVERLOAD(double, d)
+#undef RBX_CONV_OVERLOAD
+template <typename From, typename To>
+void convert(const From &src, To &dst, const char *name) {
+    static_assert(std::is_same<typename From::scalar_t, typename To::scalar_t>::value, "same scalar type");
+    auto a = src.c();
+    auto b = dst.c();
+    if (conv(&a, &b, (typename From::scalar_t *)nullptr) != RBX_OK)
+        throw Error(std::string(name) + ": " + rbx_last_error_message());
+    dst.nnz = b.nnz;
+}
+}  // namespace formats
+
+template <typename T>
+void csr_to_coo(const CSRMatrix<T, int64_t> &csr, COOMatrix<T, int64_t> &coo) {
+    formats::convert(csr, coo, "csr_to_coo");
+}
+template <typename T>
+void csc_to_coo(const CSCMatrix<T, int64_t> &csc, COOMatrix<T, int64_t> &coo) {
+    formats::convert(csc, coo, "csc_to_coo");
+}
+template <typename T>
+void coo_to_csr(const COOMatrix<T, int64_t> &coo, CSRMatrix<T, int64_t> &csr) {
+    formats::convert(coo, csr, "coo_to_csr");
+}
+template <typename T>
+void coo_to_csc(const COOMatrix<T, int64_t> &coo, CSCMatrix<T, int64_t> &csc) {
+    formats::convert(coo, csc, "coo_to_csc");
+}
 
 namespace formats {
 // One C entry point per (precision, format) pair.

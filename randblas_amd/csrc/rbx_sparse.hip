@@ -682,10 +682,14 @@ __global__ __launch_bounds__(256) void csr_rows_kernel(const int64_t *rowptr, co
     }
 }
 
-__global__ __launch_bounds__(256) void csr_to_coo_rows_kernel(const int64_t *rowptr, int64_t nrows, int64_t *rows) {
+// rows[p] = i for p in [rowptr[i], rowptr[i+1]); positions outside [0, nnz)
+// are never written
+__global__ __launch_bounds__(256) void csr_to_coo_rows_kernel(const int64_t *rowptr, int64_t nrows, int64_t nnz,
+                                                              int64_t *rows) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrows) return;
-    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) rows[p] = i;
+    const int64_t p0 = rowptr[i] < 0 ? 0 : rowptr[i], p1 = rowptr[i + 1] > nnz ? nnz : rowptr[i + 1];
+    for (int64_t p = p0; p < p1; ++p) rows[p] = i;
 }
 
 // op(submat(S)) of a materialised operator (S.buff in S.layout) gathered into
@@ -758,7 +762,7 @@ static const int64_t *expand_ptr(const int64_t *ptr, int64_t nmaj, int64_t nnz, 
     int64_t *out = (int64_t *)scratch(SCRATCH_SPARSE_OP, (size_t)(nnz > 0 ? nnz : 1) * 8);
     if (nnz > 0 && nmaj > 0) {
         dim3 grid((unsigned)((nmaj + 255) / 256));
-        hipLaunchKernelGGL(csr_to_coo_rows_kernel, grid, dim3(256), 0, st, ptr, nmaj, out);
+        hipLaunchKernelGGL(csr_to_coo_rows_kernel, grid, dim3(256), 0, st, ptr, nmaj, nnz, out);
         RBX_HIP(hipGetLastError());
     }
     return out;

@@ -286,6 +286,26 @@ static void gpu_checks(const std::string &out) {
         HIPCHK(hipMemcpy(h6.data(), B6, cm * cd * sizeof(double), hipMemcpyDeviceToHost));
         EXPECT(std::memcmp(h4.data(), h5.data(), h4.size() * sizeof(double)) == 0);
         EXPECT(std::memcmp(h4.data(), h6.data(), h4.size() * sizeof(double)) == 0);
+
+        // conversions: the reversed COO sorted back to CSR gives the CSR arrays;
+        // CSC -> COO is CSC-ordered
+        EXPECT(Acoo.determine_sort() == RandBLAS::sparse_data::NonzeroSort::None);
+        int64_t *rp2 = dev_alloc<int64_t>(cm + 1), *ci2 = dev_alloc<int64_t>(nnz);
+        double *v2 = dev_alloc<double>(nnz);
+        RandBLAS::sparse_data::CSRMatrix<double> Acsr2(cm, cn, 0, v2, rp2, ci2);
+        RandBLAS::sparse_data::coo_to_csr(Acoo, Acsr2);
+        std::vector<int64_t> hrp(cm + 1), hci(nnz);
+        std::vector<double> hv(nnz);
+        HIPCHK(hipMemcpy(hrp.data(), rp2, (cm + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hci.data(), ci2, nnz * sizeof(int64_t), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hv.data(), v2, nnz * sizeof(double), hipMemcpyDeviceToHost));
+        EXPECT(Acsr2.nnz == nnz && hrp == rowptr && hci == colidx && hv == vals);
+        RandBLAS::sparse_data::COOMatrix<double> Bcoo(cm, cn, 0, ov, orow, ocol);
+        RandBLAS::sparse_data::csc_to_coo(Acsc, Bcoo);
+        EXPECT(Bcoo.nnz == nnz && Bcoo.determine_sort() == RandBLAS::sparse_data::NonzeroSort::CSC);
+        Bcoo.sort_arrays(RandBLAS::sparse_data::NonzeroSort::CSR);
+        EXPECT(Bcoo.determine_sort() == RandBLAS::sparse_data::NonzeroSort::CSR);
+        for (void *p : {(void *)rp2, (void *)ci2, (void *)v2}) HIPCHK(hipFree(p));
         for (void *p : {(void *)cv, (void *)ov, (void *)B5, (void *)B6, (void *)cp, (void *)cr, (void *)orow,
                         (void *)ocol})
             HIPCHK(hipFree(p));
