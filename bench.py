@@ -71,8 +71,8 @@ def main():
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 sketch (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the sketch shards)", file=sys.stderr)
+        print("bench.py: measures one GPU only; a multi-GPU run would shard the sketch by row "
+              "blocks of S (one counter range per GPU) and is not part of this benchmark", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -346,7 +346,9 @@ def sketch_solve_leg(rb, torch, dev, reps=5):
     """The sketch-and-solve product and its adjoint: S = DenseDist(1024, 65536)
     (key 42) times [A b], A 65536 x 256 N(0,1) (key 11), FP32 and FP64; the
     adjoint S^T G for G 1024 x 257 regenerates the operator.  2*1024*65536*257
-    flop each, against the FP32 / FP64 MFMA peaks (157.3 / 78.6 TFLOP/s)."""
+    flop each, against the FP32 / FP64 MFMA peaks (157.3 / 78.6 TFLOP/s).
+    Then the whole device solve x = argmin ||S(Ax - b)|| (the two sketches,
+    FP64 QR of the 1024 x 256 system, triangular solve) and its adjoint."""
     d, m, n = 1024, 65536, 257
     S = rb.DenseSkOp(rb.DenseDist(d, m), rb.RNGState(42))
     fl = 2.0 * d * m * n
@@ -361,13 +363,24 @@ def sketch_solve_leg(rb, torch, dev, reps=5):
         fwd_ms, fk = _time(rb, torch, lambda: rb.lskge3("C", "N", "N", d, n, m, 1.0, S, 0, 0, Ab, m, 0.0, SA, d), reps)
         adj_ms, ak = _time(rb, torch, lambda: rb.lskge3_adjoint("C", "N", d, n, m, 1.0, S, 0, 0, G, d, 0.0, Abar, m),
                            reps)
+        nA = n - 1
+        A, b = Ab[:m * nA], Ab[m * nA:]
+        x = torch.empty(nA, dtype=dt, device=dev)
+        xbar = torch.ones(nA, dtype=dt, device=dev)
+        bbar = torch.empty(m, dtype=dt, device=dev)
+        solve_ms, sk = _time(rb, torch, lambda: rb.sketch_solve("C", d, m, nA, S, 0, 0, A, m, b, x), reps)
+        sadj_ms, sak = _time(rb, torch, lambda: rb.sketch_solve_adjoint("C", d, m, nA, S, 0, 0, A, m, b, xbar,
+                                                                        Abar, m, bbar), reps)
         res[name] = {"fwd_ms": round(fwd_ms, 4), "fwd_tflops": round(fl / (fwd_ms * 1e-3) / 1e12, 2),
                      "fwd_frac_peak": round(fl / (fwd_ms * 1e-3) / 1e12 / peak, 4),
                      "adj_ms": round(adj_ms, 4), "adj_tflops": round(fl / (adj_ms * 1e-3) / 1e12, 2),
                      "adj_frac_peak": round(fl / (adj_ms * 1e-3) / 1e12 / peak, 4),
-                     "fwd_kernels_ms": fk, "adj_kernels_ms": ak}
-        log(f"sketch-and-solve {name}: fwd {fwd_ms:.3f} ms, adjoint {adj_ms:.3f} ms")
-        del Ab, SA, G, Abar
+                     "fwd_kernels_ms": fk, "adj_kernels_ms": ak,
+                     "solve_ms": round(solve_ms, 4), "solve_adjoint_ms": round(sadj_ms, 4),
+                     "solve_kernels_ms": sk, "solve_adjoint_kernels_ms": sak}
+        log(f"sketch-and-solve {name}: fwd {fwd_ms:.3f} ms, adjoint {adj_ms:.3f} ms, "
+            f"solve {solve_ms:.3f} ms, solve adjoint {sadj_ms:.3f} ms")
+        del Ab, SA, G, Abar, x, xbar, bbar
         torch.cuda.synchronize()
     return res
 

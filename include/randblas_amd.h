@@ -440,6 +440,36 @@ rbx_status rbx_dsample_indices_iid(int64_t n, const double *cdf, int64_t k, int6
 rbx_status rbx_sweights_to_cdf(int64_t n, float *w, float error_if_below, void *stream);
 rbx_status rbx_dweights_to_cdf(int64_t n, double *w, double error_if_below, void *stream);
 
+/* ---------------------------------------------------------------- sketch-and-solve
+ * The downstream use of a left sketch: examples/total-least-squares/
+ * tls_dense_skop.cc:138-171 sketches [A b] with sketch_general and solves the
+ * small sketched problem on the host.  These do the sketched least-squares
+ * solve on the device:
+ *   x = argmin_x || S (A x - b) ||_2,   S = submat(S) at (ro_s, co_s), d x m,
+ * A m x n in `layout` (leading dimension lda), b and x device vectors of m and
+ * n entries, d >= n.  SA and Sb come from the fused sketch (lskge3, opS = opA
+ * = 'N'; S is never stored), the QR factorisation of the d x n sketched system
+ * and the triangular solve run in FP64 for both precisions.  A rank-deficient
+ * SA gives non-finite x.  The adjoint returns the gradient of <xbar, x(A, b)>:
+ *   Abar = S^T (r z^T - w x^T),  bbar = S^T w,
+ * with SA = QR, u = R^{-T} xbar, w = Q u, z = R^{-1} u, r = Sb - SA x; Abar is
+ * m x n in `layout` (leading dimension ldab), bbar has m entries.  Both keep
+ * their workspace between calls. */
+rbx_status rbx_ssketch_solve(char layout, int64_t d, int64_t m, int64_t n, const rbx_dense_skop *S,
+                             int64_t ro_s, int64_t co_s, const float *A, int64_t lda, const float *b,
+                             float *x, void *stream);
+rbx_status rbx_dsketch_solve(char layout, int64_t d, int64_t m, int64_t n, const rbx_dense_skop *S,
+                             int64_t ro_s, int64_t co_s, const double *A, int64_t lda, const double *b,
+                             double *x, void *stream);
+rbx_status rbx_ssketch_solve_adjoint(char layout, int64_t d, int64_t m, int64_t n, const rbx_dense_skop *S,
+                                     int64_t ro_s, int64_t co_s, const float *A, int64_t lda,
+                                     const float *b, const float *xbar, float *Abar, int64_t ldab,
+                                     float *bbar, void *stream);
+rbx_status rbx_dsketch_solve_adjoint(char layout, int64_t d, int64_t m, int64_t n, const rbx_dense_skop *S,
+                                     int64_t ro_s, int64_t co_s, const double *A, int64_t lda,
+                                     const double *b, const double *xbar, double *Abar, int64_t ldab,
+                                     double *bbar, void *stream);
+
 /* ---------------------------------------------------------------- diagnostics
  * (no reference counterpart) Kernel timing for benchmarks: while enabled,
  * every kernel the library launches is bracketed by HIP events on its stream.

@@ -177,6 +177,9 @@ _SIG = {
     "rbx_sweights_to_cdf": [_I64, _P, _F, _P],
     "rbx_dweights_to_cdf": [_I64, _P, _D, _P],
     "rbx_drequire_symmetric": [_CH, _P, _I64, _I64, _D, _P],
+    **{f"rbx_{p}sketch_solve": [_CH, _I64, _I64, _I64, _DS, _I64, _I64, _P, _I64, _P, _P, _P] for p in ("s", "d")},
+    **{f"rbx_{p}sketch_solve_adjoint": [_CH, _I64, _I64, _I64, _DS, _I64, _I64, _P, _I64, _P, _P, _P, _I64, _P, _P]
+       for p in ("s", "d")},
     "rbx_profile_enable": [C.c_int],
     "rbx_profile_collect": [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int],
 }
@@ -741,6 +744,22 @@ def weights_to_cdf(n, w, error_if_below=None) -> None:
         import numpy as _np
         error_if_below = -float(_np.sqrt(_np.finfo(_np.float32 if _prefix(w) == "s" else _np.float64).eps))
     _call(f"rbx_{_prefix(w)}weights_to_cdf", n, _ptr(w), error_if_below, _stream())
+
+
+def sketch_solve(layout, d, m, n, S: DenseSkOp, ro_s, co_s, A, lda, b, x) -> None:
+    """x = argmin || submat(S) (A x - b) ||_2 on the device, submat(S) d x m at
+    (ro_s, co_s), A m x n in `layout`: the sketch of [A b] as in
+    examples/total-least-squares/tls_dense_skop.cc:138-171 with a QR solve of
+    the sketched system (FP64 for both precisions)."""
+    _call(f"rbx_{_prefix(x)}sketch_solve", _c(layout), d, m, n, C.byref(S.c()), ro_s, co_s, _ptr(A), lda, _ptr(b),
+          _ptr(x), _stream())
+
+
+def sketch_solve_adjoint(layout, d, m, n, S: DenseSkOp, ro_s, co_s, A, lda, b, xbar, Abar, ldab, bbar) -> None:
+    """Gradient of <xbar, sketch_solve(A, b)> with respect to A (into Abar, m x n
+    in `layout`) and b (into bbar); the operator is regenerated, never stored."""
+    _call(f"rbx_{_prefix(Abar)}sketch_solve_adjoint", _c(layout), d, m, n, C.byref(S.c()), ro_s, co_s, _ptr(A), lda,
+          _ptr(b), _ptr(xbar), _ptr(Abar), ldab, _ptr(bbar), _stream())
 
 
 def profile_enable(on: bool = True) -> None:

@@ -4,8 +4,8 @@ Bars: SASO / LASO index arrays and values bit-exact (LASO vectors with
 duplicates are written in first-occurrence order by both sides); lskges /
 rskges bit-exact with +-1 operators (same per-entry summation order as the
 reference's fallback kernels), FP32 and FP64; rsksp3 bit-exact in the
-reference's dot-product form.  The SASO left sketch of BASELINE's sparse
-configuration (S 2000 x 100000, vec_nnz 8, key 7; A 100000 x 500) runs at
+reference's dot-product form.  The SASO left sketch of the sparse
+benchmark workload (S 2000 x 100000, vec_nnz 8, key 7; A 100000 x 500) runs at
 full size.
 """
 import numpy as np
