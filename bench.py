@@ -184,6 +184,8 @@ def main():
         out["saso_lskges"] = saso_leg(rb, torch, dev)
         out["generator"] = generator_leg(rb, torch, dev)
         out["sketch_and_solve"] = sketch_solve_leg(rb, torch, dev)
+        out["sparse_formats"] = formats_leg(rb, torch, dev)
+        out["sampling"] = sampling_leg(rb, torch, dev)
 
     # ---- CPU: FP64 oracle rows (also the timed CPU baseline), errors, distortion ----
     # All reference linear algebra runs on the host (numpy/OpenBLAS, scipy).
@@ -382,6 +384,85 @@ def sketch_solve_leg(rb, torch, dev, reps=5):
             f"solve {solve_ms:.3f} ms, solve adjoint {sadj_ms:.3f} ms")
         del Ab, SA, G, Abar, x, xbar, bbar
         torch.cuda.synchronize()
+    return res
+
+
+def formats_leg(rb, torch, dev, reps=5):
+    """The sparse-data path around the sketches, on the 200000 x 50000
+    power-law matrix of the sparse leg (FP32 values, int64 indices):
+    conversions CSR -> COO, COO -> CSC and COO -> CSR (stable sorts on the
+    device), and the left sketch lsksp3 of that tall sparse matrix,
+    B = S * A with S = DenseDist(256, 200000) Gaussian (key 42), B 256 x 50000
+    column-major, given as CSR, CSC and COO.  All bound by memory traffic:
+    achieved GB/s of the algorithmic bytes (index and value arrays read and
+    written once; lsksp3: A's arrays, the generated operator's rows it
+    touches, B written once) against HBM."""
+    from randblas_amd.datagen import power_law_csr
+    m, n, d = 200000, 50000, 256
+    rowptr, colidx, vals = power_law_csr(m, n, 20, 2.1, 17)
+    nnz = len(vals)
+    i64 = dict(dtype=torch.int64, device=dev)
+    csr = rb.CSRMatrix(m, n, torch.from_numpy(vals).to(dev), torch.from_numpy(rowptr).to(dev),
+                       torch.from_numpy(colidx).to(dev))
+    coo = rb.COOMatrix(m, n, torch.empty(nnz, dtype=torch.float32, device=dev), torch.empty(nnz, **i64),
+                       torch.empty(nnz, **i64))
+    csc = rb.CSCMatrix(m, n, torch.empty(nnz, dtype=torch.float32, device=dev), torch.empty(nnz, **i64),
+                       torch.empty(n + 1, **i64))
+    csr2 = rb.CSRMatrix(m, n, torch.empty(nnz, dtype=torch.float32, device=dev), torch.empty(m + 1, **i64),
+                        torch.empty(nnz, **i64))
+    res = {"workload": "CSR 200000x50000 power-law (exp 2.1, ~20/row) FP32, int64 indices", "nnz": nnz}
+
+    def rate(name, fn, nbytes):
+        ms, k = _time(rb, torch, fn, reps)
+        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac_hbm": round(nbytes / (ms * 1e-3) / 8e12, 4), "kernels_ms": k}
+        log(f"formats: {name} {ms:.3f} ms")
+
+    entry = 8 + 8 + 4  # row, column, value of a COO entry
+    rate("csr_to_coo", lambda: rb.csr_to_coo(csr, coo), (m + 1) * 8 + nnz * 12 + nnz * entry)
+    rate("coo_to_csc", lambda: rb.coo_to_csc(coo, csc), nnz * entry + nnz * 12 + (n + 1) * 8)
+    rate("coo_to_csr", lambda: rb.coo_to_csr(coo, csr2), nnz * entry + nnz * 12 + (m + 1) * 8)
+    S = rb.DenseSkOp(rb.DenseDist(d, m), rb.RNGState(42))
+    B = torch.empty(d * n, dtype=torch.float32, device=dev)
+    # the operator is generated, not read: its bytes are not counted
+    sk_bytes = {"CSR": (m + 1) * 8 + nnz * 12, "CSC": (n + 1) * 8 + nnz * 12, "COO": nnz * entry}
+    for name, A in (("CSR", csr), ("CSC", csc), ("COO", coo)):
+        rate(f"lsksp3_{name}", lambda A=A: rb.lsksp3("C", "N", "N", d, n, m, 1.0, S, 0, 0, A, 0, 0, 0.0, B, d),
+             sk_bytes[name] + d * n * 4)
+    res["lsksp3_workload"] = "S DenseDist(256, 200000) Gaussian (key 42) * A -> B 256x50000 ColMajor FP32"
+    del csr, coo, csc, csr2, B
+    torch.cuda.synchronize()
+    return res
+
+
+def sampling_leg(rb, torch, dev, reps=5):
+    """Index sampling: 2^27 iid uniform indices in [0, 10^9) (int64 out), the
+    same with FP32 Rademacher signs, and 2^26 draws from a CDF over 10^6
+    indices (weights_to_cdf of power-law weights first, one sequential pass,
+    timed on its own).  Bound by the HBM writes of the samples (8 B per index,
+    +4 B per sign)."""
+    k, kc, nc = 1 << 27, 1 << 26, 1000000
+    res = {"workload": "2^27 uniform indices in [0, 1e9); 2^26 draws from a 1e6-entry CDF"}
+    s = torch.empty(k, dtype=torch.int64, device=dev)
+    r = torch.empty(k, dtype=torch.float32, device=dev)
+    st = rb.RNGState(9)
+
+    def rate(name, fn, nbytes):
+        ms, kk = _time(rb, torch, fn, reps)
+        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac_hbm": round(nbytes / (ms * 1e-3) / 8e12, 4), "kernels_ms": kk}
+        log(f"sampling: {name} {ms:.3f} ms")
+
+    rate("iid_uniform", lambda: rb.sample_indices_iid_uniform(10 ** 9, k, s, st), k * 8)
+    rate("iid_uniform_signed", lambda: rb.sample_indices_iid_uniform(10 ** 9, k, s, st, rademachers=r), k * 12)
+    w = 1.0 / torch.arange(1, nc + 1, dtype=torch.float64, device=dev) ** 1.1
+    cdf = w.clone()
+    rb.weights_to_cdf(nc, cdf)
+    ms_cdf, _ = _time(rb, torch, lambda: (cdf.copy_(w), rb.weights_to_cdf(nc, cdf)), 2)
+    res["weights_to_cdf_ms"] = round(ms_cdf, 4)
+    rate("iid_from_cdf", lambda: rb.sample_indices_iid(nc, cdf, kc, s[:kc], st), kc * 8)
+    del s, r, w, cdf
+    torch.cuda.synchronize()
     return res
 
 

@@ -129,6 +129,42 @@ __global__ __launch_bounds__(256) void fill_dense_kernel(GenMap g, int64_t n_p, 
     }
 }
 
+// The same entries for an output whose unit stride runs along the minor
+// (a) direction, where one thread per counter would write 4 strided values:
+// a workgroup generates a tile of 64 minor indices x 16 counters (64 major
+// indices) into LDS and writes it 64 minor-contiguous values at a time.
+constexpr int FT_A = 64, FT_C = 16, FT_B = 4 * FT_C;
+template <typename T>
+__global__ __launch_bounds__(256) void fill_dense_tr_kernel(GenMap g, int64_t n_p, int64_t n_q, T *buff, int64_t rs,
+                                                            int64_t cs, int64_t cell_first, int64_t n_cells,
+                                                            int64_t n_a) {
+    __shared__ T tile[FT_B][FT_A + 1];
+    const int tid = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * FT_A;            // first minor index of the tile
+    const int64_t c0 = cell_first + (int64_t)blockIdx.y * FT_C;  // first counter cell of the tile
+    const int64_t n_b = g.b_along_rows ? n_p : n_q;
+    const int64_t bbase = 4 * c0 - g.b_off;                    // major index of the tile's first entry
+#pragma unroll
+    for (int it = 0; it < FT_A * FT_C / 256; ++it) {
+        const int e = tid + 256 * it;
+        const int va = e / FT_C, cl = e % FT_C;  // neighbouring threads: neighbouring counters
+        const int64_t v = v0 + va, cell = c0 + cl;
+        if (v < n_a && cell < cell_first + n_cells) {
+            const float4 w = gen4(g.gauss != 0, ctr_add(g.ctr, (uint64_t)((g.a_off + v) * g.P + cell)), g.k0, g.k1);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) tile[4 * cl + t][va] = entry_value<T>(f4_get(w, t), g.gauss != 0);
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < FT_B * FT_A; e += 256) {
+        const int bl = e / FT_A, va = e % FT_A;  // lanes walk the minor (unit-stride) direction
+        const int64_t bb = bbase + bl, v = v0 + va;
+        if (bb < 0 || bb >= n_b || v >= n_a) continue;
+        const int64_t p = g.b_along_rows ? bb : v, q = g.b_along_rows ? v : bb;
+        buff[p * rs + q * cs] = tile[bl][va];
+    }
+}
+
 // Write the generated matrix G (n_p x n_q) to buff[p*rs + q*cs].
 template <typename T>
 void fill_generated(const GenMap &g, int64_t n_p, int64_t n_q, T *buff, int64_t rs, int64_t cs, hipStream_t stream) {
@@ -137,13 +173,21 @@ void fill_generated(const GenMap &g, int64_t n_p, int64_t n_q, T *buff, int64_t 
     int64_t n_a = g.b_along_rows ? n_q : n_p;
     int64_t cell_first = g.b_off / 4;
     int64_t cells = (g.b_off + n_b - 1) / 4 - cell_first + 1;
-    int64_t total = safe_mul(cells, n_a);
-    int64_t blocks = (total + 255) / 256;
-    int64_t cap = (int64_t)num_cus() * 64;
-    if (blocks > cap) blocks = cap;
+    const int64_t stride_a = g.b_along_rows ? cs : rs, stride_b = g.b_along_rows ? rs : cs;
     ProfScope prof("fill_dense_kernel", stream);
-    hipLaunchKernelGGL(fill_dense_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, g, n_p, n_q, buff, rs, cs,
-                       cell_first, cells, n_a);
+    if (stride_a == 1 && stride_b != 1 && n_a > 1) {
+        const int64_t gx = (n_a + FT_A - 1) / FT_A, gy = (cells + FT_C - 1) / FT_C;
+        RBX_REQUIRE(gx < (1ll << 31) && gy <= 65535);
+        hipLaunchKernelGGL(fill_dense_tr_kernel<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, g, n_p,
+                           n_q, buff, rs, cs, cell_first, cells, n_a);
+    } else {
+        int64_t total = safe_mul(cells, n_a);
+        int64_t blocks = (total + 255) / 256;
+        int64_t cap = (int64_t)num_cus() * 64;
+        if (blocks > cap) blocks = cap;
+        hipLaunchKernelGGL(fill_dense_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, g, n_p, n_q, buff, rs,
+                           cs, cell_first, cells, n_a);
+    }
     RBX_HIP(hipGetLastError());
 }
 template void fill_generated<float>(const GenMap &, int64_t, int64_t, float *, int64_t, int64_t, hipStream_t);
@@ -174,14 +218,7 @@ static void fill_dense_impl(char layout, const rbx_dense_dist *D, int64_t n_rows
     int64_t cell_last = (g.b_off + n_b - 1) / 4;
     int64_t cells = cell_last - cell_first + 1;
     int64_t rs = layout == 'C' ? 1 : n_cols, cs = layout == 'C' ? n_rows : 1;
-    int64_t total = safe_mul(cells, n_a);
-    int64_t blocks = (total + 255) / 256;
-    int64_t cap = (int64_t)num_cus() * 64;
-    if (blocks > cap) blocks = cap;
-    ProfScope prof("fill_dense_kernel", stream);
-    hipLaunchKernelGGL(fill_dense_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, g, n_rows,
-                       n_cols, buff, rs, cs, cell_first, cells, n_a);
-    RBX_HIP(hipGetLastError());
+    fill_generated<T>(g, n_rows, n_cols, buff, rs, cs, stream);
     if (next_state) {
         // fill_dense_submat_impl returns seed + ctr_mat_start + n_srows*stride
         // (dense_skops.hh:109-169): ctr_mat_start = a_off*P + b_off/4.

@@ -149,10 +149,11 @@ int num_cus();
 // or generated for the sparse-data products; SCRATCH_SPARSE_G: the row-major
 // copy of the dense operand inside left_spmm_coo; SCRATCH_SPLITK: the K-range
 // partial products of the fused dense sketch; SCRATCH_SOLVE / SCRATCH_SOLVE2:
-// the sketched system of a sketch-and-solve call and its adjoint's vectors).
+// the sketched system of a sketch-and-solve call and its adjoint's vectors;
+// SCRATCH_LONG_ROWS: the segment sums of the sparse row kernels' long rows).
 enum ScratchSlot { SCRATCH_BF16 = 0, SCRATCH_SPARSE_OP = 1, SCRATCH_SPARSE_CSR = 2, SCRATCH_DENSE_OP = 3,
                    SCRATCH_SMALL = 4, SCRATCH_SPARSE_G = 5, SCRATCH_SPLITK = 6, SCRATCH_SOLVE = 7,
-                   SCRATCH_SOLVE2 = 8, SCRATCH_NSLOTS = 9 };
+                   SCRATCH_SOLVE2 = 8, SCRATCH_LONG_ROWS = 9, SCRATCH_NSLOTS = 10 };
 void *scratch(int slot, size_t bytes);
 
 // Write the generated matrix G (n_p x n_q) of a GenMap to buff[p*rs + q*cs].

@@ -9,6 +9,7 @@
 // number of counters the loop touched.  One thread per counter.
 #include <cmath>
 #include <limits>
+#include <type_traits>
 
 #include "rbx_internal.hh"
 
@@ -66,31 +67,97 @@ __global__ __launch_bounds__(256) void iid_cdf_kernel(Ctr128 ctr, uint32_t k0, u
     }
 }
 
-// Sequential, in the reference's order, so the partial sums round exactly as
-// its loop does.  status: 0 ok; 1 + i: w[i] < error_if_below (w[0..i) hold the
-// partial sums, as when the reference throws there); -1: the total is below
-// sqrt(n) * eps (w holds the partial sums, unscaled).
+// The prefix sums in the reference's order, so they round exactly as its
+// loop does: one workgroup streams the weights through LDS in blocks of
+// CDF_BLK (double-buffered: waves 1-3 load block b + 1 while thread 0 adds
+// up block b), thread 0 runs the sequential chain sum = sum + max(w[i], 0)
+// over LDS, and the block's partial sums are written back coalesced.
+// status: 0 ok (total[0] = 1 / sum); 1 + i: w[i] < error_if_below (w[0..i)
+// hold the partial sums, as when the reference throws there); -1: the total
+// is below sqrt(n) * eps (w holds the partial sums, unscaled).
+constexpr int CDF_BLK = 2048;
 template <typename T>
-__global__ void weights_to_cdf_kernel(int64_t n, T *w, T error_if_below, long long *status) {
-    T sum = (T)0;
-    for (int64_t i = 0; i < n; ++i) {
-        T val = w[i];
-        if (!(val >= error_if_below)) {
-            *status = 1 + i;
+__global__ __launch_bounds__(256) void weights_to_cdf_kernel(int64_t n, T *w, T error_if_below, long long *status,
+                                                             T *inv_total) {
+    __shared__ __attribute__((aligned(16))) T buf[2][CDF_BLK];
+    __shared__ int stop_sh;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int64_t nb = (n + CDF_BLK - 1) / CDF_BLK;
+    auto load = [&](int64_t b, T *dst, int t0, int nt) {
+        const int64_t i0 = b * CDF_BLK;
+        for (int e = t0; e < CDF_BLK; e += nt)
+            if (i0 + e < n) dst[e] = w[i0 + e];
+    };
+    if (nb > 0) load(0, buf[0], tid, 256);
+    T sum = (T)0;  // thread 0's running sum
+    for (int64_t b = 0; b < nb; ++b) {
+        T *cur = buf[b & 1];
+        const int64_t i0 = b * CDF_BLK;
+        const int len = (int)(n - i0 < CDF_BLK ? n - i0 : CDF_BLK);
+        if (tid == 0) stop_sh = len;
+        __syncthreads();  // block b loaded; the previous write-back done
+        for (int e = tid; e < len; e += 256)
+            if (!(cur[e] >= error_if_below)) atomicMin(&stop_sh, e);
+        __syncthreads();
+        const int stop = stop_sh;  // entries [0, stop) of the block are summed
+        // val = max(val, 0) (std::max order), in parallel, off the chain
+        for (int e = tid; e < stop; e += 256)
+            if (cur[e] < (T)0) cur[e] = (T)0;
+        __syncthreads();
+        if (wave != 0 && stop == len && b + 1 < nb) load(b + 1, buf[(b + 1) & 1], tid - 64, 192);
+        if (tid == 0) {
+            // the chain alone: 16-byte LDS reads and writes, 8 in flight
+            using V = std::conditional_t<sizeof(T) == 8, double2, float4>;
+            constexpr int PV = 16 / sizeof(T);  // values per vector
+            T s_ = sum;
+            int e = 0;
+            for (; e + 8 * PV <= stop; e += 8 * PV) {
+                V v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = *(const V *)(cur + e + u * PV);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    T *x = reinterpret_cast<T *>(&v[u]);
+#pragma unroll
+                    for (int q = 0; q < PV; ++q) {
+                        s_ = s_ + x[q];
+                        x[q] = s_;
+                    }
+                    *(V *)(cur + e + u * PV) = v[u];
+                }
+            }
+            for (; e < stop; ++e) {
+                s_ = s_ + cur[e];
+                cur[e] = s_;
+            }
+            sum = s_;
+        }
+        __syncthreads();
+        for (int e = tid; e < stop; e += 256) w[i0 + e] = cur[e];
+        if (stop < len) {
+            if (tid == 0) *status = 1 + i0 + stop;
             return;
         }
-        val = val > (T)0 ? val : (T)0;
-        sum = sum + val;
-        w[i] = sum;
     }
-    const T floor_sum = (T)sqrt((double)n) * std::numeric_limits<T>::epsilon();
-    if (!(sum >= floor_sum)) {
-        *status = -1;
-        return;
+    if (tid == 0) {
+        const T floor_sum = (T)sqrt((double)n) * std::numeric_limits<T>::epsilon();
+        if (!(sum >= floor_sum)) {
+            *status = -1;
+        } else {
+            *inv_total = (T)1 / sum;
+            *status = 0;
+        }
     }
-    const T inv = (T)1 / sum;
-    for (int64_t i = 0; i < n; ++i) w[i] = w[i] * inv;
-    *status = 0;
+}
+
+// w[i] = w[i] * (1 / sum) (blas::scal), when the sums succeeded
+template <typename T>
+__global__ __launch_bounds__(256) void cdf_scale_kernel(int64_t n, T *w, const long long *status,
+                                                        const T *inv_total) {
+    if (*status != 0) return;
+    const T inv = *inv_total;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        w[i] = w[i] * inv;
 }
 
 __global__ void i64_zero_kernel(long long *p) { *p = 0; }
@@ -145,9 +212,16 @@ template <typename T>
 static void weights_to_cdf_impl(int64_t n, T *w, T error_if_below, hipStream_t st) {
     RBX_REQUIRE(n >= 0);
     RBX_REQUIRE(n == 0 || w != nullptr);
-    long long *status = (long long *)scratch(SCRATCH_SMALL, sizeof(long long));
+    long long *status = (long long *)scratch(SCRATCH_SMALL, 2 * sizeof(long long));
+    T *inv_total = (T *)(status + 1);
     hipLaunchKernelGGL(i64_zero_kernel, dim3(1), dim3(1), 0, st, status);
-    hipLaunchKernelGGL((weights_to_cdf_kernel<T>), dim3(1), dim3(1), 0, st, n, w, error_if_below, status);
+    hipLaunchKernelGGL((weights_to_cdf_kernel<T>), dim3(1), dim3(256), 0, st, n, w, error_if_below, status,
+                       inv_total);
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks > 0)
+        hipLaunchKernelGGL((cdf_scale_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, n, w,
+                           (const long long *)status, (const T *)inv_total);
     RBX_HIP(hipGetLastError());
     long long h = 0;
     RBX_HIP(hipMemcpyAsync(&h, status, sizeof(h), hipMemcpyDeviceToHost, st));

@@ -18,6 +18,8 @@
 // bit-identical results.  The COO operator is transposed once into row-major
 // CSR (rocPRIM radix sort on the (row, column) key); the apply kernel stages a
 // chunk of op(B) in LDS and lets every thread walk its rows' sorted entries.
+#include <type_traits>
+
 #include "rbx_internal.hh"
 
 #include <math.h>
@@ -323,12 +325,12 @@ __device__ __forceinline__ T vget(const typename VecOf<T>::type &v, int e) {
 // (beta = 0 reads nothing and starts from 0; alpha = 0 leaves RN(beta*c).)
 enum { CRV_JIK = 0, CRV_KIB = 1, CRV_JKI = 2 };
 
-// NW waves per workgroup (4, or 16 when the rows are few and long).
+// NW waves per workgroup (8, or 16 when the rows are few).
 template <typename T, typename IDX, bool COLMAJ, int NW>
 __global__ __launch_bounds__(64 * NW) void csr_rows_vec_kernel(const int64_t *rowptr, const IDX *colidx, const T *vals,
                                                            int64_t mrows, int64_t ncols_out, T alpha, T beta,
                                                            const T *G, int64_t ldg, T *C, int64_t ldc, int form,
-                                                           int cvec) {
+                                                           int cvec, int64_t lrow) {
     using V = typename VecOf<T>::type;
     constexpr int VEC = VecOf<T>::n, CW = 64 * VEC;
     __shared__ T tile[COLMAJ ? CRV_ROWS : 1][COLMAJ ? CW + 1 : 1];
@@ -350,6 +352,8 @@ __global__ __launch_bounds__(64 * NW) void csr_rows_vec_kernel(const int64_t *ro
     for (int rr = wave; rr < CRV_ROWS; rr += NW) {
         const int64_t i = r0 + rr;
         if (i >= mrows) break;
+        const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
+        if (p1 - p0 > lrow) continue;  // a long row: summed in segments elsewhere
         T acc[VEC];
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
@@ -361,7 +365,6 @@ __global__ __launch_bounds__(64 * NW) void csr_rows_vec_kernel(const int64_t *ro
             acc[e] = c0;
         }
         if (alpha != (T)0) {
-            const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
             for (int64_t pb = p0; pb < p1; pb += 64) {
                 const int cnt = (int)(p1 - pb < 64 ? p1 - pb : 64);
                 const int64_t myc = lane < cnt ? (int64_t)colidx[pb + lane] : 0;
@@ -432,9 +435,10 @@ __global__ __launch_bounds__(64 * NW) void csr_rows_vec_kernel(const int64_t *ro
         __syncthreads();
         const int rr = tid & 31;
         const int64_t i = r0 + rr;
+        const bool mine = i < mrows && rowptr[i + 1] - rowptr[i] <= lrow;
         for (int cc = tid >> 5; cc < CW; cc += 2 * NW) {
             const int64_t j = col0 + cc;
-            if (i < mrows && j < ncols_out) C[i + j * ldc] = tile[rr][cc];
+            if (mine && j < ncols_out) C[i + j * ldc] = tile[rr][cc];
         }
     }
 }
@@ -502,13 +506,239 @@ static void coo_to_csr(const int64_t *rows, const int64_t *cols, const T *vals, 
     *val_out = cval;
 }
 
-// Launch csr_rows_vec_kernel over mrows x ncols_out: 16 waves per workgroup
-// when 4-wave workgroups would leave the GPU short of waves.
+// ---------------------------------------------------------------- long rows
+// The row kernels give each row to one wave, which applies the row's entries
+// in stored order: a row of 10^5 entries (the transposed view of a matrix with
+// power-law column degrees) then keeps one wave busy, latency-bound, long
+// after the rest of the GPU has finished.  Rows of more than
+//   lrow = max(256, 4 x the mean row length)
+// entries (so rows of similar lengths, as a sparse operator's, are never
+// split) are cut into segments of at most `seg` entries instead: each segment
+// is summed by its own wave in FP64 (sum of v * g in stored order), and the
+// row's outputs are combined from the segment sums in segment order,
+//   c = RN(RN(beta * c) + RN(alpha * sum_s P_s))   (FP64, then rounded to T).
+// Deterministic; for such rows the rounding differs from the reference's
+// sequential forms by at most the rounding of the sum.  Shorter rows keep the
+// row kernels' exact forms.
+constexpr int64_t LONG_ROW_MIN = 256, LONG_ROW_MEAN_FACTOR = 4, LONG_SEG = 256;
+
+__global__ void long_rows_clear_kernel(int64_t *count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0;
+}
+
+// list[0 .. *count) = the rows longer than lrow (in no particular order)
+__global__ __launch_bounds__(256) void long_rows_find_kernel(const int64_t *rowptr, int64_t mrows, int64_t lrow,
+                                                             int64_t cap, int64_t *count, int64_t *list) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < mrows; i += (int64_t)gridDim.x * blockDim.x)
+        if (rowptr[i + 1] - rowptr[i] > lrow) {
+            const int64_t pos = (int64_t)atomicAdd((unsigned long long *)count, 1ull);
+            if (pos < cap) list[pos] = i;
+        }
+}
+
+// seg_off[k] = sum over list entries before k of ceil(len / seg); one
+// workgroup, 256 entries per pass
+__global__ __launch_bounds__(256) void long_rows_plan_kernel(const int64_t *rowptr, const int64_t *count,
+                                                             const int64_t *list, int64_t cap, int64_t seg,
+                                                             int64_t *seg_off) {
+    __shared__ int64_t sh[256];
+    const int tid = threadIdx.x;
+    const int64_t n = *count < cap ? *count : cap;
+    int64_t carry = 0;
+    for (int64_t base = 0; base < n; base += 256) {
+        const int64_t k = base + tid;
+        int64_t v = 0;
+        if (k < n) {
+            const int64_t row = list[k];
+            v = (rowptr[row + 1] - rowptr[row] + seg - 1) / seg;
+        }
+        sh[tid] = v;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int64_t add = tid >= off ? sh[tid - off] : 0;
+            __syncthreads();
+            sh[tid] += add;
+            __syncthreads();
+        }
+        if (k < n) seg_off[k] = carry + sh[tid] - v;
+        carry += sh[255];
+        __syncthreads();
+    }
+    if (tid == 0) seg_off[n] = carry;
+}
+
+// part[gs * ldp + j] = sum over segment gs's entries of v * G[col, j], in
+// FP64.  Waves take segments grid-stride; a lane covers VEC consecutive
+// columns (VEC > 1: 16-byte rows of G, ncols_out a multiple of VEC) and
+// blockIdx.y picks 64 VEC columns.
+template <typename T, typename IDX, int VEC>
+__global__ __launch_bounds__(256) void long_rows_partial_kernel(const int64_t *rowptr, const IDX *colidx,
+                                                                const T *vals, int64_t ncols_out, const T *G,
+                                                                int64_t ldg, const int64_t *count, const int64_t *list,
+                                                                const int64_t *seg_off, int64_t cap, int64_t nseg_cap,
+                                                                int64_t seg, double *part, int64_t ldp) {
+    using V = std::conditional_t<(VEC > 1), typename VecOf<T>::type, T>;
+    constexpr int NB = VEC > 1 ? 12 : 16;  // gathers in flight per lane
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t n = *count < cap ? *count : cap;
+    if (n == 0) return;
+    const int64_t total = seg_off[n] < nseg_cap ? seg_off[n] : nseg_cap;
+    const int64_t j = ((int64_t)blockIdx.y * 64 + lane) * VEC;
+    const bool jin = j < ncols_out;
+    for (int64_t gs = (int64_t)blockIdx.x * 4 + wave; gs < total; gs += (int64_t)gridDim.x * 4) {
+        int64_t lo = 0, hi = n - 1;  // last list entry whose segments start at or before gs
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (seg_off[mid] <= gs) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t row = list[lo];
+        const int64_t pend = rowptr[row + 1];
+        const int64_t p0 = rowptr[row] + (gs - seg_off[lo]) * seg;
+        const int64_t p1 = p0 + seg < pend ? p0 + seg : pend;
+        double acc[VEC];
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[e] = 0.0;
+        auto load = [&](int64_t c) {
+            V g;
+            if (jin) {
+                g = *(const V *)(G + c * ldg + j);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) reinterpret_cast<T *>(&g)[e] = (T)0;
+            }
+            return g;
+        };
+        auto add = [&](T v, const V &g) {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) acc[e] += (double)v * (double)reinterpret_cast<const T *>(&g)[e];
+        };
+        for (int64_t pb = p0; pb < p1; pb += 64) {
+            const int cnt = (int)(p1 - pb < 64 ? p1 - pb : 64);
+            const int64_t myc = lane < cnt ? (int64_t)colidx[pb + lane] : 0;
+            const T myv = lane < cnt ? vals[pb + lane] : (T)0;
+            int q = 0;
+            for (; q + NB <= cnt; q += NB) {
+                V g[NB];
+                T v[NB];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    v[u] = __shfl(myv, q + u);
+                    g[u] = load(__shfl(myc, q + u));
+                }
+#pragma unroll
+                for (int u = 0; u < NB; ++u) add(v[u], g[u]);
+            }
+            for (; q < cnt; ++q) add(__shfl(myv, q), load(__shfl(myc, q)));
+        }
+        if (jin) {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) part[gs * ldp + j + e] = acc[e];
+        }
+    }
+}
+
+// C[row, j] = RN(RN(beta * C) + RN(alpha * sum of the row's segment sums))
+template <typename T>
+__global__ __launch_bounds__(256) void long_rows_combine_kernel(const int64_t *count, const int64_t *list,
+                                                                const int64_t *seg_off, int64_t cap, int64_t nseg_cap,
+                                                                const double *part, int64_t ldp, int64_t ncols_out,
+                                                                T alpha, T beta, T *C, int64_t crs, int64_t ccs) {
+    const int64_t n = *count < cap ? *count : cap;
+    const int64_t j = (int64_t)blockIdx.y * 256 + threadIdx.x;
+    if (j >= ncols_out) return;
+    for (int64_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const int64_t row = list[k];
+        const int64_t s1 = seg_off[k + 1] < nseg_cap ? seg_off[k + 1] : nseg_cap;
+        double sum = 0.0;
+        int64_t sgi = seg_off[k];
+        for (; sgi + 8 <= s1; sgi += 8) {  // eight loads in flight, added in order
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = part[(sgi + u) * ldp + j];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) sum += x[u];
+        }
+        for (; sgi < s1; ++sgi) sum += part[sgi * ldp + j];
+        T *c = C + row * crs + j * ccs;
+        double out = (double)alpha * sum;
+        if (beta != (T)0) out = (double)beta * (double)(*c) + out;
+        *c = (T)out;
+    }
+}
+
+// The long-row threshold of a row-kernel call (rows longer than it are split).
+static int64_t long_row_threshold(int64_t nnz, int64_t mrows) {
+    const int64_t t = LONG_ROW_MEAN_FACTOR * ((nnz + mrows - 1) / mrows);
+    return t > LONG_ROW_MIN ? t : LONG_ROW_MIN;
+}
+
+// Sets up and launches the long-row path for a row-kernel call; returns the
+// length above which the row kernels must skip a row (INT64_MAX: no split).
+// `nnz` bounds the total number of entries the rows hold.  G's rows are
+// unit-stride (G[c * ldg + j]); C[i, j] is at C[i * crs + j * ccs].
+template <typename T, typename IDX>
+static int64_t launch_long_rows(const int64_t *rowptr, const IDX *colidx, const T *vals, int64_t mrows, int64_t nnz,
+                                int64_t ncols_out, T alpha, T beta, const T *G, int64_t ldg, T *C, int64_t crs,
+                                int64_t ccs, hipStream_t st) {
+    if (alpha == (T)0 || mrows == 0 || ncols_out == 0) return INT64_MAX;
+    const int64_t lrow = long_row_threshold(nnz, mrows);
+    if (nnz <= lrow) return INT64_MAX;  // no row can be long
+    constexpr int VN = VecOf<T>::n;
+    const bool vec = ncols_out % VN == 0 && ldg % VN == 0 && ((uintptr_t)G % 16) == 0;
+    const int64_t cw = vec ? 64 * VN : 64;
+    const int64_t ldp = (ncols_out + cw - 1) / cw * cw;
+    // keep the segment sums within 512 MiB: longer segments for wide outputs
+    int64_t seg = LONG_SEG;
+    const int64_t cap = nnz / (lrow + 1) + 1;  // rows longer than lrow
+    while ((nnz / seg + cap + 1) * ldp * 8 > (512ll << 20)) seg *= 2;
+    const int64_t nseg_cap = nnz / seg + cap + 1;  // their segments
+    const size_t b_list = align256((size_t)cap * 8), b_off = align256((size_t)(cap + 1) * 8);
+    char *ws = (char *)scratch(SCRATCH_LONG_ROWS,
+                               256 + b_list + b_off + (size_t)safe_mul(safe_mul(nseg_cap, ldp), 8));
+    int64_t *count = (int64_t *)ws, *list = (int64_t *)(ws + 256), *seg_off = (int64_t *)(ws + 256 + b_list);
+    double *part = (double *)(ws + 256 + b_list + b_off);
+    ProfScope prof("long_rows", st);
+    hipLaunchKernelGGL(long_rows_clear_kernel, dim3(1), dim3(64), 0, st, count);
+    int64_t fb = (mrows + 255) / 256;
+    if (fb > 4096) fb = 4096;
+    hipLaunchKernelGGL(long_rows_find_kernel, dim3((unsigned)fb), dim3(256), 0, st, rowptr, mrows, lrow, cap, count,
+                       list);
+    hipLaunchKernelGGL(long_rows_plan_kernel, dim3(1), dim3(256), 0, st, rowptr, (const int64_t *)count,
+                       (const int64_t *)list, cap, seg, seg_off);
+    const int64_t ny = ldp / cw;
+    RBX_REQUIRE(ny <= 65535 && (ncols_out + 255) / 256 <= 65535);
+    int64_t px = (nseg_cap + 3) / 4;
+    if (px > 4096) px = 4096;
+    const dim3 pgrid((unsigned)px, (unsigned)ny);
+    if (vec)
+        hipLaunchKernelGGL((long_rows_partial_kernel<T, IDX, VN>), pgrid, dim3(256), 0, st, rowptr, colidx, vals,
+                           ncols_out, G, ldg, (const int64_t *)count, (const int64_t *)list, (const int64_t *)seg_off,
+                           cap, nseg_cap, seg, part, ldp);
+    else
+        hipLaunchKernelGGL((long_rows_partial_kernel<T, IDX, 1>), pgrid, dim3(256), 0, st, rowptr, colidx, vals,
+                           ncols_out, G, ldg, (const int64_t *)count, (const int64_t *)list, (const int64_t *)seg_off,
+                           cap, nseg_cap, seg, part, ldp);
+    int64_t cx = cap < 1024 ? cap : 1024;
+    hipLaunchKernelGGL((long_rows_combine_kernel<T>), dim3((unsigned)cx, (unsigned)((ncols_out + 255) / 256)),
+                       dim3(256), 0, st, (const int64_t *)count, (const int64_t *)list, (const int64_t *)seg_off, cap,
+                       nseg_cap, (const double *)part, ldp, ncols_out, alpha, beta, C, crs, ccs);
+    RBX_HIP(hipGetLastError());
+    return lrow;
+}
+
+// Launch csr_rows_vec_kernel over mrows x ncols_out: 8 waves per workgroup
+// (4 rows per wave: rows of similar length that cluster in one 32-row block
+// are shared by more waves than with 4), 16 when 8-wave workgroups would
+// leave the GPU short of waves.  Rows longer than the long-row threshold go
+// to the segment kernels.
 template <typename T, typename IDX>
 static void launch_rows_vec(bool colmaj, const int64_t *rowptr, const IDX *colidx, const T *vals, int64_t mrows,
-                            int64_t ncols_out, T alpha, T beta, const T *G, int64_t ldg, T *C, int64_t ldc, int form,
-                            int cvec, hipStream_t st) {
+                            int64_t nnz, int64_t ncols_out, T alpha, T beta, const T *G, int64_t ldg, T *C,
+                            int64_t ldc, int form, int cvec, hipStream_t st) {
     constexpr int VEC = VecOf<T>::n;
+    const int64_t lrow = launch_long_rows<T, IDX>(rowptr, colidx, vals, mrows, nnz, ncols_out, alpha, beta, G, ldg, C,
+                                                  colmaj ? 1 : ldc, colmaj ? ldc : 1, st);
     const int64_t nchunk = (ncols_out + 64 * VEC - 1) / (64 * VEC);
     const int64_t nrb = (mrows + CRV_ROWS - 1) / CRV_ROWS;
     RBX_REQUIRE(nchunk <= 65535 && nrb < (1ll << 31));
@@ -516,11 +746,11 @@ static void launch_rows_vec(bool colmaj, const int64_t *rowptr, const IDX *colid
     const bool wide = nrb * nchunk < 4 * (int64_t)num_cus();
 #define RBX_ROWS_VEC(CM, NW)                                                                                  \
     hipLaunchKernelGGL((csr_rows_vec_kernel<T, IDX, CM, NW>), grid, dim3(64 * NW), 0, st, rowptr, colidx, vals, \
-                       mrows, ncols_out, alpha, beta, G, ldg, C, ldc, form, cvec)
+                       mrows, ncols_out, alpha, beta, G, ldg, C, ldc, form, cvec, lrow)
     if (colmaj && wide) RBX_ROWS_VEC(true, 16);
-    else if (colmaj) RBX_ROWS_VEC(true, 4);
+    else if (colmaj) RBX_ROWS_VEC(true, 8);
     else if (wide) RBX_ROWS_VEC(false, 16);
-    else RBX_ROWS_VEC(false, 4);
+    else RBX_ROWS_VEC(false, 8);
 #undef RBX_ROWS_VEC
     RBX_HIP(hipGetLastError());
 }
@@ -596,7 +826,7 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
             Gp = Gs;
         }
         const int cvec = layout == 'R' && ldc % VEC == 0 && ((uintptr_t)C % 16) == 0;
-        launch_rows_vec<T, int32_t>(layout == 'C', row_ptr, ccol, cval, d, n, alpha, beta, Gp, ldg, C, ldc,
+        launch_rows_vec<T, int32_t>(layout == 'C', row_ptr, ccol, cval, d, nnz, n, alpha, beta, Gp, ldg, C, ldc,
                                     kib ? CRV_KIB : CRV_JKI, cvec, st);
         RBX_HIP(hipGetLastError());
     }
@@ -661,11 +891,12 @@ template <typename T>
 __global__ __launch_bounds__(256) void csr_rows_kernel(const int64_t *rowptr, const int64_t *colidx, const T *vals,
                                                        int64_t mrows, int64_t ncols_out, T alpha, T beta, const T *B,
                                                        int64_t brs, int64_t bcs, T *C, int64_t crs, int64_t ccs,
-                                                       int ikb) {
+                                                       int ikb, int64_t lrow) {
     const int lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= mrows) return;
     const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
+    if (p1 - p0 > lrow) return;  // a long row: summed in segments elsewhere
     for (int64_t j = lane; j < ncols_out; j += 64) {
         T *c = C + i * crs + j * ccs;
         T acc = (beta == (T)0) ? (T)0 : beta * (*c);
@@ -820,12 +1051,14 @@ static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, i
         // 256-byte base); a row-major B takes vector stores when aligned too
         const int cvec = layout == 'R' && ldb % VEC == 0 && ((uintptr_t)B % 16) == 0;
         if (d % VEC == 0) {
-            launch_rows_vec<T, int64_t>(layout == 'C', A.ptr, A.idx, (const T *)A.vals, m, d, alpha, beta,
+            launch_rows_vec<T, int64_t>(layout == 'C', A.ptr, A.idx, (const T *)A.vals, m, A.nnz, d, alpha, beta,
                                         (const T *)G, d, B, ldb, kib ? CRV_KIB : CRV_JIK, cvec, st);
         } else {
+            const int64_t lrow = launch_long_rows<T, int64_t>(A.ptr, A.idx, (const T *)A.vals, m, A.nnz, d, alpha,
+                                                              beta, (const T *)G, d, B, crs, ccs, st);
             dim3 grid((unsigned)((m + 3) / 4));
             hipLaunchKernelGGL(csr_rows_kernel<T>, grid, dim3(256), 0, st, A.ptr, A.idx, (const T *)A.vals, m, d,
-                               alpha, beta, (const T *)G, (int64_t)d, (int64_t)1, B, crs, ccs, kib);
+                               alpha, beta, (const T *)G, (int64_t)d, (int64_t)1, B, crs, ccs, kib, lrow);
         }
         RBX_HIP(hipGetLastError());
         return;

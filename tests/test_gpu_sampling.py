@@ -77,3 +77,37 @@ def test_weights_to_cdf_errors_on_device(rb, orc):
     assert host(w).tolist()[:2] == [1.0, 3.0]
     with pytest.raises(rb.Error, match="sum"):
         rb.weights_to_cdf(5, torch.zeros(5, dtype=torch.float32, device=DEV))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [2048, 4097, 3_000_001])
+def test_weights_to_cdf_blocks_bit_exact(rb, orc, dtype, n):
+    """Sizes at, just past and far beyond the device's 2048-entry blocks, with
+    negative-zero, zero and tiny negative weights, against the oracle's
+    sequential sums."""
+    rng = np.random.default_rng(n)
+    w = rng.random(n).astype(dtype) ** 3
+    w[::7] = 0
+    w[1::13] = -0.0
+    w[2::17] = -np.finfo(dtype).eps / 100
+    wd = torch.from_numpy(w.copy()).to(DEV)
+    rb.weights_to_cdf(n, wd)
+    ref, st = orc.weights_to_cdf(w)
+    assert st == 0
+    assert np.array_equal(host(wd), ref)
+
+
+def test_weights_to_cdf_error_in_a_later_block(rb, orc):
+    """A weight below error_if_below at index 5000 (the third block): the
+    entries before it hold the sequential partial sums, the rest are
+    untouched, as when the reference throws there."""
+    rng = np.random.default_rng(8)
+    w = rng.random(7000)
+    w[5000] = -1.0
+    w[6500] = -2.0
+    wd = torch.from_numpy(w.copy()).to(DEV)
+    with pytest.raises(rb.Error, match="w\\[5000\\]"):
+        rb.weights_to_cdf(7000, wd)
+    got = host(wd)
+    assert np.array_equal(got[:5000], np.cumsum(w[:5000]))
+    assert np.array_equal(got[5000:], w[5000:])

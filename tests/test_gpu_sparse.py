@@ -455,3 +455,106 @@ def test_rsksp3_row_kernel_widths(rb, orc, fmt, layout, dtype, d):
     _ref_left_spmm(orc, fmt, "d" if dtype == np.float64 else "s", layout, opA, opS_ref, m, d, n, -1.5, shape, parts,
                    0, 0, Sbuf, lds, 0.25, ref, ldb)
     assert np.array_equal(host(Bd), ref)
+
+
+# ---------------------------------------------------------------- long rows
+# Rows of the row kernels' orientation with more than max(256, 4 x mean)
+# stored entries are summed in FP64 segments (deterministic, not the
+# reference's sequential rounding): shorter rows stay bit-exact against the
+# oracle, the long ones are checked against the FP64 product at the
+# precision bars.
+def _long_row_threshold(nnz, nrows):
+    return max(256, 4 * -(-nnz // nrows))
+
+
+def _csr_long(rng, nrows, ncols, mean, long_rows):
+    """CSR with Poisson(mean) rows, rows in `long_rows` (row -> entries) long;
+    columns random (repeats allowed), N(0,1) values."""
+    deg = rng.poisson(mean, nrows)
+    for r, k in long_rows.items():
+        deg[r] = k
+    rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.int64)
+    cols = rng.integers(0, ncols, int(rowptr[-1])).astype(np.int64)
+    return rowptr, cols, rng.standard_normal(len(cols))
+
+
+def _dense(rowptr, cols, vals, shape):
+    A = np.zeros(shape)
+    np.add.at(A, (np.repeat(np.arange(shape[0]), np.diff(rowptr)), cols), vals.astype(np.float64))
+    return A
+
+
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-6)])
+@pytest.mark.parametrize("d", [64, 37])
+def test_rsksp3_long_rows(rb, orc, layout, dtype, tol, d):
+    """CSR row kernel (vector and scalar widths) with rows of 300, 5000 and
+    40000 entries among Poisson(12) rows."""
+    rng = np.random.default_rng(53)
+    m, n = 400, 20000
+    longs = {3: 300, 10: 5000, 399: 40000}
+    rowptr, colidx, vals = _csr_long(rng, m, n, 12, longs)
+    vals = vals.astype(dtype)
+    S = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(61))
+    A = rb.CSRMatrix(m, n, torch.from_numpy(vals).to(DEV), torch.from_numpy(rowptr).to(DEV),
+                     torch.from_numpy(colidx).to(DEV))
+    ldb = m if layout == "C" else d
+    B0 = rng.standard_normal(m * d).astype(dtype)
+    Bd = torch.from_numpy(B0.copy()).to(DEV)
+    rb.rsksp3(layout, "N", "N", m, d, n, 0.75, A, 0, 0, S, 0, 0, -0.5, Bd, ldb)
+    Sm, _ = orc.fill_dense(n, d, [0, 0, 0, 0], [61, 0], dtype=dtype)
+    nat = orc.natural_layout(n, d, "L")
+    Sbuf = np.ascontiguousarray(Sm.reshape(-1, order="F" if nat == "C" else "C"))
+    opS_ref = "N" if nat == layout else "T"
+    ref = B0.copy()
+    dt = "d" if dtype == np.float64 else "s"
+    orc.left_spmm_csr(dt, layout, opS_ref, m, d, n, 0.75, vals, rowptr, colidx, Sbuf, n if nat == "C" else d, -0.5,
+                      ref, ldb)
+    shape = (d, m) if layout == "C" else (m, d)
+    out = host(Bd).reshape(shape)
+    ref = ref.reshape(shape)
+    out, ref = (out.T, ref.T) if layout == "C" else (out, ref)   # m x d
+    short = np.diff(rowptr) <= _long_row_threshold(len(vals), m)
+    assert short[3] and not short[10] and not short[399]
+    assert np.array_equal(out[short], ref[short])
+    B0m = B0.reshape(shape).T if layout == "C" else B0.reshape(shape)
+    exact = 0.75 * (_dense(rowptr, colidx, vals, (m, n)) @ Sm.astype(np.float64)) - 0.5 * B0m.astype(np.float64)
+    for r in longs:
+        err = np.linalg.norm(out[r].astype(np.float64) - exact[r]) / np.linalg.norm(exact[r])
+        assert err <= tol, (r, err)
+
+
+@pytest.mark.parametrize("fmt", ["csc", "coo"])
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-6)])
+def test_lsksp3_long_columns(rb, orc, fmt, layout, dtype, tol):
+    """lsksp3 of a tall sparse matrix whose first columns hold 30000, 3000
+    and 400 entries (the left sketch of data with power-law column degrees):
+    CSC reaches the row kernel on its transpose, COO the sorted-CSR kernel."""
+    rng = np.random.default_rng(59)
+    m, n, d = 40000, 60, 48
+    longs = {0: 30000, 1: 3000, 7: 400}
+    # built as the CSR of A^T (n x m): its rows are A's columns
+    tptr, trows, tvals = _csr_long(rng, n, m, 8, longs)
+    tvals = tvals.astype(dtype)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+    cols = np.repeat(np.arange(n, dtype=np.int64), np.diff(tptr))
+    if fmt == "csc":
+        A = rb.CSCMatrix(m, n, t(tvals), t(trows), t(tptr))
+    else:
+        perm = rng.permutation(len(tvals))
+        A = rb.COOMatrix(m, n, t(tvals[perm]), t(trows[perm]), t(cols[perm]))
+    S = rb.DenseSkOp(rb.DenseDist(d, m), rb.RNGState(67))
+    ldb = d if layout == "C" else n
+    B0 = rng.standard_normal(d * n).astype(dtype)
+    Bd = torch.from_numpy(B0.copy()).to(DEV)
+    rb.lsksp3(layout, "N", "N", d, n, m, 1.25, S, 0, 0, A, 0, 0, 0.5, Bd, ldb)
+    shape = (n, d) if layout == "C" else (d, n)
+    out = host(Bd).reshape(shape)
+    out = out.T if layout == "C" else out          # d x n
+    Sm, _ = orc.fill_dense(d, m, [0, 0, 0, 0], [67, 0], dtype=dtype)
+    B0m = B0.reshape(shape).T if layout == "C" else B0.reshape(shape)
+    exact = 1.25 * (Sm.astype(np.float64) @ _dense(tptr, trows, tvals, (n, m)).T) + 0.5 * B0m.astype(np.float64)
+    for j in range(n):
+        err = np.linalg.norm(out[:, j].astype(np.float64) - exact[:, j]) / np.linalg.norm(exact[:, j])
+        assert err <= (tol if j in longs else max(tol, 1e-6 if dtype == np.float32 else 1e-14)), (j, err)
