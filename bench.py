@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--err-rows", type=int, default=64, help="rows of B checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip fp64 / error / distortion legs")
+    ap.add_argument("--legs", default="all",
+                    help="comma list of extra legs: fp64,sparse,saso,generator,solve,formats,sampling (default all)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="first run one plain device copy of this many GiB (calibrates traffic counters)")
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 sketch (profiling)")
@@ -150,7 +152,12 @@ def main():
 
     B_h = B.view(N, D).t().cpu().numpy().astype(np.float64)
     B64_h = B64_rows = None
-    if not args.no_extras:
+    all_legs = ("fp64", "sparse", "saso", "generator", "solve", "formats", "sampling")
+    legs = set(all_legs) if args.legs == "all" else set(args.legs.split(","))
+    unknown = legs - set(all_legs)
+    if unknown:
+        ap.error(f"unknown legs: {sorted(unknown)}")
+    if not args.no_extras and "fp64" in legs:
         # ---- plain FP64 fused sketch ----
         log("fp64 leg")
         A64 = A.double()
@@ -179,20 +186,21 @@ def main():
         del A64, B64
         torch.cuda.synchronize()
 
-        # ---- sparse data: B = A_csr * S (rsksp3) and its adjoint w.r.t. A's values ----
-        out["sparse_rsksp3"] = sparse_leg(rb, torch, dev)
-        out["saso_lskges"] = saso_leg(rb, torch, dev)
-        out["generator"] = generator_leg(rb, torch, dev)
-        out["sketch_and_solve"] = sketch_solve_leg(rb, torch, dev)
-        out["sparse_formats"] = formats_leg(rb, torch, dev)
-        out["sampling"] = sampling_leg(rb, torch, dev)
+    if not args.no_extras:
+        # ---- sparse data, sparse operators, generator, consumers ----
+        for key, name, fn in (("sparse", "sparse_rsksp3", sparse_leg), ("saso", "saso_lskges", saso_leg),
+                              ("generator", "generator", generator_leg),
+                              ("solve", "sketch_and_solve", sketch_solve_leg),
+                              ("formats", "sparse_formats", formats_leg), ("sampling", "sampling", sampling_leg)):
+            if key in legs:
+                out[name] = fn(rb, torch, dev)
 
     # ---- CPU: FP64 oracle rows (also the timed CPU baseline), errors, distortion ----
     # All reference linear algebra runs on the host (numpy/OpenBLAS, scipy).
     if not args.no_cpu:
         base, ref_rows, A_h = cpu_baseline(A, args.cpu_rows, np, torch)
         out["cpu_baseline"] = base
-        if not args.no_extras:
+        if not args.no_extras and B64_rows is not None:
             import oracle
             rows = min(args.err_rows, args.cpu_rows)
             Sblk = ref_rows["S"][:rows]
