@@ -42,28 +42,85 @@ __global__ __launch_bounds__(256) void iid_uniform_kernel(Ctr128 ctr, uint32_t k
 }
 
 // u = ((T) uneg11(w) + 1) / 2 in T (uneg11_to_u01<T>, util.hh:474-477), then
-// the first index whose CDF value is not below u (std::lower_bound).
+// the first index whose CDF value is not below u, by std::lower_bound's own
+// probe sequence (half = len / 2, mid = lo + half).  The first CDF_LEVELS
+// levels of that sequence probe at most 2^CDF_LEVELS - 1 positions, the
+// same for every u: each workgroup loads them into LDS as a heap-ordered
+// tree (node t, children 2t for cdf[mid] >= u and 2t + 1 for cdf[mid] < u),
+// so only the remaining levels read global memory; results are identical
+// to the plain search for any array.  The grid is a few workgroups per CU,
+// each looping over counters, so a tree serves many draws; a thread's four
+// searches of a counter run in lockstep, their global probes in flight
+// together.
+constexpr int CDF_LEVELS = 12, CDF_NODES = (1 << CDF_LEVELS) - 1;
 template <typename T>
 __global__ __launch_bounds__(256) void iid_cdf_kernel(Ctr128 ctr, uint32_t k0, uint32_t k1, int64_t n, const T *cdf,
                                                       int64_t k, int64_t *samples) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t i0 = c * 4;
-    if (i0 >= k) return;
-    const uint4 w4 = philox4x32_10(ctr_add(ctr, (uint64_t)c), k0, k1);
-    const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-    for (int q = 0; q < 4 && i0 + q < k; ++q) {
-        const T u = ((T)uneg11(w[q]) + (T)1) / (T)2;
+    __shared__ T tree[CDF_NODES + 1];
+    for (int t = threadIdx.x + 1; t <= CDF_NODES; t += blockDim.x) {
         int64_t lo = 0, len = n;
-        while (len > 0) {
+        const int depth = 31 - __builtin_clz((unsigned)t);
+        for (int bit = depth - 1; bit >= 0; --bit) {
             const int64_t half = len / 2, mid = lo + half;
-            if (cdf[mid] < u) {
+            if ((t >> bit) & 1) {
                 lo = mid + 1;
                 len -= half + 1;
             } else {
                 len = half;
             }
         }
-        samples[i0 + q] = lo;
+        tree[t] = len > 0 ? cdf[lo + len / 2] : (T)0;  // unused when len == 0
+    }
+    __syncthreads();
+    const int64_t ncounters = (k + 3) / 4;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncounters;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i0 = c * 4;
+        const uint4 w4 = philox4x32_10(ctr_add(ctr, (uint64_t)c), k0, k1);
+        const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+        T u[4];
+        int64_t lo[4], len[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            u[q] = ((T)uneg11(w[q]) + (T)1) / (T)2;
+            lo[q] = 0;
+            len[q] = n;
+            int t = 1;
+            for (int lev = 0; lev < CDF_LEVELS && len[q] > 0; ++lev) {
+                const int64_t half = len[q] / 2;
+                if (tree[t] < u[q]) {
+                    lo[q] += half + 1;
+                    len[q] -= half + 1;
+                    t = 2 * t + 1;
+                } else {
+                    len[q] = half;
+                    t = 2 * t;
+                }
+            }
+        }
+        bool more = true;
+        while (more) {
+            more = false;
+            T v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = len[q] > 0 ? cdf[lo[q] + len[q] / 2] : (T)0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (len[q] > 0) {
+                    const int64_t half = len[q] / 2;
+                    if (v[q] < u[q]) {
+                        lo[q] += half + 1;
+                        len[q] -= half + 1;
+                    } else {
+                        len[q] = half;
+                    }
+                    more |= len[q] > 0;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (i0 + q < k) samples[i0 + q] = lo[q];
     }
 }
 
@@ -198,8 +255,8 @@ static void iid_cdf_impl(int64_t n, const T *cdf, int64_t k, int64_t *samples, c
     if (k > 0) {
         RBX_REQUIRE(samples != nullptr && (n == 0 || cdf != nullptr));
         const Ctr128 ctr = ctr_from_words(state->counter);
-        const int64_t blocks = (counters + 255) / 256;
-        RBX_REQUIRE(blocks < (1ll << 31));
+        int64_t blocks = (counters + 255) / 256;
+        if (blocks > 4 * (int64_t)num_cus()) blocks = 4 * (int64_t)num_cus();
         ProfScope prof("iid_cdf_kernel", st);
         hipLaunchKernelGGL((iid_cdf_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, ctr, state->key[0],
                            state->key[1], n, cdf, k, samples);

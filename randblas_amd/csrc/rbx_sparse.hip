@@ -682,16 +682,22 @@ static int64_t launch_long_rows(const int64_t *rowptr, const IDX *colidx, const 
                                 int64_t ncols_out, T alpha, T beta, const T *G, int64_t ldg, T *C, int64_t crs,
                                 int64_t ccs, hipStream_t st) {
     if (alpha == (T)0 || mrows == 0 || ncols_out == 0) return INT64_MAX;
-    const int64_t lrow = long_row_threshold(nnz, mrows);
-    if (nnz <= lrow) return INT64_MAX;  // no row can be long
+    int64_t lrow = long_row_threshold(nnz, mrows);
     constexpr int VN = VecOf<T>::n;
     const bool vec = ncols_out % VN == 0 && ldg % VN == 0 && ((uintptr_t)G % 16) == 0;
     const int64_t cw = vec ? 64 * VN : 64;
     const int64_t ldp = (ncols_out + cw - 1) / cw * cw;
-    // keep the segment sums within 512 MiB: longer segments for wide outputs
+    // keep the segment sums within 512 MiB: longer segments (and a higher
+    // threshold) for wide outputs; no split when even that cannot fit
     int64_t seg = LONG_SEG;
-    const int64_t cap = nnz / (lrow + 1) + 1;  // rows longer than lrow
-    while ((nnz / seg + cap + 1) * ldp * 8 > (512ll << 20)) seg *= 2;
+    auto sums_bytes = [&]() { return ((double)(nnz / seg) + (double)(nnz / (lrow + 1)) + 2.0) * (double)ldp * 8.0; };
+    while (sums_bytes() > (double)(512ll << 20)) {
+        if (seg >= nnz) return INT64_MAX;
+        seg *= 2;
+        if (lrow < seg) lrow = seg;
+    }
+    if (nnz <= lrow) return INT64_MAX;  // no row can be long
+    const int64_t cap = nnz / (lrow + 1) + 1;      // rows longer than lrow
     const int64_t nseg_cap = nnz / seg + cap + 1;  // their segments
     const size_t b_list = align256((size_t)cap * 8), b_off = align256((size_t)(cap + 1) * 8);
     char *ws = (char *)scratch(SCRATCH_LONG_ROWS,

@@ -111,3 +111,33 @@ def test_weights_to_cdf_error_in_a_later_block(rb, orc):
     got = host(wd)
     assert np.array_equal(got[:5000], np.cumsum(w[:5000]))
     assert np.array_equal(got[5000:], w[5000:])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 2046, 2047, 2048, 2049, 1048583])
+def test_cdf_search_sizes_bit_exact(rb, orc, dtype, n):
+    """Sizes around the device's 2047-node probe tree (11 levels in LDS) and
+    far beyond it: indices identical to the oracle's lower_bound, including
+    draws above the last CDF value (a total below 1)."""
+    rng = np.random.default_rng(n + 5)
+    cdf = (np.sort(rng.random(n)) * 0.999).astype(dtype)
+    k = 40001
+    s = torch.empty(k, dtype=torch.int64, device=DEV)
+    rb.sample_indices_iid(n, torch.from_numpy(cdf).to(DEV), k, s, rb.RNGState(key=6, counter=(7, 0, 0, 0)))
+    ref, _ = orc.sample_indices_iid(n, cdf, k, [7, 0, 0, 0], [6, 0])
+    assert np.array_equal(host(s), ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_cdf_search_unsorted_array_bit_exact(rb, orc, dtype):
+    """An array that is not a CDF (random order, NaNs): the device follows
+    lower_bound's probe sequence exactly, as the oracle does."""
+    rng = np.random.default_rng(12)
+    n = 100003
+    a = rng.random(n).astype(dtype)
+    a[::97] = np.nan
+    k = 20000
+    s = torch.empty(k, dtype=torch.int64, device=DEV)
+    rb.sample_indices_iid(n, torch.from_numpy(a).to(DEV), k, s, rb.RNGState(key=3))
+    ref, _ = orc.sample_indices_iid(n, a, k, [0, 0, 0, 0], [3, 0])
+    assert np.array_equal(host(s), ref)
