@@ -10,7 +10,8 @@ never materialised.  One step is one call of the BF16-split sketch
 MFMA accumulation, FP64 combination of the K-splits), inputs resident in HBM.
 
 value = algorithmic 2*d*m*n / time of the whole step (TFLOP/s).  Beside it:
-  * fp64: the plain FP64 fused sketch (rbx_dlskge3 on A promoted to FP64);
+  * fp64 / fp32: the plain FP64 fused sketch (rbx_dlskge3 on A promoted to
+    FP64) and the plain FP32 one (rbx_slskge3);
   * errors of the BF16 result against FP64 products (same bf16 operator, and
     the unrounded operator) on a block of rows, and the subspace-embedding
     distortion of (1/sqrt d) S Q for A = QR;
@@ -32,6 +33,7 @@ KEY_S, KEY_A = 42, 5
 METRIC = "Gaussian sketch SA TFLOP/s & %roofline, BF16-split vs FP64, at stated rel err"
 PEAK_BF16 = 2500.0   # TFLOP/s, MI355X dense BF16 MFMA
 PEAK_FP64 = 78.6     # TFLOP/s, MI355X FP64 MFMA
+PEAK_FP32 = 157.3    # TFLOP/s, MI355X FP32 MFMA
 
 
 PROFILE_MD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -184,6 +186,35 @@ def main():
         log(f"fp64: {ms64:.2f} ms, {tf64:.2f} TFLOP/s, bf16 speedup {ms64 / ms:.1f}x")
 
         del A64, B64
+        torch.cuda.synchronize()
+
+        # ---- plain FP32 fused sketch of the same product ----
+        B32 = torch.empty(D * N, dtype=torch.float32, device=dev)
+        rb.lskge3("C", "N", "N", D, N, M, 1.0, S, 0, 0, A, M, 0.0, B32, D)
+        torch.cuda.synchronize()
+        rb.profile_collect()
+        rb.profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(args.fp64_steps):
+            rb.lskge3("C", "N", "N", D, N, M, 1.0, S, 0, 0, A, M, 0.0, B32, D)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rb.profile_enable(False)
+        p32 = rb.profile_collect()
+        ms32 = (t1 - t0) * 1e3 / args.fp64_steps
+        main32 = "f32_wide_kernel" if "f32_wide_kernel" in p32 else "skge_kernel_f32"
+        c32, k32 = p32.get(main32, (1, float("nan")))
+        B64v = torch.empty(D * N, dtype=torch.float64, device=dev)
+        rb.lskge3("C", "N", "N", D, N, M, 1.0, S, 0, 0, A.double(), M, 0.0, B64v, D)
+        err32 = float(torch.linalg.norm(B32.double() - B64v) / torch.linalg.norm(B64v))
+        del B64v
+        out["fp32"] = {"ms_per_step": round(ms32, 3), "tflops": round(flops / (ms32 * 1e-3) / 1e12, 3),
+                       "kernel": main32, "frac_of_fp32_peak": round(flops / (k32 / c32 * 1e-3) / 1e12 / PEAK_FP32, 4),
+                       "bf16_speedup": round(ms32 / ms, 2), "rel_err_vs_fp64_path": err32,
+                       "per_kernel_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in p32.items()}}
+        log(f"fp32: {ms32:.2f} ms, {flops / (ms32 * 1e-3) / 1e12:.2f} TFLOP/s, rel err vs FP64 path {err32:.2e}, "
+            f"kernels {p32}")
+        del B32
         torch.cuda.synchronize()
 
     if not args.no_extras:

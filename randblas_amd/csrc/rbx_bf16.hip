@@ -418,10 +418,17 @@ constexpr int BF_RB = RBX_BF16_TALL_RB, BF_CB = 8 / BF_RB;  // blocks per wave: 
 #define RBX_BF16_PRIO 1
 #endif
 
+// K depth of the operator chunks the pair kernel generates per barrier.
+#ifndef RBX_BF16_KC
+#define RBX_BF16_KC 32
+#endif
+constexpr int PKC = RBX_BF16_KC, PLDK = PKC + 8, PCPR = PKC / 4;  // PCPR: Philox cells per operator row
+
 template <int TERMS, bool GAUSS, int RB, int CB>
 __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem pb) {
-    constexpr int MT = 32 * RB, NT = 256 * CB, NCELL = MT * 8 / BF_FTHR;  // cells per thread per chunk
-    __shared__ __attribute__((aligned(16))) uint16_t Ss[2][MT * BF_LDK];
+    constexpr int MT = 32 * RB, NT = 256 * CB;
+    constexpr int RPP = BF_FTHR / PCPR, NCELL = MT / RPP;  // rows per generation pass, cells per thread
+    __shared__ __attribute__((aligned(16))) uint16_t Ss[2][MT * PLDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     // XCD-aware tile order: consecutive tiles (same K-split) land on one XCD
@@ -437,7 +444,7 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
     const int64_t kbeg = (int64_t)sp * pb.kw;
     int64_t kend = kbeg + pb.kw;
     if (kend > pb.Kp) kend = pb.Kp;
-    const int nchunks = (int)((kend - kbeg) / BF_KC);
+    const int nchunks = (int)((kend - kbeg) / PKC);
     const GenMap g = pb.g;
 
     f32x16 acc[RB][CB];
@@ -456,14 +463,14 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
     const int64_t bstep_blk = KS * (TERMS * 512);
     const int64_t s_first = kbeg / 16, s_last = kend / 16 - 1;
 
-    // generator: thread owns cell (tid & 7) of operator rows tid / 8 + 64 i
-    const int frow = tid >> 3, fcell = tid & 7;
+    // generator: thread owns cell (tid % PCPR) of operator rows tid / PCPR + RPP i
+    const int frow = tid / PCPR, fcell = tid % PCPR;
     const uint64_t foff = (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
     auto gen_chunk = [&](int c, uint16_t *lds) {
 #pragma unroll
         for (int i = 0; i < NCELL; ++i) {
-            *(uint2 *)(lds + (frow + 64 * i) * BF_LDK + fcell * 4) =
-                gen4_bf16<GAUSS>(ctr_add(g.ctr, foff + (uint64_t)(64 * i) * g.P + 8 * (uint64_t)c), g.k0, g.k1);
+            *(uint2 *)(lds + (frow + RPP * i) * PLDK + fcell * 4) = gen4_bf16<GAUSS>(
+                ctr_add(g.ctr, foff + (uint64_t)(RPP * i) * g.P + (uint64_t)PCPR * (uint64_t)c), g.k0, g.k1);
             __builtin_amdgcn_sched_barrier(0);
         }
     };
@@ -488,7 +495,7 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
         bf16x8 a[RB];
 #pragma unroll
         for (int i = 0; i < RB; ++i)
-            a[i] = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (32 * i + (lane & 31)) * BF_LDK + koff));
+            a[i] = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (32 * i + (lane & 31)) * PLDK + koff));
         // term-outer: consecutive MFMAs on one accumulator are RB x CB apart,
         // and each accumulator still adds its terms in order hi, lo, lo2
 #pragma unroll
@@ -514,11 +521,11 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
     for (int c = 0; c < nchunks; ++c) {
         const uint16_t *S = Ss[c & 1];
         uint16_t *Sn = Ss[(c + 1) & 1];
-        const int64_t s0 = s_first + 2 * c;
+        const int64_t s0 = s_first + (PKC / 16) * c;
         const bool more = c + 1 < nchunks;
         if (more && !late) gen_chunk(c + 1, Sn);
-        step(S, 0, s0 + 1);
-        step(S, 1, s0 + 2 > s_last ? s_last : s0 + 2);
+#pragma unroll
+        for (int ks = 0; ks < PKC / 16; ++ks) step(S, ks, s0 + ks + 1 > s_last ? s_last : s0 + ks + 1);
         if (more && late) gen_chunk(c + 1, Sn);
         __syncthreads();
     }
@@ -639,8 +646,8 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     int64_t want_splits = (4 * (int64_t)num_cus() + tiles - 1) / tiles;
     int64_t kw_fill = (pb.Kp + want_splits - 1) / want_splits;
     if (kw_fill < kw) kw = kw_fill;
-    kw = (kw + BF_KC - 1) / BF_KC * BF_KC;
-    if (kw < BF_KC) kw = BF_KC;
+    kw = (kw + PKC - 1) / PKC * PKC;  // a whole number of the pair kernel's chunks (and of BF_KC)
+    if (kw < PKC) kw = PKC;
     pb.kw = kw;
     pb.nS = (int)((pb.Kp + kw - 1) / kw);
     RBX_REQUIRE((int64_t)pb.nMB * pb.nNB * pb.nS < (1ll << 31));
@@ -679,7 +686,7 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         // interior row blocks on the two-wave kernel when the shape allows
         // it, the rest (or everything) on the bounds-checked kernel
         constexpr int TMT = 32 * BF_RB, TNT = 256 * BF_CB;
-        const bool fast_ok = K % BF_KC == 0 && N % TNT == 0 && !pb.g.b_along_rows && (pb.g.b_off & 3) == 0;
+        const bool fast_ok = K % PKC == 0 && N % TNT == 0 && !pb.g.b_along_rows && (pb.g.b_off & 3) == 0;
         const int nfast = fast_ok ? (int)(M / TMT) : 0;   // row blocks of TMT rows
         const int nMB_all = pb.nMB;                        // row blocks of BF_MT rows
         const int nfast64 = nfast * (TMT / BF_MT);
@@ -719,6 +726,295 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
                            (double)alpha, (double)beta, B, brs, bcs);
         RBX_HIP(hipGetLastError());
     }
+}
+
+// ---------------------------------------------------------------- wide FP32 tiles
+// The plain FP32 fused sketch C = alpha * X * Y + beta * C with X the
+// generated operator (M x K, rows along K) and Y the dense operand (K x N),
+// for large interior shapes: the pair kernel's structure with FP32 operands.
+// Y is packed once into FP32 fragment runs; one 512-thread workgroup per
+// (64-row block, 1024-column block, K-split) generates each 32-deep operator
+// chunk once into LDS (exact FP32 values, as the other kernels) and applies it
+// to all 1024 columns with v_mfma_f32_32x32x2f32, so every operator entry is
+// generated once per 1024 columns instead of once per 256.  Partials go to
+// scratch and are combined by bf16_reduce_kernel (FP64, split order).
+//
+// Packed Y: for 32-column block cb and 16-deep K step s, lane l's eight
+// floats are Y[16 s + 2 q + l / 32][32 cb + l % 32], q = 0..7 (the B
+// operands of the step's eight MFMAs), at ((cb * KS + s) * 64 + l) * 8 + q.
+// The operator chunk in LDS uses the same order within each row: k = 16 st +
+// kk sits at 16 st + 8 (kk % 2) + kk / 2.
+constexpr int FW_LDK = 36;  // 32 k per row + 4 pad (floats)
+static size_t align256b(size_t x) { return (x + 255) / 256 * 256; }
+
+__device__ __forceinline__ int fw_pos(int k) { return (k & ~15) + 8 * (k & 1) + ((k & 15) >> 1); }
+
+// Unit-stride K: one workgroup per (32-column block, 256-deep K range), the
+// tile staged through LDS (as split_kunit_kernel).
+template <bool VEC>
+__global__ __launch_bounds__(256) void pack_f32_kunit_kernel(const float *A, int64_t cs, int64_t K, int64_t N,
+                                                             int64_t Kp, float *packed) {
+    __shared__ __attribute__((aligned(16))) float tile[32][SPL_LD];
+    const int tid = threadIdx.x;
+    const int64_t KS = Kp / 16;
+    const int64_t cb = blockIdx.y;
+    const int64_t kt0 = (int64_t)blockIdx.x * SPL_KT;
+    const int q4 = tid & 63, c0 = tid >> 6;
+    const int64_t k = kt0 + 4 * q4;
+    float4 v[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        const int64_t j = cb * 32 + c0 + 4 * p;
+        const float *src = A + k + j * cs;
+        if (VEC && j < N && k + 3 < K) {
+            v[p] = *(const float4 *)src;
+        } else {
+            v[p].x = (j < N && k < K) ? src[0] : 0.0f;
+            v[p].y = (j < N && k + 1 < K) ? src[1] : 0.0f;
+            v[p].z = (j < N && k + 2 < K) ? src[2] : 0.0f;
+            v[p].w = (j < N && k + 3 < K) ? src[3] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 8; ++p) *(float4 *)&tile[c0 + 4 * p][4 * q4] = v[p];
+    __syncthreads();
+    const int64_t steps = (KS - kt0 / 16) < 16 ? (KS - kt0 / 16) : 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int task = tid + 256 * i;
+        const int st = task >> 6, l = task & 63;
+        if (st >= steps) break;
+        const float *src = &tile[l & 31][16 * st + (l >> 5)];
+        float4 lo, hi;
+        lo.x = src[0]; lo.y = src[2]; lo.z = src[4]; lo.w = src[6];
+        hi.x = src[8]; hi.y = src[10]; hi.z = src[12]; hi.w = src[14];
+        float *dst = packed + ((cb * KS + kt0 / 16 + st) * 64 + l) * 8;
+        *(float4 *)dst = lo;
+        *(float4 *)(dst + 4) = hi;
+    }
+}
+
+// Any strides: one thread per packed float.
+__global__ __launch_bounds__(256) void pack_f32_kernel(const float *A, int64_t rs, int64_t cs, int64_t K, int64_t N,
+                                                       int64_t Kp, int64_t Np, float *packed) {
+    const int64_t KS = Kp / 16, total = (Np / 32) * KS * 512;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int q = (int)(e & 7), l = (int)((e >> 3) & 63);
+        const int64_t run = e >> 9, s = run % KS, cb = run / KS;
+        const int64_t k = 16 * s + 2 * q + (l >> 5), j = 32 * cb + (l & 31);
+        packed[e] = (k < K && j < N) ? A[k * rs + j * cs] : 0.0f;
+    }
+}
+
+struct F32WideProblem {
+    int64_t M, N, Kp;  // M, N multiples of 64 / 1024; Kp of the 32-deep chunk
+    GenMap g;
+    const float *packed;
+    float *part;   // nS x M x N, row-major
+    int64_t kw;    // K per split (multiple of 32)
+    int nMB, nNB, nS;
+};
+
+template <bool GAUSS>
+__global__ __launch_bounds__(BF_FTHR, 1) void f32_wide_kernel(F32WideProblem pb) {
+    constexpr int RB = 2, CB = 4, MT = 64, NT = 1024;
+    __shared__ __attribute__((aligned(16))) float Ss[2][MT * FW_LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int W = pb.nMB * pb.nNB * pb.nS;
+    const int L = blockIdx.x;
+    const int t = (W % 8 == 0) ? (L % 8) * (W / 8) + L / 8 : L;  // XCD-aware: one K-split per XCD run
+    const int mb = t % pb.nMB;
+    const int rest = t / pb.nMB;
+    const int nb = rest % pb.nNB;
+    const int sp = rest / pb.nNB;
+    const int64_t m0 = (int64_t)mb * MT, n0 = (int64_t)nb * NT;
+    const int64_t kbeg = (int64_t)sp * pb.kw;
+    int64_t kend = kbeg + pb.kw;
+    if (kend > pb.Kp) kend = pb.Kp;
+    const int nchunks = (int)((kend - kbeg) / 32);
+    const GenMap g = pb.g;
+
+    f32x16 acc[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int64_t wcol = n0 + wave * (32 * CB);
+    const int64_t KS = pb.Kp / 16;
+    const float *bbase = pb.packed + ((wcol >> 5) * KS * 64 + lane) * 8;
+    const int64_t bblk = KS * 512;  // floats between column blocks
+    const int64_t s_first = kbeg / 16, s_last = kend / 16 - 1;
+
+    // generator: thread owns cell (tid & 7) of operator row tid / 8
+    const int frow = tid >> 3, fcell = tid & 7;
+    const uint64_t foff = (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
+    auto gen_chunk = [&](int c, float *lds) {
+        const float4 v = gen4(GAUSS, ctr_add(g.ctr, foff + 8 * (uint64_t)c), g.k0, g.k1);
+        float *row = lds + frow * FW_LDK;
+        const int k0 = 4 * fcell;  // even: k0, k0 + 2 share the even half; k0 + 1, k0 + 3 the odd half
+        float2 ev, od;
+        ev.x = entry_value<float>(v.x, GAUSS);
+        od.x = entry_value<float>(v.y, GAUSS);
+        ev.y = entry_value<float>(v.z, GAUSS);
+        od.y = entry_value<float>(v.w, GAUSS);
+        *(float2 *)(row + fw_pos(k0)) = ev;
+        *(float2 *)(row + fw_pos(k0 + 1)) = od;
+    };
+
+    // B operands of one step: lo = q 0..3, hi = q 4..7 of block j
+    float4 blo[CB], bhi[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        blo[j] = *(const float4 *)(bbase + j * bblk + s_first * 512);
+        bhi[j] = *(const float4 *)(bbase + j * bblk + s_first * 512 + 4);
+    }
+
+    auto step = [&](const float *S, int ks, int64_t s_next) {
+        __builtin_amdgcn_s_setprio(1);
+        float4 alo[RB], ahi[RB];
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const float *src = S + (32 * i + (lane & 31)) * FW_LDK + 16 * ks + 8 * (lane >> 5);
+            alo[i] = *(const float4 *)src;
+            ahi[i] = *(const float4 *)(src + 4);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int j = 0; j < CB; ++j)
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(f4_get(alo[i], q), f4_get(blo[j], q), acc[i][j],
+                                                                     0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < CB; ++j) blo[j] = *(const float4 *)(bbase + j * bblk + s_next * 512);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int j = 0; j < CB; ++j)
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(f4_get(ahi[i], q), f4_get(bhi[j], q), acc[i][j],
+                                                                     0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < CB; ++j) bhi[j] = *(const float4 *)(bbase + j * bblk + s_next * 512 + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(0);
+    };
+
+    const bool late = (wave & 4) != 0;
+    gen_chunk(0, Ss[0]);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const float *S = Ss[c & 1];
+        float *Sn = Ss[(c + 1) & 1];
+        const int64_t s0 = s_first + 2 * c;
+        const bool more = c + 1 < nchunks;
+        if (more && !late) gen_chunk(c + 1, Sn);
+        step(S, 0, s0 + 1);
+        step(S, 1, s0 + 2 > s_last ? s_last : s0 + 2);
+        if (more && late) gen_chunk(c + 1, Sn);
+        __syncthreads();
+    }
+
+    float *P = pb.part + (int64_t)sp * pb.M * pb.N;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = m0 + i * 32 + out_row32(lane, r);
+                const int64_t gj = wcol + j * 32 + (lane & 31);
+                P[gi * pb.N + gj] = acc[i][j][r];
+            }
+}
+
+// Runs the part of an FP32 fused-sketch problem that the wide kernel covers
+// (rows [0, Mf) x columns [0, Nf), Mf and Nf multiples of 64 and 1024) and
+// returns in rest[] the remaining sub-problems, for the caller's tile kernels
+// (-1: the wide kernel does not apply and nothing ran).
+int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream) {
+    if (!pb.gen_is_x || pb.sbuf != nullptr || pb.g.b_along_rows || (pb.g.b_off & 3) != 0) return -1;
+    if (pb.K < 4096 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 1024) return -1;
+    const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / 1024 * 1024, K = pb.K;
+    F32WideProblem w{};
+    w.M = Mf;
+    w.N = Nf;
+    w.Kp = K;
+    w.g = pb.g;
+    w.nMB = (int)(Mf / 64);
+    w.nNB = (int)(Nf / 1024);
+    int64_t kw = 4096;
+    const int64_t tiles = (int64_t)w.nMB * w.nNB;
+    const int64_t want = (4 * (int64_t)num_cus() + tiles - 1) / tiles;
+    const int64_t kw_fill = (K + want - 1) / want;
+    if (kw_fill < kw) kw = kw_fill;
+    kw = (kw + 31) / 32 * 32;
+    if (kw < 32) kw = 32;
+    w.kw = kw;
+    w.nS = (int)((K + kw - 1) / kw);
+    if ((int64_t)w.nMB * w.nNB * w.nS >= (1ll << 31)) return -1;
+    const size_t packed_bytes = align256b((size_t)safe_mul(K, Nf) * 4);
+    const size_t part_bytes = (size_t)safe_mul(safe_mul(w.nS, Mf), Nf) * 4;
+    char *ws = (char *)scratch(SCRATCH_BF16, packed_bytes + part_bytes);
+    w.packed = (const float *)ws;
+    w.part = (float *)(ws + packed_bytes);
+    const float *Y = (const float *)pb.D;
+    {
+        ProfScope prof("f32_pack_kernel", stream);
+        if (pb.drs == 1) {
+            const int64_t kt = (K + SPL_KT - 1) / SPL_KT;
+            RBX_REQUIRE(kt < (1ll << 31) && Nf / 32 <= 65535);
+            const dim3 grid((unsigned)kt, (unsigned)(Nf / 32));
+            if (pb.dcs % 4 == 0 && ((uintptr_t)Y & 15) == 0)
+                hipLaunchKernelGGL(pack_f32_kunit_kernel<true>, grid, dim3(256), 0, stream, Y, pb.dcs, K, Nf, K,
+                                   (float *)w.packed);
+            else
+                hipLaunchKernelGGL(pack_f32_kunit_kernel<false>, grid, dim3(256), 0, stream, Y, pb.dcs, K, Nf, K,
+                                   (float *)w.packed);
+        } else {
+            hipLaunchKernelGGL(pack_f32_kernel, dim3(65536), dim3(256), 0, stream, Y, pb.drs, pb.dcs, K, Nf, K, Nf,
+                               (float *)w.packed);
+        }
+        RBX_HIP(hipGetLastError());
+    }
+    {
+        ProfScope prof("f32_wide_kernel", stream);
+        const dim3 grid((unsigned)(w.nMB * w.nNB * w.nS));
+        if (w.g.gauss) hipLaunchKernelGGL(f32_wide_kernel<true>, grid, dim3(BF_FTHR), 0, stream, w);
+        else hipLaunchKernelGGL(f32_wide_kernel<false>, grid, dim3(BF_FTHR), 0, stream, w);
+        RBX_HIP(hipGetLastError());
+    }
+    {
+        ProfScope prof("f32_reduce_kernel", stream);
+        const dim3 grid((unsigned)(Nf / 32), (unsigned)(Mf / 32));
+        RBX_REQUIRE(grid.y <= 65535u);
+        hipLaunchKernelGGL(bf16_reduce_kernel, grid, dim3(256), 0, stream, (const float *)w.part, w.nS, Mf, Nf,
+                           pb.alpha, pb.beta, (float *)pb.C, pb.crs, pb.ccs);
+        RBX_HIP(hipGetLastError());
+    }
+    int nr = 0;
+    if (pb.M > Mf) {  // rows [Mf, M) x all columns: the operator's rows advance along a
+        GemmProblem r = pb;
+        r.M = pb.M - Mf;
+        r.g.a_off = pb.g.a_off + Mf;
+        r.C = (float *)pb.C + Mf * pb.crs;
+        rest[nr++] = r;
+    }
+    if (pb.N > Nf) {  // rows [0, Mf) x columns [Nf, N)
+        GemmProblem r = pb;
+        r.M = Mf;
+        r.N = pb.N - Nf;
+        r.D = Y + Nf * pb.dcs;
+        r.C = (float *)pb.C + Nf * pb.ccs;
+        rest[nr++] = r;
+    }
+    return nr;
 }
 
 }  // namespace rbx

@@ -154,4 +154,10 @@ GemmProblem plan_left(char layout, char opS, char opA, int64_t d, int64_t n, int
                       const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, const T *A, int64_t lda, double beta,
                       T *B, int64_t ldb);
 
+// FP32 problems with a generated operator on the X side and large interior
+// shapes: runs the covered part on the wide-tile kernel and returns the
+// remaining sub-problems in rest[] (their count), or -1 when it does not
+// apply (rbx_bf16.hip).
+int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream);
+
 }  // namespace rbx

@@ -366,7 +366,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const T *part, int n
 }
 
 template <typename T>
-static void launch_skge(const GemmProblem &pb_in, hipStream_t stream) {
+static void launch_skge_tiles(const GemmProblem &pb_in, hipStream_t stream) {
     const bool mem = pb_in.sbuf != nullptr;
     ProfScope prof(sizeof(T) == 8 ? "skge_kernel_f64" : "skge_kernel_f32", stream);
     GemmProblem pb = pb_in;
@@ -401,6 +401,22 @@ static void launch_skge(const GemmProblem &pb_in, hipStream_t stream) {
                            pb.nsplit, pb.M, pb.N, pb.alpha, pb.beta, (T *)pb.C, pb.crs, pb.ccs);
         RBX_HIP(hipGetLastError());
     }
+}
+
+// Large FP32 problems with a generated operator take the wide-tile kernel
+// (each operator entry generated once per 1024 columns) for their interior
+// and the tile kernels for the ragged rest.
+template <typename T>
+static void launch_skge(const GemmProblem &pb, hipStream_t stream) {
+    if constexpr (sizeof(T) == 4) {
+        GemmProblem rest[2];
+        const int nr = f32_wide_run(pb, rest, stream);
+        if (nr >= 0) {
+            for (int i = 0; i < nr; ++i) launch_skge_tiles<T>(rest[i], stream);
+            return;
+        }
+    }
+    launch_skge_tiles<T>(pb, stream);
 }
 
 // C = beta * C on an M x N strided matrix (util::safe_scal semantics).

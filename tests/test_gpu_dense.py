@@ -293,3 +293,36 @@ def test_rskge3_split_k(rb, orc, layout, opS):
     ref = B0.copy()
     orc.rskge3("d", layout, "N", opS, m, d, n, 1.5, A, lda, Sshape, [0, 0, 0, 0], [23, 0], ro, co, 0.25, ref, ldb)
     assert rel_fro(strided(host(Bd), m, d, layout, ldb), strided(ref, m, d, layout, ldb)) <= 1e-13
+
+
+# ---------------------------------------------------------------- wide FP32 tiles
+# Large FP32 problems (K >= 4096, K % 32 == 0, >= 64 rows, >= 1024 columns,
+# generated operator with rows along K from a counter boundary) run their
+# interior on the 64 x 1024 wide-tile kernel and the ragged rows / columns on
+# the tile kernels; results within the FP32 bar of the FP64 oracle.
+@pytest.mark.parametrize("layout,opS,opA,d,n,co", [("C", "N", "N", 130, 1030, 4),   # ragged rows and columns
+                                                   ("C", "N", "T", 128, 2048, 0),   # op(A) = T: strided packing
+                                                   ("C", "N", "N", 64, 1024, 3),    # operator offset off a counter
+                                                   ("R", "T", "N", 96, 1100, 0)])
+@pytest.mark.parametrize("fam", ["G", "U"])
+def test_lskge3_fp32_wide_tiles(rb, orc, layout, opS, opA, d, n, co, fam):
+    m = 8224
+    got, ref = run_lskge3(rb, orc, np.float32, layout, opS, opA, d, n, m, 0.75, -0.5, ro=2, co=co, fam=fam)
+    err = rel_fro(got, ref)
+    print(f"wide FP32 {layout}{opS}{opA} {d}x{n}x{m} {fam}: {err:.2e}")
+    assert err <= 1e-5
+
+
+def test_rskge3_fp32_wide_tiles(rb, orc):
+    rng = np.random.default_rng(6)
+    m, d, n = 1100, 80, 8192   # row-major: the generated operator is the left factor of B^T
+    Sshape = (n, d)
+    A = rng.standard_normal(m * n).astype(np.float32)
+    B0 = rng.standard_normal(m * d).astype(np.float32)
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape), rb.RNGState(29))
+    Bd = dev(B0)
+    rb.rskge3("R", "N", "N", m, d, n, 1.0, dev(A), n, S, 0, 0, 1.0, Bd, d)
+    ref = B0.astype(np.float64).copy()
+    orc.rskge3("d", "R", "N", "N", m, d, n, 1.0, A.astype(np.float64), n, Sshape, [0, 0, 0, 0], [29, 0], 0, 0, 1.0,
+               ref, d)
+    assert rel_fro(strided(host(Bd), m, d, "R", d), strided(ref, m, d, "R", d)) <= 1e-5
