@@ -177,12 +177,14 @@ def main():
         p64 = rb.profile_collect()
         ms64 = (t1 - t0) * 1e3 / args.fp64_steps
         tf64 = flops / (ms64 * 1e-3) / 1e12
-        c64, k64 = p64.get("skge_kernel_f64", (1, float("nan")))
+        main64 = "f64_wide_kernel" if "f64_wide_kernel" in p64 else "skge_kernel_f64"
+        c64, k64 = p64.get(main64, (1, float("nan")))
         B64_h = B64.view(N, D).t().cpu().numpy()
         B64_rows = B64_h[:args.cpu_rows].copy()
-        out["fp64"] = {"ms_per_step": round(ms64, 3), "tflops": round(tf64, 3),
+        out["fp64"] = {"ms_per_step": round(ms64, 3), "tflops": round(tf64, 3), "kernel": main64,
                        "frac_of_fp64_peak": round(flops / (k64 / c64 * 1e-3) / 1e12 / PEAK_FP64, 4),
-                       "bf16_speedup": round(ms64 / ms, 2)}
+                       "bf16_speedup": round(ms64 / ms, 2),
+                       "per_kernel_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in p64.items()}}
         log(f"fp64: {ms64:.2f} ms, {tf64:.2f} TFLOP/s, bf16 speedup {ms64 / ms:.1f}x")
 
         del A64, B64

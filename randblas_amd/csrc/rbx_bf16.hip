@@ -547,8 +547,9 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
 // B[i,j] = alpha * sum_s part[s][i*N + j] + beta * B[i*brs + j*bcs], sums in
 // FP64 in split order.  32 x 32 tiles: partials are read along j; a
 // column-major B is written along i after a transpose through LDS.
-__global__ __launch_bounds__(256) void bf16_reduce_kernel(const float *part, int nS, int64_t M, int64_t N,
-                                                          double alpha, double beta, float *B, int64_t brs,
+template <typename PT, typename OT>
+__global__ __launch_bounds__(256) void bf16_reduce_kernel(const PT *part, int nS, int64_t M, int64_t N,
+                                                          double alpha, double beta, OT *B, int64_t brs,
                                                           int64_t bcs) {
     __shared__ double tile[32][33];
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -560,10 +561,10 @@ __global__ __launch_bounds__(256) void bf16_reduce_kernel(const float *part, int
         double v = 0.0;
         if (i < M && j < N) {
             // loads of eight splits are issued together; the sum stays in split order
-            const float *p = part + i * N + j;
+            const PT *p = part + i * N + j;
             int s = 0;
             for (; s + 8 <= nS; s += 8) {
-                float x[8];
+                PT x[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) x[u] = p[(int64_t)(s + u) * total];
 #pragma unroll
@@ -582,10 +583,10 @@ __global__ __launch_bounds__(256) void bf16_reduce_kernel(const float *part, int
         const int64_t j = cm ? j0 + ty + 8 * q : j0 + tx;
         if (i < M && j < N) {
             const double v = cm ? tile[tx][ty + 8 * q] : tile[ty + 8 * q][tx];
-            float *b = B + i * brs + j * bcs;
+            OT *b = B + i * brs + j * bcs;
             double out = alpha * v;
             if (beta != 0.0) out += beta * (double)(*b);
-            *b = (float)out;
+            *b = (OT)out;
         }
     }
 }
@@ -722,7 +723,7 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         dim3 grid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32));
         RBX_REQUIRE(grid.y <= 65535u);
         ProfScope prof("bf16_reduce_kernel", stream);
-        hipLaunchKernelGGL(bf16_reduce_kernel, grid, dim3(256), 0, stream, pb.part, pb.nS, M, N,
+        hipLaunchKernelGGL((bf16_reduce_kernel<float, float>), grid, dim3(256), 0, stream, pb.part, pb.nS, M, N,
                            (double)alpha, (double)beta, B, brs, bcs);
         RBX_HIP(hipGetLastError());
     }
@@ -994,7 +995,7 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
         ProfScope prof("f32_reduce_kernel", stream);
         const dim3 grid((unsigned)(Nf / 32), (unsigned)(Mf / 32));
         RBX_REQUIRE(grid.y <= 65535u);
-        hipLaunchKernelGGL(bf16_reduce_kernel, grid, dim3(256), 0, stream, (const float *)w.part, w.nS, Mf, Nf,
+        hipLaunchKernelGGL((bf16_reduce_kernel<float, float>), grid, dim3(256), 0, stream, (const float *)w.part, w.nS, Mf, Nf,
                            pb.alpha, pb.beta, (float *)pb.C, pb.crs, pb.ccs);
         RBX_HIP(hipGetLastError());
     }
@@ -1012,6 +1013,221 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
         r.N = pb.N - Nf;
         r.D = Y + Nf * pb.dcs;
         r.C = (float *)pb.C + Nf * pb.ccs;
+        rest[nr++] = r;
+    }
+    return nr;
+}
+
+// ---------------------------------------------------------------- wide FP64 tiles
+// The plain FP64 fused sketch on the same plan as f32_wide_kernel, with
+// v_mfma_f64_16x16x4f64: 64 x 512 tiles (the FP64 accumulators of a 64 x 1024
+// tile would not fit), so each operator entry is generated once per 512
+// columns (the tile kernels: once per 256).  Packed Y: for 16-column block jb
+// and 16-deep step s, lane l's four doubles are Y[16 s + 4 q + l / 16][16 jb +
+// l % 16], q = 0..3, at ((jb * KS + s) * 64 + l) * 4 + q; the operator chunk
+// in LDS holds k = 16 st + kk at 16 st + 4 (kk % 4) + kk / 4.  Partials are
+// FP64; K-splits are longer (FP64 chains need no bounding).
+constexpr int DW_LDK = 36;  // doubles per LDS row: 32 k + 4 pad
+
+__device__ __forceinline__ int dw_pos(int k) { return (k & ~15) + 4 * (k & 3) + ((k & 15) >> 2); }
+
+__global__ __launch_bounds__(256) void pack_f64_kernel(const double *A, int64_t rs, int64_t cs, int64_t K, int64_t N,
+                                                       int64_t Kp, int64_t Np, double *packed) {
+    const int64_t KS = Kp / 16, total = (Np / 16) * KS * 256;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int q = (int)(e & 3), l = (int)((e >> 2) & 63);
+        const int64_t run = e >> 8, s = run % KS, jb = run / KS;
+        const int64_t k = 16 * s + 4 * q + (l >> 4), j = 16 * jb + (l & 15);
+        packed[e] = (k < K && j < N) ? A[k * rs + j * cs] : 0.0;
+    }
+}
+
+struct F64WideProblem {
+    int64_t M, N, Kp;
+    GenMap g;
+    const double *packed;
+    double *part;  // nS x M x N, row-major
+    int64_t kw;
+    int nMB, nNB, nS;
+};
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <bool GAUSS>
+__global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb) {
+    constexpr int RB = 4, CB = 4, MT = 64, NT = 512;
+    __shared__ __attribute__((aligned(16))) double Ss[2][MT * DW_LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int W = pb.nMB * pb.nNB * pb.nS;
+    const int L = blockIdx.x;
+    const int t = (W % 8 == 0) ? (L % 8) * (W / 8) + L / 8 : L;
+    const int mb = t % pb.nMB;
+    const int rest = t / pb.nMB;
+    const int nb = rest % pb.nNB;
+    const int sp = rest / pb.nNB;
+    const int64_t m0 = (int64_t)mb * MT, n0 = (int64_t)nb * NT;
+    const int64_t kbeg = (int64_t)sp * pb.kw;
+    int64_t kend = kbeg + pb.kw;
+    if (kend > pb.Kp) kend = pb.Kp;
+    const int nchunks = (int)((kend - kbeg) / 32);
+    const GenMap g = pb.g;
+
+    f64x4 acc[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+    const int64_t wcol = n0 + wave * (16 * CB);
+    const int64_t KS = pb.Kp / 16;
+    const double *bbase = pb.packed + ((wcol >> 4) * KS * 64 + lane) * 4;
+    const int64_t bblk = KS * 256;  // doubles between 16-column blocks
+    const int64_t s_first = kbeg / 16, s_last = kend / 16 - 1;
+
+    const int frow = tid >> 3, fcell = tid & 7;
+    const uint64_t foff = (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
+    auto gen_chunk = [&](int c, double *lds) {
+        const float4 v = gen4(GAUSS, ctr_add(g.ctr, foff + 8 * (uint64_t)c), g.k0, g.k1);
+        double *row = lds + frow * DW_LDK;
+        const int k0 = 4 * fcell;
+        row[dw_pos(k0)] = entry_value<double>(v.x, GAUSS);
+        row[dw_pos(k0 + 1)] = entry_value<double>(v.y, GAUSS);
+        row[dw_pos(k0 + 2)] = entry_value<double>(v.z, GAUSS);
+        row[dw_pos(k0 + 3)] = entry_value<double>(v.w, GAUSS);
+    };
+
+    // B operands of one step: lo = q 0..1, hi = q 2..3 of block j
+    f64x2 blo[CB], bhi[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        blo[j] = *(const f64x2 *)(bbase + j * bblk + s_first * 256);
+        bhi[j] = *(const f64x2 *)(bbase + j * bblk + s_first * 256 + 2);
+    }
+
+    auto step = [&](const double *S, int ks, int64_t s_next) {
+        __builtin_amdgcn_s_setprio(1);
+        f64x2 alo[RB], ahi[RB];
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const double *src = S + (16 * i + (lane & 15)) * DW_LDK + 16 * ks + 4 * (lane >> 4);
+            alo[i] = *(const f64x2 *)src;
+            ahi[i] = *(const f64x2 *)(src + 2);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int j = 0; j < CB; ++j)
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(alo[i][q], blo[j][q], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < CB; ++j) blo[j] = *(const f64x2 *)(bbase + j * bblk + s_next * 256);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int j = 0; j < CB; ++j)
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ahi[i][q], bhi[j][q], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < CB; ++j) bhi[j] = *(const f64x2 *)(bbase + j * bblk + s_next * 256 + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(0);
+    };
+
+    const bool late = (wave & 4) != 0;
+    gen_chunk(0, Ss[0]);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const double *S = Ss[c & 1];
+        double *Sn = Ss[(c + 1) & 1];
+        const int64_t s0 = s_first + 2 * c;
+        const bool more = c + 1 < nchunks;
+        if (more && !late) gen_chunk(c + 1, Sn);
+        step(S, 0, s0 + 1);
+        step(S, 1, s0 + 2 > s_last ? s_last : s0 + 2);
+        if (more && late) gen_chunk(c + 1, Sn);
+        __syncthreads();
+    }
+
+    // element r of a 16 x 16 FP64 block: row l / 16 + 4 r, column l % 16
+    double *P = pb.part + (int64_t)sp * pb.M * pb.N;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gi = m0 + i * 16 + (lane >> 4) + 4 * r;
+                const int64_t gj = wcol + j * 16 + (lane & 15);
+                P[gi * pb.N + gj] = acc[i][j][r];
+            }
+}
+
+int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream) {
+    if (!pb.gen_is_x || pb.sbuf != nullptr || pb.g.b_along_rows || (pb.g.b_off & 3) != 0) return -1;
+    if (pb.K < 4096 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 512) return -1;
+    const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / 512 * 512, K = pb.K;
+    F64WideProblem w{};
+    w.M = Mf;
+    w.N = Nf;
+    w.Kp = K;
+    w.g = pb.g;
+    w.nMB = (int)(Mf / 64);
+    w.nNB = (int)(Nf / 512);
+    int64_t kw = 16384;
+    const int64_t tiles = (int64_t)w.nMB * w.nNB;
+    const int64_t want = (4 * (int64_t)num_cus() + tiles - 1) / tiles;
+    const int64_t kw_fill = (K + want - 1) / want;
+    if (kw_fill < kw) kw = kw_fill;
+    kw = (kw + 31) / 32 * 32;
+    if (kw < 32) kw = 32;
+    w.kw = kw;
+    w.nS = (int)((K + kw - 1) / kw);
+    if ((int64_t)w.nMB * w.nNB * w.nS >= (1ll << 31)) return -1;
+    const size_t packed_bytes = align256b((size_t)safe_mul(K, Nf) * 8);
+    const size_t part_bytes = (size_t)safe_mul(safe_mul(w.nS, Mf), Nf) * 8;
+    char *ws = (char *)scratch(SCRATCH_BF16, packed_bytes + part_bytes);
+    w.packed = (const double *)ws;
+    w.part = (double *)(ws + packed_bytes);
+    const double *Y = (const double *)pb.D;
+    {
+        ProfScope prof("f64_pack_kernel", stream);
+        hipLaunchKernelGGL(pack_f64_kernel, dim3(65536), dim3(256), 0, stream, Y, pb.drs, pb.dcs, K, Nf, K, Nf,
+                           (double *)w.packed);
+        RBX_HIP(hipGetLastError());
+    }
+    {
+        ProfScope prof("f64_wide_kernel", stream);
+        const dim3 grid((unsigned)(w.nMB * w.nNB * w.nS));
+        if (w.g.gauss) hipLaunchKernelGGL(f64_wide_kernel<true>, grid, dim3(BF_FTHR), 0, stream, w);
+        else hipLaunchKernelGGL(f64_wide_kernel<false>, grid, dim3(BF_FTHR), 0, stream, w);
+        RBX_HIP(hipGetLastError());
+    }
+    {
+        ProfScope prof("f64_reduce_kernel", stream);
+        const dim3 grid((unsigned)(Nf / 32), (unsigned)(Mf / 32));
+        RBX_REQUIRE(grid.y <= 65535u);
+        hipLaunchKernelGGL((bf16_reduce_kernel<double, double>), grid, dim3(256), 0, stream, (const double *)w.part,
+                           w.nS, Mf, Nf, pb.alpha, pb.beta, (double *)pb.C, pb.crs, pb.ccs);
+        RBX_HIP(hipGetLastError());
+    }
+    int nr = 0;
+    if (pb.M > Mf) {
+        GemmProblem r = pb;
+        r.M = pb.M - Mf;
+        r.g.a_off = pb.g.a_off + Mf;
+        r.C = (double *)pb.C + Mf * pb.crs;
+        rest[nr++] = r;
+    }
+    if (pb.N > Nf) {
+        GemmProblem r = pb;
+        r.M = Mf;
+        r.N = pb.N - Nf;
+        r.D = Y + Nf * pb.dcs;
+        r.C = (double *)pb.C + Nf * pb.ccs;
         rest[nr++] = r;
     }
     return nr;

@@ -403,18 +403,16 @@ static void launch_skge_tiles(const GemmProblem &pb_in, hipStream_t stream) {
     }
 }
 
-// Large FP32 problems with a generated operator take the wide-tile kernel
-// (each operator entry generated once per 1024 columns) for their interior
-// and the tile kernels for the ragged rest.
+// Large problems with a generated operator take the wide-tile kernels (each
+// operator entry generated once per 1024 (FP32) / 512 (FP64) columns) for
+// their interior and the tile kernels for the ragged rest.
 template <typename T>
 static void launch_skge(const GemmProblem &pb, hipStream_t stream) {
-    if constexpr (sizeof(T) == 4) {
-        GemmProblem rest[2];
-        const int nr = f32_wide_run(pb, rest, stream);
-        if (nr >= 0) {
-            for (int i = 0; i < nr; ++i) launch_skge_tiles<T>(rest[i], stream);
-            return;
-        }
+    GemmProblem rest[2];
+    const int nr = sizeof(T) == 4 ? f32_wide_run(pb, rest, stream) : f64_wide_run(pb, rest, stream);
+    if (nr >= 0) {
+        for (int i = 0; i < nr; ++i) launch_skge_tiles<T>(rest[i], stream);
+        return;
     }
     launch_skge_tiles<T>(pb, stream);
 }

@@ -326,3 +326,17 @@ def test_rskge3_fp32_wide_tiles(rb, orc):
     orc.rskge3("d", "R", "N", "N", m, d, n, 1.0, A.astype(np.float64), n, Sshape, [0, 0, 0, 0], [29, 0], 0, 0, 1.0,
                ref, d)
     assert rel_fro(strided(host(Bd), m, d, "R", d), strided(ref, m, d, "R", d)) <= 1e-5
+
+
+# FP64: 64 x 512 wide tiles for K >= 4096 (K % 32 == 0), >= 64 rows,
+# >= 512 columns; the FP64 bar (1e-13) holds with the FP64 partial sums.
+@pytest.mark.parametrize("layout,opS,opA,d,n,co", [("C", "N", "N", 130, 530, 4),    # ragged rows and columns
+                                                   ("C", "N", "T", 64, 1024, 0),    # op(A) = T
+                                                   ("C", "N", "N", 64, 512, 3),     # operator offset off a counter
+                                                   ("R", "T", "N", 96, 600, 0)])
+@pytest.mark.parametrize("fam", ["G", "U"])
+def test_lskge3_fp64_wide_tiles(rb, orc, layout, opS, opA, d, n, co, fam):
+    got, ref = run_lskge3(rb, orc, np.float64, layout, opS, opA, d, n, 8224, 0.75, -0.5, ro=2, co=co, fam=fam)
+    err = rel_fro(got, ref)
+    print(f"wide FP64 {layout}{opS}{opA} {d}x{n} {fam}: {err:.2e}")
+    assert err <= 1e-13
