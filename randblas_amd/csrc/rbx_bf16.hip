@@ -1028,6 +1028,8 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
 // in LDS holds k = 16 st + kk at 16 st + 4 (kk % 4) + kk / 4.  Partials are
 // FP64; K-splits are longer (FP64 chains need no bounding).
 constexpr int DW_LDK = 36;  // doubles per LDS row: 32 k + 4 pad
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ int dw_pos(int k) { return (k & ~15) + 4 * (k & 3) + ((k & 15) >> 2); }
 
@@ -1042,6 +1044,53 @@ __global__ __launch_bounds__(256) void pack_f64_kernel(const double *A, int64_t 
     }
 }
 
+// Unit-stride K (column-major A): one workgroup per (16-column block, 256-deep
+// K range); each column's 256 doubles are 16 threads' contiguous 16-byte
+// loads, staged through LDS and written as the block's 16 runs.
+constexpr int DPK_LD = 258;  // doubles per LDS row (256 + pad)
+template <bool VEC>
+__global__ __launch_bounds__(256) void pack_f64_kunit_kernel(const double *A, int64_t cs, int64_t K, int64_t N,
+                                                             int64_t Kp, double *packed) {
+    __shared__ __attribute__((aligned(16))) double tile[16][DPK_LD];
+    const int tid = threadIdx.x;
+    const int64_t KS = Kp / 16;
+    const int64_t jb = blockIdx.y;
+    const int64_t kt0 = (int64_t)blockIdx.x * SPL_KT;
+    const int c = tid >> 4, p0 = tid & 15;
+    const int64_t j = jb * 16 + c;
+    f64x2 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int64_t k = kt0 + 2 * (16 * i + p0);
+        const double *src = A + k + j * cs;
+        if (VEC && j < N && k + 1 < K) {
+            v[i] = *(const f64x2 *)src;
+        } else {
+            v[i][0] = (j < N && k < K) ? src[0] : 0.0;
+            v[i][1] = (j < N && k + 1 < K) ? src[1] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) *(f64x2 *)&tile[c][2 * (16 * i + p0)] = v[i];
+    __syncthreads();
+    const int64_t steps = (KS - kt0 / 16) < 16 ? (KS - kt0 / 16) : 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int task = tid + 256 * i;
+        const int st = task >> 6, l = task & 63;
+        if (st >= steps) break;
+        const double *src = &tile[l & 15][16 * st + (l >> 4)];
+        f64x2 lo, hi;
+        lo[0] = src[0];
+        lo[1] = src[4];
+        hi[0] = src[8];
+        hi[1] = src[12];
+        double *dst = packed + ((jb * KS + kt0 / 16 + st) * 64 + l) * 4;
+        *(f64x2 *)dst = lo;
+        *(f64x2 *)(dst + 2) = hi;
+    }
+}
+
 struct F64WideProblem {
     int64_t M, N, Kp;
     GenMap g;
@@ -1051,8 +1100,6 @@ struct F64WideProblem {
     int nMB, nNB, nS;
 };
 
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 template <bool GAUSS>
 __global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb) {
@@ -1195,8 +1242,20 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     const double *Y = (const double *)pb.D;
     {
         ProfScope prof("f64_pack_kernel", stream);
-        hipLaunchKernelGGL(pack_f64_kernel, dim3(65536), dim3(256), 0, stream, Y, pb.drs, pb.dcs, K, Nf, K, Nf,
-                           (double *)w.packed);
+        if (pb.drs == 1) {
+            const int64_t kt = (K + SPL_KT - 1) / SPL_KT;
+            RBX_REQUIRE(kt < (1ll << 31) && Nf / 16 <= 65535);
+            const dim3 grid((unsigned)kt, (unsigned)(Nf / 16));
+            if (pb.dcs % 2 == 0 && ((uintptr_t)Y & 15) == 0)
+                hipLaunchKernelGGL(pack_f64_kunit_kernel<true>, grid, dim3(256), 0, stream, Y, pb.dcs, K, Nf, K,
+                                   (double *)w.packed);
+            else
+                hipLaunchKernelGGL(pack_f64_kunit_kernel<false>, grid, dim3(256), 0, stream, Y, pb.dcs, K, Nf, K,
+                                   (double *)w.packed);
+        } else {
+            hipLaunchKernelGGL(pack_f64_kernel, dim3(65536), dim3(256), 0, stream, Y, pb.drs, pb.dcs, K, Nf, K, Nf,
+                               (double *)w.packed);
+        }
         RBX_HIP(hipGetLastError());
     }
     {
