@@ -406,14 +406,35 @@ static void launch_skge_tiles(const GemmProblem &pb_in, hipStream_t stream) {
 // Large problems with a generated operator take the wide-tile kernels (each
 // operator entry generated once per 1024 (FP32) / 512 (FP64) columns) for
 // their interior and the tile kernels for the ragged rest.
+// The same product described transposed: C^T = Y^T X^T, so a generated
+// operator on the Y side becomes the X side (its orientation flag flips).
+static GemmProblem transposed_problem(const GemmProblem &pb) {
+    GemmProblem t = pb;
+    t.M = pb.N;
+    t.N = pb.M;
+    t.gen_is_x = pb.gen_is_x ? 0 : 1;
+    t.g.b_along_rows = pb.g.b_along_rows ? 0 : 1;
+    t.drs = pb.dcs;
+    t.dcs = pb.drs;
+    t.crs = pb.ccs;
+    t.ccs = pb.crs;
+    if (pb.sbuf) {
+        t.srs = pb.scs;
+        t.scs = pb.srs;
+    }
+    return t;
+}
+
 template <typename T>
 static void launch_skge(const GemmProblem &pb, hipStream_t stream) {
     GemmProblem rest[2];
-    const int nr = sizeof(T) == 4 ? f32_wide_run(pb, rest, stream) : f64_wide_run(pb, rest, stream);
-    if (nr >= 0) {
+    auto wide = [&](const GemmProblem &q) {
+        const int nr = sizeof(T) == 4 ? f32_wide_run(q, rest, stream) : f64_wide_run(q, rest, stream);
         for (int i = 0; i < nr; ++i) launch_skge_tiles<T>(rest[i], stream);
-        return;
-    }
+        return nr >= 0;
+    };
+    if (wide(pb)) return;
+    if (!pb.gen_is_x && pb.sbuf == nullptr && wide(transposed_problem(pb))) return;
     launch_skge_tiles<T>(pb, stream);
 }
 

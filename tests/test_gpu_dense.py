@@ -340,3 +340,32 @@ def test_lskge3_fp64_wide_tiles(rb, orc, layout, opS, opA, d, n, co, fam):
     err = rel_fro(got, ref)
     print(f"wide FP64 {layout}{opS}{opA} {d}x{n} {fam}: {err:.2e}")
     assert err <= 1e-13
+
+
+# Right sketches in column-major (B = A S, S tall): the operator is the right
+# factor, so the product is run transposed (B^T = S^T A^T) on the wide tiles.
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-5), (np.float64, 1e-13)])
+@pytest.mark.parametrize("opS,m", [("N", 2100), ("T", 1030)])
+def test_rskge3_colmajor_wide_tiles(rb, orc, dtype, tol, opS, m):
+    rng = np.random.default_rng(8)
+    d, n = 96, 8224
+    Sshape = (n, d) if opS == "N" else (d, n)
+    lda, ldb = m + 1, m + 2
+    A = rng.standard_normal(lda * n).astype(dtype)
+    B0 = rng.standard_normal(ldb * d).astype(dtype)
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape), rb.RNGState(31))
+    Bd = dev(B0)
+    Ad = dev(A)
+    torch.cuda.synchronize()
+    rb.profile_collect()
+    rb.profile_enable(True)
+    rb.rskge3("C", "N", opS, m, d, n, 0.5, Ad, lda, S, 0, 0, -1.0, Bd, ldb)
+    rb.profile_enable(False)
+    ran = rb.profile_collect()
+    assert ("f32_wide_kernel" if dtype == np.float32 else "f64_wide_kernel") in ran, ran
+    ref = B0.astype(np.float64).copy()
+    orc.rskge3("d", "C", "N", opS, m, d, n, 0.5, A.astype(np.float64), lda, Sshape, [0, 0, 0, 0], [31, 0], 0, 0,
+               -1.0, ref, ldb)
+    err = rel_fro(strided(host(Bd), m, d, "C", ldb), strided(ref, m, d, "C", ldb))
+    print(f"rskge3 C {opS} {m}x{d}x{n} {np.dtype(dtype).name}: {err:.2e}")
+    assert err <= tol
