@@ -746,6 +746,9 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
 // The operator chunk in LDS uses the same order within each row: k = 16 st +
 // kk sits at 16 st + 8 (kk % 2) + kk / 2.
 constexpr int FW_LDK = 36;  // 32 k per row + 4 pad (floats)
+// Largest packed operand + partials the wide kernels take in scratch; bigger
+// problems run on the tile kernels, which need none.
+constexpr size_t WIDE_SCRATCH_MAX = 16ull << 30;
 static size_t align256b(size_t x) { return (x + 255) / 256 * 256; }
 
 __device__ __forceinline__ int fw_pos(int k) { return (k & ~15) + 8 * (k & 1) + ((k & 15) >> 1); }
@@ -962,6 +965,7 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     if ((int64_t)w.nMB * w.nNB * w.nS >= (1ll << 31)) return -1;
     const size_t packed_bytes = align256b((size_t)safe_mul(K, Nf) * 4);
     const size_t part_bytes = (size_t)safe_mul(safe_mul(w.nS, Mf), Nf) * 4;
+    if (packed_bytes + part_bytes > WIDE_SCRATCH_MAX) return -1;  // the tile kernels need no scratch
     char *ws = (char *)scratch(SCRATCH_BF16, packed_bytes + part_bytes);
     w.packed = (const float *)ws;
     w.part = (float *)(ws + packed_bytes);
@@ -1236,6 +1240,7 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     if ((int64_t)w.nMB * w.nNB * w.nS >= (1ll << 31)) return -1;
     const size_t packed_bytes = align256b((size_t)safe_mul(K, Nf) * 8);
     const size_t part_bytes = (size_t)safe_mul(safe_mul(w.nS, Mf), Nf) * 8;
+    if (packed_bytes + part_bytes > WIDE_SCRATCH_MAX) return -1;
     char *ws = (char *)scratch(SCRATCH_BF16, packed_bytes + part_bytes);
     w.packed = (const double *)ws;
     w.part = (double *)(ws + packed_bytes);
