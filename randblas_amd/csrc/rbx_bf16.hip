@@ -36,6 +36,8 @@ namespace rbx {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 constexpr int BF_MT = 64, BF_NT = 1024, BF_KC = 32, BF_LDK = 40, BF_NTHR = 256;
 
@@ -231,26 +233,25 @@ __global__ __launch_bounds__(256) void split_kunit_kernel(const float *A, int64_
         const int st = task >> 6, l = task & 63;
         if (st >= steps) break;
         const float *src = &tile[l & 31][16 * st + 8 * (l >> 5)];
-        uint16_t out[3][8];
+        // pairs rounded by the hardware RNE conversion (v_cvt_pk_bf16_f32,
+        // the same bits as bf16_rne for every non-NaN input), each
+        // remainder exact in FP32
+        f32x2 a[4];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            float a = src[e];
+        for (int p = 0; p < 4; ++p) a[p] = f32x2{src[2 * p], src[2 * p + 1]};
+        u32x4 out[3];
 #pragma unroll
-            for (int t = 0; t < 3; ++t) {
-                const uint16_t h = bf16_rne(a);
-                out[t][e] = h;
-                a = __fsub_rn(a, bf16_to_f32(h));  // exact
+        for (int t = 0; t < 3; ++t) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(a[p], bf16x2));
+                out[t][p] = h;
+                a[p][0] = __fsub_rn(a[p][0], __uint_as_float(h << 16));
+                a[p][1] = __fsub_rn(a[p][1], __uint_as_float(h & 0xffff0000u));
             }
         }
         const int64_t dst = (cb * KS + kt0 / 16 + st) * terms * 512 + l * 8;
-        for (int t = 0; t < terms; ++t) {
-            u32x4 v;
-            v.x = (uint32_t)out[t][0] | ((uint32_t)out[t][1] << 16);
-            v.y = (uint32_t)out[t][2] | ((uint32_t)out[t][3] << 16);
-            v.z = (uint32_t)out[t][4] | ((uint32_t)out[t][5] << 16);
-            v.w = (uint32_t)out[t][6] | ((uint32_t)out[t][7] << 16);
-            *(u32x4 *)(planes + dst + t * 512) = v;
-        }
+        for (int t = 0; t < terms; ++t) *(u32x4 *)(planes + dst + t * 512) = out[t];
     }
 }
 
