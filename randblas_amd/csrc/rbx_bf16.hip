@@ -556,24 +556,40 @@ __global__ __launch_bounds__(256) void bf16_reduce_kernel(const PT *part, int nS
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     const int64_t i0 = (int64_t)blockIdx.y * 32, j0 = (int64_t)blockIdx.x * 32;
     const int64_t total = M * N;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int64_t i = i0 + ty + 8 * q, j = j0 + tx;
-        double v = 0.0;
-        if (i < M && j < N) {
-            // loads of eight splits are issued together; the sum stays in split order
+    {
+        // thread: row r, columns c4 .. c4 + 3, 16-byte loads when whole and
+        // aligned; eight splits' loads are issued together and every sum stays
+        // in split order
+        const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
+        const int64_t i = i0 + r, j = j0 + c4;
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        if (i < M) {
             const PT *p = part + i * N + j;
-            int s = 0;
-            for (; s + 8 <= nS; s += 8) {
-                PT x[8];
+            if (j + 3 < N && N % 4 == 0) {
+                typedef PT pt4 __attribute__((ext_vector_type(4)));
+                int s = 0;
+                for (; s + 8 <= nS; s += 8) {
+                    pt4 x[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) x[u] = p[(int64_t)(s + u) * total];
+                    for (int u = 0; u < 8; ++u) x[u] = *(const pt4 *)(p + (int64_t)(s + u) * total);
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v += (double)x[u];
+                    for (int u = 0; u < 8; ++u)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) v[e] += (double)x[u][e];
+                }
+                for (; s < nS; ++s) {
+                    const pt4 x = *(const pt4 *)(p + (int64_t)s * total);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) v[e] += (double)x[e];
+                }
+            } else {
+                for (int e = 0; e < 4; ++e)
+                    if (j + e < N)
+                        for (int s = 0; s < nS; ++s) v[e] += (double)p[(int64_t)s * total + e];
             }
-            for (; s < nS; ++s) v += (double)p[(int64_t)s * total];
         }
-        tile[ty + 8 * q][tx] = v;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) tile[r][c4 + e] = v[e];
     }
     __syncthreads();
 #pragma unroll
