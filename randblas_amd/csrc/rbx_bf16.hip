@@ -836,7 +836,7 @@ struct F32WideProblem {
     int nMB, nNB, nS;
 };
 
-template <bool GAUSS>
+template <bool GAUSS, bool ROWS>
 __global__ __launch_bounds__(BF_FTHR, 1) void f32_wide_kernel(F32WideProblem pb) {
     constexpr int RB = 2, CB = 4, MT = 64, NT = 1024;
     __shared__ __attribute__((aligned(16))) float Ss[2][MT * FW_LDK];
@@ -869,20 +869,30 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f32_wide_kernel(F32WideProblem pb)
     const int64_t bblk = KS * 512;  // floats between column blocks
     const int64_t s_first = kbeg / 16, s_last = kend / 16 - 1;
 
-    // generator: thread owns cell (tid & 7) of operator row tid / 8
-    const int frow = tid >> 3, fcell = tid & 7;
-    const uint64_t foff = (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
+    // generator, operator rows along K (!ROWS): thread owns cell (tid & 7) of
+    // operator row tid / 8; counters along the rows (ROWS): thread owns rows
+    // 4 (tid & 15) .. + 3 of operator column tid / 16 of the chunk
+    const int frow = ROWS ? 4 * (tid & 15) : tid >> 3, fcell = ROWS ? tid >> 4 : tid & 7;
+    const uint64_t foff = ROWS ? (uint64_t)((g.a_off + kbeg + fcell) * g.P + ((g.b_off + m0) >> 2) + (tid & 15))
+                               : (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
+    const uint64_t fstep = ROWS ? 32 * (uint64_t)g.P : 8;  // counters per chunk
     auto gen_chunk = [&](int c, float *lds) {
-        const float4 v = gen4(GAUSS, ctr_add(g.ctr, foff + 8 * (uint64_t)c), g.k0, g.k1);
-        float *row = lds + frow * FW_LDK;
-        const int k0 = 4 * fcell;  // even: k0, k0 + 2 share the even half; k0 + 1, k0 + 3 the odd half
-        float2 ev, od;
-        ev.x = entry_value<float>(v.x, GAUSS);
-        od.x = entry_value<float>(v.y, GAUSS);
-        ev.y = entry_value<float>(v.z, GAUSS);
-        od.y = entry_value<float>(v.w, GAUSS);
-        *(float2 *)(row + fw_pos(k0)) = ev;
-        *(float2 *)(row + fw_pos(k0 + 1)) = od;
+        const float4 v = gen4(GAUSS, ctr_add(g.ctr, foff + fstep * (uint64_t)c), g.k0, g.k1);
+        if constexpr (ROWS) {
+            const int pos = fw_pos(fcell);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) lds[(frow + e) * FW_LDK + pos] = entry_value<float>(f4_get(v, e), GAUSS);
+        } else {
+            float *row = lds + frow * FW_LDK;
+            const int k0 = 4 * fcell;  // even: k0, k0 + 2 share the even half; k0 + 1, k0 + 3 the odd half
+            float2 ev, od;
+            ev.x = entry_value<float>(v.x, GAUSS);
+            od.x = entry_value<float>(v.y, GAUSS);
+            ev.y = entry_value<float>(v.z, GAUSS);
+            od.y = entry_value<float>(v.w, GAUSS);
+            *(float2 *)(row + fw_pos(k0)) = ev;
+            *(float2 *)(row + fw_pos(k0 + 1)) = od;
+        }
     };
 
     // B operands of one step: lo = q 0..3, hi = q 4..7 of block j
@@ -960,7 +970,7 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f32_wide_kernel(F32WideProblem pb)
 // returns in rest[] the remaining sub-problems, for the caller's tile kernels
 // (-1: the wide kernel does not apply and nothing ran).
 int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream) {
-    if (!pb.gen_is_x || pb.sbuf != nullptr || pb.g.b_along_rows || (pb.g.b_off & 3) != 0) return -1;
+    if (!pb.gen_is_x || pb.sbuf != nullptr || (pb.g.b_off & 3) != 0) return -1;
     if (pb.K < 4096 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 1024) return -1;
     const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / 1024 * 1024, K = pb.K;
     F32WideProblem w{};
@@ -1008,8 +1018,11 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     {
         ProfScope prof("f32_wide_kernel", stream);
         const dim3 grid((unsigned)(w.nMB * w.nNB * w.nS));
-        if (w.g.gauss) hipLaunchKernelGGL(f32_wide_kernel<true>, grid, dim3(BF_FTHR), 0, stream, w);
-        else hipLaunchKernelGGL(f32_wide_kernel<false>, grid, dim3(BF_FTHR), 0, stream, w);
+        const bool rows = w.g.b_along_rows != 0;
+        if (w.g.gauss && rows) hipLaunchKernelGGL((f32_wide_kernel<true, true>), grid, dim3(BF_FTHR), 0, stream, w);
+        else if (w.g.gauss) hipLaunchKernelGGL((f32_wide_kernel<true, false>), grid, dim3(BF_FTHR), 0, stream, w);
+        else if (rows) hipLaunchKernelGGL((f32_wide_kernel<false, true>), grid, dim3(BF_FTHR), 0, stream, w);
+        else hipLaunchKernelGGL((f32_wide_kernel<false, false>), grid, dim3(BF_FTHR), 0, stream, w);
         RBX_HIP(hipGetLastError());
     }
     {
@@ -1024,7 +1037,8 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     if (pb.M > Mf) {  // rows [Mf, M) x all columns: the operator's rows advance along a
         GemmProblem r = pb;
         r.M = pb.M - Mf;
-        r.g.a_off = pb.g.a_off + Mf;
+        if (pb.g.b_along_rows) r.g.b_off = pb.g.b_off + Mf;
+        else r.g.a_off = pb.g.a_off + Mf;
         r.C = (float *)pb.C + Mf * pb.crs;
         rest[nr++] = r;
     }
@@ -1122,7 +1136,7 @@ struct F64WideProblem {
 };
 
 
-template <bool GAUSS>
+template <bool GAUSS, bool ROWS>
 __global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb) {
     constexpr int RB = 4, CB = 4, MT = 64, NT = 512;
     __shared__ __attribute__((aligned(16))) double Ss[2][MT * DW_LDK];
@@ -1153,16 +1167,24 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb)
     const int64_t bblk = KS * 256;  // doubles between 16-column blocks
     const int64_t s_first = kbeg / 16, s_last = kend / 16 - 1;
 
-    const int frow = tid >> 3, fcell = tid & 7;
-    const uint64_t foff = (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
+    const int frow = ROWS ? 4 * (tid & 15) : tid >> 3, fcell = ROWS ? tid >> 4 : tid & 7;
+    const uint64_t foff = ROWS ? (uint64_t)((g.a_off + kbeg + fcell) * g.P + ((g.b_off + m0) >> 2) + (tid & 15))
+                               : (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
+    const uint64_t fstep = ROWS ? 32 * (uint64_t)g.P : 8;
     auto gen_chunk = [&](int c, double *lds) {
-        const float4 v = gen4(GAUSS, ctr_add(g.ctr, foff + 8 * (uint64_t)c), g.k0, g.k1);
-        double *row = lds + frow * DW_LDK;
-        const int k0 = 4 * fcell;
-        row[dw_pos(k0)] = entry_value<double>(v.x, GAUSS);
-        row[dw_pos(k0 + 1)] = entry_value<double>(v.y, GAUSS);
-        row[dw_pos(k0 + 2)] = entry_value<double>(v.z, GAUSS);
-        row[dw_pos(k0 + 3)] = entry_value<double>(v.w, GAUSS);
+        const float4 v = gen4(GAUSS, ctr_add(g.ctr, foff + fstep * (uint64_t)c), g.k0, g.k1);
+        if constexpr (ROWS) {
+            const int pos = dw_pos(fcell);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) lds[(frow + e) * DW_LDK + pos] = entry_value<double>(f4_get(v, e), GAUSS);
+        } else {
+            double *row = lds + frow * DW_LDK;
+            const int k0 = 4 * fcell;
+            row[dw_pos(k0)] = entry_value<double>(v.x, GAUSS);
+            row[dw_pos(k0 + 1)] = entry_value<double>(v.y, GAUSS);
+            row[dw_pos(k0 + 2)] = entry_value<double>(v.z, GAUSS);
+            row[dw_pos(k0 + 3)] = entry_value<double>(v.w, GAUSS);
+        }
     };
 
     // B operands of one step: lo = q 0..1, hi = q 2..3 of block j
@@ -1235,7 +1257,7 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb)
 }
 
 int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream) {
-    if (!pb.gen_is_x || pb.sbuf != nullptr || pb.g.b_along_rows || (pb.g.b_off & 3) != 0) return -1;
+    if (!pb.gen_is_x || pb.sbuf != nullptr || (pb.g.b_off & 3) != 0) return -1;
     if (pb.K < 4096 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 512) return -1;
     const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / 512 * 512, K = pb.K;
     F64WideProblem w{};
@@ -1283,8 +1305,11 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     {
         ProfScope prof("f64_wide_kernel", stream);
         const dim3 grid((unsigned)(w.nMB * w.nNB * w.nS));
-        if (w.g.gauss) hipLaunchKernelGGL(f64_wide_kernel<true>, grid, dim3(BF_FTHR), 0, stream, w);
-        else hipLaunchKernelGGL(f64_wide_kernel<false>, grid, dim3(BF_FTHR), 0, stream, w);
+        const bool rows = w.g.b_along_rows != 0;
+        if (w.g.gauss && rows) hipLaunchKernelGGL((f64_wide_kernel<true, true>), grid, dim3(BF_FTHR), 0, stream, w);
+        else if (w.g.gauss) hipLaunchKernelGGL((f64_wide_kernel<true, false>), grid, dim3(BF_FTHR), 0, stream, w);
+        else if (rows) hipLaunchKernelGGL((f64_wide_kernel<false, true>), grid, dim3(BF_FTHR), 0, stream, w);
+        else hipLaunchKernelGGL((f64_wide_kernel<false, false>), grid, dim3(BF_FTHR), 0, stream, w);
         RBX_HIP(hipGetLastError());
     }
     {
@@ -1299,7 +1324,8 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     if (pb.M > Mf) {
         GemmProblem r = pb;
         r.M = pb.M - Mf;
-        r.g.a_off = pb.g.a_off + Mf;
+        if (pb.g.b_along_rows) r.g.b_off = pb.g.b_off + Mf;
+        else r.g.a_off = pb.g.a_off + Mf;
         r.C = (double *)pb.C + Mf * pb.crs;
         rest[nr++] = r;
     }

@@ -369,3 +369,35 @@ def test_rskge3_colmajor_wide_tiles(rb, orc, dtype, tol, opS, m):
     err = rel_fro(strided(host(Bd), m, d, "C", ldb), strided(ref, m, d, "C", ldb))
     print(f"rskge3 C {opS} {m}x{d}x{n} {np.dtype(dtype).name}: {err:.2e}")
     assert err <= tol
+
+
+# Counters along the output rows (short-axis-major operators; the adjoints
+# S^T G of the natural operators): the wide kernels generate four rows of one
+# K column per Philox counter.
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-5), (np.float64, 1e-13)])
+@pytest.mark.parametrize("fam", ["G", "U"])
+def test_lskge3_short_axis_wide_tiles(rb, orc, dtype, tol, fam):
+    got, ref = run_lskge3(rb, orc, dtype, "C", "N", "N", 130, 1030, 8224, 1.0, 0.25, ro=4, co=8, axis="S", fam=fam)
+    assert rel_fro(got, ref) <= tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-5), (np.float64, 1e-13)])
+def test_lskge3_adjoint_wide_tiles(rb, orc, dtype, tol):
+    rng = np.random.default_rng(10)
+    d, n, m = 4096, 1030, 8200   # Abar (m x n) = S^T (m x d) G (d x n): K = d
+    S = rb.DenseSkOp(rb.DenseDist(d, m), rb.RNGState(37))
+    G = rng.standard_normal(d * n).astype(dtype)
+    A0 = rng.standard_normal(m * n).astype(dtype)
+    Abar = dev(A0)
+    torch.cuda.synchronize()
+    rb.profile_collect()
+    rb.profile_enable(True)
+    rb.lskge3_adjoint("C", "N", d, n, m, 0.5, S, 0, 0, dev(G), d, -2.0, Abar, m)
+    rb.profile_enable(False)
+    ran = rb.profile_collect()
+    assert ("f32_wide_kernel" if dtype == np.float32 else "f64_wide_kernel") in ran, ran
+    Sm, _ = orc.fill_dense(d, m, [0, 0, 0, 0], [37, 0], dtype=dtype)
+    ref = 0.5 * (Sm.astype(np.float64).T @ G.astype(np.float64).reshape(n, d).T) - 2.0 * A0.astype(np.float64).reshape(n, m).T
+    err = rel_fro(host(Abar).reshape(n, m).T, ref)
+    print(f"adjoint wide {np.dtype(dtype).name}: {err:.2e}")
+    assert err <= tol
