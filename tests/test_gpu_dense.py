@@ -401,3 +401,14 @@ def test_lskge3_adjoint_wide_tiles(rb, orc, dtype, tol):
     err = rel_fro(host(Abar).reshape(n, m).T, ref)
     print(f"adjoint wide {np.dtype(dtype).name}: {err:.2e}")
     assert err <= tol
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_wide_tiles_beta_zero_ignores_nan(rb, dtype):
+    """beta = 0 leaves the output unread on the wide-tile path (NaN in, finite out)."""
+    d, n, m = 128, 1024, 8192
+    S = rb.DenseSkOp(rb.DenseDist(d, m), rb.RNGState(3))
+    A = torch.randn(m * n, dtype=dtype, device=DEV, generator=torch.Generator(device=DEV).manual_seed(1))
+    B = torch.full((d * n,), float("nan"), dtype=dtype, device=DEV)
+    rb.lskge3("C", "N", "N", d, n, m, 1.0, S, 0, 0, A, m, 0.0, B, d)
+    assert bool(torch.isfinite(B).all())

@@ -558,3 +558,20 @@ def test_lsksp3_long_columns(rb, orc, fmt, layout, dtype, tol):
     for j in range(n):
         err = np.linalg.norm(out[:, j].astype(np.float64) - exact[:, j]) / np.linalg.norm(exact[:, j])
         assert err <= (tol if j in longs else max(tol, 1e-6 if dtype == np.float32 else 1e-14)), (j, err)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_long_rows_beta_zero_ignores_nan(rb, dtype):
+    """beta = 0 leaves the output of the long-row segments unread."""
+    rng = np.random.default_rng(61)
+    m, n, d = 64, 5000, 32
+    rowptr, colidx, vals = _csr_long(rng, m, n, 4, {5: 20000, 40: 3000})
+    vals = vals.astype(dtype)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    A = rb.CSRMatrix(m, n, torch.from_numpy(vals).to(DEV), torch.from_numpy(rowptr).to(DEV),
+                     torch.from_numpy(colidx).to(DEV))
+    S = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(2))
+    for layout in ("C", "R"):
+        B = torch.full((m * d,), float("nan"), dtype=tdt, device=DEV)
+        rb.rsksp3(layout, "N", "N", m, d, n, 1.0, A, 0, 0, S, 0, 0, 0.0, B, m if layout == "C" else d)
+        assert bool(torch.isfinite(B).all()), layout
