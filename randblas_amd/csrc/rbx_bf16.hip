@@ -419,6 +419,10 @@ constexpr int BF_RB = RBX_BF16_TALL_RB, BF_CB = 8 / BF_RB;  // blocks per wave: 
 #define RBX_BF16_PRIO 1
 #endif
 
+// B-operand buffering depth of the pair kernel, in 16-deep K steps.
+#ifndef RBX_BF16_BD
+#define RBX_BF16_BD 1
+#endif
 // K depth of the operator chunks the pair kernel generates per barrier.
 #ifndef RBX_BF16_KC
 #define RBX_BF16_KC 32
@@ -478,25 +482,36 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
 
     // first B operands, loaded term-major: the order the loop reloads them
     // in, so the wait before each MFMA group covers only that group's load
-    bf16x8 bq[CB][TERMS];
+    // B operands, BD steps deep: step s uses buffer s % BD (BD divides the
+    // chunk's two steps), reloaded right after its MFMAs with step s + BD
+    constexpr int BD = RBX_BF16_BD;
+    static_assert(BD == 1 || BD == 2, "B operand depth");
+    bf16x8 bq[BD][CB][TERMS];
 #pragma unroll
-    for (int tt = 0; tt < TERMS; ++tt)
+    for (int bd = 0; bd < BD; ++bd)
 #pragma unroll
-        for (int j = 0; j < CB; ++j)
-            bq[j][tt] = __builtin_bit_cast(
-                bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + s_first * (TERMS * 512) + tt * 512));
+        for (int tt = 0; tt < TERMS; ++tt)
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                int64_t s_ld = s_first + bd;
+                if (s_ld > s_last) s_ld = s_last;
+                bq[bd][j][tt] = __builtin_bit_cast(
+                    bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + s_ld * (TERMS * 512) + tt * 512));
+            }
 
-    // one 16-deep K step: block j's B operands for the next step are reloaded
-    // right after its MFMAs, a whole step ahead of their use
-    auto step = [&](const uint16_t *S, int ks, int64_t s_next) {
-#if RBX_BF16_PRIO
-        __builtin_amdgcn_s_setprio(RBX_BF16_PRIO);
-#endif
+    // one 16-deep K step: block j's B operands for step s + BD are reloaded
+    // right after its MFMAs, BD steps ahead of their use
+    auto load_a = [&](const uint16_t *S, int ks, bf16x8 *a) {
         const int koff = ks * 16 + 8 * (lane >> 5);
-        bf16x8 a[RB];
 #pragma unroll
         for (int i = 0; i < RB; ++i)
             a[i] = __builtin_bit_cast(bf16x8, *(const u32x4 *)(S + (32 * i + (lane & 31)) * PLDK + koff));
+    };
+    auto step = [&](const bf16x8 *a, int ks, int64_t s_next) {
+        const int bb = ks % BD;
+#if RBX_BF16_PRIO
+        __builtin_amdgcn_s_setprio(RBX_BF16_PRIO);
+#endif
         // term-outer: consecutive MFMAs on one accumulator are RB x CB apart,
         // and each accumulator still adds its terms in order hi, lo, lo2
 #pragma unroll
@@ -505,8 +520,8 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
             for (int j = 0; j < CB; ++j) {
 #pragma unroll
                 for (int i = 0; i < RB; ++i)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bq[j][tt], acc[i][j], 0, 0, 0);
-                bq[j][tt] = __builtin_bit_cast(
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bq[bb][j][tt], acc[i][j], 0, 0, 0);
+                bq[bb][j][tt] = __builtin_bit_cast(
                     bf16x8, *(const u32x4 *)(bbase + j * bstep_blk + s_next * (TERMS * 512) + tt * 512));
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -525,8 +540,13 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
         const int64_t s0 = s_first + (PKC / 16) * c;
         const bool more = c + 1 < nchunks;
         if (more && !late) gen_chunk(c + 1, Sn);
+        // both steps' operator fragments are read from LDS up front, so the
+        // second step's reads overlap the first step's MFMAs
+        bf16x8 a[PKC / 16][RB];
 #pragma unroll
-        for (int ks = 0; ks < PKC / 16; ++ks) step(S, ks, s0 + ks + 1 > s_last ? s_last : s0 + ks + 1);
+        for (int ks = 0; ks < PKC / 16; ++ks) load_a(S, ks, a[ks]);
+#pragma unroll
+        for (int ks = 0; ks < PKC / 16; ++ks) step(a[ks], ks, s0 + ks + BD > s_last ? s_last : s0 + ks + BD);
         if (more && late) gen_chunk(c + 1, Sn);
         __syncthreads();
     }
