@@ -628,6 +628,17 @@ __global__ __launch_bounds__(256) void bf16_reduce_kernel(const PT *part, int nS
     }
 }
 
+template <int CB>
+static void launch_pair(int n_terms, bool gs, dim3 grid, const Bf16Problem &q, hipStream_t stream) {
+    if (n_terms == 1 && gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<1, true, BF_RB, CB>), grid, dim3(BF_FTHR), 0, stream, q);
+    else if (n_terms == 1) hipLaunchKernelGGL((bf16_skge_pair_kernel<1, false, BF_RB, CB>), grid, dim3(BF_FTHR), 0, stream, q);
+    else if (n_terms == 2 && gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<2, true, BF_RB, CB>), grid, dim3(BF_FTHR), 0, stream, q);
+    else if (n_terms == 2) hipLaunchKernelGGL((bf16_skge_pair_kernel<2, false, BF_RB, CB>), grid, dim3(BF_FTHR), 0, stream, q);
+    else if (gs) hipLaunchKernelGGL((bf16_skge_pair_kernel<3, true, BF_RB, CB>), grid, dim3(BF_FTHR), 0, stream, q);
+    else hipLaunchKernelGGL((bf16_skge_pair_kernel<3, false, BF_RB, CB>), grid, dim3(BF_FTHR), 0, stream, q);
+    RBX_HIP(hipGetLastError());
+}
+
 static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, float alpha,
                               const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, const float *A, int64_t lda,
                               float beta, float *B, int64_t ldb, int n_terms, hipStream_t stream) {
@@ -723,27 +734,29 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         ProfScope prof("bf16_skge_kernel", stream);
         // interior row blocks on the two-wave kernel when the shape allows
         // it, the rest (or everything) on the bounds-checked kernel
-        constexpr int TMT = 32 * BF_RB, TNT = 256 * BF_CB;
-        const bool fast_ok = K % PKC == 0 && N % TNT == 0 && !pb.g.b_along_rows && (pb.g.b_off & 3) == 0;
+        // the widest tile (256 CB columns, CB = BF_CB, BF_CB / 2, BF_CB / 4)
+        // that divides N: narrow sketches keep the two-wave kernel
+        constexpr int TMT = 32 * BF_RB;
+        int cb = 0;
+        for (int c = BF_CB; c >= 1; c /= 2)
+            if (N % (256 * c) == 0) {
+                cb = c;
+                break;
+            }
+        const bool fast_ok = K % PKC == 0 && cb > 0 && !pb.g.b_along_rows && (pb.g.b_off & 3) == 0;
         const int nfast = fast_ok ? (int)(M / TMT) : 0;   // row blocks of TMT rows
         const int nMB_all = pb.nMB;                        // row blocks of BF_MT rows
         const int nfast64 = nfast * (TMT / BF_MT);
         if (nfast > 0) {
             Bf16Problem q = pb;
             q.nMB = nfast;
-            q.nNB = (int)(N / TNT);
+            q.nNB = (int)(N / (256 * cb));
             q.mb0 = 0;
             dim3 grid((unsigned)(q.nMB * q.nNB * q.nS));
             const bool gs = q.g.gauss != 0;
-#define RBX_BF16_MAIN bf16_skge_pair_kernel
-            if (n_terms == 1 && gs) hipLaunchKernelGGL((RBX_BF16_MAIN<1, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (n_terms == 1) hipLaunchKernelGGL((RBX_BF16_MAIN<1, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (n_terms == 2 && gs) hipLaunchKernelGGL((RBX_BF16_MAIN<2, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (n_terms == 2) hipLaunchKernelGGL((RBX_BF16_MAIN<2, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-            else if (gs) hipLaunchKernelGGL((RBX_BF16_MAIN<3, true, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-            else hipLaunchKernelGGL((RBX_BF16_MAIN<3, false, BF_RB, BF_CB>), grid, dim3(BF_FTHR), 0, stream, q);
-#undef RBX_BF16_MAIN
-            RBX_HIP(hipGetLastError());
+            if (cb == BF_CB) launch_pair<BF_CB>(n_terms, gs, grid, q, stream);
+            else if (cb == BF_CB / 2) launch_pair<(BF_CB / 2 > 0 ? BF_CB / 2 : 1)>(n_terms, gs, grid, q, stream);
+            else launch_pair<(BF_CB / 4 > 0 ? BF_CB / 4 : 1)>(n_terms, gs, grid, q, stream);
         }
         if (nMB_all > nfast64) {
             Bf16Problem q = pb;

@@ -118,3 +118,24 @@ def test_materialised_operator_rejected(rb):
     B = torch.zeros(8 * 4, dtype=torch.float32, device=DEV)
     with pytest.raises(rb.Unsupported):
         rb.lskge3_bf16split("C", "N", "N", 8, 4, 16, 1.0, S, 0, 0, A, 16, 0.0, B, 8)
+
+
+@pytest.mark.parametrize("n", [256, 512, 768])
+def test_narrow_tiles_identity_exact(rb, orc, n):
+    """Sketch widths that are multiples of 256 but not of 1024 run the
+    two-wave kernel with 256- or 512-column tiles: bf16(S) exact through an
+    identity of that width."""
+    d = 192
+    S = rb.DenseSkOp(rb.DenseDist(d, n), rb.RNGState(78))
+    I = torch.eye(n, dtype=torch.float32, device=DEV).reshape(-1)
+    B = torch.empty(d * n, dtype=torch.float32, device=DEV)
+    rb.lskge3_bf16split("C", "N", "N", d, n, n, 1.0, S, 0, 0, I, n, 0.0, B, d, n_terms=3)
+    got = host(B).reshape(n, d).T
+    ref, _ = orc.fill_dense(d, n, [0, 0, 0, 0], [78, 0], dtype=np.float32)
+    assert np.array_equal(got, orc.bf16_round(ref))
+
+
+@pytest.mark.parametrize("n", [256, 512, 768])
+def test_narrow_tiles_error(rb, orc, n):
+    got, same, _ = _case(rb, orc, "C", "N", "N", 130, n, 8192, 3, alpha=0.75, beta=0.5, seed=4)
+    assert rel_fro(got, same) <= 1e-6
