@@ -784,11 +784,12 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
 // generated operator (M x K, rows along K) and Y the dense operand (K x N),
 // for large interior shapes: the pair kernel's structure with FP32 operands.
 // Y is packed once into FP32 fragment runs; one 512-thread workgroup per
-// (64-row block, 1024-column block, K-split) generates each 32-deep operator
-// chunk once into LDS (exact FP32 values, as the other kernels) and applies it
-// to all 1024 columns with v_mfma_f32_32x32x2f32, so every operator entry is
-// generated once per 1024 columns instead of once per 256.  Partials go to
-// scratch and are combined by bf16_reduce_kernel (FP64, split order).
+// (64-row block, 256 CB-column block, K-split) generates each 32-deep
+// operator chunk once into LDS (exact FP32 values, as the other kernels) and
+// applies it to all of the block's columns with v_mfma_f32_32x32x2f32, so
+// every operator entry is generated once per 1024 columns (CB = 4; 512 / 256
+// for narrower outputs) instead of once per 256.  Partials go to scratch and
+// are combined by bf16_reduce_kernel (FP64, split order).
 //
 // Packed Y: for 32-column block cb and 16-deep K step s, lane l's eight
 // floats are Y[16 s + 2 q + l / 32][32 cb + l % 32], q = 0..7 (the B
@@ -861,7 +862,7 @@ __global__ __launch_bounds__(256) void pack_f32_kernel(const float *A, int64_t r
 }
 
 struct F32WideProblem {
-    int64_t M, N, Kp;  // M, N multiples of 64 / 1024; Kp of the 32-deep chunk
+    int64_t M, N, Kp;  // M, N multiples of 64 / the tile width; Kp of the 32-deep chunk
     GenMap g;
     const float *packed;
     float *part;   // nS x M x N, row-major
@@ -869,9 +870,9 @@ struct F32WideProblem {
     int nMB, nNB, nS;
 };
 
-template <bool GAUSS, bool ROWS>
+template <bool GAUSS, bool ROWS, int CB>
 __global__ __launch_bounds__(BF_FTHR, 1) void f32_wide_kernel(F32WideProblem pb) {
-    constexpr int RB = 2, CB = 4, MT = 64, NT = 1024;
+    constexpr int RB = 2, MT = 64, NT = 256 * CB;  // CB 32-column blocks per wave
     __shared__ __attribute__((aligned(16))) float Ss[2][MT * FW_LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int W = pb.nMB * pb.nNB * pb.nS;
@@ -998,21 +999,34 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f32_wide_kernel(F32WideProblem pb)
             }
 }
 
+// Launches a wide-tile FP32 kernel instantiation by (family, orientation).
+template <int CB>
+static void launch_f32_wide(bool gauss, bool rows, dim3 grid, const F32WideProblem &w, hipStream_t st) {
+    if (gauss && rows) hipLaunchKernelGGL((f32_wide_kernel<true, true, CB>), grid, dim3(BF_FTHR), 0, st, w);
+    else if (gauss) hipLaunchKernelGGL((f32_wide_kernel<true, false, CB>), grid, dim3(BF_FTHR), 0, st, w);
+    else if (rows) hipLaunchKernelGGL((f32_wide_kernel<false, true, CB>), grid, dim3(BF_FTHR), 0, st, w);
+    else hipLaunchKernelGGL((f32_wide_kernel<false, false, CB>), grid, dim3(BF_FTHR), 0, st, w);
+}
+
 // Runs the part of an FP32 fused-sketch problem that the wide kernel covers
 // (rows [0, Mf) x columns [0, Nf), Mf and Nf multiples of 64 and 1024) and
 // returns in rest[] the remaining sub-problems, for the caller's tile kernels
 // (-1: the wide kernel does not apply and nothing ran).
 int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream) {
     if (!pb.gen_is_x || pb.sbuf != nullptr || (pb.g.b_off & 3) != 0) return -1;
-    if (pb.K < 4096 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 1024) return -1;
-    const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / 1024 * 1024, K = pb.K;
+    if (pb.K < 4096 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 256) return -1;
+    // the widest tile (1024, 512 or 256 columns) the output holds; the
+    // columns past the last whole tile run on the tile kernels
+    const int cb = pb.N >= 1024 ? 4 : (pb.N >= 512 ? 2 : 1);
+    const int64_t NT = 256 * cb;
+    const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / NT * NT, K = pb.K;
     F32WideProblem w{};
     w.M = Mf;
     w.N = Nf;
     w.Kp = K;
     w.g = pb.g;
     w.nMB = (int)(Mf / 64);
-    w.nNB = (int)(Nf / 1024);
+    w.nNB = (int)(Nf / NT);
     int64_t kw = 4096;
     const int64_t tiles = (int64_t)w.nMB * w.nNB;
     const int64_t want = (4 * (int64_t)num_cus() + tiles - 1) / tiles;
@@ -1052,10 +1066,9 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
         ProfScope prof("f32_wide_kernel", stream);
         const dim3 grid((unsigned)(w.nMB * w.nNB * w.nS));
         const bool rows = w.g.b_along_rows != 0;
-        if (w.g.gauss && rows) hipLaunchKernelGGL((f32_wide_kernel<true, true>), grid, dim3(BF_FTHR), 0, stream, w);
-        else if (w.g.gauss) hipLaunchKernelGGL((f32_wide_kernel<true, false>), grid, dim3(BF_FTHR), 0, stream, w);
-        else if (rows) hipLaunchKernelGGL((f32_wide_kernel<false, true>), grid, dim3(BF_FTHR), 0, stream, w);
-        else hipLaunchKernelGGL((f32_wide_kernel<false, false>), grid, dim3(BF_FTHR), 0, stream, w);
+        if (cb == 4) launch_f32_wide<4>(w.g.gauss != 0, rows, grid, w, stream);
+        else if (cb == 2) launch_f32_wide<2>(w.g.gauss != 0, rows, grid, w, stream);
+        else launch_f32_wide<1>(w.g.gauss != 0, rows, grid, w, stream);
         RBX_HIP(hipGetLastError());
     }
     {
@@ -1088,9 +1101,9 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
 
 // ---------------------------------------------------------------- wide FP64 tiles
 // The plain FP64 fused sketch on the same plan as f32_wide_kernel, with
-// v_mfma_f64_16x16x4f64: 64 x 512 tiles (the FP64 accumulators of a 64 x 1024
-// tile would not fit), so each operator entry is generated once per 512
-// columns (the tile kernels: once per 256).  Packed Y: for 16-column block jb
+// v_mfma_f64_16x16x4f64: 64 x 128 CB tiles, 512 columns at CB = 4 (the FP64
+// accumulators of a 64 x 1024 tile would not fit), so each operator entry is
+// generated once per 512 columns (the tile kernels: once per 256).  Packed Y: for 16-column block jb
 // and 16-deep step s, lane l's four doubles are Y[16 s + 4 q + l / 16][16 jb +
 // l % 16], q = 0..3, at ((jb * KS + s) * 64 + l) * 4 + q; the operator chunk
 // in LDS holds k = 16 st + kk at 16 st + 4 (kk % 4) + kk / 4.  Partials are
@@ -1169,9 +1182,9 @@ struct F64WideProblem {
 };
 
 
-template <bool GAUSS, bool ROWS>
+template <bool GAUSS, bool ROWS, int CB>
 __global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb) {
-    constexpr int RB = 4, CB = 4, MT = 64, NT = 512;
+    constexpr int RB = 4, MT = 64, NT = 128 * CB;  // CB 16-column blocks per wave
     __shared__ __attribute__((aligned(16))) double Ss[2][MT * DW_LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int W = pb.nMB * pb.nNB * pb.nS;
@@ -1289,17 +1302,27 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb)
             }
 }
 
+template <int CB>
+static void launch_f64_wide(bool gauss, bool rows, dim3 grid, const F64WideProblem &w, hipStream_t st) {
+    if (gauss && rows) hipLaunchKernelGGL((f64_wide_kernel<true, true, CB>), grid, dim3(BF_FTHR), 0, st, w);
+    else if (gauss) hipLaunchKernelGGL((f64_wide_kernel<true, false, CB>), grid, dim3(BF_FTHR), 0, st, w);
+    else if (rows) hipLaunchKernelGGL((f64_wide_kernel<false, true, CB>), grid, dim3(BF_FTHR), 0, st, w);
+    else hipLaunchKernelGGL((f64_wide_kernel<false, false, CB>), grid, dim3(BF_FTHR), 0, st, w);
+}
+
 int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream) {
     if (!pb.gen_is_x || pb.sbuf != nullptr || (pb.g.b_off & 3) != 0) return -1;
-    if (pb.K < 4096 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 512) return -1;
-    const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / 512 * 512, K = pb.K;
+    if (pb.K < 4096 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 128) return -1;
+    const int cb = pb.N >= 512 ? 4 : (pb.N >= 256 ? 2 : 1);
+    const int64_t NT = 128 * cb;
+    const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / NT * NT, K = pb.K;
     F64WideProblem w{};
     w.M = Mf;
     w.N = Nf;
     w.Kp = K;
     w.g = pb.g;
     w.nMB = (int)(Mf / 64);
-    w.nNB = (int)(Nf / 512);
+    w.nNB = (int)(Nf / NT);
     int64_t kw = 16384;
     const int64_t tiles = (int64_t)w.nMB * w.nNB;
     const int64_t want = (4 * (int64_t)num_cus() + tiles - 1) / tiles;
@@ -1339,10 +1362,9 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
         ProfScope prof("f64_wide_kernel", stream);
         const dim3 grid((unsigned)(w.nMB * w.nNB * w.nS));
         const bool rows = w.g.b_along_rows != 0;
-        if (w.g.gauss && rows) hipLaunchKernelGGL((f64_wide_kernel<true, true>), grid, dim3(BF_FTHR), 0, stream, w);
-        else if (w.g.gauss) hipLaunchKernelGGL((f64_wide_kernel<true, false>), grid, dim3(BF_FTHR), 0, stream, w);
-        else if (rows) hipLaunchKernelGGL((f64_wide_kernel<false, true>), grid, dim3(BF_FTHR), 0, stream, w);
-        else hipLaunchKernelGGL((f64_wide_kernel<false, false>), grid, dim3(BF_FTHR), 0, stream, w);
+        if (cb == 4) launch_f64_wide<4>(w.g.gauss != 0, rows, grid, w, stream);
+        else if (cb == 2) launch_f64_wide<2>(w.g.gauss != 0, rows, grid, w, stream);
+        else launch_f64_wide<1>(w.g.gauss != 0, rows, grid, w, stream);
         RBX_HIP(hipGetLastError());
     }
     {

@@ -159,7 +159,7 @@ GemmProblem plan_left(char layout, char opS, char opA, int64_t d, int64_t n, int
 // remaining sub-problems in rest[] (their count), or -1 when it does not
 // apply (rbx_bf16.hip).
 int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream);
-// The same for FP64 problems (64 x 512 tiles).
+// The same for FP64 problems (tiles of up to 64 x 512).
 int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream);
 
 }  // namespace rbx

@@ -404,8 +404,8 @@ static void launch_skge_tiles(const GemmProblem &pb_in, hipStream_t stream) {
 }
 
 // Large problems with a generated operator take the wide-tile kernels (each
-// operator entry generated once per 1024 (FP32) / 512 (FP64) columns) for
-// their interior and the tile kernels for the ragged rest.
+// operator entry generated once per up to 1024 (FP32) / 512 (FP64) columns)
+// for their interior and the tile kernels for the ragged rest.
 // The same product described transposed: C^T = Y^T X^T, so a generated
 // operator on the Y side becomes the X side (its orientation flag flips).
 static GemmProblem transposed_problem(const GemmProblem &pb) {

@@ -412,3 +412,13 @@ def test_wide_tiles_beta_zero_ignores_nan(rb, dtype):
     B = torch.full((d * n,), float("nan"), dtype=dtype, device=DEV)
     rb.lskge3("C", "N", "N", d, n, m, 1.0, S, 0, 0, A, m, 0.0, B, d)
     assert bool(torch.isfinite(B).all())
+
+
+# Narrower outputs take the wide kernels with 512- / 256-column (FP32) and
+# 256- / 128-column (FP64) tiles; the columns past the last whole tile run on
+# the tile kernels.
+@pytest.mark.parametrize("dtype,tol,n", [(np.float32, 1e-5, 300), (np.float32, 1e-5, 600),
+                                         (np.float64, 1e-13, 200), (np.float64, 1e-13, 300)])
+def test_lskge3_narrow_wide_tiles(rb, orc, dtype, tol, n):
+    got, ref = run_lskge3(rb, orc, dtype, "C", "N", "N", 130, n, 8224, 0.75, -0.5, ro=2, co=4)
+    assert rel_fro(got, ref) <= tol
