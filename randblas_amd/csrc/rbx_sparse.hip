@@ -512,8 +512,9 @@ static void coo_to_csr(const int64_t *rows, const int64_t *cols, const T *vals, 
 // power-law column degrees) then keeps one wave busy, latency-bound, long
 // after the rest of the GPU has finished.  Rows of more than
 //   lrow = max(256, 4 x the mean row length)
-// entries (so rows of similar lengths, as a sparse operator's, are never
-// split) are cut into segments of at most `seg` entries instead: each segment
+// entries (so rows of similar lengths, as a sparse operator's, are not split,
+// unless they are too few to fill the GPU: long_row_threshold) are cut into
+// segments of at most `seg` entries instead: each segment
 // is summed by its own wave in FP64 (sum of v * g in stored order), and the
 // row's outputs are combined from the segment sums in segment order,
 //   c = RN(RN(beta * c) + RN(alpha * sum_s P_s))   (FP64, then rounded to T).
@@ -668,8 +669,14 @@ __global__ __launch_bounds__(256) void long_rows_combine_kernel(const int64_t *c
 }
 
 // The long-row threshold of a row-kernel call (rows longer than it are split).
+// Fewer rows than eight per CU, averaging more than 4096 entries (a sparse
+// operator with a small sketch dimension: 1000 rows of 8000 entries), keep
+// each of the few waves on a long latency-bound chain even when the rows are
+// alike: then every row longer than LONG_ROW_MIN is split.
 static int64_t long_row_threshold(int64_t nnz, int64_t mrows) {
-    const int64_t t = LONG_ROW_MEAN_FACTOR * ((nnz + mrows - 1) / mrows);
+    const int64_t mean = (nnz + mrows - 1) / mrows;
+    if (mrows < 8 * (int64_t)num_cus() && mean > 4096) return LONG_ROW_MIN;
+    const int64_t t = LONG_ROW_MEAN_FACTOR * mean;
     return t > LONG_ROW_MIN ? t : LONG_ROW_MIN;
 }
 

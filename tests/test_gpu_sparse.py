@@ -575,3 +575,17 @@ def test_long_rows_beta_zero_ignores_nan(rb, dtype):
         B = torch.full((m * d,), float("nan"), dtype=tdt, device=DEV)
         rb.rsksp3(layout, "N", "N", m, d, n, 1.0, A, 0, 0, S, 0, 0, 0.0, B, m if layout == "C" else d)
         assert bool(torch.isfinite(B).all()), layout
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-5)])
+@pytest.mark.parametrize("layout", ["C", "R"])
+def test_lskges_small_sketch_dimension(rb, orc, dtype, tol, layout):
+    """A SASO operator with few, long rows (d = 60, m = 200000, vec_nnz 8:
+    rows of ~26700 entries) is split into segments to fill the GPU; the
+    result is within the precision bar of the oracle's sequential product
+    (not bit-exact; in FP32 the oracle's own sequential rounding over 26700
+    terms is ~3e-6)."""
+    d, m, n = 60, 200000, 40
+    got, ref = _lsk(rb, orc, dtype, layout, "N", "N", (d, m), 8, "S", d, n, m, 0, 0, 1.0, 0.0, prefill=False)
+    exact_err = np.linalg.norm(np.asarray(got, np.float64) - np.asarray(ref, np.float64)) / np.linalg.norm(ref)
+    assert exact_err <= tol, exact_err
