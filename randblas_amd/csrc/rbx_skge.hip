@@ -406,6 +406,114 @@ static void launch_skge_tiles(const GemmProblem &pb_in, hipStream_t stream) {
 // Large problems with a generated operator take the wide-tile kernels (each
 // operator entry generated once per up to 1024 (FP32) / 512 (FP64) columns)
 // for their interior and the tile kernels for the ragged rest.
+// ---------------------------------------------------------------- one column
+// C (M x 1) = alpha * X * y + beta * C with X the generated operator: a
+// vector sketch.  The tile kernels would run their MFMAs over 255 empty
+// columns; here each workgroup takes GV_ROWS rows and a K range of GV_KW,
+// every thread generates whole Philox cells of those rows (exact values, as
+// everywhere) and accumulates v * y in FP64, a fixed-order workgroup
+// reduction gives the range's partial, and gemv_reduce_kernel sums the ranges
+// in order: deterministic, bound by the generator.  BR = 0: K runs along the
+// counter's major index b (one cell = 4 consecutive k of one row); BR = 1: the
+// rows run along b (one cell = 4 consecutive rows at one k).
+constexpr int GV_ROWS = 8;
+constexpr int64_t GV_KW = 16384;
+
+template <typename T, int BR>
+__global__ __launch_bounds__(256) void gemv_gen_kernel(GenMap g, int64_t M, int64_t K, const T *y, int64_t ys,
+                                                       double *part) {
+    __shared__ double red[GV_ROWS][256 / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * GV_ROWS, k0 = (int64_t)blockIdx.y * GV_KW;
+    int64_t k1 = k0 + GV_KW;
+    if (k1 > K) k1 = K;
+    const bool gauss = g.gauss != 0;
+    double acc[GV_ROWS];
+#pragma unroll
+    for (int r = 0; r < GV_ROWS; ++r) acc[r] = 0.0;
+    if (BR == 0) {
+        const int64_t c0 = k0 / 4, c1 = (k1 + 3) / 4;  // cells of this K range (k0 % 4 == 0)
+        for (int64_t c = c0 + tid; c < c1; c += 256) {
+            const int64_t k = 4 * c;
+            T yv[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) yv[t] = k + t < K ? y[(k + t) * ys] : (T)0;
+#pragma unroll
+            for (int r = 0; r < GV_ROWS; ++r) {
+                if (r0 + r >= M) break;
+                const float4 v = gen4(gauss, ctr_add(g.ctr, (uint64_t)((g.a_off + r0 + r) * g.P + (g.b_off >> 2) + c)),
+                                      g.k0, g.k1);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[r] += (double)entry_value<T>(f4_get(v, t), gauss) * (double)yv[t];
+            }
+        }
+    } else {
+        // b_off % 4 == 0 and r0 % 8 == 0: the block's rows are cells j = 0, 1
+        const int64_t cb = (g.b_off + r0) >> 2;
+        for (int64_t k = k0 + tid; k < k1; k += 256) {
+            const double yv = (double)y[k * ys];
+#pragma unroll
+            for (int j = 0; j < GV_ROWS / 4; ++j) {
+                if (r0 + 4 * j >= M) break;
+                const float4 v = gen4(gauss, ctr_add(g.ctr, (uint64_t)((g.a_off + k) * g.P + cb + j)), g.k0, g.k1);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[4 * j + t] += (double)entry_value<T>(f4_get(v, t), gauss) * yv;
+            }
+        }
+    }
+    // fixed-order reduction: butterfly within each wave, then the waves in order
+#pragma unroll
+    for (int r = 0; r < GV_ROWS; ++r) {
+        double v = acc[r];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) red[r][wave] = v;
+    }
+    __syncthreads();
+    if (tid < GV_ROWS && r0 + tid < M) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) v += red[tid][w];
+        part[(int64_t)blockIdx.y * M + r0 + tid] = v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_reduce_kernel(const double *part, int nk, int64_t M, double alpha,
+                                                          double beta, T *C, int64_t crs) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int z = 0; z < nk; ++z) v += part[(int64_t)z * M + i];
+        T *c = C + i * crs;
+        double out = alpha * v;
+        if (beta != 0.0) out += beta * (double)(*c);
+        *c = (T)out;
+    }
+}
+
+// Runs a one-column problem on gemv_gen_kernel; false when it does not apply.
+template <typename T>
+static bool gemv_run(const GemmProblem &pb, hipStream_t stream) {
+    if (pb.N != 1 || !pb.gen_is_x || pb.sbuf != nullptr || (pb.g.b_off & 3) != 0) return false;
+    if (pb.K < 1024) return false;
+    const int64_t nb = (pb.M + GV_ROWS - 1) / GV_ROWS, nk = (pb.K + GV_KW - 1) / GV_KW;
+    if (nb >= (1ll << 31) || nk > 65535) return false;
+    double *part = (double *)scratch(SCRATCH_SPLITK, (size_t)safe_mul(nk, pb.M) * sizeof(double));
+    ProfScope prof("gemv_gen_kernel", stream);
+    if (pb.g.b_along_rows)
+        hipLaunchKernelGGL((gemv_gen_kernel<T, 1>), dim3((unsigned)nb, (unsigned)nk), dim3(256), 0, stream, pb.g, pb.M,
+                           pb.K, (const T *)pb.D, pb.drs, part);
+    else
+        hipLaunchKernelGGL((gemv_gen_kernel<T, 0>), dim3((unsigned)nb, (unsigned)nk), dim3(256), 0, stream, pb.g, pb.M,
+                           pb.K, (const T *)pb.D, pb.drs, part);
+    int64_t blocks = (pb.M + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gemv_reduce_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, (const double *)part,
+                       (int)nk, pb.M, pb.alpha, pb.beta, (T *)pb.C, pb.crs);
+    RBX_HIP(hipGetLastError());
+    return true;
+}
+
 // The same product described transposed: C^T = Y^T X^T, so a generated
 // operator on the Y side becomes the X side (its orientation flag flips).
 static GemmProblem transposed_problem(const GemmProblem &pb) {
@@ -435,6 +543,8 @@ static void launch_skge(const GemmProblem &pb, hipStream_t stream) {
     };
     if (wide(pb)) return;
     if (!pb.gen_is_x && pb.sbuf == nullptr && wide(transposed_problem(pb))) return;
+    if (gemv_run<T>(pb, stream)) return;
+    if (!pb.gen_is_x && pb.sbuf == nullptr && pb.M == 1 && gemv_run<T>(transposed_problem(pb), stream)) return;
     launch_skge_tiles<T>(pb, stream);
 }
 

@@ -422,3 +422,73 @@ def test_wide_tiles_beta_zero_ignores_nan(rb, dtype):
 def test_lskge3_narrow_wide_tiles(rb, orc, dtype, tol, n):
     got, ref = run_lskge3(rb, orc, dtype, "C", "N", "N", 130, n, 8224, 0.75, -0.5, ro=2, co=4)
     assert rel_fro(got, ref) <= tol
+
+
+# One-column products (vector sketches through lskge3 / rskge3) with K >= 1024
+# take the generate-and-dot kernel when the submatrix's offset along the
+# counter's major index is a multiple of 4; other offsets run on the tiles.
+def _ran(rb, fn):
+    torch.cuda.synchronize()
+    rb.profile_collect()
+    rb.profile_enable(True)
+    fn()
+    rb.profile_enable(False)
+    return rb.profile_collect()
+
+
+@pytest.mark.parametrize("layout,opS,opA", LSK_CASES)
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-5)])
+@pytest.mark.parametrize("ro,co", [(4, 8), (3, 5)])
+def test_lskge3_one_column(rb, orc, layout, opS, opA, dtype, tol, ro, co):
+    got, ref = run_lskge3(rb, orc, dtype, layout, opS, opA, 37, 1, 20011, 0.75, -0.5, ro=ro, co=co, pad=(1, 2))
+    assert rel_fro(got, ref) <= tol
+
+
+@pytest.mark.parametrize("axis,fam", [("L", "G"), ("S", "G"), ("L", "U"), ("S", "U")])
+@pytest.mark.parametrize("opS", ["N", "T"])
+def test_lskge3_one_column_kernel(rb, orc, axis, fam, opS):
+    # both counter orientations (rows of the operator along or across the
+    # counter's major index), several K ranges, and the kernel named
+    d, m = 45, 40000
+    Sshape = (d, m) if opS == "N" else (m, d)
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(m)
+    y0 = rng.standard_normal(d)
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape, rb.ScalarDist(fam), rb.Axis(axis)), rb.RNGState(77))
+    yd = dev(y0)
+    ran = _ran(rb, lambda: rb.lskge3("C", opS, "N", d, 1, m, 1.5, S, 0, 0, dev(x), m, 2.0, yd, d))
+    assert "gemv_gen_kernel" in ran, ran
+    ref = y0.copy()
+    orc.lskge3("d", "C", opS, "N", d, 1, m, 1.5, Sshape, [0, 0, 0, 0], [77, 0], 0, 0, x, m, 2.0, ref, d,
+               family=fam, major_axis=axis)
+    err = rel_fro(host(yd), ref)
+    print(f"one column {axis}{fam}{opS}: {err:.2e}")
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("opS", ["N", "T"])
+@pytest.mark.parametrize("layout", ["C", "R"])
+def test_rskge3_one_row(rb, orc, layout, opS):
+    # B (1 x d) = x^T S: the product runs transposed onto the same kernel
+    n, d = 9001, 33
+    Sshape = (n, d) if opS == "N" else (d, n)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n)
+    b0 = rng.standard_normal(d)
+    S = rb.DenseSkOp(rb.DenseDist(*Sshape), rb.RNGState(8))
+    bd = dev(b0)
+    lda = 1 if layout == "C" else n
+    ldb = 1 if layout == "C" else d
+    ran = _ran(rb, lambda: rb.rskge3(layout, "N", opS, 1, d, n, -0.5, dev(x), lda, S, 0, 0, 0.25, bd, ldb))
+    assert "gemv_gen_kernel" in ran, ran
+    ref = b0.copy()
+    orc.rskge3("d", layout, "N", opS, 1, d, n, -0.5, x, lda, Sshape, [0, 0, 0, 0], [8, 0], 0, 0, 0.25, ref, ldb)
+    assert rel_fro(host(bd), ref) <= 1e-13
+
+
+def test_one_column_beta_zero_ignores_nan(rb):
+    S = rb.DenseSkOp(rb.DenseDist(20, 5000), rb.RNGState(3))
+    x = torch.ones(5000, dtype=torch.float64, device=DEV)
+    y = torch.full((20,), float("nan"), dtype=torch.float64, device=DEV)
+    rb.lskge3("C", "N", "N", 20, 1, 5000, 1.0, S, 0, 0, x, 5000, 0.0, y, 20)
+    assert bool(torch.isfinite(y).all())
