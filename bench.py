@@ -69,7 +69,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip fp64 / error / distortion legs")
     ap.add_argument("--legs", default="all",
-                    help="comma list of extra legs: fp64,sparse,saso,generator,solve,formats,sampling (default all)")
+                    help="comma list of extra legs: fp64,sparse,saso,generator,vector,solve,formats,sampling "
+                         "(default all)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="first run one plain device copy of this many GiB (calibrates traffic counters)")
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 sketch (profiling)")
@@ -154,7 +155,7 @@ def main():
 
     B_h = B.view(N, D).t().cpu().numpy().astype(np.float64)
     B64_h = B64_rows = None
-    all_legs = ("fp64", "sparse", "saso", "generator", "solve", "formats", "sampling")
+    all_legs = ("fp64", "sparse", "saso", "generator", "vector", "solve", "formats", "sampling")
     legs = set(all_legs) if args.legs == "all" else set(args.legs.split(","))
     unknown = legs - set(all_legs)
     if unknown:
@@ -222,7 +223,7 @@ def main():
     if not args.no_extras:
         # ---- sparse data, sparse operators, generator, consumers ----
         for key, name, fn in (("sparse", "sparse_rsksp3", sparse_leg), ("saso", "saso_lskges", saso_leg),
-                              ("generator", "generator", generator_leg),
+                              ("generator", "generator", generator_leg), ("vector", "sketch_vector", vector_leg),
                               ("solve", "sketch_and_solve", sketch_solve_leg),
                               ("formats", "sparse_formats", formats_leg), ("sampling", "sampling", sampling_leg)):
             if key in legs:
@@ -382,6 +383,23 @@ def generator_leg(rb, torch, dev, reps=5):
         log(f"generator {name}: {ms:.3f} ms")
         del buf
         torch.cuda.synchronize()
+    return res
+
+
+def vector_leg(rb, torch, dev, reps=5):
+    """One-column sketches: sketch_vector with the headline operator,
+    y = S x for S = DenseDist(4096, 262144) Gaussian (key 42) and x of 262144
+    N(0,1) entries, FP32 and FP64.  Bound by generating the 1.07e9 operator
+    entries (each used once), reported as operator entries per second."""
+    res = {"workload": "sketch_vector S DenseDist(4096, 262144) Gaussian (key 42) * x (262144)"}
+    S = rb.DenseSkOp(rb.DenseDist(D, M), rb.RNGState(KEY_S))
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        x = torch.empty(M, dtype=dt, device=dev)
+        rb.fill_dense(rb.DenseDist(M, 1), x, rb.RNGState(9))
+        y = torch.empty(D, dtype=dt, device=dev)
+        ms, k = _time(rb, torch, lambda: rb.sketch_vector("N", 1.0, S, x, 1, 0.0, y, 1), reps)
+        res[name] = {"ms": round(ms, 4), "entries_per_s": round(D * M / (ms * 1e-3), 1), "kernels_ms": k}
+        log(f"vector {name}: {ms:.3f} ms")
     return res
 
 

@@ -409,7 +409,8 @@ static void launch_skge_tiles(const GemmProblem &pb_in, hipStream_t stream) {
 // ---------------------------------------------------------------- one column
 // C (M x 1) = alpha * X * y + beta * C with X the generated operator: a
 // vector sketch.  The tile kernels would run their MFMAs over 255 empty
-// columns; here each workgroup takes GV_ROWS rows and a K range of GV_KW,
+// columns; here each workgroup takes GV_ROWS rows and a K range of kw (a
+// multiple of 4, chosen so the grid holds several workgroups per CU),
 // every thread generates whole Philox cells of those rows (exact values, as
 // everywhere) and accumulates v * y in FP64, a fixed-order workgroup
 // reduction gives the range's partial, and gemv_reduce_kernel sums the ranges
@@ -417,15 +418,14 @@ static void launch_skge_tiles(const GemmProblem &pb_in, hipStream_t stream) {
 // counter's major index b (one cell = 4 consecutive k of one row); BR = 1: the
 // rows run along b (one cell = 4 consecutive rows at one k).
 constexpr int GV_ROWS = 8;
-constexpr int64_t GV_KW = 16384;
 
 template <typename T, int BR>
-__global__ __launch_bounds__(256) void gemv_gen_kernel(GenMap g, int64_t M, int64_t K, const T *y, int64_t ys,
-                                                       double *part) {
+__global__ __launch_bounds__(256) void gemv_gen_kernel(GenMap g, int64_t M, int64_t K, int64_t kw, const T *y,
+                                                       int64_t ys, double *part) {
     __shared__ double red[GV_ROWS][256 / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t r0 = (int64_t)blockIdx.x * GV_ROWS, k0 = (int64_t)blockIdx.y * GV_KW;
-    int64_t k1 = k0 + GV_KW;
+    const int64_t r0 = (int64_t)blockIdx.x * GV_ROWS, k0 = (int64_t)blockIdx.y * kw;
+    int64_t k1 = k0 + kw;
     if (k1 > K) k1 = K;
     const bool gauss = g.gauss != 0;
     double acc[GV_ROWS];
@@ -496,16 +496,23 @@ template <typename T>
 static bool gemv_run(const GemmProblem &pb, hipStream_t stream) {
     if (pb.N != 1 || !pb.gen_is_x || pb.sbuf != nullptr || (pb.g.b_off & 3) != 0) return false;
     if (pb.K < 1024) return false;
-    const int64_t nb = (pb.M + GV_ROWS - 1) / GV_ROWS, nk = (pb.K + GV_KW - 1) / GV_KW;
-    if (nb >= (1ll << 31) || nk > 65535) return false;
+    const int64_t nb = (pb.M + GV_ROWS - 1) / GV_ROWS;
+    if (nb >= (1ll << 31)) return false;
+    // K ranges: enough for ~16 workgroups per CU, each 1024 to 16384 deep
+    const int64_t want = (16 * (int64_t)num_cus() + nb - 1) / nb;
+    int64_t kw = (pb.K + want - 1) / want;
+    kw = (kw + 1023) / 1024 * 1024;
+    if (kw > 16384) kw = 16384;
+    const int64_t nk = (pb.K + kw - 1) / kw;
+    if (nk > 65535) return false;
     double *part = (double *)scratch(SCRATCH_SPLITK, (size_t)safe_mul(nk, pb.M) * sizeof(double));
     ProfScope prof("gemv_gen_kernel", stream);
     if (pb.g.b_along_rows)
         hipLaunchKernelGGL((gemv_gen_kernel<T, 1>), dim3((unsigned)nb, (unsigned)nk), dim3(256), 0, stream, pb.g, pb.M,
-                           pb.K, (const T *)pb.D, pb.drs, part);
+                           pb.K, kw, (const T *)pb.D, pb.drs, part);
     else
         hipLaunchKernelGGL((gemv_gen_kernel<T, 0>), dim3((unsigned)nb, (unsigned)nk), dim3(256), 0, stream, pb.g, pb.M,
-                           pb.K, (const T *)pb.D, pb.drs, part);
+                           pb.K, kw, (const T *)pb.D, pb.drs, part);
     int64_t blocks = (pb.M + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(gemv_reduce_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, (const double *)part,
