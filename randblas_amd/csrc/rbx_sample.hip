@@ -163,15 +163,23 @@ __global__ __launch_bounds__(256) void weights_to_cdf_kernel(int64_t n, T *w, T 
         __syncthreads();
         if (wave != 0 && stop == len && b + 1 < nb) load(b + 1, buf[(b + 1) & 1], tid - 64, 192);
         if (tid == 0) {
-            // the chain alone: 16-byte LDS reads and writes, 8 in flight
+            // the chain alone: 16-byte LDS reads and writes; the next group
+            // of 8 vectors is read before this group's sums are formed, so
+            // the chain never waits on LDS latency (the read position is
+            // clamped inside the block buffer; values past `stop` are unused)
             using V = std::conditional_t<sizeof(T) == 8, double2, float4>;
             constexpr int PV = 16 / sizeof(T);  // values per vector
+            constexpr int GV = 8 * PV;          // values per group
             T s_ = sum;
             int e = 0;
-            for (; e + 8 * PV <= stop; e += 8 * PV) {
-                V v[8];
+            V v[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = *(const V *)(cur + e + u * PV);
+            for (int u = 0; u < 8; ++u) v[u] = *(const V *)(cur + u * PV);
+            for (; e + GV <= stop; e += GV) {
+                const int ne = e + GV < CDF_BLK - GV ? e + GV : CDF_BLK - GV;
+                V nx[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) nx[u] = *(const V *)(cur + ne + u * PV);
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     T *x = reinterpret_cast<T *>(&v[u]);
@@ -182,6 +190,8 @@ __global__ __launch_bounds__(256) void weights_to_cdf_kernel(int64_t n, T *w, T 
                     }
                     *(V *)(cur + e + u * PV) = v[u];
                 }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = nx[u];
             }
             for (; e < stop; ++e) {
                 s_ = s_ + cur[e];
