@@ -5,10 +5,20 @@
 //   x = argmin_x || S (A x - b) ||_2,   S = submat(S) (d x m), A m x n, b m,
 // SA and Sb by the fused sketch (the operator is never stored), then the QR
 // of the d x n sketched system and the triangular solve in FP64 with
-// rocSOLVER / rocBLAS for both precisions.  The adjoint gives the gradient of
+// rocSOLVER / rocBLAS for both precisions.  The QR is CholeskyQR2 (Yamamoto,
+// Nakatsukasa, Yanagisawa, Fukaya 2015: R1 = chol(Y^T Y), Q1 = Y R1^{-1},
+// R2 = chol(Q1^T Q1), Q = Q1 R2^{-1}, R = R2 R1; explicit Q, a few large
+// BLAS-3 calls) when the second Gram matrix shows Q1 orthonormal to 1e-3
+// (|R2_ii - 1| <= 1e-3, both Cholesky factorizations succeeded), i.e. a
+// well-conditioned sketched system; otherwise Householder QR (geqrf), which
+// is stable for any conditioning.  RBX_SOLVE_QR=householder in the
+// environment selects Householder QR for every call.  The adjoint gives the
+// gradient of
 // <xbar, x(A, b)>:
 //   u = R^{-T} xbar,  w = Q [u; 0],  z = R^{-1} u,  r = Sb - SA x,
 //   bbar = S^T w,     Abar = S^T (r z^T - w x^T).
+#include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -74,6 +84,24 @@ __global__ __launch_bounds__(256) void rank2_kernel(const double *r, const doubl
     }
 }
 
+// zero the strictly lower triangle of an n x n column-major matrix
+__global__ __launch_bounds__(256) void zero_lower_kernel(double *R, int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x)
+        if (e % n > e / n) R[e] = 0.0;
+}
+
+// ok[0] = 1 when both Cholesky factorizations succeeded and every diagonal
+// entry of R2 is within 1e-3 of 1 (one workgroup)
+__global__ __launch_bounds__(256) void cholqr_check_kernel(const double *R2, int64_t n, const int *info, int *ok) {
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = (info[0] != 0) || (info[1] != 0);
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        if (!(fabs(R2[i * n + i] - 1.0) <= 1e-3)) atomicOr(&bad, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) ok[0] = !bad;
+}
+
 __global__ void zero_tail_kernel(double *v, int64_t from, int64_t to) {
     for (int64_t i = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < to; i += (int64_t)gridDim.x * blockDim.x)
         v[i] = 0.0;
@@ -104,29 +132,83 @@ static rbx_status lsk_adj(char layout, int64_t d, int64_t n, int64_t m, const rb
 template <typename T>
 struct SolveWork {
     double *SA, *QR, *Sb, *y, *tau;  // d x n, d x n, d, d, n
+    double *G, *R2, *Rc;             // n x n each: R1 (CholeskyQR2), R2, R = R2 R1
+    int *info;                       // potrf status of the two passes, and the check flag
     T *tmpT;                         // d x n in T (the sketches before promotion)
+    // the factorization: explicit Q in QR (CholeskyQR2) or Householder
+    // vectors in QR with tau; R upper triangular at R with leading dim ldr
+    bool explicit_q = false;
+    const double *R = nullptr;
+    int64_t ldr = 0;
 };
 
 template <typename T>
 static SolveWork<T> solve_work(int64_t d, int64_t n) {
     auto a = [](size_t b) { return (b + 255) / 256 * 256; };
-    const size_t dn = (size_t)safe_mul(d, n);
-    const size_t bytes = 2 * a(dn * 8) + 2 * a((size_t)d * 8) + a((size_t)n * 8) + a(dn * sizeof(T) + 8);
+    const size_t dn = (size_t)safe_mul(d, n), nn = (size_t)safe_mul(n, n);
+    const size_t bytes = 2 * a(dn * 8) + 2 * a((size_t)d * 8) + a((size_t)n * 8) + 3 * a(nn * 8) + a(16) +
+                         a(dn * sizeof(T) + 8);
     char *ws = (char *)scratch(SCRATCH_SOLVE, bytes);
     SolveWork<T> w;
-    w.SA = (double *)ws;
-    w.QR = (double *)(ws + a(dn * 8));
-    w.Sb = (double *)(ws + 2 * a(dn * 8));
-    w.y = (double *)(ws + 2 * a(dn * 8) + a((size_t)d * 8));
-    w.tau = (double *)(ws + 2 * a(dn * 8) + 2 * a((size_t)d * 8));
-    w.tmpT = (T *)(ws + 2 * a(dn * 8) + 2 * a((size_t)d * 8) + a((size_t)n * 8));
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        char *p = ws + off;
+        off += a(b);
+        return p;
+    };
+    w.SA = (double *)take(dn * 8);
+    w.QR = (double *)take(dn * 8);
+    w.Sb = (double *)take((size_t)d * 8);
+    w.y = (double *)take((size_t)d * 8);
+    w.tau = (double *)take((size_t)n * 8);
+    w.G = (double *)take(nn * 8);
+    w.R2 = (double *)take(nn * 8);
+    w.Rc = (double *)take(nn * 8);
+    w.info = (int *)take(16);
+    w.tmpT = (T *)take(dn * sizeof(T) + 8);
     return w;
 }
 
-// SA, Sb (FP64), QR = geqrf(SA), y = Q^T Sb, x64 = R^{-1} y[0:n] (left in y)
+// CholeskyQR2 of the d x n FP64 matrix in w.QR (overwritten by Q), R in w.Rc;
+// true when accepted (see the file comment), false when Householder QR must
+// be used instead (w.QR then holds no usable factor).
+template <typename T>
+static bool cholqr2(int64_t d, int64_t n, SolveWork<T> &w, hipStream_t st) {
+    ProfScope prof("solve_cholqr2", st);
+    rocblas_handle h = blas_handle(st);
+    const rocblas_int D = (rocblas_int)d, Nn = (rocblas_int)n;
+    const double one = 1.0, zero = 0.0;
+    double *Rk[2] = {w.G, w.R2};
+    for (int pass = 0; pass < 2; ++pass) {
+        blas_ok(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, Nn, D, &one, w.QR, D, &zero, Rk[pass],
+                              Nn),
+                "syrk");
+        blas_ok(rocsolver_dpotrf(h, rocblas_fill_upper, Nn, Rk[pass], Nn, w.info + pass), "potrf");
+        blas_ok(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, D, Nn, &one, Rk[pass], Nn, w.QR, D),
+                "trsm");
+    }
+    hipLaunchKernelGGL(cholqr_check_kernel, dim3(1), dim3(256), 0, st, (const double *)w.R2, n, (const int *)w.info,
+                       w.info + 2);
+    RBX_HIP(hipGetLastError());
+    int ok = 0;
+    RBX_HIP(hipMemcpyAsync(&ok, w.info + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+    RBX_HIP(hipStreamSynchronize(st));
+    if (!ok) return false;
+    // R = R2 R1 (R1's strictly lower triangle holds syrk's untouched entries)
+    hipLaunchKernelGGL(zero_lower_kernel, dim3(grid_of(n * n)), dim3(256), 0, st, w.G, n);
+    RBX_HIP(hipGetLastError());
+    blas_ok(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                          Nn, Nn, &one, w.R2, Nn, w.G, Nn, w.Rc, Nn),
+            "trmm");
+    return true;
+}
+
+// SA, Sb (FP64), the QR of SA (CholeskyQR2, else Householder), y = Q^T Sb,
+// x64 = R^{-1} y[0:n] (left in y)
 template <typename T>
 static void factor_and_solve(char layout, int64_t d, int64_t m, int64_t n, const rbx_dense_skop *S, int64_t ro_s,
-                             int64_t co_s, const T *A, int64_t lda, const T *b, const SolveWork<T> &w,
+                             int64_t co_s, const T *A, int64_t lda, const T *b, SolveWork<T> &w,
                              hipStream_t st) {
     RBX_REQUIRE(d <= INT32_MAX && n <= INT32_MAX);
     // SA in T, in `layout` with leading dimension d (C) or n (R), promoted
@@ -141,14 +223,32 @@ static void factor_and_solve(char layout, int64_t d, int64_t m, int64_t n, const
                        (int64_t)1, w.Sb);
     RBX_HIP(hipGetLastError());
     RBX_HIP(hipMemcpyAsync(w.QR, w.SA, (size_t)(d * n) * 8, hipMemcpyDeviceToDevice, st));
-    RBX_HIP(hipMemcpyAsync(w.y, w.Sb, (size_t)d * 8, hipMemcpyDeviceToDevice, st));
     rocblas_handle h = blas_handle(st);
-    blas_ok(rocsolver_dgeqrf(h, (rocblas_int)d, (rocblas_int)n, w.QR, (rocblas_int)d, w.tau), "geqrf");
-    blas_ok(rocsolver_dormqr(h, rocblas_side_left, rocblas_operation_transpose, (rocblas_int)d, 1, (rocblas_int)n,
-                             w.QR, (rocblas_int)d, w.tau, w.y, (rocblas_int)d),
-            "ormqr");
+    const char *qr_env = std::getenv("RBX_SOLVE_QR");
+    const bool householder_only = qr_env != nullptr && std::strcmp(qr_env, "householder") == 0;
+    if (!householder_only && cholqr2<T>(d, n, w, st)) {
+        w.explicit_q = true;
+        w.R = w.Rc;
+        w.ldr = n;
+        // y[0:n] = Q^T Sb
+        const double one = 1.0, zero = 0.0;
+        blas_ok(rocblas_dgemv(h, rocblas_operation_transpose, (rocblas_int)d, (rocblas_int)n, &one, w.QR,
+                              (rocblas_int)d, w.Sb, 1, &zero, w.y, 1),
+                "gemv");
+    } else {
+        ProfScope prof("solve_householder", st);
+        w.explicit_q = false;
+        w.R = w.QR;
+        w.ldr = d;
+        RBX_HIP(hipMemcpyAsync(w.QR, w.SA, (size_t)(d * n) * 8, hipMemcpyDeviceToDevice, st));
+        RBX_HIP(hipMemcpyAsync(w.y, w.Sb, (size_t)d * 8, hipMemcpyDeviceToDevice, st));
+        blas_ok(rocsolver_dgeqrf(h, (rocblas_int)d, (rocblas_int)n, w.QR, (rocblas_int)d, w.tau), "geqrf");
+        blas_ok(rocsolver_dormqr(h, rocblas_side_left, rocblas_operation_transpose, (rocblas_int)d, 1, (rocblas_int)n,
+                                 w.QR, (rocblas_int)d, w.tau, w.y, (rocblas_int)d),
+                "ormqr");
+    }
     blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit, (rocblas_int)n,
-                          w.QR, (rocblas_int)d, w.y, 1),
+                          w.R, (rocblas_int)w.ldr, w.y, 1),
             "trsv");
 }
 
@@ -197,23 +297,29 @@ static void sketch_solve_adjoint_impl(char layout, int64_t d, int64_t m, int64_t
     hipLaunchKernelGGL((to_f64_kernel<T>), dim3(grid_of(n)), dim3(256), 0, st, xbar, (int64_t)1, n, n, (int64_t)1, u);
     RBX_HIP(hipGetLastError());
     blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, rocblas_operation_transpose, rocblas_diagonal_non_unit, (rocblas_int)n,
-                          w.QR, (rocblas_int)d, u, 1),
+                          w.R, (rocblas_int)w.ldr, u, 1),
             "trsv");
+    const double one = 1.0, mone = -1.0, zero = 0.0;
     // w = Q [u; 0]
-    RBX_HIP(hipMemcpyAsync(wv, u, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(zero_tail_kernel, dim3(grid_of(d - n)), dim3(256), 0, st, wv, n, d);
-    RBX_HIP(hipGetLastError());
-    blas_ok(rocsolver_dormqr(h, rocblas_side_left, rocblas_operation_none, (rocblas_int)d, 1, (rocblas_int)n, w.QR,
-                             (rocblas_int)d, w.tau, wv, (rocblas_int)d),
-            "ormqr");
+    if (w.explicit_q) {
+        blas_ok(rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)d, (rocblas_int)n, &one, w.QR, (rocblas_int)d, u,
+                              1, &zero, wv, 1),
+                "gemv");
+    } else {
+        RBX_HIP(hipMemcpyAsync(wv, u, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(zero_tail_kernel, dim3(grid_of(d - n)), dim3(256), 0, st, wv, n, d);
+        RBX_HIP(hipGetLastError());
+        blas_ok(rocsolver_dormqr(h, rocblas_side_left, rocblas_operation_none, (rocblas_int)d, 1, (rocblas_int)n, w.QR,
+                                 (rocblas_int)d, w.tau, wv, (rocblas_int)d),
+                "ormqr");
+    }
     // z = R^{-1} u
     RBX_HIP(hipMemcpyAsync(z, u, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
     blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit, (rocblas_int)n,
-                          w.QR, (rocblas_int)d, z, 1),
+                          w.R, (rocblas_int)w.ldr, z, 1),
             "trsv");
     // r = Sb - SA x
     RBX_HIP(hipMemcpyAsync(r, w.Sb, (size_t)d * 8, hipMemcpyDeviceToDevice, st));
-    const double one = 1.0, mone = -1.0;
     blas_ok(rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)d, (rocblas_int)n, &mone, w.SA, (rocblas_int)d, x64, 1,
                           &one, r, 1),
             "gemv");

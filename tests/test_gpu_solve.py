@@ -12,6 +12,11 @@ examples/total-least-squares/tls_dense_skop.cc:138-171).
     differences of the device forward (Richardson-extrapolated, <= 1e-6 of the
     largest directional derivative);
   * argument errors: d < n, short leading dimensions.
+
+The QR of the sketched system is CholeskyQR2 when the system is well
+conditioned and Householder otherwise; the accuracy tests run both
+(RBX_SOLVE_QR=householder forces Householder), and an ill-conditioned system
+(cond 1e10) checks that the Householder fallback is taken.
 """
 import numpy as np
 import pytest
@@ -24,6 +29,24 @@ DEV = "cuda"
 def host(t):
     torch.cuda.synchronize()
     return t.cpu().numpy()
+
+
+@pytest.fixture(params=["auto", "householder"])
+def qr(request, monkeypatch):
+    if request.param == "householder":
+        monkeypatch.setenv("RBX_SOLVE_QR", "householder")
+    else:
+        monkeypatch.delenv("RBX_SOLVE_QR", raising=False)
+    return request.param
+
+
+def _ran(rb, fn):
+    torch.cuda.synchronize()
+    rb.profile_collect()
+    rb.profile_enable(True)
+    fn()
+    rb.profile_enable(False)
+    return rb.profile_collect()
 
 
 def _dev(a, dtype=torch.float64):
@@ -46,7 +69,7 @@ def _problem(m, n, seed):
 @pytest.mark.parametrize("layout", ["C", "R"])
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-11), (torch.float32, 1e-4)])
 @pytest.mark.parametrize("d,m,n,ro,co", [(200, 3000, 40, 0, 0), (150, 2501, 60, 3, 17), (513, 4099, 1, 1, 2)])
-def test_solution_matches_lstsq(rb, orc, layout, dtype, tol, d, m, n, ro, co):
+def test_solution_matches_lstsq(rb, qr, orc, layout, dtype, tol, d, m, n, ro, co):
     A, b = _problem(m, n, 7)
     Dr, Dc = d + ro + 5, m + co + 9
     S = rb.DenseSkOp(rb.DenseDist(Dr, Dc), rb.RNGState(key=19, counter=(4, 0, 0, 0)))
@@ -63,7 +86,7 @@ def test_solution_matches_lstsq(rb, orc, layout, dtype, tol, d, m, n, ro, co):
     assert err <= tol, err
 
 
-def test_normal_equations_at_size(rb):
+def test_normal_equations_at_size(rb, qr):
     m, n, d = 262144, 256, 1024
     A = torch.empty(m * n, dtype=torch.float64, device=DEV)
     rb.fill_dense(rb.DenseDist(m, n), A, rb.RNGState(11))  # column-major m x n
@@ -97,7 +120,7 @@ def _torch_grads(Sm, A, b, xbar):
 
 
 @pytest.mark.parametrize("layout", ["C", "R"])
-def test_adjoint_matches_torch_autograd(rb, orc, layout):
+def test_adjoint_matches_torch_autograd(rb, qr, orc, layout):
     d, m, n, ro, co = 150, 2000, 30, 2, 5
     A, b = _problem(m, n, 8)
     xbar = np.random.default_rng(9).standard_normal(n)
@@ -116,7 +139,7 @@ def test_adjoint_matches_torch_autograd(rb, orc, layout):
     assert np.linalg.norm(host(bbar) - gb_ref) / np.linalg.norm(gb_ref) <= 1e-10
 
 
-def test_adjoint_matches_finite_differences(rb):
+def test_adjoint_matches_finite_differences(rb, qr):
     m, n, d = 16384, 64, 256
     A, b = _problem(m, n, 10)
     rng = np.random.default_rng(11)
@@ -149,7 +172,7 @@ def test_adjoint_matches_finite_differences(rb):
     assert err <= 1e-6
 
 
-def test_fp32_tracks_fp64(rb):
+def test_fp32_tracks_fp64(rb, qr):
     m, n, d = 20000, 50, 400
     A, b = _problem(m, n, 13)
     A = A.astype(np.float32).astype(np.float64)
@@ -182,3 +205,26 @@ def test_argument_errors(rb):
         rb.sketch_solve("R", 10, 100, 5, S, 0, 0, A, 4, b, x)  # lda < n
     with pytest.raises(rb.Error):
         rb.sketch_solve("C", 10, 100, 5, S, 5, 0, A, 100, b, x)  # submatrix outside S
+
+
+@pytest.mark.parametrize("cond,path", [(1e2, "solve_cholqr2"), (1e10, "solve_householder")])
+def test_factorization_path(rb, orc, monkeypatch, cond, path):
+    # a consistent system A x* = b with singular values 1 .. 1/cond: x within
+    # cond * eps of numpy's lstsq of the oracle's sketch, and the QR chosen by
+    # the conditioning (CholeskyQR2's second Gram matrix is far from I at 1e10)
+    monkeypatch.delenv("RBX_SOLVE_QR", raising=False)
+    d, m, n = 300, 4000, 40
+    rng = np.random.default_rng(3)
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (U * np.logspace(0, -np.log10(cond), n)) @ V.T
+    b = A @ rng.standard_normal(n)
+    S = rb.DenseSkOp(rb.DenseDist(d, m), rb.RNGState(29))
+    Sm, _ = orc.fill_dense_unpacked(orc.natural_layout(d, m, "L"), d, m, d, m, 0, 0, [0, 0, 0, 0], [29, 0])
+    x_ref = np.linalg.lstsq(Sm @ A, Sm @ b, rcond=None)[0]
+    x = torch.empty(n, dtype=torch.float64, device=DEV)
+    ran = _ran(rb, lambda: rb.sketch_solve("C", d, m, n, S, 0, 0, _dev(_store(A, "C")), m, _dev(b), x))
+    assert path in ran, ran
+    err = np.linalg.norm(host(x) - x_ref) / np.linalg.norm(x_ref)
+    print(f"cond {cond:.0e} via {path}: {err:.2e}")
+    assert err <= (1e-11 if cond < 1e3 else 1e-4), err
