@@ -180,9 +180,11 @@ static bool cholqr2(int64_t d, int64_t n, SolveWork<T> &w, hipStream_t st) {
     const double one = 1.0, zero = 0.0;
     double *Rk[2] = {w.G, w.R2};
     for (int pass = 0; pass < 2; ++pass) {
-        blas_ok(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, Nn, D, &one, w.QR, D, &zero, Rk[pass],
-                              Nn),
-                "syrk");
+        // the full Gram matrix by GEMM (faster than rocBLAS's SYRK at these
+        // shapes); potrf reads its upper triangle
+        blas_ok(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Nn, Nn, D, &one, w.QR, D, w.QR,
+                              D, &zero, Rk[pass], Nn),
+                "gemm");
         blas_ok(rocsolver_dpotrf(h, rocblas_fill_upper, Nn, Rk[pass], Nn, w.info + pass), "potrf");
         blas_ok(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                               rocblas_diagonal_non_unit, D, Nn, &one, Rk[pass], Nn, w.QR, D),
@@ -195,7 +197,7 @@ static bool cholqr2(int64_t d, int64_t n, SolveWork<T> &w, hipStream_t st) {
     RBX_HIP(hipMemcpyAsync(&ok, w.info + 2, sizeof(int), hipMemcpyDeviceToHost, st));
     RBX_HIP(hipStreamSynchronize(st));
     if (!ok) return false;
-    // R = R2 R1 (R1's strictly lower triangle holds syrk's untouched entries)
+    // R = R2 R1 (R1's strictly lower triangle still holds the Gram matrix)
     hipLaunchKernelGGL(zero_lower_kernel, dim3(grid_of(n * n)), dim3(256), 0, st, w.G, n);
     RBX_HIP(hipGetLastError());
     blas_ok(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
