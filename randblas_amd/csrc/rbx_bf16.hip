@@ -566,21 +566,24 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
 }
 
 // B[i,j] = alpha * sum_s part[s][i*N + j] + beta * B[i*brs + j*bcs], sums in
-// FP64 in split order.  32 x 32 tiles: partials are read along j; a
-// column-major B is written along i after a transpose through LDS.
+// FP64 in split order.  RED_R x RED_C tiles (8 x 128): each row of a tile is
+// one 512-byte run of every partial (FP32), long enough for full HBM bursts;
+// a column-major B is written along i after a transpose through LDS.  A 1-D
+// grid of tiles, row-tile major.
+constexpr int RED_R = 8, RED_C = 128;
 template <typename PT, typename OT>
 __global__ __launch_bounds__(256) void bf16_reduce_kernel(const PT *part, int nS, int64_t M, int64_t N,
                                                           double alpha, double beta, OT *B, int64_t brs,
                                                           int64_t bcs) {
-    __shared__ double tile[32][33];
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    const int64_t i0 = (int64_t)blockIdx.y * 32, j0 = (int64_t)blockIdx.x * 32;
+    __shared__ double tile[RED_R][RED_C + 1];
+    const int64_t ntc = (N + RED_C - 1) / RED_C;
+    const int64_t i0 = (int64_t)(blockIdx.x / ntc) * RED_R, j0 = (int64_t)(blockIdx.x % ntc) * RED_C;
     const int64_t total = M * N;
     {
         // thread: row r, columns c4 .. c4 + 3, 16-byte loads when whole and
         // aligned; eight splits' loads are issued together and every sum stays
         // in split order
-        const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
+        const int r = threadIdx.x / (RED_C / 4), c4 = 4 * (threadIdx.x % (RED_C / 4));
         const int64_t i = i0 + r, j = j0 + c4;
         double v[4] = {0.0, 0.0, 0.0, 0.0};
         if (i < M) {
@@ -612,20 +615,28 @@ __global__ __launch_bounds__(256) void bf16_reduce_kernel(const PT *part, int nS
         for (int e = 0; e < 4; ++e) tile[r][c4 + e] = v[e];
     }
     __syncthreads();
+    const bool cm = brs == 1;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        // column-major: lanes walk i; row-major: lanes walk j
-        const bool cm = brs == 1;
-        const int64_t i = cm ? i0 + tx : i0 + ty + 8 * q;
-        const int64_t j = cm ? j0 + ty + 8 * q : j0 + tx;
+    for (int q = 0; q < RED_R * RED_C / 256; ++q) {
+        // column-major: consecutive threads walk i; row-major: they walk j
+        const int e = threadIdx.x + 256 * q;
+        const int ti = cm ? e % RED_R : e / RED_C, tj = cm ? e / RED_R : e % RED_C;
+        const int64_t i = i0 + ti, j = j0 + tj;
         if (i < M && j < N) {
-            const double v = cm ? tile[tx][ty + 8 * q] : tile[ty + 8 * q][tx];
+            const double v = tile[ti][tj];
             OT *b = B + i * brs + j * bcs;
             double out = alpha * v;
             if (beta != 0.0) out += beta * (double)(*b);
             *b = (OT)out;
         }
     }
+}
+
+// 1-D grid of bf16_reduce_kernel over an M x N output
+static dim3 reduce_grid(int64_t M, int64_t N) {
+    const int64_t nt = ((M + RED_R - 1) / RED_R) * ((N + RED_C - 1) / RED_C);
+    RBX_REQUIRE(nt < (1ll << 31));
+    return dim3((unsigned)nt);
 }
 
 template <int CB>
@@ -770,8 +781,7 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         }
     }
     {
-        dim3 grid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32));
-        RBX_REQUIRE(grid.y <= 65535u);
+        const dim3 grid = reduce_grid(M, N);
         ProfScope prof("bf16_reduce_kernel", stream);
         hipLaunchKernelGGL((bf16_reduce_kernel<float, float>), grid, dim3(256), 0, stream, pb.part, pb.nS, M, N,
                            (double)alpha, (double)beta, B, brs, bcs);
@@ -1073,8 +1083,7 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     }
     {
         ProfScope prof("f32_reduce_kernel", stream);
-        const dim3 grid((unsigned)(Nf / 32), (unsigned)(Mf / 32));
-        RBX_REQUIRE(grid.y <= 65535u);
+        const dim3 grid = reduce_grid(Mf, Nf);
         hipLaunchKernelGGL((bf16_reduce_kernel<float, float>), grid, dim3(256), 0, stream, (const float *)w.part, w.nS, Mf, Nf,
                            pb.alpha, pb.beta, (float *)pb.C, pb.crs, pb.ccs);
         RBX_HIP(hipGetLastError());
@@ -1369,8 +1378,7 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     }
     {
         ProfScope prof("f64_reduce_kernel", stream);
-        const dim3 grid((unsigned)(Nf / 32), (unsigned)(Mf / 32));
-        RBX_REQUIRE(grid.y <= 65535u);
+        const dim3 grid = reduce_grid(Mf, Nf);
         hipLaunchKernelGGL((bf16_reduce_kernel<double, double>), grid, dim3(256), 0, stream, (const double *)w.part,
                            w.nS, Mf, Nf, pb.alpha, pb.beta, (double *)pb.C, pb.crs, pb.ccs);
         RBX_HIP(hipGetLastError());
