@@ -90,6 +90,97 @@ __global__ __launch_bounds__(256) void zero_lower_kernel(double *R, int64_t n) {
         if (e % n > e / n) R[e] = 0.0;
 }
 
+// Upper Cholesky A = R^T R of an n x n column-major FP64 matrix, n <= 256,
+// in one 1024-thread workgroup (the upper triangle is read and R written over
+// it; the strict lower triangle is untouched).  Right-looking over 32-wide
+// blocks, everything a block step reuses held in LDS:
+//   1. the diagonal block is factored in LDS (row k scaled by sqrt of its
+//      pivot, then the block's entries below row k updated);
+//   2. the block row to its right is solved with the block's R^T, one thread
+//      per column (forward substitution in registers), and kept in LDS;
+//   3. the trailing upper triangle is updated from LDS, A -= P^T P, a 4 x 4
+//      tile of entries per thread.
+// info = 0, or k + 1 when pivot k is not positive (R then incomplete).
+constexpr int CH_NB = 32, CH_MAX = 256, CH_THR = 1024;
+__global__ __launch_bounds__(CH_THR) void chol_blocked_kernel(double *A, int n, int *info) {
+    __shared__ double D[CH_NB][CH_NB + 1];  // D[i][j] = A(r0 + i, r0 + j), upper part
+    __shared__ double P[CH_NB][CH_MAX];     // P[t][c] = R(r0 + t, c0 + c)
+    const int tid = threadIdx.x;
+    const int di = tid % CH_NB, dj = tid / CH_NB;  // 32 x 32 threads over the diagonal block
+    for (int r0 = 0; r0 < n; r0 += CH_NB) {
+        const int nb = n - r0 < CH_NB ? n - r0 : CH_NB;
+        const bool mine = di < nb && dj < nb && di <= dj;
+        if (mine) D[di][dj] = A[(r0 + di) + (int64_t)(r0 + dj) * n];
+        __syncthreads();
+        for (int k = 0; k < nb; ++k) {
+            const double dkk = D[k][k];
+            if (!(dkk > 0.0)) {  // every thread reads the same pivot: a uniform exit
+                if (tid == 0) *info = r0 + k + 1;
+                return;
+            }
+            const double rkk = sqrt(dkk);
+            __syncthreads();  // the pivot is read before row k is overwritten
+            if (di == k && dj >= k && dj < nb) D[k][dj] = dj == k ? rkk : D[k][dj] / rkk;
+            __syncthreads();
+            if (di > k && di <= dj && dj < nb) D[di][dj] -= D[k][di] * D[k][dj];
+            __syncthreads();
+        }
+        if (mine) A[(r0 + di) + (int64_t)(r0 + dj) * n] = D[di][dj];
+        const int c0 = r0 + nb, m = n - c0;
+        if (tid < m) {
+            // R^T x = a for column c0 + tid of the block row
+            const int64_t cb = (int64_t)(c0 + tid) * n + r0;
+            double x[CH_NB];
+#pragma unroll
+            for (int t = 0; t < CH_NB; ++t) x[t] = t < nb ? A[cb + t] : 0.0;
+#pragma unroll
+            for (int t = 0; t < CH_NB; ++t) {
+                if (t < nb) {
+                    double v = x[t];
+#pragma unroll
+                    for (int u = 0; u < t; ++u) v -= D[u][t] * x[u];
+                    x[t] = v / D[t][t];
+                    P[t][tid] = x[t];
+                    A[cb + t] = x[t];
+                }
+            }
+        }
+        __syncthreads();
+        // trailing update of the upper triangle, 4 x 4 tiles
+        const int mt = (m + 3) / 4;
+        for (int e = tid; e < mt * mt; e += CH_THR) {
+            const int ti = e % mt, tj = e / mt;
+            if (ti > tj) continue;
+            double acc[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+            for (int t = 0; t < nb; ++t) {
+                double pi[4], pj[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    pi[a] = 4 * ti + a < m ? P[t][4 * ti + a] : 0.0;
+                    pj[a] = 4 * tj + a < m ? P[t][4 * tj + a] : 0.0;
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc[a][b] += pi[a] * pj[b];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int i = 4 * ti + a, j = 4 * tj + b;
+                    if (i < m && j < m && i <= j) A[(c0 + i) + (int64_t)(c0 + j) * n] -= acc[a][b];
+                }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *info = 0;
+}
+
 // ok[0] = 1 when both Cholesky factorizations succeeded and every diagonal
 // entry of R2 is within 1e-3 of 1 (one workgroup)
 __global__ __launch_bounds__(256) void cholqr_check_kernel(const double *R2, int64_t n, const int *info, int *ok) {
@@ -185,7 +276,12 @@ static bool cholqr2(int64_t d, int64_t n, SolveWork<T> &w, hipStream_t st) {
         blas_ok(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Nn, Nn, D, &one, w.QR, D, w.QR,
                               D, &zero, Rk[pass], Nn),
                 "gemm");
-        blas_ok(rocsolver_dpotrf(h, rocblas_fill_upper, Nn, Rk[pass], Nn, w.info + pass), "potrf");
+        if (n <= CH_MAX) {
+            hipLaunchKernelGGL(chol_blocked_kernel, dim3(1), dim3(CH_THR), 0, st, Rk[pass], (int)n, w.info + pass);
+            RBX_HIP(hipGetLastError());
+        } else {
+            blas_ok(rocsolver_dpotrf(h, rocblas_fill_upper, Nn, Rk[pass], Nn, w.info + pass), "potrf");
+        }
         blas_ok(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                               rocblas_diagonal_non_unit, D, Nn, &one, Rk[pass], Nn, w.QR, D),
                 "trsm");
