@@ -1024,7 +1024,7 @@ static void launch_f32_wide(bool gauss, bool rows, dim3 grid, const F32WideProbl
 // (-1: the wide kernel does not apply and nothing ran).
 int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream) {
     if (!pb.gen_is_x || pb.sbuf != nullptr || (pb.g.b_off & 3) != 0) return -1;
-    if (pb.K < 4096 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 256) return -1;
+    if (pb.K < 1024 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 256) return -1;
     // the widest tile (1024, 512 or 256 columns) the output holds; the
     // columns past the last whole tile run on the tile kernels
     const int cb = pb.N >= 1024 ? 4 : (pb.N >= 512 ? 2 : 1);
@@ -1044,6 +1044,10 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     if (kw_fill < kw) kw = kw_fill;
     kw = (kw + 31) / 32 * 32;
     if (kw < 32) kw = 32;
+    // short K over a large output: thin K-splits would cost more partial
+    // traffic than the tile kernels' plain pass (measured at 4096 x 1024,
+    // K = 1024)
+    if (K < 2048 && kw < 256 && Mf * Nf > (1ll << 20)) return -1;
     w.kw = kw;
     w.nS = (int)((K + kw - 1) / kw);
     if ((int64_t)w.nMB * w.nNB * w.nS >= (1ll << 31)) return -1;
@@ -1321,7 +1325,7 @@ static void launch_f64_wide(bool gauss, bool rows, dim3 grid, const F64WideProbl
 
 int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream) {
     if (!pb.gen_is_x || pb.sbuf != nullptr || (pb.g.b_off & 3) != 0) return -1;
-    if (pb.K < 4096 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 128) return -1;
+    if (pb.K < 1024 || pb.K % 32 != 0 || pb.M < 64 || pb.N < 128) return -1;
     const int cb = pb.N >= 512 ? 4 : (pb.N >= 256 ? 2 : 1);
     const int64_t NT = 128 * cb;
     const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / NT * NT, K = pb.K;
@@ -1339,6 +1343,7 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     if (kw_fill < kw) kw = kw_fill;
     kw = (kw + 31) / 32 * 32;
     if (kw < 32) kw = 32;
+    if (K < 2048 && kw < 256 && Mf * Nf > (1ll << 20)) return -1;  // as in f32_wide_run
     w.kw = kw;
     w.nS = (int)((K + kw - 1) / kw);
     if ((int64_t)w.nMB * w.nNB * w.nS >= (1ll << 31)) return -1;

@@ -545,7 +545,8 @@ static void launch_skge(const GemmProblem &pb, hipStream_t stream) {
     GemmProblem rest[2];
     auto wide = [&](const GemmProblem &q) {
         const int nr = sizeof(T) == 4 ? f32_wide_run(q, rest, stream) : f64_wide_run(q, rest, stream);
-        for (int i = 0; i < nr; ++i) launch_skge_tiles<T>(rest[i], stream);
+        for (int i = 0; i < nr; ++i)
+            if (!gemv_run<T>(rest[i], stream)) launch_skge_tiles<T>(rest[i], stream);
         return nr >= 0;
     };
     if (wide(pb)) return;

@@ -418,8 +418,11 @@ def test_wide_tiles_beta_zero_ignores_nan(rb, dtype):
 # 256- / 128-column (FP64) tiles; the columns past the last whole tile run on
 # the tile kernels.
 @pytest.mark.parametrize("dtype,tol,n", [(np.float32, 1e-5, 300), (np.float32, 1e-5, 600),
-                                         (np.float64, 1e-13, 200), (np.float64, 1e-13, 300)])
+                                         (np.float64, 1e-13, 200), (np.float64, 1e-13, 300),
+                                         (np.float32, 1e-5, 257), (np.float64, 1e-13, 129)])
 def test_lskge3_narrow_wide_tiles(rb, orc, dtype, tol, n):
+    # n = 257 / 129: the one column past the last whole tile runs on the
+    # one-column kernel
     got, ref = run_lskge3(rb, orc, dtype, "C", "N", "N", 130, n, 8224, 0.75, -0.5, ro=2, co=4)
     assert rel_fro(got, ref) <= tol
 
