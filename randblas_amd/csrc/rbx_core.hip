@@ -129,6 +129,30 @@ __global__ __launch_bounds__(256) void fill_dense_kernel(GenMap g, int64_t n_p, 
     }
 }
 
+// The same entries when the major (b) direction is unit-stride, every cell
+// is whole (b_off and n_b multiples of 4) and rows start 16-byte aligned:
+// a 2-D grid (cells x minor indices, no index division) and one 16-byte
+// store per counter (FP32; two for FP64).
+template <typename T>
+__global__ __launch_bounds__(256) void fill_dense_vec_kernel(GenMap g, T *buff, int64_t stride_a, int64_t cells,
+                                                             int64_t n_a) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const bool gauss = g.gauss != 0;
+    const int64_t cell = g.b_off / 4 + c;
+    for (int64_t v = blockIdx.y; v < n_a; v += gridDim.y) {
+        const float4 w = gen4(gauss, ctr_add(g.ctr, (uint64_t)((g.a_off + v) * g.P + cell)), g.k0, g.k1);
+        T *dst = buff + v * stride_a + 4 * c;
+        if (sizeof(T) == 4) {
+            *(float4 *)dst = make_float4(entry_value<float>(w.x, gauss), entry_value<float>(w.y, gauss),
+                                         entry_value<float>(w.z, gauss), entry_value<float>(w.w, gauss));
+        } else {
+            *(double2 *)dst = make_double2(entry_value<double>(w.x, gauss), entry_value<double>(w.y, gauss));
+            *(double2 *)(dst + 2) = make_double2(entry_value<double>(w.z, gauss), entry_value<double>(w.w, gauss));
+        }
+    }
+}
+
 // The same entries for an output whose unit stride runs along the minor
 // (a) direction, where one thread per counter would write 4 strided values:
 // a workgroup generates a tile of 64 minor indices x 16 counters (64 major
@@ -175,7 +199,14 @@ void fill_generated(const GenMap &g, int64_t n_p, int64_t n_q, T *buff, int64_t 
     int64_t cells = (g.b_off + n_b - 1) / 4 - cell_first + 1;
     const int64_t stride_a = g.b_along_rows ? cs : rs, stride_b = g.b_along_rows ? rs : cs;
     ProfScope prof("fill_dense_kernel", stream);
-    if (stride_a == 1 && stride_b != 1 && n_a > 1) {
+    if (stride_b == 1 && g.b_off % 4 == 0 && n_b % 4 == 0 && stride_a % 4 == 0 && ((uintptr_t)buff & 15) == 0 &&
+        (n_a == 1 || stride_a >= n_b)) {
+        const int64_t gx = (cells + 255) / 256;
+        const int64_t gy = n_a < 4096 ? n_a : 4096;
+        RBX_REQUIRE(gx < (1ll << 31));
+        hipLaunchKernelGGL(fill_dense_vec_kernel<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, g, buff,
+                           stride_a, cells, n_a);
+    } else if (stride_a == 1 && stride_b != 1 && n_a > 1) {
         const int64_t gx = (n_a + FT_A - 1) / FT_A, gy = (cells + FT_C - 1) / FT_C;
         RBX_REQUIRE(gx < (1ll << 31) && gy <= 65535);
         hipLaunchKernelGGL(fill_dense_tr_kernel<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, g, n_p,
