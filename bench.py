@@ -90,10 +90,8 @@ def main():
 
     # ---- inputs (resident in HBM before timing) ----
     log("generating A")
-    A = torch.empty(M * N, dtype=torch.float32, device=dev)
-    rb.fill_dense(rb.DenseDist(M, N), A, rb.RNGState(KEY_A))   # natural ColMajor
-    scales = torch.pow(10.0, -6.0 * torch.arange(N, device=dev, dtype=torch.float64) / (N - 1)).float()
-    A.view(N, M).mul_(scales.view(N, 1))
+    from randblas_amd.datagen import conditioned_gaussian
+    A = conditioned_gaussian(M, N, 1e6, KEY_A, dev)   # natural ColMajor
     S = rb.DenseSkOp(rb.DenseDist(D, M), rb.RNGState(KEY_S))
     B = torch.empty(D * N, dtype=torch.float32, device=dev)
     if args.copy_probe_gib > 0:
@@ -191,6 +189,14 @@ def main():
         del A64, B64
         torch.cuda.synchronize()
 
+        # ---- BF16 result against the FP64 product with the same (bf16) operator, all rows ----
+        out["rel_err_vs_fp64_same_operator_all_rows"] = bf16_error_all_rows(rb, torch, dev, A, B)
+        log(f"bf16 vs same-operator FP64, all {D} rows: {out['rel_err_vs_fp64_same_operator_all_rows']:.3e}")
+        meas, assumed = rb.bf16_fastgen_bounds()
+        out["bf16_fast_generator"] = {"max_abs_sin_cos_err": max(meas[0], meas[1]), "max_rel_radius_err": meas[2],
+                                      "zero_mismatches": meas[3], "assumed_sin_cos": assumed[0],
+                                      "assumed_radius": assumed[1], "words": 2 ** 32}
+
         # ---- plain FP32 fused sketch of the same product ----
         B32 = torch.empty(D * N, dtype=torch.float32, device=dev)
         rb.lskge3("C", "N", "N", D, N, M, 1.0, S, 0, 0, A, M, 0.0, B32, D)
@@ -247,7 +253,9 @@ def main():
             out["rel_err_vs_fp64_same_operator"] = rel(got, ref_same)
             out["rel_err_vs_fp64_reference_operator"] = rel(got, ref_op)
             out["fp64_rel_err_vs_oracle"] = rel(got64, ref_op)
-            out["error_sample"] = f"rows 0..{rows - 1} of B against the CPU oracle in FP64"
+            out["error_sample"] = (f"rel_err_*: rows 0..{rows - 1} of B against the CPU oracle in FP64; "
+                                   f"_all_rows: all rows against an FP64 GPU GEMM of the bf16-rounded operator")
+            out["gaussian_vs_host_libm"] = libm_distance(rb, torch, dev, np, oracle)
             log(f"rel err: bf16 vs same-operator FP64 {out['rel_err_vs_fp64_same_operator']:.3e}, vs reference "
                 f"operator {out['rel_err_vs_fp64_reference_operator']:.3e}; fp64 path {out['fp64_rel_err_vs_oracle']:.3e}")
             # distortion of (1/sqrt d) S Q with A = QR: the eigenvalues of the
@@ -262,6 +270,52 @@ def main():
             log(f"distortion bf16 {out['distortion_bf16']}, fp64 {out['distortion_fp64']}")
 
     print(json.dumps(out), flush=True)
+
+
+def _bf16_round(torch, x):
+    """FP32 -> bfloat16 (nearest, ties to even), returned as FP32."""
+    u = x.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    u = torch.where(u >= 2 ** 31, u - 2 ** 32, u)
+    return u.to(torch.int32).view(torch.float32)
+
+
+def bf16_error_all_rows(rb, torch, dev, A, B, rows=256):
+    """Relative Frobenius error of the BF16-split result B over all D rows
+    against the FP64 product of the same operator: S generated in FP32 by
+    fill_dense, rounded to bfloat16, promoted, times A in FP64 (rocBLAS
+    through torch), 256 rows at a time."""
+    got = B.view(N, D).t().double()
+    A64 = A.view(N, M).double()
+    Sf = torch.empty(rows * M, dtype=torch.float32, device=dev)
+    num = den = 0.0
+    for r0 in range(0, D, rows):
+        rb.fill_dense_unpacked("R", rb.DenseDist(D, M), rows, M, r0, 0, Sf, rb.RNGState(KEY_S))
+        ref = _bf16_round(torch, Sf).view(rows, M).double() @ A64.t()
+        num += float(torch.linalg.norm(got[r0:r0 + rows] - ref)) ** 2
+        den += float(torch.linalg.norm(ref)) ** 2
+        del ref
+    del got, A64, Sf
+    torch.cuda.synchronize()
+    return (num / den) ** 0.5
+
+
+def libm_distance(rb, torch, dev, np, oracle, rows=64):
+    """Distance in float32 ulps of the device Gaussians (rows 0..63 of the
+    headline operator, 16.8M entries) from the Box-Muller the reference
+    computes over this host's C library (the oracle's mathmode 1)."""
+    buf = torch.empty(rows * M, dtype=torch.float32, device=dev)
+    rb.fill_dense_unpacked("R", rb.DenseDist(D, M), rows, M, 0, 0, buf, rb.RNGState(KEY_S))
+    got = buf.cpu().numpy()
+    ref, _ = oracle.fill_dense_unpacked("R", D, M, rows, M, 0, 0, [0, 0, 0, 0], [KEY_S, 0], dtype=np.float32,
+                                        mathmode=1)
+
+    def key(x):
+        i = np.asarray(x, np.float32).view(np.int32).astype(np.int64)
+        return np.where(i < 0, -(i & 0x7FFFFFFF), i)
+    dist = np.abs(key(got) - key(ref.reshape(-1)))
+    return {"entries": int(dist.size), "max_ulp": int(dist.max()), "frac_above_2_ulp": float((dist > 2).mean()),
+            "frac_bit_identical": float((dist == 0).mean())}
 
 
 def sparse_leg(rb, torch, dev, reps=10):

@@ -242,6 +242,18 @@ rbx_status rbx_slskge3_bf16split(char layout, char opS, char opA, int64_t d, int
                                  int64_t co_s, const float *A, int64_t lda, float beta,
                                  float *B, int64_t ldb, int n_terms, void *stream);
 
+/* Self-check of the BF16 operator's fast generator (no reference
+ * counterpart).  The BF16-split kernels draw Gaussian entries with the
+ * hardware sin / cos / log / sqrt and keep a value only when it provably
+ * rounds to the same bfloat16 as the portable (exact) path, using the
+ * per-factor error bounds assumed[0] (|s' - s|, |c' - c|, absolute) and
+ * assumed[1] (|r' - r| / r).  This call sweeps all 2^32 input words on the
+ * device and returns measured[0..3] = max |s' - s|, max |c' - c|,
+ * max |r' - r| / r and the number of words where exactly one of r, r' is
+ * zero.  The generator is exact when measured[0..1] <= assumed[0],
+ * measured[2] <= assumed[1] and measured[3] == 0.  Synchronises `stream`. */
+rbx_status rbx_bf16_fastgen_bounds(double *measured, double *assumed, void *stream);
+
 /* ---------------------------------------------------------------- sparse operators
  * SparseSkOp(dist, seed_state) (sparse_skops.hh:356-365) with next_state from
  * compute_next_state (sparse_skops.hh:266-283); nnz = -1, arrays NULL. */
@@ -469,6 +481,19 @@ rbx_status rbx_dsketch_solve_adjoint(char layout, int64_t d, int64_t m, int64_t 
                                      int64_t ro_s, int64_t co_s, const double *A, int64_t lda,
                                      const double *b, const double *xbar, double *Abar, int64_t ldab,
                                      double *bbar, void *stream);
+
+/* ---------------------------------------------------------------- workspaces
+ * (no reference counterpart: the reference allocates its temporaries per call,
+ * skge.hh:175-181)  The library keeps grow-only device scratch and a rocBLAS
+ * handle per (device, stream), so calls on different streams are independent
+ * and may run concurrently; host threads that use one stream take turns.
+ * rbx_release_workspace frees the calling device's workspace of `stream`
+ * (stream-ordered: after the work already enqueued there); call it before
+ * destroying a stream the library has used.  rbx_release_all_workspaces
+ * synchronises every device the library used and frees everything; call it
+ * with no library work in flight. */
+rbx_status rbx_release_workspace(void *stream);
+rbx_status rbx_release_all_workspaces(void);
 
 /* ---------------------------------------------------------------- diagnostics
  * (no reference counterpart) Kernel timing for benchmarks: while enabled,

@@ -180,6 +180,9 @@ _SIG = {
     **{f"rbx_{p}sketch_solve": [_CH, _I64, _I64, _I64, _DS, _I64, _I64, _P, _I64, _P, _P, _P] for p in ("s", "d")},
     **{f"rbx_{p}sketch_solve_adjoint": [_CH, _I64, _I64, _I64, _DS, _I64, _I64, _P, _I64, _P, _P, _P, _I64, _P, _P]
        for p in ("s", "d")},
+    "rbx_bf16_fastgen_bounds": [_P, _P, _P],
+    "rbx_release_workspace": [_P],
+    "rbx_release_all_workspaces": [],
     "rbx_profile_enable": [C.c_int],
     "rbx_profile_collect": [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int],
 }
@@ -231,11 +234,77 @@ def _ptr(t) -> C.c_void_p:
 
 def _prefix(t) -> str:
     import torch
+    if not isinstance(t, torch.Tensor):
+        raise Error(f"expected a torch tensor, got {type(t).__name__}")
     if t.dtype == torch.float64:
         return "d"
     if t.dtype == torch.float32:
         return "s"
     raise Error(f"unsupported dtype {t.dtype}")
+
+
+# ---------------------------------------------------------------- operand checks
+# The C ABI takes raw device pointers, so a wrong dtype or a short buffer
+# would be read as other bits or out of bounds.  Every wrapper checks its
+# operands here, before any pointer reaches the library (the reference's
+# typed DenseSkOp<T> / SparseSkOp<T> rule out the dtype mismatch at compile
+# time).
+def _extent(rows, cols, layout, ld) -> int:
+    """Elements spanned by a rows x cols matrix stored in `layout` with
+    leading dimension ld (0 for an empty matrix)."""
+    rows, cols, ld = int(rows), int(cols), int(ld)
+    if rows <= 0 or cols <= 0:
+        return 0
+    return (cols - 1) * ld + rows if _c(layout) == b"C" else (rows - 1) * ld + cols
+
+
+def _need(name, t, dtype, numel) -> None:
+    """t must be a contiguous CUDA tensor of `dtype` (a dtype or a tuple of
+    them) holding at least `numel` elements."""
+    import torch
+    numel = int(numel)
+    if t is None:
+        if numel > 0:
+            raise Error(f"{name}: a buffer of {numel} elements is required")
+        return
+    if not isinstance(t, torch.Tensor):
+        raise Error(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    allowed = dtype if isinstance(dtype, tuple) else (dtype,)
+    if t.dtype not in allowed:
+        raise Error(f"{name}: dtype {t.dtype}, expected {' or '.join(str(d) for d in allowed)}")
+    if not t.is_contiguous():
+        raise Error(f"{name}: the tensor must be contiguous")
+    if t.numel() < numel:
+        raise Error(f"{name}: {t.numel()} elements, the call addresses {numel}")
+    # the device itself is checked by _ptr, when the pointer is taken
+
+
+def _fp(prefix):
+    import torch
+    return torch.float64 if prefix == "d" else torch.float32
+
+
+def _i64():
+    import torch
+    return torch.int64
+
+
+def _need_dense_op(S: "DenseSkOp", prefix: str) -> None:
+    if S.buff is not None:
+        _need("S.buff", S.buff, _fp(prefix), S.n_rows * S.n_cols)
+
+
+def _need_sparse_op(S: "SparseSkOp", prefix: str) -> None:
+    if S.vals is None or S.nnz < 0:
+        return  # unsampled: the library samples it into its own scratch
+    _need("S.vals", S.vals, _fp(prefix), S.nnz)
+    _need("S.rows", S.rows, _i64(), S.nnz)
+    _need("S.cols", S.cols, _i64(), S.nnz)
+
+
+def _opdims(op, r, c):
+    """(rows, cols) of the stored matrix whose op() is r x c."""
+    return (r, c) if _c(op) == b"N" else (c, r)
 
 
 # ---------------------------------------------------------------- RNGState
@@ -274,7 +343,8 @@ class RNGState:
 
 def philox4x32_10(state: RNGState, n_counters: int, out) -> None:
     """Raw words for counters state.counter + i into a uint32/int32 GPU tensor of 4*n words."""
-    assert out.numel() >= 4 * n_counters
+    import torch
+    _need("out", out, (torch.int32, torch.uint32), 4 * n_counters)
     s = state.c()
     _call("rbx_philox4x32_10", C.byref(s), n_counters, _ptr(out), _stream())
 
@@ -317,9 +387,17 @@ class DenseDist:
 class DenseSkOp:
     """DenseSkOp<T> (RandBLAS/dense_skops.hh:357-478).  buff is None until
     fill_dense(S) materialises it; sketches with buff None generate the operator
-    inside the GEMM kernels."""
+    inside the GEMM kernels.  dtype plays the role of T: it is the dtype of
+    buff (float64 when neither is given), and a materialised operator can only
+    be applied to matrices of that dtype."""
 
-    def __init__(self, dist: DenseDist, seed_state: RNGState, buff=None, layout: Optional[str] = None):
+    def __init__(self, dist: DenseDist, seed_state: RNGState, buff=None, layout: Optional[str] = None,
+                 dtype=None):
+        import torch
+        if buff is not None and dtype is not None and buff.dtype != dtype:
+            raise Error(f"DenseSkOp: buff has dtype {buff.dtype}, dtype says {dtype}")
+        self.dtype = buff.dtype if buff is not None else (dtype or torch.float64)
+        self._dtype_given = buff is not None or dtype is not None
         self.dist = dist
         self.seed_state = seed_state
         out = c_dense_skop()
@@ -343,6 +421,7 @@ class DenseSkOp:
 def fill_dense_unpacked(layout, D: DenseDist, n_rows, n_cols, ro_s, co_s, buff, seed: RNGState) -> RNGState:
     """fill_dense_unpacked (dense_skops.hh:563-606); buff is a GPU tensor."""
     nxt = c_rng_state()
+    _need("buff", buff, _fp(_prefix(buff)), int(n_rows) * int(n_cols))
     _call(f"rbx_{_prefix(buff)}fill_dense_unpacked", _c(layout), C.byref(D.c()), n_rows, n_cols, ro_s, co_s,
           _ptr(buff), C.byref(seed.c()), C.byref(nxt), _stream())
     return RNGState.from_c(nxt)
@@ -354,8 +433,12 @@ def fill_dense(D_or_S, buff=None, seed: Optional[RNGState] = None, dtype=None):
     import torch
     if isinstance(D_or_S, DenseSkOp):
         S = D_or_S
+        if dtype is not None and not S._dtype_given:
+            S.dtype = dtype
+        if dtype is not None and dtype != S.dtype:
+            raise Error(f"fill_dense: the operator holds {S.dtype} entries, not {dtype}")
         if S.buff is None:
-            S.buff = torch.empty(S.n_rows * S.n_cols, dtype=dtype or torch.float64, device="cuda")
+            S.buff = torch.empty(S.n_rows * S.n_cols, dtype=S.dtype, device="cuda")
         fill_dense_unpacked(S.layout, S.dist, S.n_rows, S.n_cols, 0, 0, S.buff, S.seed_state)
         return None
     D = D_or_S
@@ -364,24 +447,44 @@ def fill_dense(D_or_S, buff=None, seed: Optional[RNGState] = None, dtype=None):
 
 def lskge3(layout, opS, opA, d, n, m, alpha, S: DenseSkOp, ro_s, co_s, A, lda, beta, B, ldb):
     """lskge3 (skge.hh:154-202)."""
+    p = _prefix(B)
+    _need_dense_op(S, p)
+    if alpha != 0:
+        _need("A", A, _fp(p), _extent(*_opdims(opA, m, n), layout, lda))
+    _need("B", B, _fp(p), _extent(d, n, layout, ldb))
     _call(f"rbx_{_prefix(B)}lskge3", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
           _ptr(A), lda, beta, _ptr(B), ldb, _stream())
 
 
 def rskge3(layout, opA, opS, m, d, n, alpha, A, lda, S: DenseSkOp, ro_s, co_s, beta, B, ldb):
     """rskge3 (skge.hh:307-355)."""
+    p = _prefix(B)
+    _need_dense_op(S, p)
+    if alpha != 0:
+        _need("A", A, _fp(p), _extent(*_opdims(opA, m, n), layout, lda))
+    _need("B", B, _fp(p), _extent(m, d, layout, ldb))
     _call(f"rbx_{_prefix(B)}rskge3", _c(layout), _c(opA), _c(opS), m, d, n, alpha, _ptr(A), lda, C.byref(S.c()),
           ro_s, co_s, beta, _ptr(B), ldb, _stream())
 
 
 def lskge3_adjoint(layout, opS, d, n, m, alpha, S: DenseSkOp, ro_s, co_s, Bbar, ldbb, beta, Abar, ldab):
     """Abar = alpha * op(submat(S))^T * Bbar + beta * Abar (operator regenerated)."""
+    p = _prefix(Abar)
+    _need_dense_op(S, p)
+    if alpha != 0:
+        _need("Bbar", Bbar, _fp(p), _extent(d, n, layout, ldbb))
+    _need("Abar", Abar, _fp(p), _extent(m, n, layout, ldab))
     _call(f"rbx_{_prefix(Abar)}lskge3_adjoint", _c(layout), _c(opS), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
           _ptr(Bbar), ldbb, beta, _ptr(Abar), ldab, _stream())
 
 
 def rskge3_adjoint(layout, opS, m, d, n, alpha, Bbar, ldbb, S: DenseSkOp, ro_s, co_s, beta, Abar, ldab):
     """Abar = alpha * Bbar * op(submat(S))^T + beta * Abar (operator regenerated)."""
+    p = _prefix(Abar)
+    _need_dense_op(S, p)
+    if alpha != 0:
+        _need("Bbar", Bbar, _fp(p), _extent(m, d, layout, ldbb))
+    _need("Abar", Abar, _fp(p), _extent(m, n, layout, ldab))
     _call(f"rbx_{_prefix(Abar)}rskge3_adjoint", _c(layout), _c(opS), m, d, n, alpha, _ptr(Bbar), ldbb,
           C.byref(S.c()), ro_s, co_s, beta, _ptr(Abar), ldab, _stream())
 
@@ -389,6 +492,10 @@ def rskge3_adjoint(layout, opS, m, d, n, alpha, Bbar, ldbb, S: DenseSkOp, ro_s, 
 def lskge3_bf16split(layout, opS, opA, d, n, m, alpha, S: DenseSkOp, ro_s, co_s, A, lda, beta, B, ldb,
                      n_terms: int = 3):
     """B = alpha * op(bf16(submat S)) * op(A) + beta * B on the BF16 matrix cores."""
+    import torch
+    if alpha != 0:
+        _need("A", A, torch.float32, _extent(*_opdims(opA, m, n), layout, lda))
+    _need("B", B, torch.float32, _extent(d, n, layout, ldb))
     _call("rbx_slskge3_bf16split", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
           _ptr(A), lda, beta, _ptr(B), ldb, n_terms, _stream())
 
@@ -466,6 +573,9 @@ def fill_sparse(S: SparseSkOp) -> None:
     nnz = C.c_int64()
     nxt = c_rng_state()
     pre = "d" if S.dtype == torch.float64 else "s"
+    _need("S.vals", S.vals, S.dtype, full)
+    _need("S.rows", S.rows, torch.int64, full)
+    _need("S.cols", S.cols, torch.int64, full)
     _call(f"rbx_{pre}fill_sparse_unpacked_nosub", C.byref(S.dist.c()), C.byref(nnz), _ptr(S.vals), _ptr(S.rows),
           _ptr(S.cols), C.byref(S.seed_state.c()), C.byref(nxt), _stream())
     S.nnz = nnz.value
@@ -473,6 +583,10 @@ def fill_sparse(S: SparseSkOp) -> None:
 
 def fill_sparse_unpacked_nosub(D: SparseDist, vals, rows, cols, seed: RNGState):
     """Returns (nnz, next_state)."""
+    full = D.full_nnz
+    _need("vals", vals, _fp(_prefix(vals)), full)
+    _need("rows", rows, _i64(), full)
+    _need("cols", cols, _i64(), full)
     nnz = C.c_int64()
     nxt = c_rng_state()
     _call(f"rbx_{_prefix(vals)}fill_sparse_unpacked_nosub", C.byref(D.c()), C.byref(nnz), _ptr(vals), _ptr(rows),
@@ -482,12 +596,26 @@ def fill_sparse_unpacked_nosub(D: SparseDist, vals, rows, cols, seed: RNGState):
 
 def lskges(layout, opS, opA, d, n, m, alpha, S: SparseSkOp, ro_s, co_s, A, lda, beta, B, ldb):
     """lskges (skge.hh:465-492)."""
+    p = _prefix(B)
+    if S.vals is not None and S.dtype != _fp(p):
+        raise Error(f"lskges: the operator holds {S.dtype} values, B is {B.dtype}")
+    _need_sparse_op(S, p)
+    if alpha != 0:
+        _need("A", A, _fp(p), _extent(*_opdims(opA, m, n), layout, lda))
+    _need("B", B, _fp(p), _extent(d, n, layout, ldb))
     _call(f"rbx_{_prefix(B)}lskges", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
           _ptr(A), lda, beta, _ptr(B), ldb, _stream())
 
 
 def rskges(layout, opA, opS, m, d, n, alpha, A, lda, S: SparseSkOp, ro_s, co_s, beta, B, ldb):
     """rskges (skge.hh:598-626)."""
+    p = _prefix(B)
+    if S.vals is not None and S.dtype != _fp(p):
+        raise Error(f"rskges: the operator holds {S.dtype} values, B is {B.dtype}")
+    _need_sparse_op(S, p)
+    if alpha != 0:
+        _need("A", A, _fp(p), _extent(*_opdims(opA, m, n), layout, lda))
+    _need("B", B, _fp(p), _extent(m, d, layout, ldb))
     _call(f"rbx_{_prefix(B)}rskges", _c(layout), _c(opA), _c(opS), m, d, n, alpha, _ptr(A), lda, C.byref(S.c()),
           ro_s, co_s, beta, _ptr(B), ldb, _stream())
 
@@ -535,6 +663,22 @@ class COOMatrix:
                             self.rows.data_ptr(), self.cols.data_ptr())
 
 
+def _need_sparse_data(A, prefix: str, name: str = "A") -> None:
+    """Arrays of a CSR / CSC / COO matrix: values of the call's precision,
+    int64 indices, pointer arrays of n + 1 entries."""
+    nnz = int(A.vals.numel()) if A.vals is not None else 0
+    _need(f"{name}.vals", A.vals, _fp(prefix), nnz)
+    if isinstance(A, CSRMatrix):
+        _need(f"{name}.rowptr", A.rowptr, _i64(), A.n_rows + 1)
+        _need(f"{name}.colidxs", A.colidxs, _i64(), nnz)
+    elif isinstance(A, CSCMatrix):
+        _need(f"{name}.colptr", A.colptr, _i64(), A.n_cols + 1)
+        _need(f"{name}.rowidxs", A.rowidxs, _i64(), nnz)
+    else:
+        _need(f"{name}.rows", A.rows, _i64(), nnz)
+        _need(f"{name}.cols", A.cols, _i64(), nnz)
+
+
 def _sp_suffix(A) -> str:
     if isinstance(A, CSRMatrix):
         return ""
@@ -547,6 +691,11 @@ def _sp_suffix(A) -> str:
 
 def rsksp3(layout, opA, opS, m, d, n, alpha, A, ro_a, co_a, S: DenseSkOp, ro_s, co_s, beta, B, ldb):
     """rsksp3 (sparse_data/sksp.hh:277-326) for a CSRMatrix, CSCMatrix or COOMatrix A."""
+    p = _prefix(B)
+    _sp_suffix(A)
+    _need_sparse_data(A, p)
+    _need_dense_op(S, p)
+    _need("B", B, _fp(p), _extent(m, d, layout, ldb))
     _call(f"rbx_{_prefix(B)}rsksp3{_sp_suffix(A)}", _c(layout), _c(opA), _c(opS), m, d, n, alpha, C.byref(A.c()),
           ro_a, co_a, C.byref(S.c()), ro_s, co_s, beta, _ptr(B), ldb, _stream())
 
@@ -555,6 +704,11 @@ def rsksp3(layout, opA, opS, m, d, n, alpha, A, ro_a, co_a, S: DenseSkOp, ro_s, 
 def lsksp3(layout, opS, opA, d, n, m, alpha, S: DenseSkOp, ro_s, co_s, A, ro_a, co_a, beta, B, ldb):
     """lsksp3 (sparse_data/sksp.hh:132-180): B = alpha * op(submat(S)) * op(A) + beta * B,
     A a CSRMatrix, CSCMatrix or COOMatrix."""
+    p = _prefix(B)
+    _sp_suffix(A)
+    _need_sparse_data(A, p)
+    _need_dense_op(S, p)
+    _need("B", B, _fp(p), _extent(d, n, layout, ldb))
     _call(f"rbx_{_prefix(B)}lsksp3{_sp_suffix(A)}", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()),
           ro_s, co_s, C.byref(A.c()), ro_a, co_a, beta, _ptr(B), ldb, _stream())
 
@@ -574,6 +728,13 @@ def sketch_sparse(layout, op1, op2, a1, a2, a3, alpha, X, *rest):
 def rsksp3_adjoint(layout, opS, m, d, n, alpha, A: CSRMatrix, S: DenseSkOp, ro_s, co_s, Bbar, ldbb, dvals):
     """Gradient of B = alpha * A * op(submat(S)) with respect to A's stored values:
     dvals[k] = alpha * sum_j Bbar[i_k, j] * op(submat(S))[c_k, j]."""
+    p = _prefix(dvals)
+    if not isinstance(A, CSRMatrix):
+        raise TypeError(f"rsksp3_adjoint: expected a CSRMatrix, got {type(A).__name__}")
+    _need_sparse_data(A, p)
+    _need_dense_op(S, p)
+    _need("Bbar", Bbar, _fp(p), _extent(m, d, layout, ldbb))
+    _need("dvals", dvals, _fp(p), int(A.vals.numel()))
     _call(f"rbx_{_prefix(dvals)}rsksp3_adjoint", _c(layout), _c(opS), m, d, n, alpha, C.byref(A.c()),
           C.byref(S.c()), ro_s, co_s, _ptr(Bbar), ldbb, _ptr(dvals), _stream())
 
@@ -674,10 +835,24 @@ def raise_if(cond: bool, msg: str):
 def require_symmetric(layout, A, n, lda, tol):
     """util::require_symmetric (util.hh:127-148), evaluated on the device by
     rbx_[sd]require_symmetric; raises Error with the reference's message."""
+    _need("A", A, _fp(_prefix(A)), _extent(n, n, layout, lda))
     _call(f"rbx_{_prefix(A)}require_symmetric", _c(layout), _ptr(A), n, lda, tol, _stream())
 
 
 def _conv(name, src, dst):
+    p = _prefix(src.vals)
+    _need_sparse_data(src, p, "src")
+    nnz = int(src.vals.numel())
+    _need("dst.vals", dst.vals, _fp(p), nnz)
+    if isinstance(dst, CSRMatrix):
+        _need("dst.rowptr", dst.rowptr, _i64(), src.n_rows + 1)
+        _need("dst.colidxs", dst.colidxs, _i64(), nnz)
+    elif isinstance(dst, CSCMatrix):
+        _need("dst.colptr", dst.colptr, _i64(), src.n_cols + 1)
+        _need("dst.rowidxs", dst.rowidxs, _i64(), nnz)
+    else:
+        _need("dst.rows", dst.rows, _i64(), nnz)
+        _need("dst.cols", dst.cols, _i64(), nnz)
     a, b = src.c(), dst.c()
     _call(f"rbx_{_prefix(src.vals)}{name}", C.byref(a), C.byref(b), _stream())
 
@@ -704,12 +879,14 @@ def coo_to_csc(coo: "COOMatrix", csc: "CSCMatrix") -> None:
 
 def coo_sort(coo: "COOMatrix", order: str) -> None:
     """COOMatrix::sort_arrays (coo_matrix.hh:222-226) in place: order "R" (CSR) or "C" (CSC)."""
+    _need_sparse_data(coo, _prefix(coo.vals), "coo")
     a = coo.c()
     _call(f"rbx_{_prefix(coo.vals)}coo_sort", _c(order), C.byref(a), _stream())
 
 
 def coo_determine_sort(coo: "COOMatrix") -> str:
     """coo_arrays_determine_sort (sparse_data/base.hh:142-167): "C", "R" or "N"."""
+    _need_sparse_data(coo, _prefix(coo.vals), "coo")
     out = C.create_string_buffer(1)
     _call("rbx_coo_determine_sort", C.byref(coo.c()), out, _stream())
     return out.raw.decode()
@@ -719,6 +896,9 @@ def sample_indices_iid_uniform(n, k, samples, state: RNGState, rademachers=None)
     """sample_indices_iid_uniform (util.hh:516-560): k iid indices in [0, n) into
     the int64 device tensor `samples`, and (when given) Rademacher signs into
     `rademachers`; returns the advanced state."""
+    _need("samples", samples, _i64(), k)
+    if rademachers is not None:
+        _need("rademachers", rademachers, _fp(_prefix(rademachers)), k)
     nxt = c_rng_state()
     if rademachers is None:
         _call("rbx_sample_indices_iid_uniform", n, k, _ptr(samples), C.byref(state.c()), C.byref(nxt), _stream())
@@ -731,6 +911,8 @@ def sample_indices_iid_uniform(n, k, samples, state: RNGState, rademachers=None)
 def sample_indices_iid(n, cdf, k, samples, state: RNGState) -> RNGState:
     """sample_indices_iid (util.hh:493-514): k iid draws from the CDF `cdf`
     (a float32 / float64 device tensor of n entries); returns the advanced state."""
+    _need("cdf", cdf, _fp(_prefix(cdf)), n)
+    _need("samples", samples, _i64(), k)
     nxt = c_rng_state()
     _call(f"rbx_{_prefix(cdf)}sample_indices_iid", n, _ptr(cdf), k, _ptr(samples), C.byref(state.c()),
           C.byref(nxt), _stream())
@@ -740,6 +922,7 @@ def sample_indices_iid(n, cdf, k, samples, state: RNGState) -> RNGState:
 def weights_to_cdf(n, w, error_if_below=None) -> None:
     """weights_to_cdf (util.hh:459-472) in place on a float32 / float64 device
     tensor; error_if_below defaults to -sqrt(epsilon) of w's type."""
+    _need("w", w, _fp(_prefix(w)), n)
     if error_if_below is None:
         import numpy as _np
         error_if_below = -float(_np.sqrt(_np.finfo(_np.float32 if _prefix(w) == "s" else _np.float64).eps))
@@ -751,6 +934,11 @@ def sketch_solve(layout, d, m, n, S: DenseSkOp, ro_s, co_s, A, lda, b, x) -> Non
     (ro_s, co_s), A m x n in `layout`: the sketch of [A b] as in
     examples/total-least-squares/tls_dense_skop.cc:138-171 with a QR solve of
     the sketched system (FP64 for both precisions)."""
+    p = _prefix(x)
+    _need_dense_op(S, p)
+    _need("A", A, _fp(p), _extent(m, n, layout, lda))
+    _need("b", b, _fp(p), m)
+    _need("x", x, _fp(p), n)
     _call(f"rbx_{_prefix(x)}sketch_solve", _c(layout), d, m, n, C.byref(S.c()), ro_s, co_s, _ptr(A), lda, _ptr(b),
           _ptr(x), _stream())
 
@@ -758,8 +946,38 @@ def sketch_solve(layout, d, m, n, S: DenseSkOp, ro_s, co_s, A, lda, b, x) -> Non
 def sketch_solve_adjoint(layout, d, m, n, S: DenseSkOp, ro_s, co_s, A, lda, b, xbar, Abar, ldab, bbar) -> None:
     """Gradient of <xbar, sketch_solve(A, b)> with respect to A (into Abar, m x n
     in `layout`) and b (into bbar); the operator is regenerated, never stored."""
+    p = _prefix(Abar)
+    _need_dense_op(S, p)
+    _need("A", A, _fp(p), _extent(m, n, layout, lda))
+    _need("b", b, _fp(p), m)
+    _need("xbar", xbar, _fp(p), n)
+    _need("Abar", Abar, _fp(p), _extent(m, n, layout, ldab))
+    _need("bbar", bbar, _fp(p), m)
     _call(f"rbx_{_prefix(Abar)}sketch_solve_adjoint", _c(layout), d, m, n, C.byref(S.c()), ro_s, co_s, _ptr(A), lda,
           _ptr(b), _ptr(xbar), _ptr(Abar), ldab, _ptr(bbar), _stream())
+
+
+def bf16_fastgen_bounds():
+    """Sweep of all 2^32 words through the BF16 operator's fast generator:
+    returns (measured, assumed) with measured = (max |s' - s|, max |c' - c|,
+    max |r' - r| / r, words where exactly one of r, r' is zero) and assumed =
+    (the sin / cos bound, the radius bound) the generator's exactness rests on."""
+    meas = (C.c_double * 4)()
+    assumed = (C.c_double * 2)()
+    _call("rbx_bf16_fastgen_bounds", meas, assumed, _stream())
+    return tuple(meas), tuple(assumed)
+
+
+def release_workspace(stream=None) -> None:
+    """Frees the library's scratch and rocBLAS handle kept for `stream` (a
+    torch.cuda.Stream; default the current stream), after the work already
+    enqueued there.  Call it before a stream the library has used is destroyed."""
+    _call("rbx_release_workspace", C.c_void_p(stream.cuda_stream) if stream is not None else _stream())
+
+
+def release_all_workspaces() -> None:
+    """Synchronises and frees every workspace (no library work may be in flight)."""
+    _call("rbx_release_all_workspaces")
 
 
 def profile_enable(on: bool = True) -> None:

@@ -27,6 +27,8 @@
 // WG-to-tile mapping is XCD-aware: workgroups that share a K-range are
 // dispatched to the same XCD back to back, so the A planes they stream are
 // served from that XCD's L2.
+#include <vector>
+
 #include "rbx_internal.hh"
 #include "rbx_gemm_common.hh"
 
@@ -92,11 +94,23 @@ __device__ __forceinline__ bool bf16_round_safe(float g, float D) {
 // Fast bf16 pair of one Box-Muller draw on words (w0, w1), packed low =
 // first; `ok` is cleared when either value is too close to a rounding
 // midpoint for the bound above.
+// The hardware factors of a Box-Muller draw, shared by the generator and by
+// rbx_bf16_fastgen_bounds (which measures them against the portable ones).
+// sin / cos of pi * uneg11(w0): v_sin_f32 / v_cos_f32 take revolutions.
+__device__ __forceinline__ void fast_sincos_factor(uint32_t w0, float &sf, float &cf) {
+    const float x = uneg11(w0) * 0.5f;  // revolutions: sin(2 pi x) = sin(pi u0)
+    sf = __builtin_amdgcn_sinf(x);
+    cf = __builtin_amdgcn_cosf(x);
+}
+// sqrt(-2 log u01(w1)) with v_log_f32 (log2) scaled by -2 ln 2.
+__device__ __forceinline__ float fast_radius_factor(uint32_t w1) {
+    return __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u01(w1)) * -0x1.62e430p+0f);
+}
+
 __device__ __forceinline__ uint32_t boxmuller_bf16_fast(uint32_t w0, uint32_t w1, bool &ok) {
-    const float u0 = uneg11(w0), u1 = u01(w1);
-    const float x = u0 * 0.5f;  // revolutions: sin(2 pi x) = sin(pi u0)
-    const float sf = __builtin_amdgcn_sinf(x), cf = __builtin_amdgcn_cosf(x);
-    const float rf = __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u1) * -0x1.62e430p+0f);
+    float sf, cf;
+    fast_sincos_factor(w0, sf, cf);
+    const float rf = fast_radius_factor(w1);
     const float g0 = sf * rf, g1 = cf * rf;
     const float k = rf * (1.0f + 0x1p-18f);
     // both checks evaluated (no short-circuit branch)
@@ -715,7 +729,7 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     const size_t plane_elems = (size_t)safe_mul(pb.Kp, pb.Np);
     const size_t planes_bytes = ((size_t)n_terms * plane_elems * 2 + 255) / 256 * 256;
     const size_t part_bytes = (size_t)safe_mul(safe_mul(pb.nS, M), N) * 4;
-    char *ws = (char *)scratch(SCRATCH_BF16, planes_bytes + part_bytes);
+    char *ws = (char *)scratch(SCRATCH_BF16, planes_bytes + part_bytes, stream);
     pb.planes = (const uint16_t *)ws;
     pb.part = (float *)(ws + planes_bytes);
 
@@ -786,6 +800,18 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         hipLaunchKernelGGL((bf16_reduce_kernel<float, float>), grid, dim3(256), 0, stream, pb.part, pb.nS, M, N,
                            (double)alpha, (double)beta, B, brs, bcs);
         RBX_HIP(hipGetLastError());
+    }
+}
+
+// Scratch of the wide-tile paths, or nullptr when the device cannot provide
+// it: the caller then runs the tile kernels, which need none.
+static void *wide_scratch(size_t bytes, hipStream_t stream) {
+    try {
+        return scratch(SCRATCH_BF16, bytes, stream);
+    } catch (const Fail &f) {
+        if (f.st != RBX_OUT_OF_MEMORY) throw;
+        clear_error();
+        return nullptr;
     }
 }
 
@@ -1054,7 +1080,8 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     const size_t packed_bytes = align256b((size_t)safe_mul(K, Nf) * 4);
     const size_t part_bytes = (size_t)safe_mul(safe_mul(w.nS, Mf), Nf) * 4;
     if (packed_bytes + part_bytes > WIDE_SCRATCH_MAX) return -1;  // the tile kernels need no scratch
-    char *ws = (char *)scratch(SCRATCH_BF16, packed_bytes + part_bytes);
+    char *ws = (char *)wide_scratch(packed_bytes + part_bytes, stream);
+    if (ws == nullptr) return -1;
     w.packed = (const float *)ws;
     w.part = (float *)(ws + packed_bytes);
     const float *Y = (const float *)pb.D;
@@ -1350,7 +1377,8 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     const size_t packed_bytes = align256b((size_t)safe_mul(K, Nf) * 8);
     const size_t part_bytes = (size_t)safe_mul(safe_mul(w.nS, Mf), Nf) * 8;
     if (packed_bytes + part_bytes > WIDE_SCRATCH_MAX) return -1;
-    char *ws = (char *)scratch(SCRATCH_BF16, packed_bytes + part_bytes);
+    char *ws = (char *)wide_scratch(packed_bytes + part_bytes, stream);
+    if (ws == nullptr) return -1;
     w.packed = (const double *)ws;
     w.part = (double *)(ws + packed_bytes);
     const double *Y = (const double *)pb.D;
@@ -1408,6 +1436,66 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     return nr;
 }
 
+// ---------------------------------------------------------------- fast generator bounds
+// Measures, over every 32-bit word, the distance of the hardware factors of
+// the fast generator from the portable factors that define the operator:
+//   out[4 b + 0] = max |s' - s|,  out[4 b + 1] = max |c' - c|   (w as w0)
+//   out[4 b + 2] = max |r' - r| / r over r > 0                    (w as w1)
+//   out[4 b + 3] = number of words with (r' == 0) != (r == 0)
+// per workgroup b; the host takes the maxima.  Word w = i + stride * j for
+// thread i of the grid, so every word is visited exactly once.
+constexpr int FGB_BLOCKS = 16384, FGB_THREADS = 256;
+__global__ __launch_bounds__(FGB_THREADS) void fastgen_bounds_kernel(double *out) {
+    __shared__ double red[4][FGB_THREADS];
+    const uint32_t stride = (uint32_t)FGB_BLOCKS * FGB_THREADS;  // 2^22
+    const uint32_t i = blockIdx.x * FGB_THREADS + threadIdx.x;
+    double es = 0.0, ec = 0.0, er = 0.0;
+    double zero_mismatch = 0.0;
+    for (uint32_t j = 0; j < (1u << 10); ++j) {  // 2^22 * 2^10 = 2^32 words
+        const uint32_t w = i + stride * j;
+        float sf, cf;
+        fast_sincos_factor(w, sf, cf);
+        float s, c;
+        rbm_sincos(__fmul_rn(3.1415926535897932f, uneg11(w)), s, c);
+        es = fmax(es, fabs((double)sf - (double)s));
+        ec = fmax(ec, fabs((double)cf - (double)c));
+        const float rf = fast_radius_factor(w);
+        const float r = __builtin_sqrtf(__fmul_rn(-2.0f, rbm_logf(u01(w))));
+        if (r > 0.0f) er = fmax(er, fabs((double)rf - (double)r) / (double)r);
+        if ((rf == 0.0f) != (r == 0.0f)) zero_mismatch += 1.0;
+    }
+    red[0][threadIdx.x] = es;
+    red[1][threadIdx.x] = ec;
+    red[2][threadIdx.x] = er;
+    red[3][threadIdx.x] = zero_mismatch;
+    __syncthreads();
+    for (int h = FGB_THREADS / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = fmax(red[q][threadIdx.x], red[q][threadIdx.x + h]);
+            red[3][threadIdx.x] += red[3][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+static void fastgen_bounds(double measured[4], double assumed[2], hipStream_t stream) {
+    RBX_REQUIRE(measured != nullptr && assumed != nullptr);
+    double *dev_out = (double *)scratch(SCRATCH_SMALL, (size_t)FGB_BLOCKS * 4 * sizeof(double), stream);
+    hipLaunchKernelGGL(fastgen_bounds_kernel, dim3(FGB_BLOCKS), dim3(FGB_THREADS), 0, stream, dev_out);
+    RBX_HIP(hipGetLastError());
+    std::vector<double> h((size_t)FGB_BLOCKS * 4);
+    RBX_HIP(hipMemcpyAsync(h.data(), dev_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    RBX_HIP(hipStreamSynchronize(stream));
+    for (int q = 0; q < 4; ++q) measured[q] = 0.0;
+    for (int b = 0; b < FGB_BLOCKS; ++b) {
+        for (int q = 0; q < 3; ++q) measured[q] = fmax(measured[q], h[4 * b + q]);
+        measured[3] += h[4 * b + 3];
+    }
+    assumed[0] = (double)BM_ES;
+    assumed[1] = (double)BM_ER;
+}
+
 }  // namespace rbx
 
 using namespace rbx;
@@ -1419,5 +1507,11 @@ extern "C" rbx_status rbx_slskge3_bf16split(char layout, char opS, char opA, int
     RBX_API_BEGIN
     bf16_split_sketch(layout, opS, opA, d, n, m, alpha, S, ro_s, co_s, A, lda, beta, B, ldb, n_terms,
                       (hipStream_t)stream);
+    RBX_API_END
+}
+
+extern "C" rbx_status rbx_bf16_fastgen_bounds(double *measured, double *assumed, void *stream) {
+    RBX_API_BEGIN
+    fastgen_bounds(measured, assumed, (hipStream_t)stream);
     RBX_API_END
 }

@@ -99,7 +99,7 @@ static void sort_coo(int64_t nnz, int64_t nmaj, int64_t nmnr, const T *v, const 
     RBX_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
                                       (int64_t *)nullptr, (int64_t *)nullptr, (size_t)nnz, 0, end_bit, st));
     const size_t b8 = ((size_t)nnz * 8 + 255) / 256 * 256, bt = ((size_t)nnz * sizeof(T) + 255) / 256 * 256;
-    char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, 6 * b8 + bt + sort_bytes);
+    char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, 6 * b8 + bt + sort_bytes, st);
     uint64_t *k_in = (uint64_t *)ws, *k_out = (uint64_t *)(ws + b8);
     int64_t *p_in = (int64_t *)(ws + 2 * b8), *p_out = (int64_t *)(ws + 3 * b8);
     int64_t *a_tmp = (int64_t *)(ws + 4 * b8), *b_tmp = (int64_t *)(ws + 5 * b8);
@@ -136,7 +136,7 @@ template <typename T>
 static void coo_to_compressed(int64_t nmaj, int64_t nmnr, int64_t nnz, const int64_t *maj, const int64_t *mnr,
                               const T *vals, int64_t *ptr, int64_t *mnr_out, T *vals_out, hipStream_t st) {
     RBX_REQUIRE(ptr != nullptr);
-    int64_t *maj_sorted = (int64_t *)scratch(SCRATCH_SPARSE_OP, (size_t)(nnz > 0 ? nnz : 1) * 8);
+    int64_t *maj_sorted = (int64_t *)scratch(SCRATCH_SPARSE_OP, (size_t)(nnz > 0 ? nnz : 1) * 8, st);
     if (nnz > 0) sort_coo<T>(nnz, nmaj, nmnr, vals, maj, mnr, vals_out, maj_sorted, mnr_out, st);
     hipLaunchKernelGGL(compress_idx_kernel, dim3((unsigned)((nnz + 1 + 255) / 256)), dim3(256), 0, st, maj_sorted, nnz,
                        nmaj, ptr);
@@ -217,7 +217,7 @@ rbx_status rbx_coo_determine_sort(const rbx_coo_matrix *coo, char *order, void *
     check_coo(coo);
     RBX_REQUIRE(order != nullptr);
     hipStream_t st = (hipStream_t)stream;
-    int *flags = (int *)scratch(SCRATCH_SMALL, sizeof(int));
+    int *flags = (int *)scratch(SCRATCH_SMALL, sizeof(int), st);
     hipLaunchKernelGGL(int_zero_kernel, dim3(1), dim3(1), 0, st, flags);
     if (coo->nnz > 1)
         hipLaunchKernelGGL(order_flags_kernel, dim3((unsigned)blocks_for(coo->nnz)), dim3(256), 0, st, coo->rows,

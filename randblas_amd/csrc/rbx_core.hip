@@ -4,6 +4,8 @@
 
 #include <math.h>
 #include <string.h>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -20,44 +22,136 @@ void set_error(const char *fmt, ...) {
 void clear_error() { g_err[0] = 0; }
 
 int num_cus() {
-    static int cached = 0;
-    if (cached == 0) {
-        int dev = 0;
-        hipDeviceProp_t p;
-        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
-            cached = p.multiProcessorCount;
-        else
-            cached = 256;
-    }
+    static const int cached = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            return cus;
+        return 256;
+    }();
     return cached;
 }
 
 // ---------------------------------------------------------------------------
-// Device scratch slots
-static std::mutex g_ws_mutex;
-static std::vector<std::pair<void *, size_t>> g_ws;  // index dev * SCRATCH_NSLOTS + slot
+// Per-stream workspaces.
+//
+// Every (device, stream) pair owns its own scratch slots and rocBLAS handle,
+// so calls on different streams never share memory.  A host thread that
+// touches a stream's workspace holds that workspace's (recursive) mutex until
+// its outermost API call returns (ApiScope), so two host threads that enqueue
+// work on one stream take turns and never see each other's pointers change.
+// Growing a slot is stream-ordered: the old buffer is released with
+// hipFreeAsync on the same stream (after the work already enqueued there that
+// may still read it) and the new one comes from hipMallocAsync; no device-wide
+// synchronisation, no hipFree while other streams run.
+struct StreamWs {
+    std::recursive_mutex mu;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    void *ptr[SCRATCH_NSLOTS] = {};
+    size_t size[SCRATCH_NSLOTS] = {};
+    void *blas = nullptr;
+    void (*blas_destroy)(void *) = nullptr;
+};
 
-void *scratch(int slot, size_t bytes) {
-    std::lock_guard<std::mutex> lk(g_ws_mutex);
+static std::mutex g_ws_mutex;  // guards the map, not the workspaces
+static std::map<std::pair<int, hipStream_t>, std::unique_ptr<StreamWs>> g_ws;
+static thread_local int t_api_depth = 0;
+static thread_local std::vector<StreamWs *> t_held;
+
+ApiScope::ApiScope() { ++t_api_depth; }
+ApiScope::~ApiScope() {
+    if (--t_api_depth == 0) {
+        for (StreamWs *w : t_held) w->mu.unlock();
+        t_held.clear();
+    }
+}
+
+static bool pools_supported(int dev) {
+    static std::mutex mu;
+    static std::map<int, bool> known;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(dev);
+    if (it != known.end()) return it->second;
+    int v = 0;
+    bool ok = hipDeviceGetAttribute(&v, hipDeviceAttributeMemoryPoolsSupported, dev) == hipSuccess && v != 0;
+    known[dev] = ok;
+    return ok;
+}
+
+// Stream-ordered release / allocation (a stream synchronisation where the
+// device has no memory pools).
+static void ws_free(StreamWs &w, void *p) {
+    if (p == nullptr) return;
+    if (pools_supported(w.dev)) {
+        RBX_HIP(hipFreeAsync(p, w.stream));
+    } else {
+        RBX_HIP(hipStreamSynchronize(w.stream));
+        RBX_HIP(hipFree(p));
+    }
+}
+static void *ws_alloc(StreamWs &w, size_t bytes) {
+    void *p = nullptr;
+    hipError_t e = pools_supported(w.dev) ? hipMallocAsync(&p, bytes, w.stream) : hipMalloc(&p, bytes);
+    if (e != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();  // the failed allocation is reported here, not by the next call
+        RBX_FAIL(RBX_OUT_OF_MEMORY, "scratch: allocation of %zu bytes failed", bytes);
+    }
+    return p;
+}
+
+StreamWs &stream_ws(hipStream_t stream) {
+    if (t_api_depth == 0) RBX_FAIL(RBX_INVALID_ARG, "stream_ws: used outside an API call");
     int dev = 0;
     RBX_HIP(hipGetDevice(&dev));
-    size_t idx = (size_t)dev * SCRATCH_NSLOTS + slot;
-    if (g_ws.size() <= idx) g_ws.resize(idx + 1, {nullptr, 0});
-    auto &w = g_ws[idx];
-    if (bytes == 0) bytes = 256;
-    if (w.second < bytes) {
-        if (w.first) {
-            RBX_HIP(hipDeviceSynchronize());
-            RBX_HIP(hipFree(w.first));
-            w.first = nullptr;
-            w.second = 0;
+    StreamWs *w;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mutex);
+        auto &slot = g_ws[{dev, stream}];
+        if (!slot) {
+            slot.reset(new StreamWs);
+            slot->dev = dev;
+            slot->stream = stream;
         }
-        void *p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) RBX_FAIL(RBX_OUT_OF_MEMORY, "scratch: hipMalloc of %zu bytes failed", bytes);
-        w.first = p;
-        w.second = bytes;
+        w = slot.get();
     }
-    return w.first;
+    for (StreamWs *h : t_held)
+        if (h == w) return *w;
+    w->mu.lock();
+    t_held.push_back(w);
+    return *w;
+}
+
+void *scratch(int slot, size_t bytes, hipStream_t stream) {
+    StreamWs &w = stream_ws(stream);
+    if (bytes == 0) bytes = 256;
+    if (w.size[slot] < bytes) {
+        void *old = w.ptr[slot];
+        w.ptr[slot] = nullptr;
+        w.size[slot] = 0;
+        ws_free(w, old);
+        w.ptr[slot] = ws_alloc(w, bytes);
+        w.size[slot] = bytes;
+    }
+    return w.ptr[slot];
+}
+
+void *&stream_blas_handle(hipStream_t stream, void (**destroy)(void *)) {
+    StreamWs &w = stream_ws(stream);
+    *destroy = w.blas_destroy;
+    return w.blas;
+}
+void set_stream_blas_destroy(hipStream_t stream, void (*destroy)(void *)) { stream_ws(stream).blas_destroy = destroy; }
+
+static void release_ws(StreamWs &w) {
+    for (int s = 0; s < SCRATCH_NSLOTS; ++s) {
+        void *p = w.ptr[s];
+        w.ptr[s] = nullptr;
+        w.size[s] = 0;
+        ws_free(w, p);
+    }
+    if (w.blas != nullptr && w.blas_destroy != nullptr) w.blas_destroy(w.blas);
+    w.blas = nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -286,6 +380,44 @@ extern "C" {
 const char *rbx_last_error_message(void) { return g_err; }
 
 const char *rbx_version(void) { return "randblas_amd 0.1 (gfx950)"; }
+
+rbx_status rbx_release_workspace(void *stream) {
+    RBX_API_BEGIN
+    int dev = 0;
+    RBX_HIP(hipGetDevice(&dev));
+    bool known;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mutex);
+        known = g_ws.count({dev, (hipStream_t)stream}) != 0;
+    }
+    if (known) release_ws(stream_ws((hipStream_t)stream));
+    RBX_API_END
+}
+
+rbx_status rbx_release_all_workspaces(void) {
+    RBX_API_BEGIN
+    int cur = 0;
+    RBX_HIP(hipGetDevice(&cur));
+    std::vector<StreamWs *> all;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mutex);
+        for (auto &kv : g_ws) all.push_back(kv.second.get());
+    }
+    for (StreamWs *w : all) {
+        std::lock_guard<std::recursive_mutex> lk(w->mu);
+        RBX_HIP(hipSetDevice(w->dev));
+        RBX_HIP(hipDeviceSynchronize());
+        for (int s = 0; s < SCRATCH_NSLOTS; ++s) {
+            if (w->ptr[s] != nullptr) RBX_HIP(hipFree(w->ptr[s]));
+            w->ptr[s] = nullptr;
+            w->size[s] = 0;
+        }
+        if (w->blas != nullptr && w->blas_destroy != nullptr) w->blas_destroy(w->blas);
+        w->blas = nullptr;
+    }
+    RBX_HIP(hipSetDevice(cur));
+    RBX_API_END
+}
 
 rbx_status rbx_profile_enable(int on) {
     RBX_API_BEGIN

@@ -45,7 +45,17 @@ struct Fail {
         }                                                                              \
     } while (0)
 
+// Marks one API call on this thread; when the outermost call returns, the
+// per-stream workspaces it used are released to other host threads.
+struct ApiScope {
+    ApiScope();
+    ~ApiScope();
+    ApiScope(const ApiScope &) = delete;
+    ApiScope &operator=(const ApiScope &) = delete;
+};
+
 #define RBX_API_BEGIN                                                                  \
+    ::rbx::ApiScope rbx_api_scope_;                                                    \
     ::rbx::clear_error();                                                              \
     try {
 #define RBX_API_END                                                                    \
@@ -140,21 +150,26 @@ inline GenMap gen_map(const rbx_dense_skop &S, char opS, int64_t ro_s, int64_t c
 
 int num_cus();
 
-// Library-owned device scratch, one grow-only buffer per (device, slot).  A
-// slot's contents live until the next call that uses the same slot on the
-// same device; calls that overlap on one device (different host threads or
-// streams) and use the same slot race.  Growing a slot synchronises the device
-// and frees its old buffer, so one call never holds two pointers into a slot:
-// each slot has one user per call path (SCRATCH_DENSE_OP: an operator gathered
-// or generated for the sparse-data products; SCRATCH_SPARSE_G: the row-major
-// copy of the dense operand inside left_spmm_coo; SCRATCH_SPLITK: the K-range
-// partial products of the fused dense sketch; SCRATCH_SOLVE / SCRATCH_SOLVE2:
-// the sketched system of a sketch-and-solve call and its adjoint's vectors;
+// Library-owned device scratch: grow-only buffers per (device, stream, slot).
+// Calls on different streams never share a buffer; host threads that enqueue
+// on the same stream take turns (the workspace stays locked until the
+// thread's outermost API call returns).  A slot's contents live until the
+// next call on the same stream that uses it; one call uses each slot for one
+// purpose (SCRATCH_DENSE_OP: an operator gathered or generated for the
+// sparse-data products; SCRATCH_SPARSE_G: the row-major copy of the dense
+// operand inside left_spmm_coo; SCRATCH_SPLITK: the K-range partial products
+// of the fused dense sketch; SCRATCH_SOLVE / SCRATCH_SOLVE2: the sketched
+// system of a sketch-and-solve call and its adjoint's vectors;
 // SCRATCH_LONG_ROWS: the segment sums of the sparse row kernels' long rows).
+// Growth is stream-ordered (hipFreeAsync / hipMallocAsync on `stream`).
 enum ScratchSlot { SCRATCH_BF16 = 0, SCRATCH_SPARSE_OP = 1, SCRATCH_SPARSE_CSR = 2, SCRATCH_DENSE_OP = 3,
                    SCRATCH_SMALL = 4, SCRATCH_SPARSE_G = 5, SCRATCH_SPLITK = 6, SCRATCH_SOLVE = 7,
                    SCRATCH_SOLVE2 = 8, SCRATCH_LONG_ROWS = 9, SCRATCH_NSLOTS = 10 };
-void *scratch(int slot, size_t bytes);
+void *scratch(int slot, size_t bytes, hipStream_t stream);
+// The rocBLAS handle slot of the stream's workspace (created by its user)
+// and its destroy function, called by rbx_release_workspace.
+void *&stream_blas_handle(hipStream_t stream, void (**destroy)(void *));
+void set_stream_blas_destroy(hipStream_t stream, void (*destroy)(void *));
 
 // Write the generated matrix G (n_p x n_q) of a GenMap to buff[p*rs + q*cs].
 template <typename T>

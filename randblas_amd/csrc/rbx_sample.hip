@@ -279,7 +279,7 @@ template <typename T>
 static void weights_to_cdf_impl(int64_t n, T *w, T error_if_below, hipStream_t st) {
     RBX_REQUIRE(n >= 0);
     RBX_REQUIRE(n == 0 || w != nullptr);
-    long long *status = (long long *)scratch(SCRATCH_SMALL, 2 * sizeof(long long));
+    long long *status = (long long *)scratch(SCRATCH_SMALL, 2 * sizeof(long long), st);
     T *inv_total = (T *)(status + 1);
     hipLaunchKernelGGL(i64_zero_kernel, dim3(1), dim3(1), 0, st, status);
     hipLaunchKernelGGL((weights_to_cdf_kernel<T>), dim3(1), dim3(256), 0, st, n, w, error_if_below, status,

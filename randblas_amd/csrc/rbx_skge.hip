@@ -384,7 +384,7 @@ static void launch_skge_tiles(const GemmProblem &pb_in, hipStream_t stream) {
         if (ns > 1 && (uint64_t)ns * pb.M * pb.N * sizeof(T) <= (1ull << 30)) {
             pb.nsplit = (int)ns;
             pb.kw = kw;
-            pb.part = scratch(SCRATCH_SPLITK, (size_t)ns * pb.M * pb.N * sizeof(T));
+            pb.part = scratch(SCRATCH_SPLITK, (size_t)ns * pb.M * pb.N * sizeof(T), stream);
         }
     }
     if (pb.gen_is_x) {
@@ -505,7 +505,7 @@ static bool gemv_run(const GemmProblem &pb, hipStream_t stream) {
     if (kw > 16384) kw = 16384;
     const int64_t nk = (pb.K + kw - 1) / kw;
     if (nk > 65535) return false;
-    double *part = (double *)scratch(SCRATCH_SPLITK, (size_t)safe_mul(nk, pb.M) * sizeof(double));
+    double *part = (double *)scratch(SCRATCH_SPLITK, (size_t)safe_mul(nk, pb.M) * sizeof(double), stream);
     ProfScope prof("gemv_gen_kernel", stream);
     if (pb.g.b_along_rows)
         hipLaunchKernelGGL((gemv_gen_kernel<T, 1>), dim3((unsigned)nb, (unsigned)nk), dim3(256), 0, stream, pb.g, pb.M,

@@ -29,19 +29,25 @@
 
 namespace rbx {
 
-// one rocBLAS handle per device, bound to the caller's stream on each call
+// one rocBLAS handle per (device, stream), kept in the stream's workspace and
+// bound to that stream once
+static void destroy_blas_handle(void *h) { (void)rocblas_destroy_handle((rocblas_handle)h); }
+
 static rocblas_handle blas_handle(hipStream_t st) {
-    static std::mutex mu;
-    static std::vector<rocblas_handle> handles;
-    int dev = 0;
-    RBX_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(mu);
-    if ((int)handles.size() <= dev) handles.resize(dev + 1, nullptr);
-    if (handles[dev] == nullptr && rocblas_create_handle(&handles[dev]) != rocblas_status_success)
-        RBX_FAIL(RBX_HIP_ERROR, "sketch_solve: rocblas_create_handle failed");
-    if (rocblas_set_stream(handles[dev], st) != rocblas_status_success)
-        RBX_FAIL(RBX_HIP_ERROR, "sketch_solve: rocblas_set_stream failed");
-    return handles[dev];
+    void (*destroy)(void *) = nullptr;
+    void *&h = stream_blas_handle(st, &destroy);
+    if (h == nullptr) {
+        rocblas_handle nh = nullptr;
+        if (rocblas_create_handle(&nh) != rocblas_status_success)
+            RBX_FAIL(RBX_HIP_ERROR, "sketch_solve: rocblas_create_handle failed");
+        if (rocblas_set_stream(nh, st) != rocblas_status_success) {
+            (void)rocblas_destroy_handle(nh);
+            RBX_FAIL(RBX_HIP_ERROR, "sketch_solve: rocblas_set_stream failed");
+        }
+        h = nh;
+        set_stream_blas_destroy(st, destroy_blas_handle);
+    }
+    return (rocblas_handle)h;
 }
 
 static void blas_ok(rocblas_status s, const char *what) {
@@ -234,12 +240,12 @@ struct SolveWork {
 };
 
 template <typename T>
-static SolveWork<T> solve_work(int64_t d, int64_t n) {
+static SolveWork<T> solve_work(int64_t d, int64_t n, hipStream_t st) {
     auto a = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t dn = (size_t)safe_mul(d, n), nn = (size_t)safe_mul(n, n);
     const size_t bytes = 2 * a(dn * 8) + 2 * a((size_t)d * 8) + a((size_t)n * 8) + 3 * a(nn * 8) + a(16) +
                          a(dn * sizeof(T) + 8);
-    char *ws = (char *)scratch(SCRATCH_SOLVE, bytes);
+    char *ws = (char *)scratch(SCRATCH_SOLVE, bytes, st);
     SolveWork<T> w;
     size_t off = 0;
     auto take = [&](size_t b) {
@@ -365,7 +371,7 @@ static void sketch_solve_impl(char layout, int64_t d, int64_t m, int64_t n, cons
                               int64_t co_s, const T *A, int64_t lda, const T *b, T *x, hipStream_t st) {
     check_solve<T>(layout, d, m, n, S, A, lda, b);
     RBX_REQUIRE(x != nullptr);
-    SolveWork<T> w = solve_work<T>(d, n);
+    SolveWork<T> w = solve_work<T>(d, n, st);
     factor_and_solve<T>(layout, d, m, n, S, ro_s, co_s, A, lda, b, w, st);
     hipLaunchKernelGGL((from_f64_kernel<T>), dim3(grid_of(n)), dim3(256), 0, st, (const double *)w.y, n, (int64_t)1, x,
                        (int64_t)1, (int64_t)1);
@@ -379,12 +385,12 @@ static void sketch_solve_adjoint_impl(char layout, int64_t d, int64_t m, int64_t
     check_solve<T>(layout, d, m, n, S, A, lda, b);
     RBX_REQUIRE(xbar != nullptr && Abar != nullptr && bbar != nullptr);
     RBX_REQUIRE(ldab >= (layout == 'C' ? m : n));
-    SolveWork<T> w = solve_work<T>(d, n);
+    SolveWork<T> w = solve_work<T>(d, n, st);
     factor_and_solve<T>(layout, d, m, n, S, ro_s, co_s, A, lda, b, w, st);
     // second workspace: x64 (n), u (n), z (n), r (d), wv (d), M (d x n) in FP64, M in T
     auto a = [](size_t bb) { return (bb + 255) / 256 * 256; };
     const size_t dn = (size_t)(d * n);
-    char *ws = (char *)scratch(SCRATCH_SOLVE2, 3 * a((size_t)n * 8) + 2 * a((size_t)d * 8) + a(dn * 8) + a(dn * sizeof(T)));
+    char *ws = (char *)scratch(SCRATCH_SOLVE2, 3 * a((size_t)n * 8) + 2 * a((size_t)d * 8) + a(dn * 8) + a(dn * sizeof(T)), st);
     double *x64 = (double *)ws, *u = (double *)(ws + a((size_t)n * 8)), *z = (double *)(ws + 2 * a((size_t)n * 8));
     double *r = (double *)(ws + 3 * a((size_t)n * 8)), *wv = (double *)(ws + 3 * a((size_t)n * 8) + a((size_t)d * 8));
     double *M = (double *)(ws + 3 * a((size_t)n * 8) + 2 * a((size_t)d * 8));

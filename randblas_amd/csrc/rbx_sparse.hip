@@ -192,7 +192,7 @@ static int64_t fill_sparse_impl(const rbx_sparse_dist &D, T *vals, int64_t *rows
         size_t scan_bytes = 0;
         RBX_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, (int64_t *)nullptr, (int64_t *)nullptr, (int64_t)0,
                                         (size_t)g.dim_minor, rocprim::plus<int64_t>(), st));
-        char *ws = (char *)scratch(SCRATCH_SPARSE_OP, b_idx + b_val + 2 * b_cnt + align256(scan_bytes) + 256);
+        char *ws = (char *)scratch(SCRATCH_SPARSE_OP, b_idx + b_val + 2 * b_cnt + align256(scan_bytes) + 256, st);
         int64_t *tidx = (int64_t *)ws;
         T *tval = (T *)(ws + b_idx);
         int64_t *counts = (int64_t *)(ws + b_idx + b_val);
@@ -482,7 +482,7 @@ static void coo_to_csr(const int64_t *rows, const int64_t *cols, const T *vals, 
     const size_t bk = align256((size_t)nnz * sizeof(KT)), bp = align256((size_t)nnz * sizeof(PT));
     const size_t brp = align256((size_t)(d + 1) * 8), bcol = align256((size_t)nnz * 4),
                  bval = align256((size_t)nnz * sizeof(T));
-    char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, 2 * bk + 2 * bp + brp + bcol + bval + align256(sort_bytes));
+    char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, 2 * bk + 2 * bp + brp + bcol + bval + align256(sort_bytes), st);
     KT *keys_in = (KT *)ws, *keys_out = (KT *)(ws + bk);
     PT *perm_in = (PT *)(ws + 2 * bk), *perm_out = (PT *)(ws + 2 * bk + bp);
     char *rest = ws + 2 * bk + 2 * bp;
@@ -708,7 +708,7 @@ static int64_t launch_long_rows(const int64_t *rowptr, const IDX *colidx, const 
     const int64_t nseg_cap = nnz / seg + cap + 1;  // their segments
     const size_t b_list = align256((size_t)cap * 8), b_off = align256((size_t)(cap + 1) * 8);
     char *ws = (char *)scratch(SCRATCH_LONG_ROWS,
-                               256 + b_list + b_off + (size_t)safe_mul(safe_mul(nseg_cap, ldp), 8));
+                               256 + b_list + b_off + (size_t)safe_mul(safe_mul(nseg_cap, ldp), 8), st);
     int64_t *count = (int64_t *)ws, *list = (int64_t *)(ws + 256), *seg_off = (int64_t *)(ws + 256 + b_list);
     double *part = (double *)(ws + 256 + b_list + b_off);
     ProfScope prof("long_rows", st);
@@ -830,7 +830,7 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
         int64_t ldg = ldb;
         if (!(layout_opB == 'R' && ldb % VEC == 0 && ((uintptr_t)B % 16) == 0)) {
             ldg = (n + VEC - 1) / VEC * VEC;
-            T *Gs = (T *)scratch(SCRATCH_SPARSE_G, (size_t)safe_mul(m, ldg) * sizeof(T));
+            T *Gs = (T *)scratch(SCRATCH_SPARSE_G, (size_t)safe_mul(m, ldg) * sizeof(T), st);
             const int64_t gx = (m + 31) / 32, gy = (ldg + 31) / 32;
             RBX_REQUIRE(gx < (1ll << 31) && gy <= 65535);
             hipLaunchKernelGGL(to_rowmajor_kernel<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, B, brs, bcs,
@@ -865,7 +865,7 @@ static SkopView<T> view_or_sample(const rbx_sparse_skop *S, hipStream_t st) {
     const size_t bv = align256((size_t)g.full_nnz * sizeof(T)), bi = align256((size_t)g.full_nnz * 8);
     // The sampled operator lives in the dense-operator slot: LASO sampling
     // itself uses the sparse-operator slot for its temporaries.
-    char *ws = (char *)scratch(SCRATCH_DENSE_OP, bv + 2 * bi);
+    char *ws = (char *)scratch(SCRATCH_DENSE_OP, bv + 2 * bi, st);
     T *vals = (T *)ws;
     int64_t *rows = (int64_t *)(ws + bv), *cols = (int64_t *)(ws + bv + bi);
     int64_t nnz = fill_sparse_impl<T>(S->dist, vals, rows, cols, S->seed_state, nullptr, st);
@@ -955,7 +955,7 @@ template <typename T>
 static T *materialize_op(const rbx_dense_skop *S, char opS, int64_t ro_s, int64_t co_s, int64_t n_p, int64_t n_q,
                          hipStream_t st) {
     const size_t bytes = (size_t)safe_mul(n_p > 0 ? n_p : 1, n_q > 0 ? n_q : 1) * sizeof(T);
-    T *G = (T *)scratch(SCRATCH_DENSE_OP, bytes);
+    T *G = (T *)scratch(SCRATCH_DENSE_OP, bytes, st);
     if (n_p == 0 || n_q == 0) return G;
     if (S->buff == nullptr) {
         GenMap gm = gen_map(*S, opS, ro_s, co_s, false);
@@ -1003,7 +1003,7 @@ static SpData sp_of(const rbx_coo_matrix *A) {
 // Major index of every stored entry of a compressed matrix (CSR row / CSC
 // column), expanded from its pointer array into library scratch.
 static const int64_t *expand_ptr(const int64_t *ptr, int64_t nmaj, int64_t nnz, hipStream_t st) {
-    int64_t *out = (int64_t *)scratch(SCRATCH_SPARSE_OP, (size_t)(nnz > 0 ? nnz : 1) * 8);
+    int64_t *out = (int64_t *)scratch(SCRATCH_SPARSE_OP, (size_t)(nnz > 0 ? nnz : 1) * 8, st);
     if (nnz > 0 && nmaj > 0) {
         dim3 grid((unsigned)((nmaj + 255) / 256));
         hipLaunchKernelGGL(csr_to_coo_rows_kernel, grid, dim3(256), 0, st, ptr, nmaj, nnz, out);

@@ -42,7 +42,7 @@ static void require_symmetric_impl(char layout, const T *A, int64_t n, int64_t l
     const int64_t rs = layout == 'C' ? 1 : lda, cs = layout == 'C' ? lda : 1;
     const int64_t nb = (n + 31) / 32;
     RBX_REQUIRE(nb <= 65535);
-    unsigned long long *first = (unsigned long long *)scratch(SCRATCH_SMALL, sizeof(unsigned long long));
+    unsigned long long *first = (unsigned long long *)scratch(SCRATCH_SMALL, sizeof(unsigned long long), stream);
     const unsigned long long none = ~0ull;
     hipLaunchKernelGGL(u64_set_kernel, dim3(1), dim3(1), 0, stream, first, none);
     RBX_HIP(hipGetLastError());

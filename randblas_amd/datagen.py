@@ -1,4 +1,10 @@
-"""Synthetic sparse data matrices for tests and benchmarks (host, numpy).
+"""Synthetic data for tests and benchmarks.
+
+conditioned_gaussian(m, n, cond, key) is the dense workload of the large
+fused sketch: A = G diag(cond^(-j/(n-1))) in FP32, column-major, with G the
+Gaussian DenseDist(m, n) drawn by this library's generator (key `key`), so
+the column scales decay geometrically to condition number ~cond.
+
 
 power_law_csr(rows, cols, mean_nnz, exponent, seed) draws Poisson(mean_nnz)
 column indices per row from a Zipf-law popularity over the columns, so the
@@ -26,3 +32,17 @@ def power_law_csr(rows: int, cols: int, mean_nnz: float, exponent: float, seed: 
     rowptr = np.cumsum(rowptr)
     vals = rng.standard_normal(len(c)).astype(np.float32)
     return rowptr, c, vals
+
+
+def conditioned_gaussian(m: int, n: int, cond: float, key: int, device="cuda"):
+    """A (m x n, FP32, column-major, a flat device tensor of m*n entries) =
+    G * diag(cond^(-j / (n - 1))), G = DenseDist(m, n) Gaussian with key `key`
+    (natural layout ColMajor for m >= n), the scales rounded to FP32."""
+    import torch
+    import randblas_amd as rb
+    A = torch.empty(m * n, dtype=torch.float32, device=device)
+    rb.fill_dense(rb.DenseDist(m, n), A, rb.RNGState(key))
+    j = torch.arange(n, device=device, dtype=torch.float64)
+    scales = torch.pow(torch.tensor(float(cond), dtype=torch.float64, device=device), -j / max(n - 1, 1)).float()
+    A.view(n, m).mul_(scales.view(n, 1))
+    return A

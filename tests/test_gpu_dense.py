@@ -87,16 +87,41 @@ def test_fill_dense_bit_exact(rb, orc, dtype, shape, axis, fam, layout, sub):
     assert list(nxt.counter) == [int(x) for x in ref_nxt]
 
 
+# Distance of the device Gaussians from the reference's own Box-Muller over
+# the host C library (the oracle's mathmode 1: sinf / cosf / logf of this
+# host).  The target is 2 ulp, but it is not reachable for every word by any
+# implementation other than that library itself: its chain logf (~0.8 ulp),
+# sqrtf, sinf (~0.6 ulp) and the final product can land over 3 ulp from the
+# exact value.  This build's factors are correctly rounded (<= 1 ulp from the
+# library's each), which bounds the Gaussian distance at 3 ulp; the bar
+# tested here is therefore: at most 3 ulp, and at most 1e-5 of the entries
+# above 2 ulp (measured: ~4e-6).
+def _libm_distance(rb, orc, d, m, key):
+    D = rb.DenseDist(d, m)
+    buf = torch.empty(d * m, dtype=torch.float32, device=DEV)
+    rb.fill_dense(D, buf, rb.RNGState(key))
+    got = host(buf)
+    ref, _ = orc.fill_dense(d, m, [0, 0, 0, 0], [key, 0], dtype=np.float32, mathmode=1)
+    return ulp_diff_f32(got, ref.reshape(-1))
+
+
 def test_gaussians_close_to_libm_reference(rb, orc):
     # C1's operator, float: device values vs the host-libm Box-Muller
-    D = rb.DenseDist(400, 2000)
-    buf = torch.empty(400 * 2000, dtype=torch.float32, device=DEV)
-    rb.fill_dense(D, buf, rb.RNGState(42))
-    got = host(buf)
-    ref, _ = orc.fill_dense(400, 2000, [0, 0, 0, 0], [42, 0], dtype=np.float32, mathmode=1)
-    d = ulp_diff_f32(got, ref.reshape(-1))
-    assert d.max() <= 3
-    assert (d > 2).sum() <= 8
+    dist = _libm_distance(rb, orc, 400, 2000, 42)
+    print(f"max {dist.max()} ulp, {(dist > 2).sum()} of {dist.size} above 2 ulp, "
+          f"{(dist == 0).mean():.6f} bit-identical")
+    assert dist.max() <= 3
+    assert (dist > 2).sum() <= 8
+
+
+def test_gaussians_libm_distance_large_sample(rb, orc):
+    # 16.8M Gaussians of the large sketch's operator (rows 0..63 of
+    # DenseDist(4096, 262144), key 42)
+    dist = _libm_distance(rb, orc, 64, 262144, 42)
+    frac2 = (dist > 2).mean()
+    print(f"max {dist.max()} ulp, fraction above 2 ulp {frac2:.2e}, bit-identical {(dist == 0).mean():.6f}")
+    assert dist.max() <= 3
+    assert frac2 <= 1e-5
 
 
 # ---------------------------------------------------------------- sketches
