@@ -364,7 +364,26 @@ def sparse_leg(rb, torch, dev, reps=10):
            "adj_ms": round(adj_ms, 4), "adj_GBps": round(adj_bytes / (adj_ms * 1e-3) / 1e9, 1),
            "adj_frac_hbm": round(adj_bytes / (adj_ms * 1e-3) / 8e12, 4), "adj_kernels_ms": adj_k}
     log(f"sparse: rsksp3 {fwd_ms:.3f} ms, adjoint {adj_ms:.3f} ms ({nnz} stored entries)")
-    del A, B, Bbar, dv
+    # BF16-split form: bf16 operator, A's values split into three bf16 terms
+    bf_ms, bf_k = timed(lambda: rb.rsksp3_bf16split("C", "N", "N", m, d, n, 1.0, A, 0, 0, S, 0, 0, 0.0, B, m))
+    bf_bytes = nnz * 12 + (m + 1) * 8 + n * d * 2 + m * d * 4
+    # error against the FP64 product of the same (bf16) operator: the library's
+    # FP64 rsksp3 on a materialised FP64 copy of bf16(S)
+    Sf = torch.empty(n * d, dtype=torch.float32, device=dev)
+    rb.fill_dense(rb.DenseDist(n, d), Sf, rb.RNGState(42))
+    S64 = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(42), buff=_bf16_round(torch, Sf).double(), layout="C")
+    A64 = rb.CSRMatrix(m, n, A.vals.double(), A.rowptr, A.colidxs)
+    B64 = torch.empty(m * d, dtype=torch.float64, device=dev)
+    rb.rsksp3("C", "N", "N", m, d, n, 1.0, A64, 0, 0, S64, 0, 0, 0.0, B64, m)
+    bf_err = float(torch.linalg.norm(B.double() - B64) / torch.linalg.norm(B64))
+    rb.rsksp3("C", "N", "N", m, d, n, 1.0, A, 0, 0, S, 0, 0, 0.0, B, m)
+    fp32_err = float(torch.linalg.norm(B.double() - B64) / torch.linalg.norm(B64))
+    res["bf16split"] = {"ms": round(bf_ms, 4), "GBps": round(bf_bytes / (bf_ms * 1e-3) / 1e9, 1),
+                        "frac_hbm": round(bf_bytes / (bf_ms * 1e-3) / 8e12, 4), "kernels_ms": bf_k, "terms": 3,
+                        "rel_err_vs_fp64_same_operator": bf_err,
+                        "fp32_path_rel_err_vs_fp64_bf16_operator": fp32_err}
+    log(f"sparse: rsksp3 bf16-split {bf_ms:.3f} ms, rel err vs same-operator FP64 {bf_err:.2e}")
+    del A, B, Bbar, dv, Sf, S64, A64, B64
     torch.cuda.synchronize()
     return res
 
@@ -376,9 +395,9 @@ def saso_leg(rb, torch, dev, reps=10):
     HBM: algorithmic bytes = A read once + B written once (+ the operator's
     800000 stored entries, 24 B each, read once)."""
     d, m, n = 2000, 100000, 500
-    S = rb.SparseSkOp(rb.SparseDist(d, m, 8, rb.Axis.Short), rb.RNGState(7))
     res = {"workload": "SASO 2000x100000 vec_nnz 8 (key 7) * A 100000x500 ColMajor Uniform(-1,1) (key 3)"}
     for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        S = rb.SparseSkOp(rb.SparseDist(d, m, 8, rb.Axis.Short), rb.RNGState(7), dtype=dt)
         A = torch.empty(m * n, dtype=dt, device=dev)
         rb.fill_dense(rb.DenseDist(m, n, rb.ScalarDist.Uniform), A, rb.RNGState(3))
         B = torch.empty(d * n, dtype=dt, device=dev)

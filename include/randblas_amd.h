@@ -242,6 +242,23 @@ rbx_status rbx_slskge3_bf16split(char layout, char opS, char opA, int64_t d, int
                                  int64_t co_s, const float *A, int64_t lda, float beta,
                                  float *B, int64_t ldb, int n_terms, void *stream);
 
+/* BF16-split form of srsksp3 for CSR data (no reference counterpart; the FP32
+ * analogue is rsksp3, sparse_data/sksp.hh:277-326 -> csr_spmm_impl.hh:51-119).
+ * Computes
+ *   B = alpha * A_T * op(bf16(submat(S))) + beta * B
+ * with A_T the n_terms-term bfloat16 split of A's values (n_terms in {1,2,3};
+ * three terms reproduce FP32 values) and bf16(S) the operator of
+ * rbx_slskge3_bf16split (every FP32 entry rounded to bfloat16, nearest-even).
+ * The operator is written once per call as a bfloat16 matrix (half the bytes
+ * the FP32 product gathers); each output row accumulates in FP32, flushed into
+ * FP64 every 256 stored entries.  Only op(A) = NoTrans with the whole CSR
+ * matrix (ro_a = co_a = 0) is supported; S may be generated or materialised
+ * in FP32. */
+rbx_status rbx_srsksp3_bf16split(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n,
+                                 float alpha, const rbx_csr_matrix *A, int64_t ro_a, int64_t co_a,
+                                 const rbx_dense_skop *S, int64_t ro_s, int64_t co_s, float beta,
+                                 float *B, int64_t ldb, int n_terms, void *stream);
+
 /* Self-check of the BF16 operator's fast generator (no reference
  * counterpart).  The BF16-split kernels draw Gaussian entries with the
  * hardware sin / cos / log / sqrt and keep a value only when it provably

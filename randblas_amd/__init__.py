@@ -181,6 +181,8 @@ _SIG = {
     **{f"rbx_{p}sketch_solve_adjoint": [_CH, _I64, _I64, _I64, _DS, _I64, _I64, _P, _I64, _P, _P, _P, _I64, _P, _P]
        for p in ("s", "d")},
     "rbx_bf16_fastgen_bounds": [_P, _P, _P],
+    "rbx_srsksp3_bf16split": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _CSR, _I64, _I64, _DS, _I64, _I64, _F, _P, _I64,
+                              C.c_int, _P],
     "rbx_release_workspace": [_P],
     "rbx_release_all_workspaces": [],
     "rbx_profile_enable": [C.c_int],
@@ -698,6 +700,21 @@ def rsksp3(layout, opA, opS, m, d, n, alpha, A, ro_a, co_a, S: DenseSkOp, ro_s, 
     _need("B", B, _fp(p), _extent(m, d, layout, ldb))
     _call(f"rbx_{_prefix(B)}rsksp3{_sp_suffix(A)}", _c(layout), _c(opA), _c(opS), m, d, n, alpha, C.byref(A.c()),
           ro_a, co_a, C.byref(S.c()), ro_s, co_s, beta, _ptr(B), ldb, _stream())
+
+
+def rsksp3_bf16split(layout, opA, opS, m, d, n, alpha, A: "CSRMatrix", ro_a, co_a, S: DenseSkOp, ro_s, co_s, beta,
+                     B, ldb, n_terms: int = 3):
+    """B = alpha * A_T * op(bf16(submat S)) + beta * B for FP32 CSR data: the
+    operator rounded to bfloat16, A's values split into n_terms bfloat16 terms."""
+    import torch
+    if not isinstance(A, CSRMatrix):
+        raise TypeError(f"rsksp3_bf16split: expected a CSRMatrix, got {type(A).__name__}")
+    _need_sparse_data(A, "s")
+    if S.buff is not None:
+        _need("S.buff", S.buff, torch.float32, S.n_rows * S.n_cols)
+    _need("B", B, torch.float32, _extent(m, d, layout, ldb))
+    _call("rbx_srsksp3_bf16split", _c(layout), _c(opA), _c(opS), m, d, n, alpha, C.byref(A.c()), ro_a, co_a,
+          C.byref(S.c()), ro_s, co_s, beta, _ptr(B), ldb, n_terms, _stream())
 
 
 # ---------------------------------------------------------------- dispatching wrappers

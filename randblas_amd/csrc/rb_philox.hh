@@ -200,4 +200,23 @@ __device__ __forceinline__ float entry_value<float>(float v, bool gauss) {
     return gauss ? v : __fmul_rn(v, 0x1.bb67aep+0f);  // (float)sqrt(3.0)
 }
 
+// bfloat16 (as its 16 bits) nearest to x, ties to even; x not NaN.
+__host__ __device__ __forceinline__ uint16_t bf16_rne(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t u = __float_as_uint(x);
+#else
+    uint32_t u;
+    __builtin_memcpy(&u, &x, 4);
+#endif
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Operator entry of the BF16 operator: the FP32 entry rounded to bfloat16.
+template <>
+__device__ __forceinline__ uint16_t entry_value<uint16_t>(float v, bool gauss) {
+    return bf16_rne(entry_value<float>(v, gauss));
+}
+
 }  // namespace rbx

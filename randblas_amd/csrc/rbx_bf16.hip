@@ -43,24 +43,6 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 constexpr int BF_MT = 64, BF_NT = 1024, BF_KC = 32, BF_LDK = 40, BF_NTHR = 256;
 
-__host__ __device__ __forceinline__ uint16_t bf16_rne(float x) {
-#ifdef __HIP_DEVICE_COMPILE__
-    uint32_t u = __float_as_uint(x);
-#else
-    uint32_t u;
-    memcpy(&u, &x, 4);
-#endif
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
-// Operator entry rounded to bfloat16 (the value the FP32 operator would hold,
-// then RNE).
-template <>
-__device__ __forceinline__ uint16_t entry_value<uint16_t>(float v, bool gauss) {
-    return bf16_rne(gauss ? v : __fmul_rn(v, 0x1.bb67aep+0f));
-}
 
 // ---------------------------------------------------------------- fast generator
 // Gaussian operator entries rounded to bfloat16 with the hardware

@@ -237,7 +237,12 @@ __global__ __launch_bounds__(256) void fill_dense_vec_kernel(GenMap g, T *buff, 
     for (int64_t v = blockIdx.y; v < n_a; v += gridDim.y) {
         const float4 w = gen4(gauss, ctr_add(g.ctr, (uint64_t)((g.a_off + v) * g.P + cell)), g.k0, g.k1);
         T *dst = buff + v * stride_a + 4 * c;
-        if (sizeof(T) == 4) {
+        if constexpr (sizeof(T) == 2) {
+            uint2 pk;
+            pk.x = (uint32_t)entry_value<uint16_t>(w.x, gauss) | ((uint32_t)entry_value<uint16_t>(w.y, gauss) << 16);
+            pk.y = (uint32_t)entry_value<uint16_t>(w.z, gauss) | ((uint32_t)entry_value<uint16_t>(w.w, gauss) << 16);
+            *(uint2 *)dst = pk;
+        } else if constexpr (sizeof(T) == 4) {
             *(float4 *)dst = make_float4(entry_value<float>(w.x, gauss), entry_value<float>(w.y, gauss),
                                          entry_value<float>(w.z, gauss), entry_value<float>(w.w, gauss));
         } else {
@@ -317,6 +322,7 @@ void fill_generated(const GenMap &g, int64_t n_p, int64_t n_q, T *buff, int64_t 
 }
 template void fill_generated<float>(const GenMap &, int64_t, int64_t, float *, int64_t, int64_t, hipStream_t);
 template void fill_generated<double>(const GenMap &, int64_t, int64_t, double *, int64_t, int64_t, hipStream_t);
+template void fill_generated<uint16_t>(const GenMap &, int64_t, int64_t, uint16_t *, int64_t, int64_t, hipStream_t);
 
 template <typename T>
 static void fill_dense_impl(char layout, const rbx_dense_dist *D, int64_t n_rows, int64_t n_cols,

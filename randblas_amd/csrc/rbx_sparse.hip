@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "rbx_internal.hh"
+#include "rbx_spmm.hh"
 
 #include <math.h>
 #include <string.h>
@@ -768,6 +769,11 @@ static void launch_rows_vec(bool colmaj, const int64_t *rowptr, const IDX *colid
     RBX_HIP(hipGetLastError());
 }
 
+// Longest mean row (stored entries per output row) the LDS apply takes: each
+// output is one thread's sequential chain, so rows much longer than this are
+// better served by the row kernels' segment sums.
+constexpr int64_t LDS_APPLY_MAX_MEAN_ROW = 2048;
+
 // left_spmm for a COO operator (spmm_dispatch.hh:52-179):
 // C = alpha * op(submat(A)) * op(B) + beta * C, A given by COO arrays.
 template <typename T>
@@ -816,10 +822,22 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
     T *cval;
     {
         ProfScope prof("coo_to_csr", st);
-        if ((uint64_t)d * (uint64_t)m < (1ull << 32) && nnz < (int64_t)INT32_MAX)
+        // few output rows: stable counting sort by row; otherwise a radix
+        // sort on the (row, column) key
+        if (csr_build_counting<T>(rows, cols, vals, nnz, ro_a, co_a, d, m, &row_ptr, &ccol, &cval, st)) {
+        } else if ((uint64_t)d * (uint64_t)m < (1ull << 32) && nnz < (int64_t)INT32_MAX)
             coo_to_csr<T, uint32_t, int32_t>(rows, cols, vals, nnz, ro_a, co_a, d, m, &row_ptr, &ccol, &cval, st);
         else
             coo_to_csr<T, uint64_t, int64_t>(rows, cols, vals, nnz, ro_a, co_a, d, m, &row_ptr, &ccol, &cval, st);
+    }
+    // op(B) with unit stride along the contraction (a column-major operand of
+    // a left sketch), few output rows that are not very long: every output a
+    // sequential chain, op(B) streamed once through LDS
+    if (layout_opB == 'C' && d <= CSRB_MAX_ROWS && nnz <= LDS_APPLY_MAX_MEAN_ROW * d) {
+        ProfScope prof("csr_lds_apply_kernel", st);
+        if (csr_lds_apply<T>(row_ptr, ccol, cval, d, n, m, alpha, beta, B, ldb, C, crs, ccs, kib ? SPF_KIB : SPF_JKI,
+                             st))
+            return;
     }
     {
         ProfScope prof("csr_apply_kernel", st);

@@ -589,3 +589,39 @@ def test_lskges_small_sketch_dimension(rb, orc, dtype, tol, layout):
     got, ref = _lsk(rb, orc, dtype, layout, "N", "N", (d, m), 8, "S", d, n, m, 0, 0, 1.0, 0.0, prefill=False)
     exact_err = np.linalg.norm(np.asarray(got, np.float64) - np.asarray(ref, np.float64)) / np.linalg.norm(ref)
     assert exact_err <= tol, exact_err
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("layout", ["C", "R"])
+def test_lskges_shuffled_operator_with_duplicates(rb, orc, dtype, layout):
+    """An operator given as arrays in random order, with duplicate (row,
+    column) entries and one row of ~6500 entries: the counting-sort CSR
+    builder must order each row by (column, stored position) -- sorting that
+    row, longer than one 4096-entry LDS tile, by tiles and merge passes -- so
+    the product equals the reference's (coo_spmm_impl.hh:52-105: CSC sort,
+    then ascending columns, duplicates in stored order) bit for bit.  (The
+    row stays under the row kernels' long-row threshold, 4 x the mean row,
+    above which row-major products sum in FP64 segments.)"""
+    rng = np.random.default_rng(71)
+    d, m, n = 20, 20000, 33
+    r = rng.integers(0, d, 30000)
+    c = rng.integers(0, m, 30000)
+    r = np.concatenate([r, np.full(5000, 3), r[:500]])        # a long row, and 500 duplicates
+    c = np.concatenate([c, rng.integers(0, m, 5000), c[:500]])
+    v = rng.uniform(-2, 2, len(r)).astype(dtype)
+    perm = rng.permutation(len(r))
+    r, c, v = r[perm].astype(np.int64), c[perm].astype(np.int64), v[perm]
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    S = rb.SparseSkOp(rb.SparseDist(d, m, 8), rb.RNGState(5), dtype=tdt)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+    S.vals, S.rows, S.cols, S.nnz = t(v), t(r), t(c), len(v)
+    lda = m if layout == "C" else n
+    ldb = d if layout == "C" else n
+    A = rng.uniform(-1, 1, m * n).astype(dtype)
+    B0 = rng.uniform(-1, 1, d * n).astype(dtype)
+    Bd = t(B0.copy())
+    rb.lskges(layout, "N", "N", d, n, m, 0.5, S, 0, 0, t(A), lda, 1.5, Bd, ldb)
+    ref = B0.copy()
+    orc.left_spmm_coo("d" if dtype == np.float64 else "s", layout, "N", "N", d, n, m, 0.5, (d, m), v, r, c, 0, 0, A,
+                      lda, 1.5, ref, ldb)
+    assert np.array_equal(host(Bd), ref)
