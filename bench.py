@@ -320,17 +320,16 @@ def libm_distance(rb, torch, dev, np, oracle, rows=64):
 
 def sparse_leg(rb, torch, dev, reps=10):
     """rsksp3 on a 200000 x 50000 CSR matrix with power-law column degrees
-    (exponent 2.1, ~20 stored entries per row, N(0,1) FP32 values) times a
+    (exponent 2.1, 4e6 draws, ~20 stored entries per row, N(0,1) FP32 values,
+    generated on the device from key 17) times a
     generated 50000 x 256 Gaussian operator (key 42), column-major B; and the
     gradient with respect to A's stored values (SDDMM).  Both are bound by
     memory traffic: achieved GB/s of their algorithmic bytes against HBM."""
     from randblas_amd.datagen import power_law_csr
     m, n, d = 200000, 50000, 256
-    log("sparse leg: generating the CSR matrix")
-    rowptr, colidx, vals = power_law_csr(m, n, 20, 2.1, 17)
-    nnz = len(vals)
-    A = rb.CSRMatrix(m, n, torch.from_numpy(vals).to(dev), torch.from_numpy(rowptr).to(dev),
-                     torch.from_numpy(colidx).to(dev))
+    log("sparse leg: generating the CSR matrix on the device")
+    A = power_law_csr(m, n, 20, 2.1, 17)
+    nnz = int(A.vals.numel())
     S = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(42))
     B = torch.empty(m * d, dtype=torch.float32, device=dev)
     Bbar = torch.empty(m * d, dtype=torch.float32, device=dev)
@@ -531,11 +530,9 @@ def formats_leg(rb, torch, dev, reps=5):
     touches, B written once) against HBM."""
     from randblas_amd.datagen import power_law_csr
     m, n, d = 200000, 50000, 256
-    rowptr, colidx, vals = power_law_csr(m, n, 20, 2.1, 17)
-    nnz = len(vals)
+    csr = power_law_csr(m, n, 20, 2.1, 17)
+    nnz = int(csr.vals.numel())
     i64 = dict(dtype=torch.int64, device=dev)
-    csr = rb.CSRMatrix(m, n, torch.from_numpy(vals).to(dev), torch.from_numpy(rowptr).to(dev),
-                       torch.from_numpy(colidx).to(dev))
     coo = rb.COOMatrix(m, n, torch.empty(nnz, dtype=torch.float32, device=dev), torch.empty(nnz, **i64),
                        torch.empty(nnz, **i64))
     csc = rb.CSCMatrix(m, n, torch.empty(nnz, dtype=torch.float32, device=dev), torch.empty(nnz, **i64),

@@ -469,6 +469,25 @@ rbx_status rbx_dsample_indices_iid(int64_t n, const double *cdf, int64_t k, int6
 rbx_status rbx_sweights_to_cdf(int64_t n, float *w, float error_if_below, void *stream);
 rbx_status rbx_dweights_to_cdf(int64_t n, double *w, double error_if_below, void *stream);
 
+/* Power-law sparse test data (no reference counterpart: the reference's
+ * generators, sparse_data/random_matrix.hh:136-358, draw uniform-density
+ * matrices from one sequential PhiloxStream and cannot run in parallel bit
+ * for bit).  An m x n CSR matrix built from this library's counter-addressed
+ * samplers: k column draws by sample_indices_iid over the Zipf weights
+ * w[c] = (c+1)^(-1/(exponent-1)) (host libm pow, rounded to T; CDF by
+ * weights_to_cdf), starting at `state`; k row draws by
+ * sample_indices_iid_uniform(m, k) from the state after them; the pairs
+ * sorted by (row, column) with repeated pairs kept once; the nnz values the
+ * Gaussian entries 0..nnz-1 of a DenseDist(nnz, 1) operator from the state
+ * after the row draws.  next_state = that state + ceil(nnz / 4).  Column
+ * degrees follow a power law with the given exponent (> 1).  A->rowptr
+ * (m + 1), A->colidxs and A->vals (k entries each) are caller-allocated device
+ * arrays; A->n_rows, A->n_cols and A->nnz are set.  Synchronises the stream. */
+rbx_status rbx_spower_law_csr(int64_t m, int64_t n, int64_t k, double exponent, const rbx_rng_state *state,
+                              rbx_csr_matrix *A, rbx_rng_state *next_state, void *stream);
+rbx_status rbx_dpower_law_csr(int64_t m, int64_t n, int64_t k, double exponent, const rbx_rng_state *state,
+                              rbx_csr_matrix *A, rbx_rng_state *next_state, void *stream);
+
 /* ---------------------------------------------------------------- sketch-and-solve
  * The downstream use of a left sketch: examples/total-least-squares/
  * tls_dense_skop.cc:138-171 sketches [A b] with sketch_general and solves the

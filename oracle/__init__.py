@@ -42,6 +42,7 @@ def lib() -> C.CDLL:
                                              _u32p]
         L.orc_weights_to_cdf.argtypes = [C.c_char, C.c_int64, C.c_void_p, C.c_double]
         L.orc_weights_to_cdf.restype = C.c_int64
+        L.orc_power_law_weights.argtypes = [C.c_int64, C.c_double, C.c_void_p]
         L.orc_philox_stream.argtypes = [_u32p, _u32p, C.c_int64, _u32p]
         L.orc_ctr_incr.argtypes = [_u32p, C.c_uint64]
         L.orc_uneg11.argtypes = [C.c_uint32]
@@ -311,6 +312,33 @@ def weights_to_cdf(w, error_if_below=None):
         error_if_below = -float(np.sqrt(np.finfo(a.dtype).eps))
     st = lib().orc_weights_to_cdf(_b(dt), a.size, a.ctypes.data, error_if_below)
     return a, int(st)
+
+
+def power_law_csr(m, n, k, exponent, counter, key, dtype=np.float32):
+    """The library's power-law sparse test data restated from the oracle's own
+    samplers: Zipf weights by libm pow (rounded to dtype), weights_to_cdf,
+    k column draws by sample_indices_iid from `counter`, k row draws by
+    sample_indices_iid_uniform from the counter after them, the (row, column)
+    pairs sorted with repeats kept once, and N(0,1) values from a
+    DenseDist(nnz, 1) at the counter after the row draws.
+    Returns (rowptr, colidxs, vals, next_counter)."""
+    dt = np.dtype(dtype)
+    w = np.zeros(n, np.float64)
+    lib().orc_power_law_weights(n, float(exponent), w.ctypes.data)
+    cdf, st = weights_to_cdf(w.astype(dt), -1.0)
+    assert st == 0, st
+    cols, c1 = sample_indices_iid(n, cdf, k, counter, key)
+    rows, _, c2 = sample_indices_iid_uniform(m, k, c1, key)
+    keys = np.unique(rows.astype(np.int64) * n + cols.astype(np.int64))
+    r, c = keys // n, keys % n
+    rowptr = np.zeros(m + 1, np.int64)
+    np.add.at(rowptr, r + 1, 1)
+    rowptr = np.cumsum(rowptr)
+    nnz = len(keys)
+    if nnz == 0:
+        return rowptr, c, np.zeros(0, dt), c2
+    vals, nxt = fill_dense(nnz, 1, c2, key, "G", "L", dt)
+    return rowptr, c, np.ascontiguousarray(vals[:, 0]), nxt
 
 
 # ---------------------------------------------------------------- bfloat16 helpers

@@ -884,6 +884,15 @@ static int64_t NAME(int64_t n, T *w, T error_if_below) {                        
 DEF_W2CDF(double, w2cdf_d, DBL_EPSILON)
 DEF_W2CDF(float, w2cdf_s, FLT_EPSILON)
 
+/* Zipf column weights of the library's power-law sparse test data (no
+ * reference counterpart; the weights feed weights_to_cdf, util.hh:459-472,
+ * and sample_indices_iid, util.hh:493-514).  libm pow, as the library's host
+ * side evaluates them. */
+void orc_power_law_weights(int64_t n, double exponent, double *w) {
+    const double e = -1.0 / (exponent - 1.0);
+    for (int64_t c = 0; c < n; ++c) w[c] = pow((double)(c + 1), e);
+}
+
 int64_t orc_weights_to_cdf(char dtype, int64_t n, void *w, double error_if_below) {
     if (dtype == 'd') return w2cdf_d(n, (double *)w, error_if_below);
     return w2cdf_s(n, (float *)w, (float)error_if_below);

@@ -103,6 +103,9 @@ void orc_sample_indices_iid_uniform(int64_t n, int64_t k, int64_t *samples, doub
 void orc_sample_indices_iid(char dtype, int64_t n, const void *cdf, int64_t k, int64_t *samples,
                             const uint32_t ctr[4], const uint32_t key[2], uint32_t next_ctr[4]);
 int64_t orc_weights_to_cdf(char dtype, int64_t n, void *w, double error_if_below);
+/* Zipf weights of the power-law sparse test data (no reference counterpart):
+ * w[c] = pow(c + 1, -1 / (exponent - 1)) by libm. */
+void orc_power_law_weights(int64_t n, double exponent, double *w);
 
 #ifdef __cplusplus
 }

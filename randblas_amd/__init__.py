@@ -175,6 +175,8 @@ _SIG = {
     "rbx_ssample_indices_iid": [_I64, _P, _I64, _P, _RS, _RS, _P],
     "rbx_dsample_indices_iid": [_I64, _P, _I64, _P, _RS, _RS, _P],
     "rbx_sweights_to_cdf": [_I64, _P, _F, _P],
+    "rbx_spower_law_csr": [_I64, _I64, _I64, _D, _RS, C.POINTER(c_csr_matrix), _RS, _P],
+    "rbx_dpower_law_csr": [_I64, _I64, _I64, _D, _RS, C.POINTER(c_csr_matrix), _RS, _P],
     "rbx_dweights_to_cdf": [_I64, _P, _D, _P],
     "rbx_drequire_symmetric": [_CH, _P, _I64, _I64, _D, _P],
     **{f"rbx_{p}sketch_solve": [_CH, _I64, _I64, _I64, _DS, _I64, _I64, _P, _I64, _P, _P, _P] for p in ("s", "d")},
@@ -944,6 +946,29 @@ def weights_to_cdf(n, w, error_if_below=None) -> None:
         import numpy as _np
         error_if_below = -float(_np.sqrt(_np.finfo(_np.float32 if _prefix(w) == "s" else _np.float64).eps))
     _call(f"rbx_{_prefix(w)}weights_to_cdf", n, _ptr(w), error_if_below, _stream())
+
+
+def power_law_csr(m, n, k, exponent, state: RNGState, dtype=None):
+    """Power-law sparse test data on the device (no reference counterpart; the
+    reference's random_csr, sparse_data/random_matrix.hh:136-200, draws a
+    uniform-density matrix from one sequential stream): k column draws from the
+    Zipf weights (c+1)^(-1/(exponent-1)) by sample_indices_iid, k row draws by
+    sample_indices_iid_uniform, repeated (row, column) pairs kept once, N(0,1)
+    values from a DenseDist(nnz, 1).  Returns (CSRMatrix over tensors of nnz
+    entries, next RNGState)."""
+    import torch
+    dtype = torch.float32 if dtype is None else dtype
+    pre = "d" if dtype == torch.float64 else "s"
+    kk = max(int(k), 1)
+    vals = torch.empty(kk, dtype=dtype, device="cuda")
+    colidxs = torch.empty(kk, dtype=torch.int64, device="cuda")
+    rowptr = torch.empty(int(m) + 1, dtype=torch.int64, device="cuda")
+    A = c_csr_matrix(m, n, 0, vals.data_ptr(), rowptr.data_ptr(), colidxs.data_ptr())
+    nxt = c_rng_state()
+    _call(f"rbx_{pre}power_law_csr", m, n, k, float(exponent), C.byref(state.c()), C.byref(A), C.byref(nxt),
+          _stream())
+    nnz = int(A.nnz)
+    return CSRMatrix(m, n, vals[:nnz], rowptr, colidxs[:nnz]), RNGState.from_c(nxt)
 
 
 def sketch_solve(layout, d, m, n, S: DenseSkOp, ro_s, co_s, A, lda, b, x) -> None:

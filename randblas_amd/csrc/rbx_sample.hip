@@ -10,6 +10,10 @@
 #include <cmath>
 #include <limits>
 #include <type_traits>
+#include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "rbx_internal.hh"
 
@@ -299,6 +303,141 @@ static void weights_to_cdf_impl(int64_t n, T *w, T error_if_below, hipStream_t s
     if (h < 0) RBX_FAIL(RBX_INVALID_ARG, "weights_to_cdf: the weights sum to less than sqrt(n) * epsilon");
 }
 
+// ---------------------------------------------------------------- power-law sparse test data
+// A random m x n CSR matrix whose column popularity follows a Zipf law (no
+// RandBLAS counterpart; RandBLAS's own generators, sparse_data/
+// random_matrix.hh:136-358, draw uniform-density matrices from one sequential
+// stream).  Built from the library's counter-addressed samplers, so every
+// stage is a pure function of (key, counter) and runs in parallel:
+//   w[c]  = (c + 1)^(-1 / (exponent - 1))    host libm pow, rounded to T
+//   cdf   = weights_to_cdf(w)                 (util.hh:459-472)
+//   cols  = sample_indices_iid(n, cdf, k)     from `state`      (util.hh:493-514)
+//   rows  = sample_indices_iid_uniform(m, k)  from the state after it (util.hh:516-546)
+//   the k (row, column) pairs sorted, repeated pairs kept once -> CSR
+//   vals  = the first nnz entries of a DenseDist(nnz, 1) Gaussian operator
+//           from the state after the row draws (dense_skops.hh:563-606)
+// Row degrees are Binomial(k, 1/m) before merging; the column degrees have a
+// power-law tail with the given exponent.
+
+// keys[i] = rows[i] * n + cols[i]
+__global__ __launch_bounds__(256) void plc_keys_kernel(const int64_t *rows, const int64_t *cols, int64_t k,
+                                                       int64_t n, uint64_t *keys) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) keys[i] = (uint64_t)rows[i] * (uint64_t)n + (uint64_t)cols[i];
+}
+
+// first[i] = 1 for the first of each run of equal sorted keys
+__global__ __launch_bounds__(256) void plc_first_kernel(const uint64_t *keys, int64_t k, int64_t *first) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) first[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+// Unique entry e = pos[i] (sorted key i opening a run): colidxs[e], and
+// rowptr[r] = e for every row r in (row of the previous key, row of key i].
+// Rows after the last key's row get nnz (the tail kernel).
+__global__ __launch_bounds__(256) void plc_compact_kernel(const uint64_t *keys, const int64_t *first,
+                                                          const int64_t *pos, int64_t k, int64_t n,
+                                                          int64_t *colidxs, int64_t *rowptr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k || first[i] == 0) return;
+    const int64_t e = pos[i];
+    const uint64_t key = keys[i];
+    const int64_t r = (int64_t)(key / (uint64_t)n);
+    colidxs[e] = (int64_t)(key - (uint64_t)r * (uint64_t)n);
+    const int64_t rp = i == 0 ? -1 : (int64_t)(keys[i - 1] / (uint64_t)n);
+    for (int64_t q = rp + 1; q <= r; ++q) rowptr[q] = e;
+}
+
+__global__ __launch_bounds__(256) void plc_tail_kernel(const uint64_t *keys, int64_t k, int64_t n, int64_t m,
+                                                       const int64_t *first, const int64_t *pos, int64_t *rowptr,
+                                                       int64_t *nnz_out) {
+    const int64_t r_last = k > 0 ? (int64_t)(keys[k - 1] / (uint64_t)n) : -1;
+    const int64_t nnz = k > 0 ? pos[k - 1] + first[k - 1] : 0;
+    for (int64_t q = r_last + 1 + (int64_t)threadIdx.x; q <= m; q += blockDim.x) rowptr[q] = nnz;
+    if (threadIdx.x == 0) *nnz_out = nnz;
+}
+
+static int plc_bits(uint64_t x) {
+    int b = 1;
+    while (b < 64 && (x >> b) != 0) ++b;
+    return b;
+}
+
+template <typename T>
+static void power_law_csr_impl(int64_t m, int64_t n, int64_t k, double exponent, const rbx_rng_state *state,
+                               rbx_csr_matrix *A, rbx_rng_state *next, hipStream_t st) {
+    RBX_REQUIRE(state != nullptr && A != nullptr);
+    RBX_REQUIRE(m > 0 && n > 0 && k >= 0);
+    RBX_REQUIRE(exponent > 1.0);
+    RBX_REQUIRE(A->rowptr != nullptr);
+    RBX_REQUIRE(k == 0 || (A->colidxs != nullptr && A->vals != nullptr));
+    const uint64_t mn = (uint64_t)safe_mul(m, n);
+    RBX_REQUIRE(k < (1ll << 31) && mn < (1ull << 63));
+    // column weights on the host (libm pow), then the device CDF
+    std::vector<T> w((size_t)n);
+    const double rank_exp = -1.0 / (exponent - 1.0);
+    for (int64_t c = 0; c < n; ++c) w[(size_t)c] = (T)std::pow((double)(c + 1), rank_exp);
+    const size_t b_cdf = ((size_t)n * sizeof(T) + 255) / 256 * 256, b_k = ((size_t)(k > 0 ? k : 1) * 8 + 255) / 256 * 256;
+    size_t sort_bytes = 0, scan_bytes = 0;
+    if (k > 0) {
+        RBX_HIP(rocprim::radix_sort_keys(nullptr, sort_bytes, (uint64_t *)nullptr, (uint64_t *)nullptr, (size_t)k, 0,
+                                         plc_bits(mn - 1), st));
+        RBX_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, (int64_t *)nullptr, (int64_t *)nullptr, (int64_t)0,
+                                        (size_t)k, rocprim::plus<int64_t>(), st));
+    }
+    const size_t b_tmp = ((sort_bytes > scan_bytes ? sort_bytes : scan_bytes) + 255) / 256 * 256;
+    char *ws = (char *)scratch(SCRATCH_SPARSE_OP, b_cdf + 5 * b_k + b_tmp + 256, st);
+    T *cdf = (T *)ws;
+    int64_t *rows = (int64_t *)(ws + b_cdf);
+    uint64_t *keys = (uint64_t *)(ws + b_cdf + b_k), *keys2 = (uint64_t *)(ws + b_cdf + 2 * b_k);
+    int64_t *first = (int64_t *)(ws + b_cdf + 3 * b_k), *pos = (int64_t *)(ws + b_cdf + 4 * b_k);
+    void *tmp = ws + b_cdf + 5 * b_k;
+    int64_t *d_nnz = (int64_t *)(ws + b_cdf + 5 * b_k + b_tmp);
+    RBX_HIP(hipMemcpyAsync(cdf, w.data(), (size_t)n * sizeof(T), hipMemcpyHostToDevice, st));
+    weights_to_cdf_impl<T>(n, cdf, (T)-1, st);  // synchronises the stream (w may go)
+
+    rbx_rng_state s1, s2;
+    int64_t *cols = A->colidxs;  // the column draws, overwritten by the merged columns below
+    iid_cdf_impl<T>(n, cdf, k, cols, state, &s1, st);
+    iid_uniform_impl<T>(m, k, rows, nullptr, &s1, &s2, st);
+    const int64_t blocks = (k + 255) / 256;
+    if (k > 0) {
+        hipLaunchKernelGGL(plc_keys_kernel, dim3((unsigned)blocks), dim3(256), 0, st, rows, cols, k, n, keys);
+        RBX_HIP(hipGetLastError());
+        RBX_HIP(rocprim::radix_sort_keys(tmp, sort_bytes, keys, keys2, (size_t)k, 0, plc_bits(mn - 1), st));
+        hipLaunchKernelGGL(plc_first_kernel, dim3((unsigned)blocks), dim3(256), 0, st, keys2, k, first);
+        RBX_HIP(hipGetLastError());
+        RBX_HIP(rocprim::exclusive_scan(tmp, scan_bytes, first, pos, (int64_t)0, (size_t)k, rocprim::plus<int64_t>(),
+                                        st));
+        hipLaunchKernelGGL(plc_compact_kernel, dim3((unsigned)blocks), dim3(256), 0, st, keys2, first, pos, k, n,
+                           A->colidxs, A->rowptr);
+        RBX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(plc_tail_kernel, dim3(1), dim3(256), 0, st, keys2, k, n, m, first, pos, A->rowptr, d_nnz);
+    RBX_HIP(hipGetLastError());
+    int64_t nnz = 0;
+    RBX_HIP(hipMemcpyAsync(&nnz, d_nnz, sizeof(nnz), hipMemcpyDeviceToHost, st));
+    RBX_HIP(hipStreamSynchronize(st));
+    // values: DenseDist(nnz, 1) Gaussian from s2 (natural layout ColMajor:
+    // entry i is word i % 4 of counter s2 + i / 4); next = s2 + ceil(nnz / 4)
+    rbx_rng_state s3 = s2;
+    if (nnz > 0) {
+        rbx_dense_skop V{};
+        V.dist.n_rows = nnz;
+        V.dist.n_cols = 1;
+        V.dist.family = 'G';
+        V.dist.major_axis = 'L';
+        V.seed_state = s2;
+        const GenMap g = gen_map(V, 'N', 0, 0, false);
+        fill_generated<T>(g, nnz, 1, (T *)A->vals, 1, nnz, st);
+        next_state(&s2, (uint64_t)((nnz + 3) / 4), &s3);
+    }
+    A->n_rows = m;
+    A->n_cols = n;
+    A->nnz = nnz;
+    if (next) *next = s3;
+}
+
 }  // namespace rbx
 
 using namespace rbx;
@@ -341,6 +480,20 @@ rbx_status rbx_dsample_indices_iid(int64_t n, const double *cdf, int64_t k, int6
                                    const rbx_rng_state *state, rbx_rng_state *next_state_out, void *stream) {
     RBX_API_BEGIN
     iid_cdf_impl<double>(n, cdf, k, samples, state, next_state_out, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_spower_law_csr(int64_t m, int64_t n, int64_t k, double exponent, const rbx_rng_state *state,
+                              rbx_csr_matrix *A, rbx_rng_state *next_state_out, void *stream) {
+    RBX_API_BEGIN
+    power_law_csr_impl<float>(m, n, k, exponent, state, A, next_state_out, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_dpower_law_csr(int64_t m, int64_t n, int64_t k, double exponent, const rbx_rng_state *state,
+                              rbx_csr_matrix *A, rbx_rng_state *next_state_out, void *stream) {
+    RBX_API_BEGIN
+    power_law_csr_impl<double>(m, n, k, exponent, state, A, next_state_out, (hipStream_t)stream);
     RBX_API_END
 }
 

@@ -6,32 +6,28 @@ Gaussian DenseDist(m, n) drawn by this library's generator (key `key`), so
 the column scales decay geometrically to condition number ~cond.
 
 
-power_law_csr(rows, cols, mean_nnz, exponent, seed) draws Poisson(mean_nnz)
-column indices per row from a Zipf-law popularity over the columns, so the
-column degrees have a power-law tail with the given exponent (rank exponent
-1/(exponent - 1)); duplicate entries in a row are merged, values are N(0,1)
-float32.  Returns (rowptr int64, colidxs int64, vals float32), zero-based,
-column indices sorted within each row.
+power_law_csr(rows, cols, mean_nnz, exponent, key) is the sparse data matrix
+of the sparse-data workloads, drawn on the device from Philox key `key`:
+column indices from a Zipf-law popularity over the columns (column degrees
+with a power-law tail of the given exponent), rows uniform, repeated entries
+kept once, N(0,1) values; a CSRMatrix over device tensors, column indices
+ascending within each row.
 """
 from __future__ import annotations
 
 import numpy as np
 
 
-def power_law_csr(rows: int, cols: int, mean_nnz: float, exponent: float, seed: int):
-    rng = np.random.default_rng(seed)
-    w = (np.arange(cols) + 1.0) ** (-1.0 / (exponent - 1.0))
-    cdf = np.cumsum(w) / w.sum()
-    deg = rng.poisson(mean_nnz, rows)
-    r = np.repeat(np.arange(rows, dtype=np.int64), deg)
-    c = np.minimum(np.searchsorted(cdf, rng.random(len(r))), cols - 1).astype(np.int64)
-    key = np.unique(r * cols + c)
-    r, c = key // cols, key % cols
-    rowptr = np.zeros(rows + 1, np.int64)
-    np.add.at(rowptr, r + 1, 1)
-    rowptr = np.cumsum(rowptr)
-    vals = rng.standard_normal(len(c)).astype(np.float32)
-    return rowptr, c, vals
+def power_law_csr(rows: int, cols: int, mean_nnz: float, exponent: float, key: int, dtype=None):
+    """The power-law CSR matrix of the sparse-data workloads, generated on the
+    device by rbx_[sd]power_law_csr: rows * mean_nnz (row, column) draws --
+    columns from the Zipf weights (c+1)^(-1/(exponent-1)), rows uniform --
+    repeated pairs kept once, N(0,1) values; all from Philox key `key`.
+    Returns a CSRMatrix over device tensors (FP32 values unless dtype says
+    otherwise)."""
+    import randblas_amd as rb
+    A, _ = rb.power_law_csr(rows, cols, int(round(rows * mean_nnz)), exponent, rb.RNGState(key), dtype)
+    return A
 
 
 def conditioned_gaussian(m: int, n: int, cond: float, key: int, device="cuda"):

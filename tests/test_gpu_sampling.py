@@ -141,3 +141,43 @@ def test_cdf_search_unsorted_array_bit_exact(rb, orc, dtype):
     rb.sample_indices_iid(n, torch.from_numpy(a).to(DEV), k, s, rb.RNGState(key=3))
     ref, _ = orc.sample_indices_iid(n, a, k, [0, 0, 0, 0], [3, 0])
     assert np.array_equal(host(s), ref)
+
+
+# ---------------------------------------------------------------- power-law sparse test data
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("m,n,k,expo,ctr", [(1000, 500, 20000, 2.1, 0), (37, 3, 500, 1.5, 5), (1, 10, 7, 3.0, 2),
+                                            (5000, 20000, 0, 2.1, 0), (300, 100000, 4001, 2.5, 2**32 - 3)])
+def test_power_law_csr_bit_exact(rb, orc, dtype, m, n, k, expo, ctr):
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    st = rb.RNGState(17, (ctr, 0, 0, 0))
+    A, nxt = rb.power_law_csr(m, n, k, expo, st, dtype=tdt)
+    torch.cuda.synchronize()
+    rowptr, cols, vals, onxt = orc.power_law_csr(m, n, k, expo, [ctr, 0, 0, 0], [17, 0], dtype=dtype)
+    assert A.n_rows == m and A.n_cols == n
+    assert np.array_equal(A.rowptr.cpu().numpy(), rowptr)
+    assert np.array_equal(A.colidxs.cpu().numpy(), cols)
+    assert np.array_equal(A.vals.cpu().numpy(), vals)
+    assert list(nxt.counter) == [int(x) for x in onxt]
+
+
+def test_power_law_csr_sparse_workload(rb):
+    """The 200000 x 50000 power-law matrix (4e6 draws, exponent 2.1, key 17):
+    structure of the CSR and the column-degree tail."""
+    A, _ = rb.power_law_csr(200000, 50000, 4_000_000, 2.1, rb.RNGState(17))
+    torch.cuda.synchronize()
+    rp = A.rowptr.cpu().numpy()
+    c = A.colidxs.cpu().numpy()
+    nnz = len(c)
+    assert rp[0] == 0 and rp[-1] == nnz and np.all(np.diff(rp) >= 0)
+    assert 3_500_000 < nnz <= 4_000_000
+    # columns strictly ascending within each row (sorted, repeats merged)
+    d = np.diff(c)
+    row_start = np.zeros(nnz, bool)
+    row_start[rp[:-1][rp[:-1] < nnz]] = True
+    assert np.all(d[~row_start[1:]] > 0)
+    # popularity of column c ~ (c+1)^(-1/1.1): rank-frequency slope of the top
+    # columns near -0.91 (before merging repeats flattens the very top)
+    deg = np.bincount(c, minlength=50000).astype(np.float64)
+    ranks = np.arange(100, 5000)
+    slope = np.polyfit(np.log(ranks + 1.0), np.log(deg[ranks]), 1)[0]
+    assert -1.0 < slope < -0.8, slope

@@ -137,9 +137,8 @@ def test_full_size_power_law(rb):
     the reference's order, on a materialised FP64 copy of bf16(S))."""
     from randblas_amd.datagen import power_law_csr
     m, n, d = 200000, 50000, 256
-    rowptr, cols, vals = power_law_csr(m, n, 20, 2.1, 17)
-    A32 = rb.CSRMatrix(m, n, t_(vals), t_(rowptr), t_(cols))
-    A64 = rb.CSRMatrix(m, n, t_(vals.astype(np.float64)), A32.rowptr, A32.colidxs)
+    A32 = power_law_csr(m, n, 20, 2.1, 17)
+    A64 = rb.CSRMatrix(m, n, A32.vals.double(), A32.rowptr, A32.colidxs)
     S = rb.DenseSkOp(rb.DenseDist(n, d), rb.RNGState(42))
     B = torch.empty(m * d, dtype=torch.float32, device=DEV)
     rb.rsksp3_bf16split("C", "N", "N", m, d, n, 1.0, A32, 0, 0, S, 0, 0, 0.0, B, m)

@@ -141,3 +141,33 @@ def test_kolmogorov_smirnov(orc):
     emp = np.cumsum(np.bincount(t, minlength=n)) / k
     crit = 1.95 / np.sqrt(k)   # 0.1% two-sided KS critical value
     assert np.max(np.abs(emp - cdf)) < crit
+
+
+def test_power_law_csr_structure_and_streams(orc):
+    """The power-law test data: a CSR with columns strictly ascending per row,
+    counters consumed as the three samplers state (k/4 for the columns, k/4 for
+    the rows, nnz/4 for the values), and values equal to the Box-Muller pairs
+    of the Philox words after the index draws (entry i: pair (i % 4) // 2 of
+    counter c2 + i // 4)."""
+    m, n, k = 400, 2000, 9000
+    rp, c, v, nxt = orc.power_law_csr(m, n, k, 2.1, [11, 0, 0, 0], [17, 0], dtype=np.float32)
+    nnz = len(c)
+    assert rp[0] == 0 and rp[-1] == nnz and np.all(np.diff(rp) >= 0)
+    for i in range(m):
+        seg = c[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(seg) > 0)
+    assert 0 < nnz <= k
+    c2 = 11 + 2 * ((k + 3) // 4)
+    assert list(nxt) == [c2 + (nnz + 3) // 4, 0, 0, 0]
+    words = orc.philox_stream([c2, 0, 0, 0], [17, 0], (nnz + 3) // 4).reshape(-1)
+    g = orc.boxmuller(words)  # two Gaussians per pair of words, in word order
+    assert np.array_equal(v, g[:nnz].astype(np.float32))
+    # the column draws follow the normalised Zipf weights (about one draw per
+    # row here, so merging repeats removes almost nothing)
+    m, k = 40000, 20000
+    rp, c, v, nxt = orc.power_law_csr(m, n, k, 2.1, [0, 0, 0, 0], [17, 0], dtype=np.float64)
+    w = (np.arange(n) + 1.0) ** (-1 / 1.1)
+    freq = np.bincount(c, minlength=n)
+    assert k - 200 < len(c) <= k
+    assert freq[0] > freq[10] > freq[1000]
+    assert abs(freq[:10].sum() / len(c) - w[:10].sum() / w.sum()) < 0.02
