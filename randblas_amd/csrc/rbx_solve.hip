@@ -96,12 +96,23 @@ __global__ __launch_bounds__(256) void zero_lower_kernel(double *R, int64_t n) {
         if (e % n > e / n) R[e] = 0.0;
 }
 
+// v from lane `lane` of the wave (a uniform, compile-time lane index)
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Upper Cholesky A = R^T R of an n x n column-major FP64 matrix, n <= 256,
 // in one 1024-thread workgroup (the upper triangle is read and R written over
 // it; the strict lower triangle is untouched).  Right-looking over 32-wide
 // blocks, everything a block step reuses held in LDS:
-//   1. the diagonal block is factored in LDS (row k scaled by sqrt of its
-//      pivot, then the block's entries below row k updated);
+//   1. the diagonal block is factored by one wave, in registers: lane l holds
+//      column l of the block's upper triangle; pivot k is read from lane k,
+//      row k scaled by its square root, and every later row's entries updated
+//      with row k's entries read lane by lane (v_readlane) -- no workgroup
+//      barrier inside the block;
 //   2. the block row to its right is solved with the block's R^T, one thread
 //      per column (forward substitution in registers), and kept in LDS;
 //   3. the trailing upper triangle is updated from LDS, A -= P^T P, a 4 x 4
@@ -109,29 +120,56 @@ __global__ __launch_bounds__(256) void zero_lower_kernel(double *R, int64_t n) {
 // info = 0, or k + 1 when pivot k is not positive (R then incomplete).
 constexpr int CH_NB = 32, CH_MAX = 256, CH_THR = 1024;
 __global__ __launch_bounds__(CH_THR) void chol_blocked_kernel(double *A, int n, int *info) {
-    __shared__ double D[CH_NB][CH_NB + 1];  // D[i][j] = A(r0 + i, r0 + j), upper part
+    __shared__ double D[CH_NB][CH_NB + 1];  // D[i][j] = R(r0 + i, r0 + j), upper part
     __shared__ double P[CH_NB][CH_MAX];     // P[t][c] = R(r0 + t, c0 + c)
+    __shared__ int failed;
     const int tid = threadIdx.x;
-    const int di = tid % CH_NB, dj = tid / CH_NB;  // 32 x 32 threads over the diagonal block
+    if (tid == 0) failed = 0;
     for (int r0 = 0; r0 < n; r0 += CH_NB) {
         const int nb = n - r0 < CH_NB ? n - r0 : CH_NB;
-        const bool mine = di < nb && dj < nb && di <= dj;
-        if (mine) D[di][dj] = A[(r0 + di) + (int64_t)(r0 + dj) * n];
-        __syncthreads();
-        for (int k = 0; k < nb; ++k) {
-            const double dkk = D[k][k];
-            if (!(dkk > 0.0)) {  // every thread reads the same pivot: a uniform exit
-                if (tid == 0) *info = r0 + k + 1;
-                return;
+        if (tid < 64) {
+            // lane l < nb: column l of the block; other lanes carry identity
+            // columns, which leave the real ones unchanged
+            const int l = tid;
+            double c[CH_NB];
+#pragma unroll
+            for (int i = 0; i < CH_NB; ++i) {
+                double v = i == l ? 1.0 : 0.0;
+                if (l < nb && i <= l) v = A[(r0 + i) + (int64_t)(r0 + l) * n];
+                c[i] = v;
             }
-            const double rkk = sqrt(dkk);
-            __syncthreads();  // the pivot is read before row k is overwritten
-            if (di == k && dj >= k && dj < nb) D[k][dj] = dj == k ? rkk : D[k][dj] / rkk;
-            __syncthreads();
-            if (di > k && di <= dj && dj < nb) D[di][dj] -= D[k][di] * D[k][dj];
-            __syncthreads();
+            int bad = 0;  // uniform across the wave
+#pragma unroll
+            for (int k = 0; k < CH_NB; ++k) {
+                const double p = readlane_f64(c[k], k);  // the pivot, uniform
+                if (bad == 0 && !(p > 0.0)) bad = r0 + k + 1;
+                if (bad == 0) {
+                    const double rkk = sqrt(p);
+                    if (l == k) c[k] = rkk;
+                    else if (l > k) c[k] = c[k] / rkk;  // R(k, l)
+#pragma unroll
+                    for (int i = k + 1; i < CH_NB; ++i) {
+                        const double rki = readlane_f64(c[k], i);  // R(k, i)
+                        if (i <= l) c[i] -= rki * c[k];
+                    }
+                }
+            }
+            if (bad != 0) {
+                if (l == 0) {
+                    *info = bad;
+                    failed = 1;
+                }
+            } else if (l < nb) {
+#pragma unroll
+                for (int i = 0; i < CH_NB; ++i)
+                    if (i <= l) {
+                        D[i][l] = c[i];
+                        A[(r0 + i) + (int64_t)(r0 + l) * n] = c[i];
+                    }
+            }
         }
-        if (mine) A[(r0 + di) + (int64_t)(r0 + dj) * n] = D[di][dj];
+        __syncthreads();
+        if (failed) return;  // uniform: every thread reads the same flag
         const int c0 = r0 + nb, m = n - c0;
         if (tid < m) {
             // R^T x = a for column c0 + tid of the block row

@@ -835,8 +835,10 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
     // sequential chain, op(B) streamed once through LDS
     if (layout_opB == 'C' && d <= CSRB_MAX_ROWS && nnz <= LDS_APPLY_MAX_MEAN_ROW * d) {
         ProfScope prof("csr_lds_apply_kernel", st);
-        if (csr_lds_apply<T>(row_ptr, ccol, cval, d, n, m, alpha, beta, B, ldb, C, crs, ccs, kib ? SPF_KIB : SPF_JKI,
-                             st))
+        const size_t wsb = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T));
+        char *lws = (char *)scratch(SCRATCH_SPARSE_G, wsb, st);
+        if (csr_lds_apply<T>(row_ptr, ccol, cval, d, n, m, nnz, alpha, beta, B, ldb, C, crs, ccs,
+                             kib ? SPF_KIB : SPF_JKI, lws, wsb, st))
             return;
     }
     {

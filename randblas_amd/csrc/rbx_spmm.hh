@@ -17,6 +17,17 @@ enum { SPF_JKI = 0, SPF_KIB = 1 };
 // live in LDS).
 constexpr int64_t CSRB_MAX_ROWS = 8192;
 
+// Bytes of workspace csr_build_counting_ws needs (0 when the shape is
+// outside what the builder takes).
+size_t csr_build_counting_bytes(int64_t nnz, int64_t d, size_t val_size);
+// The same build into a caller-provided workspace of ws_bytes bytes (every
+// byte it reads is written first in the same call, so the workspace needs no
+// clearing between calls).
+template <typename T>
+bool csr_build_counting_ws(const int64_t *rows, const int64_t *cols, const T *vals, int64_t nnz, int64_t ro,
+                           int64_t co, int64_t d, int64_t m, char *ws, size_t ws_bytes, int64_t **row_ptr,
+                           int32_t **col, T **val, hipStream_t st);
+
 // COO entries inside the window [ro, ro + d) x [co, co + m) -> CSR sorted by
 // (row, column, stored position): a stable counting sort by row, then rows
 // whose columns are not ascending (entries not given in column order) sorted
@@ -28,16 +39,20 @@ bool csr_build_counting(const int64_t *rows, const int64_t *cols, const T *vals,
                         int64_t d, int64_t m, int64_t **row_ptr, int32_t **col, T **val, hipStream_t st);
 
 // C (d x n, element (i, j) at C[i*crs + j*ccs]) = alpha * A_csr * Bm + beta * C
-// for a CSR A (d x m) and a dense Bm (m x n) with unit stride along m
-// (Bm[c, j] at B[c + j*ldb]).  Each output entry is one sequential chain over
-// its row's entries in stored order, in summation form `form`, so results
-// equal the reference's bit for bit.  Bm is read once: a workgroup owns a
-// block of output columns and 1024 output rows, and streams chunks of Bm's
-// columns through LDS.  Returns false when the shape is outside what the
-// kernel takes (alpha == 0 and beta-only scaling are the caller's).
+// for a CSR A (d x m, nnz stored entries, columns ascending within each row)
+// and a dense Bm (m x n) with unit stride along m (Bm[c, j] at B[c + j*ldb]).
+// Each output entry is one sequential chain over its row's entries in stored
+// order, in summation form `form`, so results equal the reference's bit for
+// bit.  Bm is read once: a workgroup owns a block of output columns and 512
+// output rows, and streams chunks of Bm's columns through LDS.  `ws` is a
+// workspace of csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T)) bytes (the
+// entries laid out by chunk and slot; every byte read is written first in the
+// same call).  Returns false when the shape is outside what the kernel takes
+// (alpha == 0 and beta-only scaling are the caller's).
+size_t csr_lds_apply_ws_bytes(int64_t d, int64_t m, int64_t nnz, size_t val_size);
 template <typename T>
 bool csr_lds_apply(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
-                   T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int form,
-                   hipStream_t st);
+                   int64_t nnz, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int form,
+                   char *ws, size_t ws_bytes, hipStream_t st);
 
 }  // namespace rbx

@@ -56,44 +56,67 @@ __global__ __launch_bounds__(CB_THR) void csrb_hist_kernel(const int64_t *rows, 
     for (int r = threadIdx.x; r < d; r += CB_THR) H[(int64_t)blockIdx.x * d + r] = h[r];
 }
 
-// One workgroup: H[b * d + r] becomes the number of row-r entries in blocks
-// before b, and row_ptr the exclusive scan of the row totals.
-__global__ __launch_bounds__(1024) void csrb_scan_kernel(int32_t *H, int nb, int d, int64_t *row_ptr) {
-    __shared__ int32_t tot[CSRB_MAX_ROWS];
-    __shared__ int32_t part[1024];
-    const int t = threadIdx.x;
-    for (int r = t; r < d; r += 1024) {
-        int32_t run = 0;
+// H[b * d + r] becomes the number of row-r entries in blocks before b, and
+// tot[r] the row's total.  32 rows per workgroup, each row's blocks cut into
+// 8 segments (one thread each): segment sums, their exclusive offsets through
+// LDS, then each segment's running offsets written back.
+constexpr int CSC_ROWS = 32, CSC_SEGS = 8;
+__global__ __launch_bounds__(CSC_ROWS * CSC_SEGS) void csrb_colscan_kernel(int32_t *H, int nb, int d, int64_t *tot) {
+    __shared__ int32_t seg[CSC_SEGS][CSC_ROWS];
+    const int t = threadIdx.x, lr = t % CSC_ROWS, g = t / CSC_ROWS;
+    const int r = blockIdx.x * CSC_ROWS + lr;
+    const int nbs = (nb + CSC_SEGS - 1) / CSC_SEGS;
+    const int b0 = g * nbs, b1 = b0 + nbs < nb ? b0 + nbs : nb;
+    int32_t sum = 0;
+    if (r < d) {
 #pragma unroll 8
-        for (int b = 0; b < nb; ++b) {
+        for (int b = b0; b < b1; ++b) sum += H[(int64_t)b * d + r];
+    }
+    seg[g][lr] = sum;
+    __syncthreads();
+    int32_t run = 0;
+    for (int q = 0; q < g; ++q) run += seg[q][lr];
+    if (r < d) {
+#pragma unroll 8
+        for (int b = b0; b < b1; ++b) {
             const int64_t i = (int64_t)b * d + r;
             const int32_t x = H[i];
             H[i] = run;
             run += x;
         }
-        tot[r] = run;
+        if (g == CSC_SEGS - 1) tot[r] = run;
     }
-    __syncthreads();
-    const int per = (d + 1023) / 1024;  // rows per thread in the scan (<= 8)
-    int32_t s = 0;
-    for (int q = 0; q < per; ++q) {
+}
+
+// One workgroup: row_ptr[0..d) holds the row totals on entry and their
+// exclusive scan on exit; row_ptr[d] = the sum.
+__global__ __launch_bounds__(1024) void csrb_rowscan_kernel(int d, int64_t *row_ptr) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (d + 1023) / 1024;  // rows per thread (<= 8)
+    int64_t v[CSRB_MAX_ROWS / 1024];
+    int64_t s = 0;
+#pragma unroll
+    for (int q = 0; q < CSRB_MAX_ROWS / 1024; ++q) {
         const int r = t * per + q;
-        if (r < d) s += tot[r];
+        v[q] = (q < per && r < d) ? row_ptr[r] : 0;
+        s += v[q];
     }
     part[t] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        const int32_t v = t >= off ? part[t - off] : 0;
+        const int64_t x = t >= off ? part[t - off] : 0;
         __syncthreads();
-        part[t] += v;
+        part[t] += x;
         __syncthreads();
     }
     int64_t run = t > 0 ? part[t - 1] : 0;
-    for (int q = 0; q < per; ++q) {
+#pragma unroll
+    for (int q = 0; q < CSRB_MAX_ROWS / 1024; ++q) {
         const int r = t * per + q;
-        if (r < d) {
+        if (q < per && r < d) {
             row_ptr[r] = run;
-            run += tot[r];
+            run += v[q];
         }
     }
     if (t == 1023) row_ptr[d] = part[1023];
@@ -256,26 +279,59 @@ __global__ __launch_bounds__(CB_THR) void csrb_finish_kernel(const int64_t *row_
     }
 }
 
-template <typename T>
-bool csr_build_counting(const int64_t *rows, const int64_t *cols, const T *vals, int64_t nnz, int64_t ro, int64_t co,
-                        int64_t d, int64_t m, int64_t **row_ptr_out, int32_t **col_out, T **val_out, hipStream_t st) {
-    if (d <= 0 || d > CSRB_MAX_ROWS || nnz <= 0 || nnz >= (1ll << 31) || m >= (1ll << 31)) return false;
+// Block layout of the counting sort: tiles of CB_TILE entries, at most
+// CB_MAX_BLOCKS workgroups, each owning `span` consecutive stored entries.
+struct CsrbPlan {
+    int nb;
+    int64_t span;
+    size_t bH, bR, bK, bC, bV;
+    size_t bytes() const { return bH + bR + 2 * bK + bC + bV; }
+};
+
+static CsrbPlan csrb_plan(int64_t nnz, int64_t d, size_t val_size) {
+    CsrbPlan p;
     const int64_t tiles = (nnz + CB_TILE - 1) / CB_TILE;
     const int64_t tpb = (tiles + CB_MAX_BLOCKS - 1) / CB_MAX_BLOCKS;  // tiles per block
-    const int nb = (int)((tiles + tpb - 1) / tpb);
-    const int64_t span = tpb * CB_TILE;
-    const size_t bH = al256((size_t)nb * d * 4), bR = al256((size_t)(d + 1) * 8), bK = al256((size_t)nnz * 8),
-                 bC = al256((size_t)nnz * 4), bV = al256((size_t)nnz * sizeof(T));
-    char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, bH + bR + 2 * bK + bC + bV, st);
+    p.nb = (int)((tiles + tpb - 1) / tpb);
+    p.span = tpb * CB_TILE;
+    p.bH = al256((size_t)p.nb * d * 4);
+    p.bR = al256((size_t)(d + 1) * 8);
+    p.bK = al256((size_t)nnz * 8);
+    p.bC = al256((size_t)nnz * 4);
+    p.bV = al256((size_t)nnz * val_size);
+    return p;
+}
+
+static bool csrb_supported(int64_t nnz, int64_t d, int64_t m) {
+    return d > 0 && d <= CSRB_MAX_ROWS && nnz > 0 && nnz < (1ll << 31) && m < (1ll << 31);
+}
+
+size_t csr_build_counting_bytes(int64_t nnz, int64_t d, size_t val_size) {
+    if (d <= 0 || d > CSRB_MAX_ROWS || nnz <= 0 || nnz >= (1ll << 31)) return 0;
+    return csrb_plan(nnz, d, val_size).bytes();
+}
+
+template <typename T>
+bool csr_build_counting_ws(const int64_t *rows, const int64_t *cols, const T *vals, int64_t nnz, int64_t ro,
+                           int64_t co, int64_t d, int64_t m, char *ws, size_t ws_bytes, int64_t **row_ptr_out,
+                           int32_t **col_out, T **val_out, hipStream_t st) {
+    if (!csrb_supported(nnz, d, m)) return false;
+    const CsrbPlan pl = csrb_plan(nnz, d, sizeof(T));
+    RBX_REQUIRE(ws != nullptr && ws_bytes >= pl.bytes());
+    const int nb = pl.nb;
+    const int64_t span = pl.span;
     int32_t *H = (int32_t *)ws;
-    int64_t *row_ptr = (int64_t *)(ws + bH);
-    uint64_t *keys = (uint64_t *)(ws + bH + bR), *keys2 = (uint64_t *)(ws + bH + bR + bK);
-    int32_t *col = (int32_t *)(ws + bH + bR + 2 * bK);
-    T *val = (T *)(ws + bH + bR + 2 * bK + bC);
+    int64_t *row_ptr = (int64_t *)(ws + pl.bH);
+    uint64_t *keys = (uint64_t *)(ws + pl.bH + pl.bR), *keys2 = (uint64_t *)(ws + pl.bH + pl.bR + pl.bK);
+    int32_t *col = (int32_t *)(ws + pl.bH + pl.bR + 2 * pl.bK);
+    T *val = (T *)(ws + pl.bH + pl.bR + 2 * pl.bK + pl.bC);
     const int di = (int)d;
     hipLaunchKernelGGL(csrb_hist_kernel, dim3(nb), dim3(CB_THR), 0, st, rows, cols, nnz, ro, co, di, m, span, H);
     RBX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(csrb_scan_kernel, dim3(1), dim3(1024), 0, st, H, nb, di, row_ptr);
+    hipLaunchKernelGGL(csrb_colscan_kernel, dim3((unsigned)((d + CSC_ROWS - 1) / CSC_ROWS)),
+                       dim3(CSC_ROWS * CSC_SEGS), 0, st, H, nb, di, row_ptr);
+    RBX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(csrb_rowscan_kernel, dim3(1), dim3(1024), 0, st, di, row_ptr);
     RBX_HIP(hipGetLastError());
     hipLaunchKernelGGL(csrb_scatter_kernel, dim3(nb), dim3(CB_THR), 0, st, rows, cols, nnz, ro, co, di, m, span,
                        (const int32_t *)H, (const int64_t *)row_ptr, bits_of((uint64_t)d), keys);
@@ -290,24 +346,139 @@ bool csr_build_counting(const int64_t *rows, const int64_t *cols, const T *vals,
     return true;
 }
 
-// ---------------------------------------------------------------- apply through LDS
-constexpr int LA_THR = 1024, LA_CK = 1024, LA_WIN = 8;
+template <typename T>
+bool csr_build_counting(const int64_t *rows, const int64_t *cols, const T *vals, int64_t nnz, int64_t ro, int64_t co,
+                        int64_t d, int64_t m, int64_t **row_ptr_out, int32_t **col_out, T **val_out, hipStream_t st) {
+    if (!csrb_supported(nnz, d, m)) return false;
+    const size_t bytes = csrb_plan(nnz, d, sizeof(T)).bytes();
+    char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, bytes, st);
+    return csr_build_counting_ws<T>(rows, cols, vals, nnz, ro, co, d, m, ws, bytes, row_ptr_out, col_out, val_out,
+                                    st);
+}
 
-template <typename T, int J, int FORM>
-__global__ __launch_bounds__(LA_THR) void csr_lds_apply_kernel(const int64_t *row_ptr, const int32_t *col,
-                                                               const T *val, int64_t d, int64_t n, int64_t m, T alpha,
-                                                               T beta, const T *B, int64_t ldb, T *C, int64_t crs,
-                                                               int64_t ccs, int RW, int ncb) {
-    __shared__ __attribute__((aligned(16))) T As[2][LA_CK * J];
+// ---------------------------------------------------------------- apply through LDS
+// A workgroup owns NT output rows (one per thread) and J output columns, and
+// streams those J columns of the dense operand through LDS in CK-deep chunks
+// (double-buffered: the next chunk's loads are in flight while the current one
+// is applied).  The stored entries are first laid out by chunk and slot:
+// Ec / Ev[(k * LA_MAXE + q) * d + r] is row r's q-th entry among those with a
+// column in chunk k, and startT[k * d + r] where the row's entries of chunk k
+// start in the CSR.  A wave's load of slot q then reads 64 consecutive rows
+// (one coalesced request instead of 64 lines), and each thread loads the next
+// chunk's slots into a second register bank while it applies the current
+// chunk's; slots past the wave's largest count are skipped, and a row with
+// more than LA_MAXE entries in a chunk reads the rest from the CSR.  Every
+// output is one chain over its row's entries in ascending column.  The row
+// groups of one column block run back to back on one XCD, so the block's
+// dense data comes from HBM once and from that XCD's L2 after.
+constexpr int LA_CKMAX = 1024, LA_MAXE = 12;
+
+// Chunk depth: a power of two in [64, LA_CKMAX] giving rows about six stored
+// entries per chunk or fewer.
+static int la_chunk(int64_t d, int64_t m, int64_t nnz) {
+    int ck = 64;
+    while (ck < LA_CKMAX && (double)(2 * ck) * (double)nnz <= 6.0 * (double)d * (double)m) ck *= 2;
+    return ck;
+}
+
+struct LaPlan {
+    int ck;
+    int64_t nk;
+    size_t bStart, bEc, bEv;
+    size_t bytes() const { return bStart + bEc + bEv; }
+};
+static LaPlan la_plan(int64_t d, int64_t m, int64_t nnz, size_t val_size) {
+    LaPlan p;
+    p.ck = la_chunk(d, m, nnz);
+    p.nk = (m + p.ck - 1) / p.ck;
+    p.bStart = al256((size_t)(p.nk + 1) * (size_t)d * 4);
+    p.bEc = al256((size_t)p.nk * LA_MAXE * (size_t)d * 4);
+    p.bEv = al256((size_t)p.nk * LA_MAXE * (size_t)d * val_size);
+    return p;
+}
+
+size_t csr_lds_apply_ws_bytes(int64_t d, int64_t m, int64_t nnz, size_t val_size) {
+    return la_plan(d, m, nnz, val_size).bytes();
+}
+
+// startT[k * d + r] = position of row r's first entry with column >= k*CK
+// (the row's end for k = nk)
+__global__ __launch_bounds__(256) void csr_chunk_table_kernel(const int64_t *row_ptr, const int32_t *col, int64_t d,
+                                                              int nk, int CK, int32_t *startT) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= d * (int64_t)(nk + 1)) return;
+    const int k = (int)(idx / d);
+    const int64_t r = idx - (int64_t)k * d;
+    int64_t lo = row_ptr[r], hi = row_ptr[r + 1];
+    if (k < nk) {
+        const int64_t c = (int64_t)k * CK;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if ((int64_t)col[mid] < c) lo = mid + 1;
+            else hi = mid;
+        }
+    } else {
+        lo = hi;
+    }
+    startT[idx] = (int32_t)lo;
+}
+
+// One workgroup per row (grid-stride): each entry p of row r goes to slot
+// q = p - startT[k * d + r] of its chunk k when q < LA_MAXE.
+template <typename T>
+__global__ __launch_bounds__(256) void csr_chunk_slots_kernel(const int64_t *row_ptr, const int32_t *col,
+                                                              const T *val, int64_t d, int CK,
+                                                              const int32_t *startT, int32_t *Ec, T *Ev) {
+    for (int64_t r = blockIdx.x; r < d; r += gridDim.x) {
+        const int64_t p1 = row_ptr[r + 1];
+        for (int64_t p = row_ptr[r] + threadIdx.x; p < p1; p += blockDim.x) {
+            const int32_t c = col[p];
+            const int64_t k = c / CK;
+            const int64_t q = p - startT[k * d + r];
+            if (q < LA_MAXE) {
+                const int64_t o = (k * LA_MAXE + q) * d + r;
+                Ec[o] = c;
+                Ev[o] = val[p];
+            }
+        }
+    }
+}
+
+// Raw buffer loads: an offset at or past the descriptor's byte count reads 0,
+// so tile loads past the operand's end need no branch around them.
+constexpr int BUF_DWORD3 = 0x00020000;  // gfx9-family raw buffer descriptor word 3
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
+template <typename T, int J, int NT, int FORM>
+__global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT, const int32_t *Ec, const T *Ev,
+                                                           int nk, int CK, const int32_t *col, const T *val,
+                                                           int64_t d, int64_t n, int64_t m, T alpha, T beta,
+                                                           const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs,
+                                                           int RW, int ncb) {
+    // chunk tiles column by column: As[buf][jj * LA_CKMAX + c] (conflict-free
+    // stores; a stored entry reads its J values at stride LA_CKMAX)
+    __shared__ __attribute__((aligned(16))) T As[2][LA_CKMAX * J];
+    constexpr int PER = LA_CKMAX / NT;  // contraction indices a thread loads per chunk, at most
     const int tid = threadIdx.x;
-    // XCD-aware order: the RW row groups of a column block are consecutive
-    // tiles on one XCD, so its columns of B stream from HBM once
-    const int W = ncb * RW, L = blockIdx.x;
-    const int tt = (W % 8 == 0) ? (L % 8) * (W / 8) + L / 8 : L;
+    // XCD-aware order over a grid padded to a multiple of 8: tile tt runs on
+    // XCD tt / (grid / 8), so a column block's row groups share one L2
+    const int W = ncb * RW, L = blockIdx.x, G8 = (int)gridDim.x / 8;
+    const int tt = (L % 8) * G8 + L / 8;
+    if (tt >= W) return;  // padding: the whole workgroup leaves before any barrier
     const int cb = tt / RW, rg = tt % RW;
     const int64_t j0 = (int64_t)cb * J;
-    const int64_t r = (int64_t)rg * LA_THR + tid;
+    const int64_t r = (int64_t)rg * NT + tid;
     const bool active = r < d;
+    const int64_t rs = active ? r : d - 1;  // a row every load may read (loads are never conditional)
 
     T acc[J];
 #pragma unroll
@@ -315,56 +486,112 @@ __global__ __launch_bounds__(LA_THR) void csr_lds_apply_kernel(const int64_t *ro
         acc[jj] = (T)0;
         if (active && beta != (T)0 && j0 + jj < n) acc[jj] = beta * C[r * crs + (j0 + jj) * ccs];
     }
-    int64_t p = active ? row_ptr[r] : 0;
-    const int64_t pend = active ? row_ptr[r + 1] : 0;
-
-    T ld[J];
-    auto load_chunk = [&](int64_t k) {
-        const int64_t c = k * LA_CK + tid;
+    auto wave_max = [](int32_t x) {
 #pragma unroll
-        for (int jj = 0; jj < J; ++jj) ld[jj] = (c < m && j0 + jj < n) ? B[c + (j0 + jj) * ldb] : (T)0;
+        for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o));
+        return x;
+    };
+    auto start_of = [&](int64_t k) {
+        const int32_t v = startT[k * d + rs];
+        return active ? v : 0;
+    };
+
+    // register banks of slots: chunk k in bank k % 2
+    int32_t bc0[LA_MAXE], bc1[LA_MAXE];
+    T bv0[LA_MAXE], bv1[LA_MAXE];
+    // every slot is loaded (a fixed number of loads, so the waits before the
+    // tile stores cover only the tile's own, older loads)
+    auto load_bank = [&](int64_t k, int32_t *bc, T *bv) {
+#pragma unroll
+        for (int q = 0; q < LA_MAXE; ++q) {
+            const int64_t o = (k * LA_MAXE + q) * d + rs;
+            bc[q] = Ec[o];
+            bv[q] = Ev[o];
+        }
+    };
+    auto apply_one = [&](const T *bb, T v) {
+#pragma unroll
+        for (int jj = 0; jj < J; ++jj) {
+            const T b = bb[jj * LA_CKMAX];
+            if (FORM == SPF_KIB) acc[jj] = acc[jj] + (alpha * v) * b;
+            else acc[jj] = acc[jj] + v * (alpha * b);
+        }
+    };
+    auto apply = [&](int64_t k, const int32_t *bc, const T *bv, int32_t s, int32_t e, int32_t wmax) {
+        const int32_t cbase = (int32_t)(k * CK);
+        const T *Ab = As[k & 1];
+        const int32_t cnt = e - s;
+#pragma unroll
+        for (int q = 0; q < LA_MAXE; ++q)
+            if (q < wmax && q < cnt) apply_one(Ab + (bc[q] - cbase), bv[q]);
+        for (int32_t p = s + LA_MAXE; p < e; ++p) apply_one(Ab + (col[p] - cbase), val[p]);
+    };
+
+    // one buffer descriptor per output column: m entries, none past n
+    __amdgpu_buffer_rsrc_t rsc[J];
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj) {
+        const bool colok = j0 + jj < n;
+        rsc[jj] = __builtin_amdgcn_make_buffer_rsrc((void *)(B + (colok ? (j0 + jj) * ldb : 0)), 0,
+                                                    colok ? (int)(m * (int64_t)sizeof(T)) : 0, BUF_DWORD3);
+    }
+    T ld[PER][J];
+    auto load_chunk = [&](int64_t k) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const uint32_t off = (uint32_t)((k * CK + u * NT + tid) * (int64_t)sizeof(T));
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) ld[u][jj] = buf_load<T>(rsc[jj], off);
+        }
     };
     auto store_chunk = [&](int buf) {
 #pragma unroll
-        for (int jj = 0; jj < J; ++jj) As[buf][tid * J + jj] = ld[jj];
+        for (int u = 0; u < PER; ++u)
+            if (u * NT + tid < CK) {
+#pragma unroll
+                for (int jj = 0; jj < J; ++jj) As[buf][jj * LA_CKMAX + u * NT + tid] = ld[u][jj];
+            }
     };
-    const int64_t nk = (m + LA_CK - 1) / LA_CK;
+
+    // chunk k's entries are [s_k, e_k) of the CSR, with e_k = s_{k+1}.  The
+    // prefetches of chunk k + 1 are issued unconditionally (indices clamped
+    // to the last chunk), so every iteration has the same loads in the same
+    // order and the waits before each use cover only older loads.
+    const int64_t klast = nk - 1;
+    int32_t s0 = start_of(0), e0 = start_of(1), s1 = 0, e1 = 0;
+    int32_t w0 = wave_max(e0 - s0), w1 = 0;
+    int32_t en = start_of(nk >= 2 ? 2 : nk);  // e_{k+1} ahead of its use
     load_chunk(0);
+    load_bank(0, bc0, bv0);
     store_chunk(0);
     __syncthreads();
-    for (int64_t k = 0; k < nk; ++k) {
-        if (k + 1 < nk) load_chunk(k + 1);
-        const int64_t c0 = k * LA_CK;
-        const int32_t c1 = (int32_t)((k + 1) * LA_CK < m ? (k + 1) * LA_CK : m);
-        const T *Ab = As[k & 1];
-        // windows of LA_WIN entries from p; stop at the chunk's end
-        while (p < pend) {
-            int32_t wc[LA_WIN];
-            T wv[LA_WIN];
-#pragma unroll
-            for (int q = 0; q < LA_WIN; ++q) {
-                const bool in = p + q < pend;
-                wc[q] = in ? col[p + q] : INT32_MAX;
-                wv[q] = in ? val[p + q] : (T)0;
-            }
-            int used = 0;
-#pragma unroll
-            for (int q = 0; q < LA_WIN; ++q) {
-                if (used == q && wc[q] < c1) {
-                    const T v = wv[q];
-                    const T *b = Ab + (wc[q] - c0) * J;
-#pragma unroll
-                    for (int jj = 0; jj < J; ++jj) {
-                        if (FORM == SPF_KIB) acc[jj] = acc[jj] + (alpha * v) * b[jj];
-                        else acc[jj] = acc[jj] + v * (alpha * b[jj]);
-                    }
-                    used = q + 1;
-                }
-            }
-            p += used;
-            if (used < LA_WIN) break;
+    for (int64_t k = 0; k < nk; k += 2) {
+        // chunk k from bank 0; chunk k + 1's tile, then its slots into bank 1
+        {
+            const int64_t kn = k + 1 < nk ? k + 1 : klast;
+            s1 = e0;
+            e1 = k + 1 < nk ? en : e0;
+            w1 = wave_max(e1 - s1);
+            en = start_of(k + 3 <= nk ? k + 3 : nk);  // first: the next wait for it is a short one
+            load_chunk(kn);
+            load_bank(kn, bc1, bv1);
         }
-        if (k + 1 < nk) store_chunk((int)((k + 1) & 1));
+        apply(k, bc0, bv0, s0, e0, w0);
+        store_chunk((int)((k + 1) & 1));
+        __syncthreads();
+        if (k + 1 >= nk) break;
+        // chunk k + 1 from bank 1; chunk k + 2's tile, then its slots into bank 0
+        {
+            const int64_t kn = k + 2 < nk ? k + 2 : klast;
+            s0 = e1;
+            e0 = k + 2 < nk ? en : e1;
+            w0 = wave_max(e0 - s0);
+            en = start_of(k + 4 <= nk ? k + 4 : nk);
+            load_chunk(kn);
+            load_bank(kn, bc0, bv0);
+        }
+        apply(k + 1, bc1, bv1, s1, e1, w1);
+        store_chunk((int)((k + 2) & 1));
         __syncthreads();
     }
     if (active) {
@@ -374,50 +601,89 @@ __global__ __launch_bounds__(LA_THR) void csr_lds_apply_kernel(const int64_t *ro
     }
 }
 
-template <typename T, int J>
-static void launch_lds_apply(int form, dim3 grid, const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d,
-                             int64_t n, int64_t m, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs,
-                             int64_t ccs, int RW, int ncb, hipStream_t st) {
+template <typename T, int J, int NT>
+static void launch_lds_apply(int form, dim3 grid, const int32_t *startT, const int32_t *Ec, const T *Ev, int nk,
+                             int ck, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m, T alpha,
+                             T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int RW, int ncb,
+                             hipStream_t st) {
     if (form == SPF_KIB)
-        hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, SPF_KIB>), grid, dim3(LA_THR), 0, st, row_ptr, col, val, d, n,
-                           m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb);
+        hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, NT, SPF_KIB>), grid, dim3(NT), 0, st, startT, Ec, Ev, nk, ck,
+                           col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb);
     else
-        hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, SPF_JKI>), grid, dim3(LA_THR), 0, st, row_ptr, col, val, d, n,
-                           m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb);
+        hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, NT, SPF_JKI>), grid, dim3(NT), 0, st, startT, Ec, Ev, nk, ck,
+                           col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb);
     RBX_HIP(hipGetLastError());
+}
+
+template <typename T, int NT>
+static void launch_lds_apply_j(int J, int form, dim3 grid, const int32_t *startT, const int32_t *Ec, const T *Ev,
+                               int nk, int ck, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
+                               T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int RW,
+                               int ncb, hipStream_t st) {
+#define RBX_LA_ARGS form, grid, startT, Ec, Ev, nk, ck, col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb, st
+    if (J == 8) launch_lds_apply<T, 8, NT>(RBX_LA_ARGS);
+    else if (J == 4) launch_lds_apply<T, 4, NT>(RBX_LA_ARGS);
+    else if (J == 2) launch_lds_apply<T, 2, NT>(RBX_LA_ARGS);
+    else launch_lds_apply<T, 1, NT>(RBX_LA_ARGS);
+#undef RBX_LA_ARGS
 }
 
 template <typename T>
 bool csr_lds_apply(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
-                   T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int form,
-                   hipStream_t st) {
-    if (d <= 0 || n <= 0 || m <= 0 || m >= (1ll << 31)) return false;
-    const int64_t RW = (d + LA_THR - 1) / LA_THR;
-    // widest column block (16 bytes of B per contraction index, i.e. 2 FP64
-    // or 4 FP32 columns, up to twice that) that still gives one workgroup per CU
-    constexpr int JW = 16 / (int)sizeof(T);
+                   int64_t nnz, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int form,
+                   char *ws, size_t ws_bytes, hipStream_t st) {
+    // buffer offsets: (m + one chunk) entries must stay below 2^31 bytes
+    if (d <= 0 || n <= 0 || m <= 0 || (m + LA_CKMAX) * (int64_t)sizeof(T) >= (1ll << 31) || ws == nullptr)
+        return false;
+    const LaPlan pl = la_plan(d, m, nnz, sizeof(T));
+    RBX_REQUIRE(ws_bytes >= pl.bytes());
+    // 512 rows per workgroup (256 for few rows), the widest column block
+    // (8, 4, 2, 1) that still gives about a workgroup per CU
+    const int NT = d > 256 ? 512 : 256;
+    const int64_t RW = (d + NT - 1) / NT;
     const int64_t cus = num_cus();
-    int J = 2 * JW;
-    if (((n + J - 1) / J) * RW < cus) J = JW;
-    if (((n + J - 1) / J) * RW < cus) J = 1;
+    int J = 8;
+    while (J > 1 && ((n + J - 1) / J) * RW < cus * 3 / 4) J /= 2;
     const int64_t ncb = (n + J - 1) / J;
-    if (ncb * RW >= (1ll << 31)) return false;
-    const dim3 grid((unsigned)(ncb * RW));
+    if (ncb * RW >= (1ll << 30) || pl.nk >= (1ll << 30)) return false;
+    int32_t *startT = (int32_t *)ws;
+    int32_t *Ec = (int32_t *)(ws + pl.bStart);
+    T *Ev = (T *)(ws + pl.bStart + pl.bEc);
+    const int64_t tcells = d * (pl.nk + 1);
+    const int nki = (int)pl.nk;
+    hipLaunchKernelGGL(csr_chunk_table_kernel, dim3((unsigned)((tcells + 255) / 256)), dim3(256), 0, st, row_ptr, col,
+                       d, nki, pl.ck, startT);
+    RBX_HIP(hipGetLastError());
+    const int64_t sg = d < 4 * cus ? d : 4 * cus;
+    hipLaunchKernelGGL((csr_chunk_slots_kernel<T>), dim3((unsigned)sg), dim3(256), 0, st, row_ptr, col, val, d, pl.ck,
+                       (const int32_t *)startT, Ec, Ev);
+    RBX_HIP(hipGetLastError());
+    const dim3 grid((unsigned)((ncb * RW + 7) / 8 * 8));
     const int rw = (int)RW, nc = (int)ncb;
-    if (J == 2 * JW) launch_lds_apply<T, 2 * JW>(form, grid, row_ptr, col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, rw, nc, st);
-    else if (J == JW) launch_lds_apply<T, JW>(form, grid, row_ptr, col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, rw, nc, st);
-    else launch_lds_apply<T, 1>(form, grid, row_ptr, col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, rw, nc, st);
+    if (NT == 512)
+        launch_lds_apply_j<T, 512>(J, form, grid, startT, Ec, Ev, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
+                                   crs, ccs, rw, nc, st);
+    else
+        launch_lds_apply_j<T, 256>(J, form, grid, startT, Ec, Ev, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
+                                   crs, ccs, rw, nc, st);
     return true;
 }
 
+template bool csr_build_counting_ws<float>(const int64_t *, const int64_t *, const float *, int64_t, int64_t,
+                                           int64_t, int64_t, int64_t, char *, size_t, int64_t **, int32_t **,
+                                           float **, hipStream_t);
+template bool csr_build_counting_ws<double>(const int64_t *, const int64_t *, const double *, int64_t, int64_t,
+                                            int64_t, int64_t, int64_t, char *, size_t, int64_t **, int32_t **,
+                                            double **, hipStream_t);
 template bool csr_build_counting<float>(const int64_t *, const int64_t *, const float *, int64_t, int64_t, int64_t,
                                         int64_t, int64_t, int64_t **, int32_t **, float **, hipStream_t);
 template bool csr_build_counting<double>(const int64_t *, const int64_t *, const double *, int64_t, int64_t, int64_t,
                                          int64_t, int64_t, int64_t **, int32_t **, double **, hipStream_t);
-template bool csr_lds_apply<float>(const int64_t *, const int32_t *, const float *, int64_t, int64_t, int64_t, float,
-                                   float, const float *, int64_t, float *, int64_t, int64_t, int, hipStream_t);
+template bool csr_lds_apply<float>(const int64_t *, const int32_t *, const float *, int64_t, int64_t, int64_t,
+                                   int64_t, float, float, const float *, int64_t, float *, int64_t, int64_t, int,
+                                   char *, size_t, hipStream_t);
 template bool csr_lds_apply<double>(const int64_t *, const int32_t *, const double *, int64_t, int64_t, int64_t,
-                                    double, double, const double *, int64_t, double *, int64_t, int64_t, int,
-                                    hipStream_t);
+                                    int64_t, double, double, const double *, int64_t, double *, int64_t, int64_t, int,
+                                    char *, size_t, hipStream_t);
 
 }  // namespace rbx
