@@ -30,6 +30,7 @@
 #ifndef RANDBLAS_AMD_H
 #define RANDBLAS_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -322,6 +323,30 @@ rbx_status rbx_drskges(char layout, char opA, char opS, int64_t m, int64_t d, in
                        double alpha, const double *A, int64_t lda, const rbx_sparse_skop *S,
                        int64_t ro_s, int64_t co_s, double beta, double *B, int64_t ldb,
                        void *stream);
+
+/* (no reference counterpart) lskges without device allocation, for callers
+ * that cannot allocate: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same arguments and results as
+ * rbx_[sd]lskges (bit-identical), restricted to: S holding its sampled arrays
+ * (filled; S->nnz >= 0), op(A) with unit stride along the contraction
+ * (layout 'C' with opA 'N', or layout 'R' with opA 'T'), d <= 8192, at most
+ * 2048 stored entries per sketch row on average, and a workspace (the
+ * operator window's CSR and its chunked layout) that fits the library's fixed
+ * per-GPU scratch array of rbx_lskges_noalloc_scratch_bytes() bytes (64 MiB;
+ * the 2000 x 100000, vec_nnz 8 SASO operator needs 53 MB in FP64); larger or
+ * other shapes return RBX_INVALID_ARG or RBX_UNSUPPORTED with a message and
+ * launch nothing.  The array is one per GPU and shared by every call on that
+ * GPU: calls whose execution overlaps on one GPU (other streams, other host
+ * threads) race. */
+rbx_status rbx_slskges_noalloc(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m,
+                               float alpha, const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s,
+                               const float *A, int64_t lda, float beta, float *B, int64_t ldb,
+                               void *stream);
+rbx_status rbx_dlskges_noalloc(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m,
+                               double alpha, const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s,
+                               const double *A, int64_t lda, double beta, double *B, int64_t ldb,
+                               void *stream);
+size_t rbx_lskges_noalloc_scratch_bytes(void);
 
 /* ---------------------------------------------------------------- sparse data
  * rsksp3 (sparse_data/sksp.hh:277-326): B = alpha * op(A) * op(submat(S)) + beta * B

@@ -149,6 +149,9 @@ _SIG = {
     "rbx_dlskges": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _SS, _I64, _I64, _P, _I64, _D, _P, _I64, _P],
     "rbx_srskges": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _P, _I64, _SS, _I64, _I64, _F, _P, _I64, _P],
     "rbx_drskges": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _P, _I64, _SS, _I64, _I64, _D, _P, _I64, _P],
+    "rbx_slskges_noalloc": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _SS, _I64, _I64, _P, _I64, _F, _P, _I64, _P],
+    "rbx_dlskges_noalloc": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _SS, _I64, _I64, _P, _I64, _D, _P, _I64, _P],
+    "rbx_lskges_noalloc_scratch_bytes": [],
     "rbx_srsksp3": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _CSR, _I64, _I64, _DS, _I64, _I64, _F, _P,
                     _I64, _P],
     "rbx_drsksp3": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _CSR, _I64, _I64, _DS, _I64, _I64, _D, _P,
@@ -190,7 +193,8 @@ _SIG = {
     "rbx_profile_enable": [C.c_int],
     "rbx_profile_collect": [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int],
 }
-_RESTYPE = {"rbx_last_error_message": C.c_char_p, "rbx_version": C.c_char_p, "rbx_profile_collect": C.c_int}
+_RESTYPE = {"rbx_last_error_message": C.c_char_p, "rbx_version": C.c_char_p, "rbx_profile_collect": C.c_int,
+            "rbx_lskges_noalloc_scratch_bytes": C.c_size_t}
 
 _lib: Optional[C.CDLL] = None
 
@@ -609,6 +613,28 @@ def lskges(layout, opS, opA, d, n, m, alpha, S: SparseSkOp, ro_s, co_s, A, lda, 
     _need("B", B, _fp(p), _extent(d, n, layout, ldb))
     _call(f"rbx_{_prefix(B)}lskges", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
           _ptr(A), lda, beta, _ptr(B), ldb, _stream())
+
+
+def lskges_noalloc(layout, opS, opA, d, n, m, alpha, S: SparseSkOp, ro_s, co_s, A, lda, beta, B, ldb):
+    """lskges without device allocation (no reference counterpart): S must be
+    filled, op(A) unit-stride along the contraction, and the operator window
+    within the library's fixed scratch (lskges_noalloc_scratch_bytes())."""
+    p = _prefix(B)
+    if S.vals is None or S.nnz is None or S.nnz < 0:
+        raise Error("lskges_noalloc: fill the operator first (fill_sparse)")
+    if S.dtype != _fp(p):
+        raise Error(f"lskges_noalloc: the operator holds {S.dtype} values, B is {B.dtype}")
+    _need_sparse_op(S, p)
+    if alpha != 0:
+        _need("A", A, _fp(p), _extent(*_opdims(opA, m, n), layout, lda))
+    _need("B", B, _fp(p), _extent(d, n, layout, ldb))
+    _call(f"rbx_{p}lskges_noalloc", _c(layout), _c(opS), _c(opA), d, n, m, alpha, C.byref(S.c()), ro_s, co_s,
+          _ptr(A), lda, beta, _ptr(B), ldb, _stream())
+
+
+def lskges_noalloc_scratch_bytes() -> int:
+    """Size of the fixed per-GPU scratch array behind lskges_noalloc."""
+    return int(lib().rbx_lskges_noalloc_scratch_bytes())
 
 
 def rskges(layout, opA, opS, m, d, n, alpha, A, lda, S: SparseSkOp, ro_s, co_s, beta, B, ldb):

@@ -20,6 +20,8 @@
 // chunk of op(B) in LDS and lets every thread walk its rows' sorted entries.
 #include <type_traits>
 
+#include <mutex>
+
 #include "rbx_internal.hh"
 #include "rbx_spmm.hh"
 
@@ -780,7 +782,7 @@ template <typename T>
 static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n, int64_t m, T alpha, int64_t A_rows,
                           int64_t A_cols, int64_t nnz, const T *vals, const int64_t *rows, const int64_t *cols,
                           int64_t ro_a, int64_t co_a, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
-                          hipStream_t st, int kib_force = -1) {
+                          hipStream_t st, int kib_force = -1, char *fixed_ws = nullptr, size_t fixed_ws_bytes = 0) {
     check_layout(layout);
     check_op(opA);
     check_op(opB);
@@ -820,6 +822,32 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
     int64_t *row_ptr;
     int32_t *ccol;
     T *cval;
+    if (fixed_ws != nullptr) {
+        // caller-provided workspace: the counting-sort build and the LDS apply
+        // only (neither allocates)
+        if (layout_opB != 'C')
+            RBX_FAIL(RBX_UNSUPPORTED, "%s: the allocation-free path needs op(A) with unit stride along the "
+                     "contraction (column-major A with opA = 'N', or row-major A with opA = 'T')", __func__);
+        if (d > CSRB_MAX_ROWS || nnz > LDS_APPLY_MAX_MEAN_ROW * d)
+            RBX_FAIL(RBX_UNSUPPORTED, "%s: the allocation-free path takes at most %lld sketch rows and %lld stored "
+                     "entries per row on average (d = %lld, nnz = %lld)", __func__, (long long)CSRB_MAX_ROWS,
+                     (long long)LDS_APPLY_MAX_MEAN_ROW, (long long)d, (long long)nnz);
+        const size_t need_csr = csr_build_counting_bytes(nnz, d, sizeof(T));
+        if (need_csr == 0) RBX_FAIL(RBX_UNSUPPORTED, "%s: nnz = %lld is outside the allocation-free path", __func__,
+                                    (long long)nnz);
+        const size_t need_apply = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T));
+        const size_t need = need_csr + need_apply;
+        if (need > fixed_ws_bytes)
+            RBX_FAIL(RBX_INVALID_ARG, "%s: the operator window needs %zu bytes of scratch; the allocation-free "
+                     "entry point holds %zu", __func__, need, fixed_ws_bytes);
+        if (!csr_build_counting_ws<T>(rows, cols, vals, nnz, ro_a, co_a, d, m, fixed_ws, need_csr, &row_ptr, &ccol,
+                                      &cval, st))
+            RBX_FAIL(RBX_UNSUPPORTED, "%s: shape outside the allocation-free path", __func__);
+        if (!csr_lds_apply<T>(row_ptr, ccol, cval, d, n, m, nnz, alpha, beta, B, ldb, C, crs, ccs,
+                              kib ? SPF_KIB : SPF_JKI, fixed_ws + need_csr, need_apply, st))
+            RBX_FAIL(RBX_UNSUPPORTED, "%s: shape outside the allocation-free path", __func__);
+        return;
+    }
     {
         ProfScope prof("coo_to_csr", st);
         // few output rows: stable counting sort by row; otherwise a radix
@@ -1307,6 +1335,49 @@ static void fill_sparse_skop(rbx_sparse_skop *S, hipStream_t st) {
     S->nnz = fill_sparse_impl<T>(S->dist, (T *)S->vals, S->rows, S->cols, S->seed_state, nullptr, st);
 }
 
+// ---------------------------------------------------------------- allocation-free lskges
+// The allocation-free entry points keep their workspace (the operator
+// window's CSR) in this file-scope device array: one copy per GPU, resident
+// from the library's load on that GPU until the process ends, shared by every
+// call on that GPU.  Its host-side name is only a stand-in; the device address
+// of each GPU's copy comes from hipGetSymbolAddress, once per device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) char noalloc_scratch[NOALLOC_SCRATCH_BYTES];
+
+static char *noalloc_scratch_ptr() {
+    static std::mutex mu;
+    static char *ptrs[NOALLOC_MAX_DEVICES] = {};
+    int dev = 0;
+    RBX_HIP(hipGetDevice(&dev));
+    RBX_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(mu);
+    if (ptrs[dev] == nullptr) {
+        void *p = nullptr;
+        RBX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+        RBX_REQUIRE(p != nullptr);
+        ptrs[dev] = (char *)p;
+    }
+    return ptrs[dev];
+}
+
+// lskges without any device allocation: S must hold its sampled arrays, op(A)
+// must have unit stride along the contraction, and the operator window must
+// fit the counting-sort build and the LDS apply; the arithmetic is lskges's.
+template <typename T>
+static void lskges_noalloc_impl(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, T alpha,
+                                const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, const T *A, int64_t lda, T beta,
+                                T *B, int64_t ldb, hipStream_t st) {
+    RBX_REQUIRE(S != nullptr);
+    if (S->nnz < 0)
+        RBX_FAIL(RBX_UNSUPPORTED, "lskges_noalloc: the SparseSkOp must hold its sampled arrays "
+                 "(rbx_[sd]fill_sparse_skop first); sampling would need device memory");
+    char *ws = noalloc_scratch_ptr();
+    SkopView<T> v = view_or_sample<T>(S, st);
+    left_spmm_coo<T>(layout, opS, opA, d, n, m, alpha, S->dist.n_rows, S->dist.n_cols, v.nnz, v.vals, v.rows, v.cols,
+                     ro_s, co_s, A, lda, beta, B, ldb, st, -1, ws, NOALLOC_SCRATCH_BYTES);
+}
+
 }  // namespace rbx
 
 using namespace rbx;
@@ -1402,6 +1473,26 @@ rbx_status rbx_dlskges(char layout, char opS, char opA, int64_t d, int64_t n, in
     lskges_impl<double>(layout, opS, opA, d, n, m, alpha, S, ro_s, co_s, A, lda, beta, B, ldb, (hipStream_t)stream);
     RBX_API_END
 }
+
+rbx_status rbx_slskges_noalloc(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, float alpha,
+                               const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, const float *A, int64_t lda,
+                               float beta, float *B, int64_t ldb, void *stream) {
+    RBX_API_BEGIN
+    lskges_noalloc_impl<float>(layout, opS, opA, d, n, m, alpha, S, ro_s, co_s, A, lda, beta, B, ldb,
+                               (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_dlskges_noalloc(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, double alpha,
+                               const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, const double *A, int64_t lda,
+                               double beta, double *B, int64_t ldb, void *stream) {
+    RBX_API_BEGIN
+    lskges_noalloc_impl<double>(layout, opS, opA, d, n, m, alpha, S, ro_s, co_s, A, lda, beta, B, ldb,
+                                (hipStream_t)stream);
+    RBX_API_END
+}
+
+size_t rbx_lskges_noalloc_scratch_bytes(void) { return NOALLOC_SCRATCH_BYTES; }
 
 rbx_status rbx_srskges(char layout, char opA, char opS, int64_t m, int64_t d, int64_t n, float alpha, const float *A,
                        int64_t lda, const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, float beta, float *B,
