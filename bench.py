@@ -132,13 +132,17 @@ def main():
     value = flops / (ms * 1e-3) / 1e12
     log(f"bf16-split x{args.terms}: {ms:.3f} ms/step, {value:.1f} TFLOP/s; kernels {prof}")
 
+    # the main product may run as several launches per step (one per K range,
+    # each 1/launches of the flops): achieved = flops per launch / mean launch time
     cnt, tot = prof.get("bf16_skge_kernel", (1, float("nan")))
+    launches = max(cnt // max(args.steps, 1), 1)
     k_ms = tot / max(cnt, 1)
-    achieved = flops / (k_ms * 1e-3) / 1e12
+    achieved = (flops / launches) / (k_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "kernel": "bf16_skge_kernel", "achieved": round(achieved, 2),
                 "peak": PEAK_BF16, "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16, 4),
                 "executed_mfma_frac": round(achieved * args.terms / PEAK_BF16, 4),
-                "kernel_ms": round(k_ms, 4),
+                "kernel_ms": round(k_ms, 4), "launches_per_step": launches,
+                "main_ms_per_step": round(tot / max(args.steps, 1), 4),
                 "traffic": profiled_traffic("bf16_skge_kernel") if args.terms == 3 else None,
                 "per_kernel_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}}
 

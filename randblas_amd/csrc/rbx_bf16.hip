@@ -64,13 +64,20 @@ constexpr int BF_MT = 64, BF_NT = 1024, BF_KC = 32, BF_LDK = 40, BF_NTHR = 256;
 constexpr float BM_ES = 5.07e-7f;   // sin / cos factor, absolute
 constexpr float BM_ER = 3.36e-7f;   // radius factor, relative
 
-// true when RNE to bfloat16 of every value within D + 1.5 ulp of g equals RNE(g)
-__device__ __forceinline__ bool bf16_round_safe(float g, float D) {
+// true when RNE to bfloat16 of every value within D + 1.5 ulp of g equals RNE(g):
+// mid is the rounding midpoint inside g's bfloat16 interval (same sign and
+// exponent as g), so g - mid = (low16(g) - 0x8000) ulp(g) exactly (Sterbenz)
+// and |g - mid| - 2 ulp(g) is exact too; the nearest other midpoint is a
+// whole bfloat16 ulp further.  A zero or subnormal g (never a fast value worth
+// keeping) fails: |g - mid| is then below any D the generator forms.
+// The test is margin > 0 with margin = (|g - mid| - 2 ulp(g)) - D: rounding
+// keeps the sign of that difference, so the minimum of several margins decides
+// for all of them with one comparison.
+__device__ __forceinline__ float bf16_round_margin(float g, float D) {
     const uint32_t b = __float_as_uint(g);
-    const uint32_t e = b & 0x7f800000u;
-    const float ulp = __uint_as_float(e) * 0x1p-23f;
-    const float dist = (float)abs((int)(b & 0xffffu) - 0x8000);
-    return ((int)(e != 0u) & (int)(e != 0x7f800000u) & (int)((dist - 2.0f) * ulp > D)) != 0;
+    const float mid = __uint_as_float((b & 0xffff0000u) | 0x8000u);
+    const float ulp2 = __uint_as_float(b & 0x7f800000u) * 0x1p-22f;  // 2 ulp(g)
+    return (fabsf(g - mid) - ulp2) - D;
 }
 
 // Fast bf16 pair of one Box-Muller draw on words (w0, w1), packed low =
@@ -79,39 +86,47 @@ __device__ __forceinline__ bool bf16_round_safe(float g, float D) {
 // The hardware factors of a Box-Muller draw, shared by the generator and by
 // rbx_bf16_fastgen_bounds (which measures them against the portable ones).
 // sin / cos of pi * uneg11(w0): v_sin_f32 / v_cos_f32 take revolutions.
+// x = uneg11(w0) / 2 in one fma (the product by a power of two is exact, so
+// this equals the portable mul-then-add halved).
 __device__ __forceinline__ void fast_sincos_factor(uint32_t w0, float &sf, float &cf) {
-    const float x = uneg11(w0) * 0.5f;  // revolutions: sin(2 pi x) = sin(pi u0)
+    const float x = __fmaf_rn((float)(int32_t)w0, 0x1p-32f, 0x1p-33f);  // revolutions: sin(2 pi x) = sin(pi u0)
     sf = __builtin_amdgcn_sinf(x);
     cf = __builtin_amdgcn_cosf(x);
 }
-// sqrt(-2 log u01(w1)) with v_log_f32 (log2) scaled by -2 ln 2.
+// sqrt(-2 log u01(w1)) with v_log_f32 (log2) scaled by -2 ln 2; u01 in one
+// fma (equal to the portable u01, the product being exact).
 __device__ __forceinline__ float fast_radius_factor(uint32_t w1) {
-    return __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u01(w1)) * -0x1.62e430p+0f);
+    const float u = __fmaf_rn((float)w1, 0x1p-32f, 0x1p-33f);
+    return __builtin_amdgcn_sqrtf(__builtin_amdgcn_logf(u) * -0x1.62e430p+0f);
 }
 
-__device__ __forceinline__ uint32_t boxmuller_bf16_fast(uint32_t w0, uint32_t w1, bool &ok) {
+__device__ __forceinline__ uint32_t boxmuller_bf16_fast(uint32_t w0, uint32_t w1, float &margin) {
     float sf, cf;
     fast_sincos_factor(w0, sf, cf);
     const float rf = fast_radius_factor(w1);
     const float g0 = sf * rf, g1 = cf * rf;
-    const float k = rf * (1.0f + 0x1p-18f);
-    // both checks evaluated (no short-circuit branch)
-    const int safe0 = bf16_round_safe(g0, k * (fabsf(sf) * BM_ER + BM_ES));
-    const int safe1 = bf16_round_safe(g1, k * (fabsf(cf) * BM_ER + BM_ES));
-    ok = ((int)ok & safe0 & safe1) != 0;
-    return (uint32_t)bf16_rne(g0) | ((uint32_t)bf16_rne(g1) << 16);
+    // D = r' (|s'| BM_ER + BM_ES) (1 + 2^-18): the margin covers the
+    // roundings of these few operations
+    const float kr = rf * (BM_ER * (1.0f + 0x1p-18f)), ks = rf * (BM_ES * (1.0f + 0x1p-18f));
+    margin = fminf(bf16_round_margin(g0, __fmaf_rn(kr, fabsf(sf), ks)),
+                   bf16_round_margin(g1, __fmaf_rn(kr, fabsf(cf), ks)));
+    // both values rounded by one hardware RNE conversion (v_cvt_pk_bf16_f32)
+    const f32x2 gg = {g0, g1};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(gg, bf16x2));
 }
 
 // One Philox counter -> four operator entries as bf16 (two per 32-bit word).
 // Branch-free part: the four entries of words w, and (Gaussian) whether each
 // pair's fast values are provably the exact path's.
 template <bool GAUSS>
-__device__ __forceinline__ uint2 gen4_bf16_fast(const uint4 &w, bool &ok0, bool &ok1) {
+__device__ __forceinline__ uint2 gen4_bf16_fast(const uint4 &w, bool &ok) {
     uint2 pk;
-    ok0 = ok1 = true;
+    ok = true;
     if (GAUSS) {
-        pk.x = boxmuller_bf16_fast(w.x, w.y, ok0);
-        pk.y = boxmuller_bf16_fast(w.z, w.w, ok1);
+        float m0, m1;
+        pk.x = boxmuller_bf16_fast(w.x, w.y, m0);
+        pk.y = boxmuller_bf16_fast(w.z, w.w, m1);
+        ok = fminf(m0, m1) > 0.0f;
     } else {
         pk.x = (uint32_t)entry_value<uint16_t>(uneg11(w.x), false) |
                ((uint32_t)entry_value<uint16_t>(uneg11(w.y), false) << 16);
@@ -128,9 +143,9 @@ __device__ __forceinline__ uint2 gen4_bf16_fast(const uint4 &w, bool &ok0, bool 
 template <bool GAUSS>
 __device__ __forceinline__ uint2 gen4_bf16(Ctr128 c, uint32_t k0, uint32_t k1) {
     const uint4 w = philox4x32_10(c, k0, k1);
-    bool ok0, ok1;
-    uint2 pk = gen4_bf16_fast<GAUSS>(w, ok0, ok1);
-    if (GAUSS && !(ok0 && ok1)) {
+    bool ok;
+    uint2 pk = gen4_bf16_fast<GAUSS>(w, ok);
+    if (GAUSS && !ok) {
         __builtin_amdgcn_sched_barrier(0);
         const float2 a = boxmuller(w.x, w.y);
         __builtin_amdgcn_sched_barrier(0);
@@ -193,12 +208,13 @@ __global__ __launch_bounds__(256) void split_kernel(const float *A, int64_t rs, 
 constexpr int SPL_KT = 256, SPL_LD = 260;
 template <bool VEC>
 __global__ __launch_bounds__(256) void split_kunit_kernel(const float *A, int64_t cs, int64_t K, int64_t N,
-                                                          int64_t Kp, int terms, uint16_t *planes) {
+                                                          int64_t Kp, int terms, uint16_t *planes,
+                                                          int64_t kt_base) {
     __shared__ __attribute__((aligned(16))) float tile[32][SPL_LD];
     const int tid = threadIdx.x;
     const int64_t KS = Kp / 16;
     const int64_t cb = blockIdx.y;
-    const int64_t kt0 = (int64_t)blockIdx.x * SPL_KT;
+    const int64_t kt0 = ((int64_t)blockIdx.x + kt_base) * SPL_KT;  // this launch's K tiles start at kt_base
     // load: a wave reads 1 KB runs of one column, k in [kt0, kt0 + 256); all
     // eight of a thread's loads are issued before the first LDS store.
     // VEC: 16-byte loads (A and the column stride are 4-float aligned)
@@ -257,8 +273,9 @@ struct Bf16Problem {
     const uint16_t *planes;  // fragment-packed terms (see split_kernel)
     float *part;  // nS x M x N, row-major
     int64_t kw;   // K per split (multiple of BF_KC)
-    int nMB, nNB, nS;
+    int nMB, nNB, nS;  // row blocks, column blocks and K-splits of the launch
     int mb0;      // first row block of the launch
+    int sp0;      // first K-split of the launch
 };
 
 // 32x32 output block, element r of lane l: row 8*(r/4) + 4*(l/32) + r%4, column l%32.
@@ -439,7 +456,7 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
     const int mb = t % pb.nMB;
     const int rest = t / pb.nMB;
     const int nb = rest % pb.nNB;
-    const int sp = rest / pb.nNB;
+    const int sp = rest / pb.nNB + pb.sp0;
     const int64_t m0 = (int64_t)(mb + pb.mb0) * MT;
     const int64_t n0 = (int64_t)nb * NT;
     const int64_t kbeg = (int64_t)sp * pb.kw;
@@ -715,57 +732,97 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     pb.planes = (const uint16_t *)ws;
     pb.part = (float *)(ws + planes_bytes);
 
-    if (ars == 1) {
-        // unit-stride K: coalesced column reads, transpose through LDS
-        const int64_t kt = (pb.Kp + SPL_KT - 1) / SPL_KT;
-        RBX_REQUIRE(kt < (1ll << 31) && pb.Np / 32 <= 65535);
-        ProfScope prof("bf16_split_kernel", stream);
-        const dim3 grid((unsigned)kt, (unsigned)(pb.Np / 32));
-        if (acs % 4 == 0 && ((uintptr_t)A & 15) == 0)
-            hipLaunchKernelGGL(split_kunit_kernel<true>, grid, dim3(256), 0, stream, A, acs, K, N, pb.Kp, n_terms,
-                               (uint16_t *)pb.planes);
-        else
-            hipLaunchKernelGGL(split_kunit_kernel<false>, grid, dim3(256), 0, stream, A, acs, K, N, pb.Kp, n_terms,
-                               (uint16_t *)pb.planes);
-        RBX_HIP(hipGetLastError());
-    } else {
-        int64_t total = (pb.Np / 32) * (pb.Kp / 16) * 64;  // one wave per (block, K step)
-        int64_t blocks = (total + 255) / 256;
-        if (blocks > 65536) blocks = 65536;
-        ProfScope prof("bf16_split_kernel", stream);
-        hipLaunchKernelGGL(split_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, A, ars, acs, K, N, pb.Kp, pb.Np,
-                           n_terms, (uint16_t *)pb.planes);
-        RBX_HIP(hipGetLastError());
-    }
-    {
-        ProfScope prof("bf16_skge_kernel", stream);
-        // interior row blocks on the two-wave kernel when the shape allows
-        // it, the rest (or everything) on the bounds-checked kernel
-        // the widest tile (256 CB columns, CB = BF_CB, BF_CB / 2, BF_CB / 4)
-        // that divides N: narrow sketches keep the two-wave kernel
-        constexpr int TMT = 32 * BF_RB;
-        int cb = 0;
-        for (int c = BF_CB; c >= 1; c /= 2)
-            if (N % (256 * c) == 0) {
-                cb = c;
-                break;
-            }
-        const bool fast_ok = K % PKC == 0 && cb > 0 && !pb.g.b_along_rows && (pb.g.b_off & 3) == 0;
-        const int nfast = fast_ok ? (int)(M / TMT) : 0;   // row blocks of TMT rows
-        const int nMB_all = pb.nMB;                        // row blocks of BF_MT rows
-        const int nfast64 = nfast * (TMT / BF_MT);
-        if (nfast > 0) {
-            Bf16Problem q = pb;
-            q.nMB = nfast;
-            q.nNB = (int)(N / (256 * cb));
-            q.mb0 = 0;
-            dim3 grid((unsigned)(q.nMB * q.nNB * q.nS));
-            const bool gs = q.g.gauss != 0;
-            if (cb == BF_CB) launch_pair<BF_CB>(n_terms, gs, grid, q, stream);
-            else if (cb == BF_CB / 2) launch_pair<(BF_CB / 2 > 0 ? BF_CB / 2 : 1)>(n_terms, gs, grid, q, stream);
-            else launch_pair<(BF_CB / 4 > 0 ? BF_CB / 4 : 1)>(n_terms, gs, grid, q, stream);
+    // interior row blocks on the two-wave kernel when the shape allows it, the
+    // rest (or everything) on the bounds-checked kernel; the widest tile (256
+    // CB columns, CB = BF_CB, BF_CB / 2, BF_CB / 4) that divides N: narrow
+    // sketches keep the two-wave kernel
+    constexpr int TMT = 32 * BF_RB;
+    int cb = 0;
+    for (int c = BF_CB; c >= 1; c /= 2)
+        if (N % (256 * c) == 0) {
+            cb = c;
+            break;
         }
+    const bool fast_ok = K % PKC == 0 && cb > 0 && !pb.g.b_along_rows && (pb.g.b_off & 3) == 0;
+    const int nfast = fast_ok ? (int)(M / TMT) : 0;   // row blocks of TMT rows
+    const int nMB_all = pb.nMB;                        // row blocks of BF_MT rows
+    const int nfast64 = nfast * (TMT / BF_MT);
+    const bool vec = acs % 4 == 0 && ((uintptr_t)A & 15) == 0;
+    const int64_t kt_total = (pb.Kp + SPL_KT - 1) / SPL_KT;
+    RBX_REQUIRE(kt_total < (1ll << 31) && pb.Np / 32 <= 65535);
+    // unit-stride K: coalesced column reads, transposed through LDS; K tiles
+    // [kt_first, kt_first + kt_count) on stream s
+    auto split_kunit = [&](int64_t kt_first, int64_t kt_count, hipStream_t s) {
+        ProfScope prof("bf16_split_kernel", s);
+        const dim3 grid((unsigned)kt_count, (unsigned)(pb.Np / 32));
+        if (vec)
+            hipLaunchKernelGGL(split_kunit_kernel<true>, grid, dim3(256), 0, s, A, acs, K, N, pb.Kp, n_terms,
+                               (uint16_t *)pb.planes, kt_first);
+        else
+            hipLaunchKernelGGL(split_kunit_kernel<false>, grid, dim3(256), 0, s, A, acs, K, N, pb.Kp, n_terms,
+                               (uint16_t *)pb.planes, kt_first);
+        RBX_HIP(hipGetLastError());
+    };
+    // the two-wave kernel over K-splits [sp_first, sp_first + sp_count)
+    auto main_fast = [&](int sp_first, int sp_count) {
+        ProfScope prof("bf16_skge_kernel", stream);
+        Bf16Problem q = pb;
+        q.nMB = nfast;
+        q.nNB = (int)(N / (256 * cb));
+        q.mb0 = 0;
+        q.sp0 = sp_first;
+        q.nS = sp_count;
+        const dim3 grid((unsigned)(q.nMB * q.nNB * q.nS));
+        const bool gs = q.g.gauss != 0;
+        if (cb == BF_CB) launch_pair<BF_CB>(n_terms, gs, grid, q, stream);
+        else if (cb == BF_CB / 2) launch_pair<(BF_CB / 2 > 0 ? BF_CB / 2 : 1)>(n_terms, gs, grid, q, stream);
+        else launch_pair<(BF_CB / 4 > 0 ? BF_CB / 4 : 1)>(n_terms, gs, grid, q, stream);
+    };
+    // K-range pipeline: when every row block runs on the two-wave kernel, the
+    // split of later K ranges runs on the workspace's helper stream while the
+    // main kernel multiplies earlier ones (the split's lean waves fit beside
+    // the main kernel's on each CU); the caller's stream waits for each range's
+    // split before that range's product, so all helper work is joined before
+    // the reduction.
+    const int64_t tiles_per_split = pb.kw / SPL_KT;
+    const bool pipelined = ars == 1 && nfast > 0 && nMB_all == nfast64 && pb.nS >= 8 && pb.kw % SPL_KT == 0;
+    if (pipelined) {
+        // four equal K ranges: only the first range's split runs alone (at the
+        // headline shape 7.34 ms per step against 7.62 without the pipeline;
+        // an eighth then seven eighths measured 7.38 ms)
+        constexpr int NCH = 4;
+        int sb[NCH + 1];
+        for (int c = 0; c <= NCH; ++c) sb[c] = (int)((int64_t)pb.nS * c / NCH);
+        auto kt_of = [&](int sp) { return (int64_t)sp * tiles_per_split < kt_total ? (int64_t)sp * tiles_per_split
+                                                                                   : kt_total; };
+        hipStream_t hs = ws_aux_stream(stream);
+        split_kunit(kt_of(sb[0]), kt_of(sb[1]) - kt_of(sb[0]), stream);
+        hipEvent_t fork = ws_event(stream, 0);
+        RBX_HIP(hipEventRecord(fork, stream));
+        RBX_HIP(hipStreamWaitEvent(hs, fork, 0));
+        for (int c = 1; c < NCH; ++c) {
+            split_kunit(kt_of(sb[c]), kt_of(sb[c + 1]) - kt_of(sb[c]), hs);
+            RBX_HIP(hipEventRecord(ws_event(stream, c), hs));
+        }
+        for (int c = 0; c < NCH; ++c) {
+            if (c > 0) RBX_HIP(hipStreamWaitEvent(stream, ws_event(stream, c), 0));
+            main_fast(sb[c], sb[c + 1] - sb[c]);
+        }
+    } else {
+        if (ars == 1) {
+            split_kunit(0, kt_total, stream);
+        } else {
+            int64_t total = (pb.Np / 32) * (pb.Kp / 16) * 64;  // one wave per (block, K step)
+            int64_t blocks = (total + 255) / 256;
+            if (blocks > 65536) blocks = 65536;
+            ProfScope prof("bf16_split_kernel", stream);
+            hipLaunchKernelGGL(split_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, A, ars, acs, K, N, pb.Kp,
+                               pb.Np, n_terms, (uint16_t *)pb.planes);
+            RBX_HIP(hipGetLastError());
+        }
+        if (nfast > 0) main_fast(0, pb.nS);
         if (nMB_all > nfast64) {
+            ProfScope prof("bf16_skge_kernel", stream);
             Bf16Problem q = pb;
             q.nMB = nMB_all - nfast64;
             q.mb0 = nfast64;

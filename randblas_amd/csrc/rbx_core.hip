@@ -52,6 +52,8 @@ struct StreamWs {
     size_t size[SCRATCH_NSLOTS] = {};
     void *blas = nullptr;
     void (*blas_destroy)(void *) = nullptr;
+    hipStream_t aux = nullptr;                  // helper stream for work that overlaps this stream's
+    hipEvent_t ev[WS_EVENTS] = {};              // fork / join events between the two
 };
 
 static std::mutex g_ws_mutex;  // guards the map, not the workspaces
@@ -143,6 +145,19 @@ void *&stream_blas_handle(hipStream_t stream, void (**destroy)(void *)) {
 }
 void set_stream_blas_destroy(hipStream_t stream, void (*destroy)(void *)) { stream_ws(stream).blas_destroy = destroy; }
 
+hipStream_t ws_aux_stream(hipStream_t stream) {
+    StreamWs &w = stream_ws(stream);
+    if (w.aux == nullptr) RBX_HIP(hipStreamCreateWithFlags(&w.aux, hipStreamNonBlocking));
+    return w.aux;
+}
+
+hipEvent_t ws_event(hipStream_t stream, int i) {
+    StreamWs &w = stream_ws(stream);
+    if (i < 0 || i >= WS_EVENTS) RBX_FAIL(RBX_INVALID_ARG, "ws_event: index %d out of range", i);
+    if (w.ev[i] == nullptr) RBX_HIP(hipEventCreateWithFlags(&w.ev[i], hipEventDisableTiming));
+    return w.ev[i];
+}
+
 static void release_ws(StreamWs &w) {
     for (int s = 0; s < SCRATCH_NSLOTS; ++s) {
         void *p = w.ptr[s];
@@ -152,6 +167,18 @@ static void release_ws(StreamWs &w) {
     }
     if (w.blas != nullptr && w.blas_destroy != nullptr) w.blas_destroy(w.blas);
     w.blas = nullptr;
+    // the helper stream's work is always joined into `stream` before the call
+    // that issued it returns, so it is finished once `stream` reaches here
+    if (w.aux != nullptr) {
+        RBX_HIP(hipStreamSynchronize(w.aux));
+        RBX_HIP(hipStreamDestroy(w.aux));
+        w.aux = nullptr;
+    }
+    for (int i = 0; i < WS_EVENTS; ++i)
+        if (w.ev[i] != nullptr) {
+            (void)hipEventDestroy(w.ev[i]);
+            w.ev[i] = nullptr;
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -420,6 +447,15 @@ rbx_status rbx_release_all_workspaces(void) {
         }
         if (w->blas != nullptr && w->blas_destroy != nullptr) w->blas_destroy(w->blas);
         w->blas = nullptr;
+        if (w->aux != nullptr) {
+            RBX_HIP(hipStreamDestroy(w->aux));
+            w->aux = nullptr;
+        }
+        for (int i = 0; i < WS_EVENTS; ++i)
+            if (w->ev[i] != nullptr) {
+                (void)hipEventDestroy(w->ev[i]);
+                w->ev[i] = nullptr;
+            }
     }
     RBX_HIP(hipSetDevice(cur));
     RBX_API_END

@@ -166,6 +166,13 @@ enum ScratchSlot { SCRATCH_BF16 = 0, SCRATCH_SPARSE_OP = 1, SCRATCH_SPARSE_CSR =
                    SCRATCH_SMALL = 4, SCRATCH_SPARSE_G = 5, SCRATCH_SPLITK = 6, SCRATCH_SOLVE = 7,
                    SCRATCH_SOLVE2 = 8, SCRATCH_LONG_ROWS = 9, SCRATCH_NSLOTS = 10 };
 void *scratch(int slot, size_t bytes, hipStream_t stream);
+// A helper stream owned by `stream`'s workspace (non-blocking, default
+// priority) and up to WS_EVENTS events, for calls that overlap part of their
+// own work with the rest; such a call joins all helper work back into
+// `stream` before it returns.  Released with the workspace.
+constexpr int WS_EVENTS = 8;
+hipStream_t ws_aux_stream(hipStream_t stream);
+hipEvent_t ws_event(hipStream_t stream, int i);
 // The rocBLAS handle slot of the stream's workspace (created by its user)
 // and its destroy function, called by rbx_release_workspace.
 void *&stream_blas_handle(hipStream_t stream, void (**destroy)(void *));
