@@ -879,6 +879,191 @@ static size_t align256b(size_t x) { return (x + 255) / 256 * 256; }
 
 __device__ __forceinline__ int fw_pos(int k) { return (k & ~15) + 8 * (k & 1) + ((k & 15) >> 1); }
 
+// Tail columns of the wide kernels: up to WIDE_TAIL output columns past the
+// last whole tile (the right-hand side of [A b], a ragged edge) are formed by
+// the threads that generate the operator, from the four entries each holds
+// in registers per chunk, with VALU FMAs: the operator is generated once for
+// the tiles and the tail together.  The generator of operator rows along K
+// holds one row's four consecutive k (one accumulator per column); the one of
+// counters along the rows holds four rows at one k (four per column).  The
+// dense values of chunk c + 1 are loaded right after chunk c's are used, into
+// the same registers, so they have a whole chunk's MFMAs to arrive (a copy
+// from a second buffer would wait for the load at once).  TW
+// columns are always loaded and accumulated (TW = 1, or 4 with the unused
+// ones repeating the last real column), so the chunk loop has no branch: a
+// load under a branch would make the compiler's memory-counter waits for
+// the MFMA operands conservative.
+constexpr int WIDE_TAIL = 4;
+// The tail's columns: dense column j's element k at col[j][k * rs[j]], its
+// output's element i at out[j][i * ors[j]]; entries nt .. WIDE_TAIL - 1 of
+// col / rs repeat entry nt - 1.
+template <typename T>
+struct TailCols {
+    const T *col[WIDE_TAIL];
+    int64_t rs[WIDE_TAIL];
+    T *out[WIDE_TAIL];
+    int64_t ors[WIDE_TAIL];
+    int nt;
+};
+template <typename T, bool ROWS, int TW>
+struct WideTail {
+    static constexpr int R = ROWS ? 4 : 1;  // output rows per thread
+    static constexpr int E = ROWS ? 1 : 4;  // dense values per column per chunk
+    T acc[TW][R];
+    T cur[TW][E];  // the dense values of the chunk being generated
+
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int j = 0; j < TW; ++j)
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[j][r] = (T)0;
+    }
+    // cur = dense values of the thread's k (k .. k + E - 1), every tail column
+    __device__ __forceinline__ void load(const TailCols<T> &tc, int64_t k) {
+#pragma unroll
+        for (int j = 0; j < TW; ++j)
+#pragma unroll
+            for (int e = 0; e < E; ++e) cur[j][e] = tc.col[j][(k + e) * tc.rs[j]];
+    }
+    // acc += the chunk's operator entries (ent[0..3]) times cur
+    __device__ __forceinline__ void step(const T (&ent)[4]) {
+#pragma unroll
+        for (int j = 0; j < TW; ++j) {
+            if constexpr (ROWS) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[j][r] = fma(ent[r], cur[j][0], acc[j][r]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[j][0] = fma(ent[e], cur[j][e], acc[j][0]);
+            }
+        }
+    }
+    // The workgroup's 64 x nt partial into out[j * M + row], out = the
+    // split's tail partials shifted to the block's first row.  Rows along K:
+    // the eight threads of a row are adjacent lanes (butterfly).  Counters
+    // along the rows: lanes l, l ^ 16, l ^ 32, l ^ 48 of a wave share rows
+    // (butterfly), then the eight waves are summed in order through `red`
+    // (8 * TW * 64 values of LDS free for reuse).  Every thread of the
+    // workgroup calls this (it has a barrier).
+    __device__ __forceinline__ void finish(int nt, int tid, int frow, int fcell, T *red, T *out, int64_t M) {
+        const int lane = tid & 63, wave = tid >> 6;
+        if constexpr (!ROWS) {
+#pragma unroll
+            for (int j = 0; j < TW; ++j) {
+                T v = acc[j][0];
+                v += __shfl_xor(v, 1);
+                v += __shfl_xor(v, 2);
+                v += __shfl_xor(v, 4);
+                if (j < nt && fcell == 0) out[j * M + frow] = v;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < TW; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    T v = acc[j][r];
+                    v += __shfl_xor(v, 16);
+                    v += __shfl_xor(v, 32);
+                    if (lane < 16) red[(wave * TW + j) * 64 + frow + r] = v;
+                }
+            __syncthreads();
+            if (tid < 64 * nt) {
+                const int j = tid >> 6, row = tid & 63;
+                T v = red[j * 64 + row];
+#pragma unroll
+                for (int w = 1; w < BF_FTHR / 64; ++w) v += red[(w * TW + j) * 64 + row];
+                out[j * M + row] = v;
+            }
+        }
+    }
+};
+
+// The tail's outputs: out[j][i * ors[j]] = alpha * (sum over splits of
+// tpart[(s * nt + j) * M + i]) + beta * out, FP64 sums.  G threads per output
+// entry: G = 1 sums the splits in order; G = 64 (many splits) has lane l sum
+// splits l, l + 64, ..., then a butterfly (a fixed order).
+template <typename T, int G>
+__global__ __launch_bounds__(256) void wide_tail_reduce_kernel(const T *tpart, int nS, int64_t M, TailCols<T> tc,
+                                                               double alpha, double beta) {
+    const int nt = tc.nt;
+    const int g = threadIdx.x % G;
+    const int64_t e = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+    if (e >= M * nt) return;  // uniform per group
+    const int64_t i = e % M;
+    const int j = (int)(e / M);
+    double v = 0.0;
+    for (int s = g; s < nS; s += G) v += (double)tpart[((int64_t)s * nt + j) * M + i];
+    if constexpr (G == 64) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        if (g != 0) return;
+    }
+    T *c = nullptr;
+    int64_t rs = 0;
+#pragma unroll
+    for (int q = 0; q < WIDE_TAIL; ++q)
+        if (q == j) {
+            c = tc.out[q];
+            rs = tc.ors[q];
+        }
+    c += i * rs;
+    double out = alpha * v;
+    if (beta != 0.0) out += beta * (double)(*c);
+    *c = (T)out;
+}
+
+template <typename T>
+static void launch_tail_reduce(const T *tpart, int nS, int64_t M, const TailCols<T> &tc, double alpha, double beta,
+                               hipStream_t st) {
+    const int64_t n = M * tc.nt;
+    if (nS <= 16)
+        hipLaunchKernelGGL((wide_tail_reduce_kernel<T, 1>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tpart,
+                           nS, M, tc, alpha, beta);
+    else
+        hipLaunchKernelGGL((wide_tail_reduce_kernel<T, 64>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, tpart,
+                           nS, M, tc, alpha, beta);
+}
+
+
+// How many of the columns past the whole tiles, [Nf, N), the wide kernel
+// forms in its tail, the last nt of them (0: none; the others go to the
+// caller as a rest problem), or -1 when the problem cannot run on the wide
+// kernel.  The widest tiles (cb = 4) have no registers to spare for a tail.
+// A problem whose last column lives apart (xD / xC) needs that column in the
+// tail and whole 64-row blocks (a rest problem cannot carry it).
+static int wide_tail_cols(const GemmProblem &pb, int64_t Mf, int64_t Nf, int cb) {
+    const int64_t rem = pb.N - Nf;
+    if (pb.xD != nullptr || pb.xC != nullptr) {
+        if (pb.M != Mf || cb == 4 || rem < 1) return -1;
+        return rem <= WIDE_TAIL ? (int)rem : 1;
+    }
+    return (rem >= 1 && rem <= WIDE_TAIL && cb < 4) ? (int)rem : 0;
+}
+
+// The tail's columns N - nt .. N - 1: dense column c at D[k * drs + c * dcs],
+// output at C[i * crs + c * ccs], the last one at xD / xC when given.
+template <typename T>
+static TailCols<T> wide_tail_io(const GemmProblem &pb, int nt) {
+    TailCols<T> tc{};
+    tc.nt = nt;
+    for (int j = 0; j < WIDE_TAIL; ++j) {
+        if (j >= nt) {  // unused slots repeat the last column (loaded, never written)
+            tc.col[j] = tc.col[nt - 1];
+            tc.rs[j] = tc.rs[nt - 1];
+            tc.out[j] = nullptr;
+            tc.ors[j] = 0;
+            continue;
+        }
+        const int64_t c = pb.N - nt + j;
+        const bool last = c == pb.N - 1;
+        tc.col[j] = (last && pb.xD) ? (const T *)pb.xD : (const T *)pb.D + c * pb.dcs;
+        tc.rs[j] = (last && pb.xD) ? pb.xdrs : pb.drs;
+        tc.out[j] = (last && pb.xC) ? (T *)pb.xC : (T *)pb.C + c * pb.ccs;
+        tc.ors[j] = (last && pb.xC) ? pb.xcrs : pb.crs;
+    }
+    return tc;
+}
+
 // Unit-stride K: one workgroup per (32-column block, 256-deep K range), the
 // tile staged through LDS (as split_kunit_kernel).
 template <bool VEC>
@@ -943,9 +1128,13 @@ struct F32WideProblem {
     float *part;   // nS x M x N, row-major
     int64_t kw;    // K per split (multiple of 32)
     int nMB, nNB, nS;
+    // tail columns (TAIL instantiations), partials at
+    // tpart[(split * tc.nt + j) * M + row]
+    TailCols<float> tc;
+    float *tpart;
 };
 
-template <bool GAUSS, bool ROWS, int CB>
+template <bool GAUSS, bool ROWS, int CB, int TW>
 __global__ __launch_bounds__(BF_FTHR, 1) void f32_wide_kernel(F32WideProblem pb) {
     constexpr int RB = 2, MT = 64, NT = 256 * CB;  // CB 32-column blocks per wave
     __shared__ __attribute__((aligned(16))) float Ss[2][MT * FW_LDK];
@@ -985,22 +1174,36 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f32_wide_kernel(F32WideProblem pb)
     const uint64_t foff = ROWS ? (uint64_t)((g.a_off + kbeg + fcell) * g.P + ((g.b_off + m0) >> 2) + (tid & 15))
                                : (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
     const uint64_t fstep = ROWS ? 32 * (uint64_t)g.P : 8;  // counters per chunk
+    // tail columns: only the first column block's workgroups form them
+    // (cb < 4: the output has one column block, nb == 0)
+    WideTail<float, ROWS, TW == 0 ? 1 : TW> tl;
+    const int64_t tk = kbeg + (ROWS ? fcell : 4 * fcell);  // the thread's first k of chunk 0
+    if constexpr (TW > 0) {
+        tl.zero();
+        tl.load(pb.tc, tk);
+    }
     auto gen_chunk = [&](int c, float *lds) {
         const float4 v = gen4(GAUSS, ctr_add(g.ctr, foff + fstep * (uint64_t)c), g.k0, g.k1);
+        const float ent[4] = {entry_value<float>(v.x, GAUSS), entry_value<float>(v.y, GAUSS),
+                              entry_value<float>(v.z, GAUSS), entry_value<float>(v.w, GAUSS)};
         if constexpr (ROWS) {
             const int pos = fw_pos(fcell);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) lds[(frow + e) * FW_LDK + pos] = entry_value<float>(f4_get(v, e), GAUSS);
+            for (int e = 0; e < 4; ++e) lds[(frow + e) * FW_LDK + pos] = ent[e];
         } else {
             float *row = lds + frow * FW_LDK;
             const int k0 = 4 * fcell;  // even: k0, k0 + 2 share the even half; k0 + 1, k0 + 3 the odd half
             float2 ev, od;
-            ev.x = entry_value<float>(v.x, GAUSS);
-            od.x = entry_value<float>(v.y, GAUSS);
-            ev.y = entry_value<float>(v.z, GAUSS);
-            od.y = entry_value<float>(v.w, GAUSS);
+            ev.x = ent[0];
+            od.x = ent[1];
+            ev.y = ent[2];
+            od.y = ent[3];
             *(float2 *)(row + fw_pos(k0)) = ev;
             *(float2 *)(row + fw_pos(k0 + 1)) = od;
+        }
+        if constexpr (TW > 0) {
+            tl.step(ent);
+            tl.load(pb.tc, tk + 32 * (int64_t)(c + 1 < nchunks ? c + 1 : c));  // the next chunk's
         }
     };
 
@@ -1049,17 +1252,45 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f32_wide_kernel(F32WideProblem pb)
     const bool late = (wave & 4) != 0;
     gen_chunk(0, Ss[0]);
     __syncthreads();
-    for (int c = 0; c < nchunks; ++c) {
-        const float *S = Ss[c & 1];
-        float *Sn = Ss[(c + 1) & 1];
-        const int64_t s0 = s_first + 2 * c;
-        const bool more = c + 1 < nchunks;
-        if (more && !late) gen_chunk(c + 1, Sn);
-        step(S, 0, s0 + 1);
-        step(S, 1, s0 + 2 > s_last ? s_last : s0 + 2);
-        if (more && late) gen_chunk(c + 1, Sn);
+    if constexpr (TW == 0) {
+        for (int c = 0; c < nchunks; ++c) {
+            const float *S = Ss[c & 1];
+            float *Sn = Ss[(c + 1) & 1];
+            const int64_t s0 = s_first + 2 * c;
+            const bool more = c + 1 < nchunks;
+            if (more && !late) gen_chunk(c + 1, Sn);
+            step(S, 0, s0 + 1);
+            step(S, 1, s0 + 2 > s_last ? s_last : s0 + 2);
+            if (more && late) gen_chunk(c + 1, Sn);
+            __syncthreads();
+        }
+    } else {
+        // with a tail: one loop per wave role and the last chunk peeled, so
+        // no branch surrounds the tail's loads inside a loop (the compiler's
+        // memory-counter waits for the MFMA operands stay exact)
+        const int last = nchunks - 1;
+        if (!late) {
+            for (int c = 0; c < last; ++c) {
+                gen_chunk(c + 1, Ss[(c + 1) & 1]);
+                step(Ss[c & 1], 0, s_first + 2 * c + 1);
+                step(Ss[c & 1], 1, s_first + 2 * c + 2);
+                __syncthreads();
+            }
+        } else {
+            for (int c = 0; c < last; ++c) {
+                step(Ss[c & 1], 0, s_first + 2 * c + 1);
+                step(Ss[c & 1], 1, s_first + 2 * c + 2);
+                gen_chunk(c + 1, Ss[(c + 1) & 1]);
+                __syncthreads();
+            }
+        }
+        step(Ss[last & 1], 0, s_first + 2 * last + 1);
+        step(Ss[last & 1], 1, s_last);
         __syncthreads();
     }
+
+    if constexpr (TW > 0)
+        if (nb == 0) tl.finish(pb.tc.nt, tid, frow, fcell, &Ss[0][0], pb.tpart + (int64_t)sp * pb.tc.nt * pb.M + m0, pb.M);
 
     float *P = pb.part + (int64_t)sp * pb.M * pb.N;
 #pragma unroll
@@ -1075,12 +1306,19 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f32_wide_kernel(F32WideProblem pb)
 }
 
 // Launches a wide-tile FP32 kernel instantiation by (family, orientation).
-template <int CB>
+template <int CB, int TW>
 static void launch_f32_wide(bool gauss, bool rows, dim3 grid, const F32WideProblem &w, hipStream_t st) {
-    if (gauss && rows) hipLaunchKernelGGL((f32_wide_kernel<true, true, CB>), grid, dim3(BF_FTHR), 0, st, w);
-    else if (gauss) hipLaunchKernelGGL((f32_wide_kernel<true, false, CB>), grid, dim3(BF_FTHR), 0, st, w);
-    else if (rows) hipLaunchKernelGGL((f32_wide_kernel<false, true, CB>), grid, dim3(BF_FTHR), 0, st, w);
-    else hipLaunchKernelGGL((f32_wide_kernel<false, false, CB>), grid, dim3(BF_FTHR), 0, st, w);
+    if (gauss && rows) hipLaunchKernelGGL((f32_wide_kernel<true, true, CB, TW>), grid, dim3(BF_FTHR), 0, st, w);
+    else if (gauss) hipLaunchKernelGGL((f32_wide_kernel<true, false, CB, TW>), grid, dim3(BF_FTHR), 0, st, w);
+    else if (rows) hipLaunchKernelGGL((f32_wide_kernel<false, true, CB, TW>), grid, dim3(BF_FTHR), 0, st, w);
+    else hipLaunchKernelGGL((f32_wide_kernel<false, false, CB, TW>), grid, dim3(BF_FTHR), 0, st, w);
+}
+// by tail width: none, one column, two to four
+template <int CB>
+static void launch_f32_wide_tw(bool gauss, bool rows, int nt, dim3 grid, const F32WideProblem &w, hipStream_t st) {
+    if (nt == 0) launch_f32_wide<CB, 0>(gauss, rows, grid, w, st);
+    else if (nt == 1) launch_f32_wide<CB, 1>(gauss, rows, grid, w, st);
+    else launch_f32_wide<CB, WIDE_TAIL>(gauss, rows, grid, w, st);
 }
 
 // Runs the part of an FP32 fused-sketch problem that the wide kernel covers
@@ -1095,6 +1333,8 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     const int cb = pb.N >= 1024 ? 4 : (pb.N >= 512 ? 2 : 1);
     const int64_t NT = 256 * cb;
     const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / NT * NT, K = pb.K;
+    const int nt = wide_tail_cols(pb, Mf, Nf, cb);
+    if (nt < 0) return -1;
     F32WideProblem w{};
     w.M = Mf;
     w.N = Nf;
@@ -1117,13 +1357,17 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     w.nS = (int)((K + kw - 1) / kw);
     if ((int64_t)w.nMB * w.nNB * w.nS >= (1ll << 31)) return -1;
     const size_t packed_bytes = align256b((size_t)safe_mul(K, Nf) * 4);
-    const size_t part_bytes = (size_t)safe_mul(safe_mul(w.nS, Mf), Nf) * 4;
-    if (packed_bytes + part_bytes > WIDE_SCRATCH_MAX) return -1;  // the tile kernels need no scratch
-    char *ws = (char *)wide_scratch(packed_bytes + part_bytes, stream);
+    const size_t part_bytes = align256b((size_t)safe_mul(safe_mul(w.nS, Mf), Nf) * 4);
+    const size_t tail_bytes = (size_t)w.nS * nt * Mf * 4;
+    // the tile kernels need no scratch
+    if (packed_bytes + part_bytes + tail_bytes > WIDE_SCRATCH_MAX) return -1;
+    char *ws = (char *)wide_scratch(packed_bytes + part_bytes + tail_bytes, stream);
     if (ws == nullptr) return -1;
     w.packed = (const float *)ws;
     w.part = (float *)(ws + packed_bytes);
     const float *Y = (const float *)pb.D;
+    w.tc = wide_tail_io<float>(pb, nt);
+    w.tpart = (float *)(ws + packed_bytes + part_bytes);
     {
         ProfScope prof("f32_pack_kernel", stream);
         if (pb.drs == 1) {
@@ -1146,9 +1390,10 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
         ProfScope prof("f32_wide_kernel", stream);
         const dim3 grid((unsigned)(w.nMB * w.nNB * w.nS));
         const bool rows = w.g.b_along_rows != 0;
-        if (cb == 4) launch_f32_wide<4>(w.g.gauss != 0, rows, grid, w, stream);
-        else if (cb == 2) launch_f32_wide<2>(w.g.gauss != 0, rows, grid, w, stream);
-        else launch_f32_wide<1>(w.g.gauss != 0, rows, grid, w, stream);
+        const bool gauss = w.g.gauss != 0;
+        if (cb == 4) launch_f32_wide<4, 0>(gauss, rows, grid, w, stream);  // no tail at cb = 4
+        else if (cb == 2) launch_f32_wide_tw<2>(gauss, rows, nt, grid, w, stream);
+        else launch_f32_wide_tw<1>(gauss, rows, nt, grid, w, stream);
         RBX_HIP(hipGetLastError());
     }
     {
@@ -1156,6 +1401,8 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
         const dim3 grid = reduce_grid(Mf, Nf);
         hipLaunchKernelGGL((bf16_reduce_kernel<float, float>), grid, dim3(256), 0, stream, (const float *)w.part, w.nS, Mf, Nf,
                            pb.alpha, pb.beta, (float *)pb.C, pb.crs, pb.ccs);
+        if (nt > 0)
+            launch_tail_reduce<float>(w.tpart, w.nS, Mf, w.tc, pb.alpha, pb.beta, stream);
         RBX_HIP(hipGetLastError());
     }
     int nr = 0;
@@ -1167,10 +1414,12 @@ int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
         r.C = (float *)pb.C + Mf * pb.crs;
         rest[nr++] = r;
     }
-    if (pb.N > Nf) {  // rows [0, Mf) x columns [Nf, N)
+    if (pb.N - nt > Nf) {  // rows [0, Mf) x columns [Nf, N - nt)
         GemmProblem r = pb;
+        r.xD = nullptr;
+        r.xC = nullptr;
         r.M = Mf;
-        r.N = pb.N - Nf;
+        r.N = pb.N - nt - Nf;
         r.D = Y + Nf * pb.dcs;
         r.C = (float *)pb.C + Nf * pb.ccs;
         rest[nr++] = r;
@@ -1258,10 +1507,12 @@ struct F64WideProblem {
     double *part;  // nS x M x N, row-major
     int64_t kw;
     int nMB, nNB, nS;
+    TailCols<double> tc;  // tail columns, as in F32WideProblem
+    double *tpart;
 };
 
 
-template <bool GAUSS, bool ROWS, int CB>
+template <bool GAUSS, bool ROWS, int CB, int TW>
 __global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb) {
     constexpr int RB = 4, MT = 64, NT = 128 * CB;  // CB 16-column blocks per wave
     __shared__ __attribute__((aligned(16))) double Ss[2][MT * DW_LDK];
@@ -1296,19 +1547,33 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb)
     const uint64_t foff = ROWS ? (uint64_t)((g.a_off + kbeg + fcell) * g.P + ((g.b_off + m0) >> 2) + (tid & 15))
                                : (uint64_t)((g.a_off + m0 + frow) * g.P + ((g.b_off + kbeg) >> 2) + fcell);
     const uint64_t fstep = ROWS ? 32 * (uint64_t)g.P : 8;
+    // tail columns: only the first column block's workgroups form them
+    // (cb < 4: the output has one column block, nb == 0)
+    WideTail<double, ROWS, TW == 0 ? 1 : TW> tl;
+    const int64_t tk = kbeg + (ROWS ? fcell : 4 * fcell);  // the thread's first k of chunk 0
+    if constexpr (TW > 0) {
+        tl.zero();
+        tl.load(pb.tc, tk);
+    }
     auto gen_chunk = [&](int c, double *lds) {
         const float4 v = gen4(GAUSS, ctr_add(g.ctr, foff + fstep * (uint64_t)c), g.k0, g.k1);
+        const double ent[4] = {entry_value<double>(v.x, GAUSS), entry_value<double>(v.y, GAUSS),
+                               entry_value<double>(v.z, GAUSS), entry_value<double>(v.w, GAUSS)};
         if constexpr (ROWS) {
             const int pos = dw_pos(fcell);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) lds[(frow + e) * DW_LDK + pos] = entry_value<double>(f4_get(v, e), GAUSS);
+            for (int e = 0; e < 4; ++e) lds[(frow + e) * DW_LDK + pos] = ent[e];
         } else {
             double *row = lds + frow * DW_LDK;
             const int k0 = 4 * fcell;
-            row[dw_pos(k0)] = entry_value<double>(v.x, GAUSS);
-            row[dw_pos(k0 + 1)] = entry_value<double>(v.y, GAUSS);
-            row[dw_pos(k0 + 2)] = entry_value<double>(v.z, GAUSS);
-            row[dw_pos(k0 + 3)] = entry_value<double>(v.w, GAUSS);
+            row[dw_pos(k0)] = ent[0];
+            row[dw_pos(k0 + 1)] = ent[1];
+            row[dw_pos(k0 + 2)] = ent[2];
+            row[dw_pos(k0 + 3)] = ent[3];
+        }
+        if constexpr (TW > 0) {
+            tl.step(ent);
+            tl.load(pb.tc, tk + 32 * (int64_t)(c + 1 < nchunks ? c + 1 : c));  // the next chunk's
         }
     };
 
@@ -1355,17 +1620,45 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb)
     const bool late = (wave & 4) != 0;
     gen_chunk(0, Ss[0]);
     __syncthreads();
-    for (int c = 0; c < nchunks; ++c) {
-        const double *S = Ss[c & 1];
-        double *Sn = Ss[(c + 1) & 1];
-        const int64_t s0 = s_first + 2 * c;
-        const bool more = c + 1 < nchunks;
-        if (more && !late) gen_chunk(c + 1, Sn);
-        step(S, 0, s0 + 1);
-        step(S, 1, s0 + 2 > s_last ? s_last : s0 + 2);
-        if (more && late) gen_chunk(c + 1, Sn);
+    if constexpr (TW == 0) {
+        for (int c = 0; c < nchunks; ++c) {
+            const double *S = Ss[c & 1];
+            double *Sn = Ss[(c + 1) & 1];
+            const int64_t s0 = s_first + 2 * c;
+            const bool more = c + 1 < nchunks;
+            if (more && !late) gen_chunk(c + 1, Sn);
+            step(S, 0, s0 + 1);
+            step(S, 1, s0 + 2 > s_last ? s_last : s0 + 2);
+            if (more && late) gen_chunk(c + 1, Sn);
+            __syncthreads();
+        }
+    } else {
+        // with a tail: one loop per wave role and the last chunk peeled, so
+        // no branch surrounds the tail's loads inside a loop (the compiler's
+        // memory-counter waits for the MFMA operands stay exact)
+        const int last = nchunks - 1;
+        if (!late) {
+            for (int c = 0; c < last; ++c) {
+                gen_chunk(c + 1, Ss[(c + 1) & 1]);
+                step(Ss[c & 1], 0, s_first + 2 * c + 1);
+                step(Ss[c & 1], 1, s_first + 2 * c + 2);
+                __syncthreads();
+            }
+        } else {
+            for (int c = 0; c < last; ++c) {
+                step(Ss[c & 1], 0, s_first + 2 * c + 1);
+                step(Ss[c & 1], 1, s_first + 2 * c + 2);
+                gen_chunk(c + 1, Ss[(c + 1) & 1]);
+                __syncthreads();
+            }
+        }
+        step(Ss[last & 1], 0, s_first + 2 * last + 1);
+        step(Ss[last & 1], 1, s_last);
         __syncthreads();
     }
+
+    if constexpr (TW > 0)
+        if (nb == 0) tl.finish(pb.tc.nt, tid, frow, fcell, &Ss[0][0], pb.tpart + (int64_t)sp * pb.tc.nt * pb.M + m0, pb.M);
 
     // element r of a 16 x 16 FP64 block: row l / 16 + 4 r, column l % 16
     double *P = pb.part + (int64_t)sp * pb.M * pb.N;
@@ -1381,12 +1674,19 @@ __global__ __launch_bounds__(BF_FTHR, 1) void f64_wide_kernel(F64WideProblem pb)
             }
 }
 
-template <int CB>
+template <int CB, int TW>
 static void launch_f64_wide(bool gauss, bool rows, dim3 grid, const F64WideProblem &w, hipStream_t st) {
-    if (gauss && rows) hipLaunchKernelGGL((f64_wide_kernel<true, true, CB>), grid, dim3(BF_FTHR), 0, st, w);
-    else if (gauss) hipLaunchKernelGGL((f64_wide_kernel<true, false, CB>), grid, dim3(BF_FTHR), 0, st, w);
-    else if (rows) hipLaunchKernelGGL((f64_wide_kernel<false, true, CB>), grid, dim3(BF_FTHR), 0, st, w);
-    else hipLaunchKernelGGL((f64_wide_kernel<false, false, CB>), grid, dim3(BF_FTHR), 0, st, w);
+    if (gauss && rows) hipLaunchKernelGGL((f64_wide_kernel<true, true, CB, TW>), grid, dim3(BF_FTHR), 0, st, w);
+    else if (gauss) hipLaunchKernelGGL((f64_wide_kernel<true, false, CB, TW>), grid, dim3(BF_FTHR), 0, st, w);
+    else if (rows) hipLaunchKernelGGL((f64_wide_kernel<false, true, CB, TW>), grid, dim3(BF_FTHR), 0, st, w);
+    else hipLaunchKernelGGL((f64_wide_kernel<false, false, CB, TW>), grid, dim3(BF_FTHR), 0, st, w);
+}
+// by tail width: none, one column, two to four
+template <int CB>
+static void launch_f64_wide_tw(bool gauss, bool rows, int nt, dim3 grid, const F64WideProblem &w, hipStream_t st) {
+    if (nt == 0) launch_f64_wide<CB, 0>(gauss, rows, grid, w, st);
+    else if (nt == 1) launch_f64_wide<CB, 1>(gauss, rows, grid, w, st);
+    else launch_f64_wide<CB, WIDE_TAIL>(gauss, rows, grid, w, st);
 }
 
 int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream) {
@@ -1395,6 +1695,8 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     const int cb = pb.N >= 512 ? 4 : (pb.N >= 256 ? 2 : 1);
     const int64_t NT = 128 * cb;
     const int64_t Mf = pb.M / 64 * 64, Nf = pb.N / NT * NT, K = pb.K;
+    const int nt = wide_tail_cols(pb, Mf, Nf, cb);
+    if (nt < 0) return -1;
     F64WideProblem w{};
     w.M = Mf;
     w.N = Nf;
@@ -1414,13 +1716,16 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     w.nS = (int)((K + kw - 1) / kw);
     if ((int64_t)w.nMB * w.nNB * w.nS >= (1ll << 31)) return -1;
     const size_t packed_bytes = align256b((size_t)safe_mul(K, Nf) * 8);
-    const size_t part_bytes = (size_t)safe_mul(safe_mul(w.nS, Mf), Nf) * 8;
-    if (packed_bytes + part_bytes > WIDE_SCRATCH_MAX) return -1;
-    char *ws = (char *)wide_scratch(packed_bytes + part_bytes, stream);
+    const size_t part_bytes = align256b((size_t)safe_mul(safe_mul(w.nS, Mf), Nf) * 8);
+    const size_t tail_bytes = (size_t)w.nS * nt * Mf * 8;
+    if (packed_bytes + part_bytes + tail_bytes > WIDE_SCRATCH_MAX) return -1;
+    char *ws = (char *)wide_scratch(packed_bytes + part_bytes + tail_bytes, stream);
     if (ws == nullptr) return -1;
     w.packed = (const double *)ws;
     w.part = (double *)(ws + packed_bytes);
     const double *Y = (const double *)pb.D;
+    w.tc = wide_tail_io<double>(pb, nt);
+    w.tpart = (double *)(ws + packed_bytes + part_bytes);
     {
         ProfScope prof("f64_pack_kernel", stream);
         if (pb.drs == 1) {
@@ -1443,9 +1748,10 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
         ProfScope prof("f64_wide_kernel", stream);
         const dim3 grid((unsigned)(w.nMB * w.nNB * w.nS));
         const bool rows = w.g.b_along_rows != 0;
-        if (cb == 4) launch_f64_wide<4>(w.g.gauss != 0, rows, grid, w, stream);
-        else if (cb == 2) launch_f64_wide<2>(w.g.gauss != 0, rows, grid, w, stream);
-        else launch_f64_wide<1>(w.g.gauss != 0, rows, grid, w, stream);
+        const bool gauss = w.g.gauss != 0;
+        if (cb == 4) launch_f64_wide<4, 0>(gauss, rows, grid, w, stream);  // no tail at cb = 4
+        else if (cb == 2) launch_f64_wide_tw<2>(gauss, rows, nt, grid, w, stream);
+        else launch_f64_wide_tw<1>(gauss, rows, nt, grid, w, stream);
         RBX_HIP(hipGetLastError());
     }
     {
@@ -1453,6 +1759,8 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
         const dim3 grid = reduce_grid(Mf, Nf);
         hipLaunchKernelGGL((bf16_reduce_kernel<double, double>), grid, dim3(256), 0, stream, (const double *)w.part,
                            w.nS, Mf, Nf, pb.alpha, pb.beta, (double *)pb.C, pb.crs, pb.ccs);
+        if (nt > 0)
+            launch_tail_reduce<double>(w.tpart, w.nS, Mf, w.tc, pb.alpha, pb.beta, stream);
         RBX_HIP(hipGetLastError());
     }
     int nr = 0;
@@ -1464,10 +1772,12 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
         r.C = (double *)pb.C + Mf * pb.crs;
         rest[nr++] = r;
     }
-    if (pb.N > Nf) {
+    if (pb.N - nt > Nf) {
         GemmProblem r = pb;
+        r.xD = nullptr;
+        r.xC = nullptr;
         r.M = Mf;
-        r.N = pb.N - Nf;
+        r.N = pb.N - nt - Nf;
         r.D = Y + Nf * pb.dcs;
         r.C = (double *)pb.C + Nf * pb.ccs;
         rest[nr++] = r;

@@ -23,6 +23,13 @@ struct GemmProblem {
     int nsplit = 1;
     int64_t kw = 0;
     void *part = nullptr;
+    // the last column of D and of C held apart from the rest (a right-hand
+    // side beside a matrix): D's column N - 1 at xD[k * xdrs], C's at
+    // xC[i * xcrs]; only the wide kernels' tail takes these
+    const void *xD = nullptr;
+    int64_t xdrs = 0;
+    void *xC = nullptr;
+    int64_t xcrs = 0;
 };
 
 // The sub-problem of K range z: the dense operand and the operator are
@@ -161,5 +168,15 @@ GemmProblem plan_left(char layout, char opS, char opA, int64_t d, int64_t n, int
 int f32_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream);
 // The same for FP64 problems (tiles of up to 64 x 512).
 int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream);
+
+// [B b] = op(S) * [A a] for column-major A (m x n, lda) and B (d x n, ldb)
+// with one more column a (m) -> b (d) held apart, alpha 1, beta 0, in one
+// pass over the generated operator (the wide kernels' tail column).  Returns
+// false, having launched nothing, when the wide kernels do not take the
+// shape (the caller then sketches A and a separately).
+template <typename T>
+bool left_sketch_with_column(char opS, int64_t d, int64_t n, int64_t m, const rbx_dense_skop *S, int64_t ro_s,
+                             int64_t co_s, const T *A, int64_t lda, const T *a, T *B, int64_t ldb, T *b,
+                             hipStream_t stream);
 
 }  // namespace rbx

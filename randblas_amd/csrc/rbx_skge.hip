@@ -692,6 +692,29 @@ static void run_problem(const GemmProblem &pb, hipStream_t stream) {
     launch_skge<T>(pb, stream);
 }
 
+template <typename T>
+bool left_sketch_with_column(char opS, int64_t d, int64_t n, int64_t m, const rbx_dense_skop *S, int64_t ro_s,
+                             int64_t co_s, const T *A, int64_t lda, const T *a, T *B, int64_t ldb, T *b,
+                             hipStream_t stream) {
+    if (d == 0 || m == 0) return false;
+    GemmProblem pb = plan_left<T>('C', opS, 'N', d, n + 1, m, 1.0, S, ro_s, co_s, A, lda, 0.0, B, ldb);
+    pb.xD = a;
+    pb.xdrs = 1;
+    pb.xC = b;
+    pb.xcrs = 1;
+    GemmProblem rest[2];
+    const int nr = sizeof(T) == 4 ? f32_wide_run(pb, rest, stream) : f64_wide_run(pb, rest, stream);
+    if (nr < 0) return false;
+    for (int i = 0; i < nr; ++i) launch_skge<T>(rest[i], stream);  // columns between the tiles and the tail
+    return true;
+}
+template bool left_sketch_with_column<float>(char, int64_t, int64_t, int64_t, const rbx_dense_skop *, int64_t,
+                                             int64_t, const float *, int64_t, const float *, float *, int64_t,
+                                             float *, hipStream_t);
+template bool left_sketch_with_column<double>(char, int64_t, int64_t, int64_t, const rbx_dense_skop *, int64_t,
+                                              int64_t, const double *, int64_t, const double *, double *, int64_t,
+                                              double *, hipStream_t);
+
 }  // namespace rbx
 
 using namespace rbx;

@@ -22,7 +22,7 @@
 #include <mutex>
 #include <vector>
 
-#include "rbx_internal.hh"
+#include "rbx_gemm_common.hh"
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -269,7 +269,7 @@ struct SolveWork {
     double *SA, *QR, *Sb, *y, *tau;  // d x n, d x n, d, d, n
     double *G, *R2, *Rc;             // n x n each: R1 (CholeskyQR2), R2, R = R2 R1
     int *info;                       // potrf status of the two passes, and the check flag
-    T *tmpT;                         // d x n in T (the sketches before promotion)
+    T *tmpT;                         // d x (n + 1) in T (SA, then Sb: the sketches before promotion)
     // the factorization: explicit Q in QR (CholeskyQR2) or Householder
     // vectors in QR with tau; R upper triangular at R with leading dim ldr
     bool explicit_q = false;
@@ -282,7 +282,7 @@ static SolveWork<T> solve_work(int64_t d, int64_t n, hipStream_t st) {
     auto a = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t dn = (size_t)safe_mul(d, n), nn = (size_t)safe_mul(n, n);
     const size_t bytes = 2 * a(dn * 8) + 2 * a((size_t)d * 8) + a((size_t)n * 8) + 3 * a(nn * 8) + a(16) +
-                         a(dn * sizeof(T) + 8);
+                         a((dn + (size_t)d) * sizeof(T));
     char *ws = (char *)scratch(SCRATCH_SOLVE, bytes, st);
     SolveWork<T> w;
     size_t off = 0;
@@ -300,7 +300,7 @@ static SolveWork<T> solve_work(int64_t d, int64_t n, hipStream_t st) {
     w.R2 = (double *)take(nn * 8);
     w.Rc = (double *)take(nn * 8);
     w.info = (int *)take(16);
-    w.tmpT = (T *)take(dn * sizeof(T) + 8);
+    w.tmpT = (T *)take((dn + (size_t)d) * sizeof(T));
     return w;
 }
 
@@ -353,15 +353,18 @@ static void factor_and_solve(char layout, int64_t d, int64_t m, int64_t n, const
                              int64_t co_s, const T *A, int64_t lda, const T *b, SolveWork<T> &w,
                              hipStream_t st) {
     RBX_REQUIRE(d <= INT32_MAX && n <= INT32_MAX);
-    // SA in T, in `layout` with leading dimension d (C) or n (R), promoted
+    // SA in T, in `layout` with leading dimension d (C) or n (R), and Sb after
+    // it; column-major [A b] in one pass over the operator where the wide
+    // kernels take the shape, else two sketches
     const int64_t ldsa = layout == 'C' ? d : n;
-    inner_ok(lsk<T>(layout, d, n, m, S, ro_s, co_s, A, lda, w.tmpT, ldsa, st));
+    T *SbT = w.tmpT + d * n;
+    if (layout != 'C' || !left_sketch_with_column<T>('N', d, n, m, S, ro_s, co_s, A, lda, b, w.tmpT, d, SbT, st)) {
+        inner_ok(lsk<T>(layout, d, n, m, S, ro_s, co_s, A, lda, w.tmpT, ldsa, st));
+        inner_ok(lsk<T>('C', d, 1, m, S, ro_s, co_s, b, m, SbT, d, st));
+    }
     hipLaunchKernelGGL((to_f64_kernel<T>), dim3(grid_of(d * n)), dim3(256), 0, st, (const T *)w.tmpT,
                        layout == 'C' ? (int64_t)1 : ldsa, layout == 'C' ? ldsa : (int64_t)1, d, n, w.SA);
-    RBX_HIP(hipGetLastError());
-    // Sb: a one-column column-major sketch
-    inner_ok(lsk<T>('C', d, 1, m, S, ro_s, co_s, b, m, w.tmpT, d, st));
-    hipLaunchKernelGGL((to_f64_kernel<T>), dim3(grid_of(d)), dim3(256), 0, st, (const T *)w.tmpT, (int64_t)1, d, d,
+    hipLaunchKernelGGL((to_f64_kernel<T>), dim3(grid_of(d)), dim3(256), 0, st, (const T *)SbT, (int64_t)1, d, d,
                        (int64_t)1, w.Sb);
     RBX_HIP(hipGetLastError());
     RBX_HIP(hipMemcpyAsync(w.QR, w.SA, (size_t)(d * n) * 8, hipMemcpyDeviceToDevice, st));
@@ -425,10 +428,12 @@ static void sketch_solve_adjoint_impl(char layout, int64_t d, int64_t m, int64_t
     RBX_REQUIRE(ldab >= (layout == 'C' ? m : n));
     SolveWork<T> w = solve_work<T>(d, n, st);
     factor_and_solve<T>(layout, d, m, n, S, ro_s, co_s, A, lda, b, w, st);
-    // second workspace: x64 (n), u (n), z (n), r (d), wv (d), M (d x n) in FP64, M in T
+    // second workspace: x64 (n), u (n), z (n), r (d), wv (d), M (d x n) in FP64, [M w] in T
     auto a = [](size_t bb) { return (bb + 255) / 256 * 256; };
     const size_t dn = (size_t)(d * n);
-    char *ws = (char *)scratch(SCRATCH_SOLVE2, 3 * a((size_t)n * 8) + 2 * a((size_t)d * 8) + a(dn * 8) + a(dn * sizeof(T)), st);
+    char *ws = (char *)scratch(SCRATCH_SOLVE2,
+                               3 * a((size_t)n * 8) + 2 * a((size_t)d * 8) + a(dn * 8) + a((dn + (size_t)d) * sizeof(T)),
+                               st);
     double *x64 = (double *)ws, *u = (double *)(ws + a((size_t)n * 8)), *z = (double *)(ws + 2 * a((size_t)n * 8));
     double *r = (double *)(ws + 3 * a((size_t)n * 8)), *wv = (double *)(ws + 3 * a((size_t)n * 8) + a((size_t)d * 8));
     double *M = (double *)(ws + 3 * a((size_t)n * 8) + 2 * a((size_t)d * 8));
@@ -471,13 +476,16 @@ static void sketch_solve_adjoint_impl(char layout, int64_t d, int64_t m, int64_t
     const int64_t ldm = layout == 'C' ? d : n;
     hipLaunchKernelGGL((from_f64_kernel<T>), dim3(grid_of(d * n)), dim3(256), 0, st, (const double *)M, d, n, MT,
                        layout == 'C' ? (int64_t)1 : ldm, layout == 'C' ? ldm : (int64_t)1);
-    RBX_HIP(hipGetLastError());
-    inner_ok(lsk_adj<T>(layout, d, n, m, S, ro_s, co_s, MT, ldm, Abar, ldab, st));
-    // bbar = S^T w (one column)
-    hipLaunchKernelGGL((from_f64_kernel<T>), dim3(grid_of(d)), dim3(256), 0, st, (const double *)wv, d, (int64_t)1, MT,
+    T *wT = MT + dn;
+    hipLaunchKernelGGL((from_f64_kernel<T>), dim3(grid_of(d)), dim3(256), 0, st, (const double *)wv, d, (int64_t)1, wT,
                        (int64_t)1, d);
     RBX_HIP(hipGetLastError());
-    inner_ok(lsk_adj<T>('C', d, 1, m, S, ro_s, co_s, MT, d, bbar, m, st));
+    // Abar = S^T M and bbar = S^T w: column-major [M w] in one pass over the
+    // operator where the wide kernels take the shape, else two sketches
+    if (layout != 'C' || !left_sketch_with_column<T>('T', m, n, d, S, ro_s, co_s, MT, ldm, wT, Abar, ldab, bbar, st)) {
+        inner_ok(lsk_adj<T>(layout, d, n, m, S, ro_s, co_s, MT, ldm, Abar, ldab, st));
+        inner_ok(lsk_adj<T>('C', d, 1, m, S, ro_s, co_s, wT, d, bbar, m, st));
+    }
 }
 
 }  // namespace rbx

@@ -446,10 +446,64 @@ def test_wide_tiles_beta_zero_ignores_nan(rb, dtype):
                                          (np.float64, 1e-13, 200), (np.float64, 1e-13, 300),
                                          (np.float32, 1e-5, 257), (np.float64, 1e-13, 129)])
 def test_lskge3_narrow_wide_tiles(rb, orc, dtype, tol, n):
-    # n = 257 / 129: the one column past the last whole tile runs on the
-    # one-column kernel
+    # n = 257 / 129: the one column past the last whole tile is the wide
+    # kernel's tail column
     got, ref = run_lskge3(rb, orc, dtype, "C", "N", "N", 130, n, 8224, 0.75, -0.5, ro=2, co=4)
     assert rel_fro(got, ref) <= tol
+
+
+# One to four columns past the last whole wide tile (the right-hand side of
+# an [A b] product, a ragged edge) are formed inside the wide kernel by the
+# threads that generate the operator, so the operator is generated once: no
+# one-column kernel and no tile kernel for them.  Both generator orientations
+# (operator rows along K: axis L; counters along the output rows: axis S),
+# strided dense columns (op(A) = T), ragged rows (d = 130) and alpha / beta.
+@pytest.mark.parametrize("dtype,tol,n", [(np.float64, 1e-13, 257), (np.float64, 1e-13, 132),
+                                         (np.float32, 1e-5, 257), (np.float32, 1e-5, 260),
+                                         (np.float32, 1e-5, 515)])
+@pytest.mark.parametrize("axis,opA", [("L", "N"), ("S", "N"), ("L", "T")])
+def test_lskge3_wide_tail_columns(rb, orc, dtype, tol, n, axis, opA):
+    torch.cuda.synchronize()
+    rb.profile_collect()
+    rb.profile_enable(True)
+    got, ref = run_lskge3(rb, orc, dtype, "C", "N", opA, 130, n, 4128, 0.75, -0.5, ro=4, co=4, axis=axis, seed=n)
+    rb.profile_enable(False)
+    ran = rb.profile_collect()
+    assert ("f32_wide_kernel" if dtype == np.float32 else "f64_wide_kernel") in ran, ran
+    assert "gemv_gen_kernel" not in ran, ran
+    err = rel_fro(got, ref)
+    # the tail columns alone, so that a wrong tail cannot hide in the norm
+    tile = (512 if n >= 512 else 256) if dtype == np.float32 else (256 if n >= 256 else 128)
+    t0 = n // tile * tile
+    err_tail = rel_fro(got[:, t0:], ref[:, t0:])
+    print(f"tail {np.dtype(dtype).name} n={n} axis={axis} op(A)={opA}: {err:.2e}, tail columns {err_tail:.2e}")
+    assert err <= tol and err_tail <= tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-5), (np.float64, 1e-13)])
+def test_lskge3_adjoint_wide_tail_column(rb, orc, dtype, tol):
+    """The adjoint S^T G of an [M w] right-hand side: K = d, counters along
+    the output rows, one tail column."""
+    rng = np.random.default_rng(12)
+    d, m = 1024, 2112
+    n = 257 if dtype == np.float32 else 129
+    S = rb.DenseSkOp(rb.DenseDist(d, m), rb.RNGState(41))
+    G = rng.standard_normal(d * n).astype(dtype)
+    A0 = rng.standard_normal(m * n).astype(dtype)
+    Abar = dev(A0)
+    torch.cuda.synchronize()
+    rb.profile_collect()
+    rb.profile_enable(True)
+    rb.lskge3_adjoint("C", "N", d, n, m, 0.5, S, 0, 0, dev(G), d, -2.0, Abar, m)
+    rb.profile_enable(False)
+    ran = rb.profile_collect()
+    assert ("f32_wide_kernel" if dtype == np.float32 else "f64_wide_kernel") in ran, ran
+    assert "gemv_gen_kernel" not in ran, ran
+    Sm, _ = orc.fill_dense(d, m, [0, 0, 0, 0], [41, 0], dtype=dtype)
+    ref = 0.5 * (Sm.astype(np.float64).T @ G.astype(np.float64).reshape(n, d).T) - 2.0 * A0.astype(np.float64).reshape(n, m).T
+    got = host(Abar).reshape(n, m).T
+    assert rel_fro(got, ref) <= tol
+    assert rel_fro(got[:, -1], ref[:, -1]) <= tol
 
 
 # One-column products (vector sketches through lskge3 / rskge3) with K >= 1024

@@ -107,6 +107,41 @@ def test_normal_equations_at_size(rb, qr):
     assert bool(torch.isfinite(x).all())
 
 
+# Column-major systems whose width n fills whole wide tiles (or leaves more
+# than the tail) sketch [A b] in one pass over the operator: b is the wide
+# kernel's tail column (n = 200: columns 128..199 of A on a rest problem,
+# b in the tail), so no one-column kernel runs.  FP64 n = 128, 200; FP32
+# n = 256, 300.  The adjoint's [M w] product (K = d = 1024) likewise.
+@pytest.mark.parametrize("dtype,tol,n", [(torch.float64, 1e-11, 128), (torch.float64, 1e-11, 200),
+                                         (torch.float32, 1e-4, 256), (torch.float32, 1e-4, 300)])
+def test_joint_sketch_of_a_and_b(rb, orc, dtype, tol, n):
+    d, m = 1024, 2112
+    A, b = _problem(m, n, 21)
+    if dtype == torch.float32:
+        A, b = A.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64)
+    S = rb.DenseSkOp(rb.DenseDist(d, m), rb.RNGState(27))
+    Sm, _ = orc.fill_dense_unpacked(orc.natural_layout(d, m, "L"), d, m, d, m, 0, 0, [0, 0, 0, 0], [27, 0])
+    x_ref = np.linalg.lstsq(Sm @ A, Sm @ b, rcond=None)[0]
+    Ad, bd = _dev(_store(A, "C"), dtype), _dev(b, dtype)
+    x = torch.full((n,), np.nan, dtype=dtype, device=DEV)
+    ran = _ran(rb, lambda: rb.sketch_solve("C", d, m, n, S, 0, 0, Ad, m, bd, x))
+    wide = "f32_wide_kernel" if dtype == torch.float32 else "f64_wide_kernel"
+    assert wide in ran and "gemv_gen_kernel" not in ran, ran
+    err = np.linalg.norm(host(x).astype(np.float64) - x_ref) / np.linalg.norm(x_ref)
+    assert err <= tol, err
+    # the adjoint: [Abar bbar] = S^T [M w] in one pass
+    xbar = np.random.default_rng(22).standard_normal(n)
+    Abar = torch.full((m * n,), np.nan, dtype=dtype, device=DEV)
+    bbar = torch.full((m,), np.nan, dtype=dtype, device=DEV)
+    ran = _ran(rb, lambda: rb.sketch_solve_adjoint("C", d, m, n, S, 0, 0, Ad, m, bd, _dev(xbar, dtype), Abar, m,
+                                                   bbar))
+    assert wide in ran and "gemv_gen_kernel" not in ran, ran
+    gA_ref, gb_ref = _torch_grads(Sm, A, b, xbar)
+    gA = host(Abar).astype(np.float64).reshape(n, m).T
+    assert np.linalg.norm(gA - gA_ref) / np.linalg.norm(gA_ref) <= (1e-9 if dtype == torch.float64 else 1e-4)
+    assert np.linalg.norm(host(bbar) - gb_ref) / np.linalg.norm(gb_ref) <= (1e-9 if dtype == torch.float64 else 1e-4)
+
+
 def _torch_grads(Sm, A, b, xbar):
     """Gradient of <xbar, x(A, b)> by torch's autograd through a host FP64 QR
     solve of the materialised sketched system."""
