@@ -105,122 +105,153 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 }
 
 // Upper Cholesky A = R^T R of an n x n column-major FP64 matrix, n <= 256,
-// in one 1024-thread workgroup (the upper triangle is read and R written over
-// it; the strict lower triangle is untouched).  Right-looking over 32-wide
-// blocks, everything a block step reuses held in LDS:
-//   1. the diagonal block is factored by one wave, in registers: lane l holds
-//      column l of the block's upper triangle; pivot k is read from lane k,
-//      row k scaled by its square root, and every later row's entries updated
-//      with row k's entries read lane by lane (v_readlane) -- no workgroup
-//      barrier inside the block;
-//   2. the block row to its right is solved with the block's R^T, one thread
-//      per column (forward substitution in registers), and kept in LDS;
-//   3. the trailing upper triangle is updated from LDS, A -= P^T P, a 4 x 4
-//      tile of entries per thread.
+// in one 512-thread workgroup (the upper triangle is read and R written over
+// it; the strict lower triangle is untouched).  The upper triangle stays in
+// registers for the whole factorization: 16 x 16 tiles, slot q of wave w
+// holding tile q * 8 + w of the row-major upper triangle, lane l rows
+// l / 16 + 4 r (r = 0..3) of column l % 16 (the v_mfma_f64_16x16x4f64
+// accumulator layout); rows and columns past n are identity padding.  Per
+// 16-wide step s:
+//   1. the owners publish tile (s, s) and the tile row (s, j > s) to LDS;
+//   2. every wave factors tile (s, s) itself, lane l < 16 holding column l
+//      (pivot k read from lane k, 1 / sqrt by the hardware estimate and two
+//      Newton steps, row k's entries broadcast with v_readlane), so no
+//      barrier follows;
+//   3. R^T X = B for the tile row, one thread per column, right-looking;
+//   4. owners reload their tile-row tiles (now R) and update their trailing
+//      tiles, T -= X_a^T X_b, with v_mfma_f64_16x16x4f64.
+// Global memory is read once and written once; two barriers per step (the
+// tile row's LDS buffer alternates between steps).
 // info = 0, or k + 1 when pivot k is not positive (R then incomplete).
-constexpr int CH_NB = 32, CH_MAX = 256, CH_THR = 1024;
-__global__ __launch_bounds__(CH_THR) void chol_blocked_kernel(double *A, int n, int *info) {
-    __shared__ double D[CH_NB][CH_NB + 1];  // D[i][j] = R(r0 + i, r0 + j), upper part
-    __shared__ double P[CH_NB][CH_MAX];     // P[t][c] = R(r0 + t, c0 + c)
-    __shared__ int failed;
-    const int tid = threadIdx.x;
-    if (tid == 0) failed = 0;
-    for (int r0 = 0; r0 < n; r0 += CH_NB) {
-        const int nb = n - r0 < CH_NB ? n - r0 : CH_NB;
-        if (tid < 64) {
-            // lane l < nb: column l of the block; other lanes carry identity
-            // columns, which leave the real ones unchanged
-            const int l = tid;
-            double c[CH_NB];
+constexpr int CH_MAX = 256, CH_THR = 512, CH_WAVES = CH_THR / 64, CH_T = CH_MAX / 16;
+constexpr int CH_SLOTS = (CH_T * (CH_T + 1) / 2 + CH_WAVES - 1) / CH_WAVES;  // 17
+constexpr int CH_PLD = CH_MAX + 16;  // LDS row of the tile row (16 doubles of pad)
+typedef double chf64x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(CH_THR) void chol_tiles_kernel(double *A, int n, int *info) {
+    __shared__ double Dg[16][17];            // the step's diagonal tile, from its owner
+    __shared__ double Dw[CH_WAVES][16][17];  // each wave's own R of that tile
+    __shared__ double dinvw[CH_WAVES][16];   // and 1 / R(i, i)
+    __shared__ __attribute__((aligned(16))) double Pn[2][16][CH_PLD];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: slot bookkeeping in scalar registers
+    const int T = (n + 15) / 16, ntiles = T * (T + 1) / 2;
+    const int lr = lane >> 4, lc = lane & 15;
+    int ti_s[CH_SLOTS], tj_s[CH_SLOTS];
+    double tile[CH_SLOTS][4];
 #pragma unroll
-            for (int i = 0; i < CH_NB; ++i) {
-                double v = i == l ? 1.0 : 0.0;
-                if (l < nb && i <= l) v = A[(r0 + i) + (int64_t)(r0 + l) * n];
-                c[i] = v;
-            }
-            int bad = 0;  // uniform across the wave
+    for (int q = 0; q < CH_SLOTS; ++q) {
+        const int e = q * CH_WAVES + wave;
+        int a = 0, rem = e;
+        while (a < T && rem >= T - a) {  // row a of the upper triangle holds T - a tiles
+            rem -= T - a;
+            ++a;
+        }
+        ti_s[q] = e < ntiles ? a : -1;
+        tj_s[q] = e < ntiles ? a + rem : -1;
 #pragma unroll
-            for (int k = 0; k < CH_NB; ++k) {
-                const double p = readlane_f64(c[k], k);  // the pivot, uniform
-                if (bad == 0 && !(p > 0.0)) bad = r0 + k + 1;
-                if (bad == 0) {
-                    const double rkk = sqrt(p);
-                    if (l == k) c[k] = rkk;
-                    else if (l > k) c[k] = c[k] / rkk;  // R(k, l)
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * a + lr + 4 * r, col = 16 * (a + rem) + lc;
+            double v = row == col ? 1.0 : 0.0;
+            if (e < ntiles && row < n && col < n && row <= col) v = A[row + (int64_t)col * n];
+            tile[q][r] = v;
+        }
+    }
+    for (int s = 0; s < T; ++s) {
+        double(*P)[CH_PLD] = Pn[s & 1];
+        const int c0 = 16 * (s + 1), m = 16 * T - c0;
+        // 1. publish
 #pragma unroll
-                    for (int i = k + 1; i < CH_NB; ++i) {
-                        const double rki = readlane_f64(c[k], i);  // R(k, i)
-                        if (i <= l) c[i] -= rki * c[k];
-                    }
-                }
-            }
-            if (bad != 0) {
-                if (l == 0) {
-                    *info = bad;
-                    failed = 1;
-                }
-            } else if (l < nb) {
+        for (int q = 0; q < CH_SLOTS; ++q) {
+            if (ti_s[q] == s && tj_s[q] == s)
 #pragma unroll
-                for (int i = 0; i < CH_NB; ++i)
-                    if (i <= l) {
-                        D[i][l] = c[i];
-                        A[(r0 + i) + (int64_t)(r0 + l) * n] = c[i];
-                    }
-            }
+                for (int r = 0; r < 4; ++r) Dg[lr + 4 * r][lc] = tile[q][r];
+            if (ti_s[q] == s && tj_s[q] > s)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) P[lr + 4 * r][16 * tj_s[q] - c0 + lc] = tile[q][r];
         }
         __syncthreads();
-        if (failed) return;  // uniform: every thread reads the same flag
-        const int c0 = r0 + nb, m = n - c0;
+        // 2. factor the diagonal tile in every wave
+        {
+            const int l = lane;
+            double c[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) c[i] = (l < 16 && i <= l) ? Dg[i][l] : (i == l ? 1.0 : 0.0);
+            int bad = 0;
+            double dinv = 1.0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const double p = readlane_f64(c[k], k);
+                if (bad == 0 && !(p > 0.0)) bad = 16 * s + k + 1;
+                double y = __builtin_amdgcn_rsq(p);  // 1 / sqrt(p) to full precision after two Newton steps
+                y = fma(0.5 * y, fma(-p, y * y, 1.0), y);
+                y = fma(0.5 * y, fma(-p, y * y, 1.0), y);
+                if (l == k) dinv = y;
+                c[k] = l == k ? p * y : c[k] * y;  // (entries below the diagonal, l < k, are never read)
+#pragma unroll
+                for (int i = 0; i < 16; ++i)  // constant trip count: c[] stays in registers
+                    if (i > k) c[i] -= readlane_f64(c[k], i) * c[k];
+            }
+            if (bad != 0) {  // every wave saw the same pivot: all leave here
+                if (tid == 0) *info = bad;
+                return;
+            }
+            if (l < 16) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) Dw[wave][i][l] = c[i];
+                dinvw[wave][l] = dinv;
+            }
+        }
+        // the wave's lanes read what its lanes < 16 wrote
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int q = 0; q < CH_SLOTS; ++q)
+            if (ti_s[q] == s && tj_s[q] == s)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) tile[q][r] = (lr + 4 * r <= lc) ? Dw[wave][lr + 4 * r][lc] : 0.0;
+        // 3. the tile row: R^T x = b per column
         if (tid < m) {
-            // R^T x = a for column c0 + tid of the block row
-            const int64_t cb = (int64_t)(c0 + tid) * n + r0;
-            double x[CH_NB];
+            double x[16];
 #pragma unroll
-            for (int t = 0; t < CH_NB; ++t) x[t] = t < nb ? A[cb + t] : 0.0;
+            for (int t = 0; t < 16; ++t) x[t] = P[t][tid];
 #pragma unroll
-            for (int t = 0; t < CH_NB; ++t) {
-                if (t < nb) {
-                    double v = x[t];
+            for (int u = 0; u < 16; ++u) {
+                x[u] *= dinvw[wave][u];
 #pragma unroll
-                    for (int u = 0; u < t; ++u) v -= D[u][t] * x[u];
-                    x[t] = v / D[t][t];
-                    P[t][tid] = x[t];
-                    A[cb + t] = x[t];
-                }
-            }
-        }
-        __syncthreads();
-        // trailing update of the upper triangle, 4 x 4 tiles
-        const int mt = (m + 3) / 4;
-        for (int e = tid; e < mt * mt; e += CH_THR) {
-            const int ti = e % mt, tj = e / mt;
-            if (ti > tj) continue;
-            double acc[4][4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-            for (int t = 0; t < nb; ++t) {
-                double pi[4], pj[4];
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    pi[a] = 4 * ti + a < m ? P[t][4 * ti + a] : 0.0;
-                    pj[a] = 4 * tj + a < m ? P[t][4 * tj + a] : 0.0;
-                }
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) acc[a][b] += pi[a] * pj[b];
+                for (int t = 0; t < 16; ++t)
+                    if (t > u) x[t] -= Dw[wave][u][t] * x[u];
             }
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int i = 4 * ti + a, j = 4 * tj + b;
-                    if (i < m && j < m && i <= j) A[(c0 + i) + (int64_t)(c0 + j) * n] -= acc[a][b];
-                }
+            for (int t = 0; t < 16; ++t) P[t][tid] = x[t];
         }
         __syncthreads();
+        // 4. reload the tile row; update the trailing tiles
+#pragma unroll
+        for (int q = 0; q < CH_SLOTS; ++q) {
+            const int a = ti_s[q], b = tj_s[q];
+            if (a == s && b > s)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) tile[q][r] = P[lr + 4 * r][16 * b - c0 + lc];
+            if (a > s) {
+                chf64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) {
+                    const double av = P[4 * ks + lr][16 * a - c0 + lc];
+                    const double bv = P[4 * ks + lr][16 * b - c0 + lc];
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) tile[q][r] -= acc[r];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < CH_SLOTS; ++q) {
+        if (ti_s[q] < 0) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * ti_s[q] + lr + 4 * r, col = 16 * tj_s[q] + lc;
+            if (row < n && col < n && row <= col) A[row + (int64_t)col * n] = tile[q][r];
+        }
     }
     if (tid == 0) *info = 0;
 }
@@ -304,11 +335,12 @@ static SolveWork<T> solve_work(int64_t d, int64_t n, hipStream_t st) {
     return w;
 }
 
-// CholeskyQR2 of the d x n FP64 matrix in w.QR (overwritten by Q), R in w.Rc;
-// true when accepted (see the file comment), false when Householder QR must
-// be used instead (w.QR then holds no usable factor).
+// CholeskyQR2 of the d x n FP64 matrix in w.QR (overwritten by Q), R in w.Rc,
+// enqueued without waiting: w.info[2] = 1 when it is accepted (see the file
+// comment); otherwise Householder QR must be used instead (w.QR and w.Rc then
+// hold no usable factor).
 template <typename T>
-static bool cholqr2(int64_t d, int64_t n, SolveWork<T> &w, hipStream_t st) {
+static void cholqr2(int64_t d, int64_t n, SolveWork<T> &w, hipStream_t st) {
     ProfScope prof("solve_cholqr2", st);
     rocblas_handle h = blas_handle(st);
     const rocblas_int D = (rocblas_int)d, Nn = (rocblas_int)n;
@@ -321,7 +353,7 @@ static bool cholqr2(int64_t d, int64_t n, SolveWork<T> &w, hipStream_t st) {
                               D, &zero, Rk[pass], Nn),
                 "gemm");
         if (n <= CH_MAX) {
-            hipLaunchKernelGGL(chol_blocked_kernel, dim3(1), dim3(CH_THR), 0, st, Rk[pass], (int)n, w.info + pass);
+            hipLaunchKernelGGL(chol_tiles_kernel, dim3(1), dim3(CH_THR), 0, st, Rk[pass], (int)n, w.info + pass);
             RBX_HIP(hipGetLastError());
         } else {
             blas_ok(rocsolver_dpotrf(h, rocblas_fill_upper, Nn, Rk[pass], Nn, w.info + pass), "potrf");
@@ -333,17 +365,12 @@ static bool cholqr2(int64_t d, int64_t n, SolveWork<T> &w, hipStream_t st) {
     hipLaunchKernelGGL(cholqr_check_kernel, dim3(1), dim3(256), 0, st, (const double *)w.R2, n, (const int *)w.info,
                        w.info + 2);
     RBX_HIP(hipGetLastError());
-    int ok = 0;
-    RBX_HIP(hipMemcpyAsync(&ok, w.info + 2, sizeof(int), hipMemcpyDeviceToHost, st));
-    RBX_HIP(hipStreamSynchronize(st));
-    if (!ok) return false;
     // R = R2 R1 (R1's strictly lower triangle still holds the Gram matrix)
     hipLaunchKernelGGL(zero_lower_kernel, dim3(grid_of(n * n)), dim3(256), 0, st, w.G, n);
     RBX_HIP(hipGetLastError());
     blas_ok(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
                           Nn, Nn, &one, w.R2, Nn, w.G, Nn, w.Rc, Nn),
             "trmm");
-    return true;
 }
 
 // SA, Sb (FP64), the QR of SA (CholeskyQR2, else Householder), y = Q^T Sb,
@@ -371,7 +398,11 @@ static void factor_and_solve(char layout, int64_t d, int64_t m, int64_t n, const
     rocblas_handle h = blas_handle(st);
     const char *qr_env = std::getenv("RBX_SOLVE_QR");
     const bool householder_only = qr_env != nullptr && std::strcmp(qr_env, "householder") == 0;
-    if (!householder_only && cholqr2<T>(d, n, w, st)) {
+    bool accepted = false;
+    if (!householder_only) {
+        // the whole CholeskyQR2 path is enqueued, and its acceptance read
+        // once at the end (no idle gap in the middle of the solve)
+        cholqr2<T>(d, n, w, st);
         w.explicit_q = true;
         w.R = w.Rc;
         w.ldr = n;
@@ -380,7 +411,15 @@ static void factor_and_solve(char layout, int64_t d, int64_t m, int64_t n, const
         blas_ok(rocblas_dgemv(h, rocblas_operation_transpose, (rocblas_int)d, (rocblas_int)n, &one, w.QR,
                               (rocblas_int)d, w.Sb, 1, &zero, w.y, 1),
                 "gemv");
-    } else {
+        blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                              (rocblas_int)n, w.R, (rocblas_int)w.ldr, w.y, 1),
+                "trsv");
+        int ok = 0;
+        RBX_HIP(hipMemcpyAsync(&ok, w.info + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+        RBX_HIP(hipStreamSynchronize(st));
+        accepted = ok != 0;
+    }
+    if (!accepted) {
         ProfScope prof("solve_householder", st);
         w.explicit_q = false;
         w.R = w.QR;
@@ -391,10 +430,10 @@ static void factor_and_solve(char layout, int64_t d, int64_t m, int64_t n, const
         blas_ok(rocsolver_dormqr(h, rocblas_side_left, rocblas_operation_transpose, (rocblas_int)d, 1, (rocblas_int)n,
                                  w.QR, (rocblas_int)d, w.tau, w.y, (rocblas_int)d),
                 "ormqr");
+        blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                              (rocblas_int)n, w.R, (rocblas_int)w.ldr, w.y, 1),
+                "trsv");
     }
-    blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit, (rocblas_int)n,
-                          w.R, (rocblas_int)w.ldr, w.y, 1),
-            "trsv");
 }
 
 template <typename T>
