@@ -256,6 +256,88 @@ __global__ __launch_bounds__(CH_THR) void chol_tiles_kernel(double *A, int n, in
     if (tid == 0) *info = 0;
 }
 
+// R x = y (TRANS false) or R^T x = y (TRANS true) in place, R n x n upper
+// triangular (column-major, leading dimension ldr, non-unit diagonal),
+// n <= TS_MAX, one 256-thread workgroup: y in LDS, 16-row blocks in the
+// order of the substitution.  Per block, lane i < 16 of wave 0 holds row i
+// of the block's triangle (R's row, or R's column when transposed) and
+// substitutes with x_k broadcast by v_readlane (1 / R(k, k) precomputed per
+// lane); then every thread of a remaining row subtracts the block's 16
+// products from its y, the R values loaded before the block's solve.
+constexpr int TS_MAX = 256;
+template <bool TRANS>
+__global__ __launch_bounds__(256) void trsv_upper_kernel(const double *R, int64_t ldr, double *y, int n) {
+    __shared__ double ys[TS_MAX];
+    __shared__ double xs[16];
+    const int tid = threadIdx.x;
+    if (tid < n) ys[tid] = y[tid];
+    __syncthreads();
+    const int nblk = (n + 15) / 16;
+    for (int bi = 0; bi < nblk; ++bi) {
+        const int kb = TRANS ? bi : nblk - 1 - bi;
+        const int k0 = 16 * kb, nk = n - k0 < 16 ? n - k0 : 16;
+        // rows still to update: above the block (back substitution), below it (forward)
+        const bool upd = TRANS ? (tid >= k0 + nk && tid < n) : (tid < k0);
+        double rv[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            rv[j] = (upd && j < nk) ? (TRANS ? R[(k0 + j) + (int64_t)tid * ldr] : R[tid + (int64_t)(k0 + j) * ldr]) : 0.0;
+        if (tid < 64) {
+            const int i = tid;
+            double r[16];  // row i of the block's triangle T (T = R_bb, or R_bb^T)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const bool in = i < nk && j < nk && (TRANS ? j <= i : j >= i);
+                r[j] = in ? (TRANS ? R[(k0 + j) + (int64_t)(k0 + i) * ldr] : R[(k0 + i) + (int64_t)(k0 + j) * ldr])
+                          : (i == j ? 1.0 : 0.0);
+            }
+            double rii = 1.0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j == i) rii = r[j];
+            const double dinv = 1.0 / rii;
+            double b = i < nk ? ys[k0 + i] : 0.0, x = 0.0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int k = TRANS ? t : 15 - t;
+                const double xk = readlane_f64(b * dinv, k);  // lane k's b is final here
+                if (i == k) x = xk;
+                b -= r[k] * xk;  // rows not yet solved; finished lanes keep x
+            }
+            if (i < nk) {
+                ys[k0 + i] = x;
+                xs[i] = x;
+            }
+        }
+        __syncthreads();
+        if (upd) {
+            double v = ys[tid];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v -= rv[j] * xs[j];
+            ys[tid] = v;
+        }
+        __syncthreads();
+    }
+    if (tid < n) y[tid] = ys[tid];
+}
+
+// x = op(R)^{-1} x in place for the solve's triangular factor: the kernel
+// above for n <= TS_MAX, rocBLAS above that.
+static void trsv_upper(rocblas_handle h, bool trans, int64_t n, const double *R, int64_t ldr, double *x,
+                       hipStream_t st) {
+    if (n <= TS_MAX) {
+        if (trans)
+            hipLaunchKernelGGL(trsv_upper_kernel<true>, dim3(1), dim3(256), 0, st, R, ldr, x, (int)n);
+        else
+            hipLaunchKernelGGL(trsv_upper_kernel<false>, dim3(1), dim3(256), 0, st, R, ldr, x, (int)n);
+        RBX_HIP(hipGetLastError());
+        return;
+    }
+    blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, trans ? rocblas_operation_transpose : rocblas_operation_none,
+                          rocblas_diagonal_non_unit, (rocblas_int)n, R, (rocblas_int)ldr, x, 1),
+            "trsv");
+}
+
 // ok[0] = 1 when both Cholesky factorizations succeeded and every diagonal
 // entry of R2 is within 1e-3 of 1 (one workgroup)
 __global__ __launch_bounds__(256) void cholqr_check_kernel(const double *R2, int64_t n, const int *info, int *ok) {
@@ -411,9 +493,7 @@ static void factor_and_solve(char layout, int64_t d, int64_t m, int64_t n, const
         blas_ok(rocblas_dgemv(h, rocblas_operation_transpose, (rocblas_int)d, (rocblas_int)n, &one, w.QR,
                               (rocblas_int)d, w.Sb, 1, &zero, w.y, 1),
                 "gemv");
-        blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
-                              (rocblas_int)n, w.R, (rocblas_int)w.ldr, w.y, 1),
-                "trsv");
+        trsv_upper(h, false, n, w.R, w.ldr, w.y, st);
         int ok = 0;
         RBX_HIP(hipMemcpyAsync(&ok, w.info + 2, sizeof(int), hipMemcpyDeviceToHost, st));
         RBX_HIP(hipStreamSynchronize(st));
@@ -430,9 +510,7 @@ static void factor_and_solve(char layout, int64_t d, int64_t m, int64_t n, const
         blas_ok(rocsolver_dormqr(h, rocblas_side_left, rocblas_operation_transpose, (rocblas_int)d, 1, (rocblas_int)n,
                                  w.QR, (rocblas_int)d, w.tau, w.y, (rocblas_int)d),
                 "ormqr");
-        blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
-                              (rocblas_int)n, w.R, (rocblas_int)w.ldr, w.y, 1),
-                "trsv");
+        trsv_upper(h, false, n, w.R, w.ldr, w.y, st);
     }
 }
 
@@ -482,9 +560,7 @@ static void sketch_solve_adjoint_impl(char layout, int64_t d, int64_t m, int64_t
     // u = R^{-T} xbar
     hipLaunchKernelGGL((to_f64_kernel<T>), dim3(grid_of(n)), dim3(256), 0, st, xbar, (int64_t)1, n, n, (int64_t)1, u);
     RBX_HIP(hipGetLastError());
-    blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, rocblas_operation_transpose, rocblas_diagonal_non_unit, (rocblas_int)n,
-                          w.R, (rocblas_int)w.ldr, u, 1),
-            "trsv");
+    trsv_upper(h, true, n, w.R, w.ldr, u, st);
     const double one = 1.0, mone = -1.0, zero = 0.0;
     // w = Q [u; 0]
     if (w.explicit_q) {
@@ -501,9 +577,7 @@ static void sketch_solve_adjoint_impl(char layout, int64_t d, int64_t m, int64_t
     }
     // z = R^{-1} u
     RBX_HIP(hipMemcpyAsync(z, u, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
-    blas_ok(rocblas_dtrsv(h, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit, (rocblas_int)n,
-                          w.R, (rocblas_int)w.ldr, z, 1),
-            "trsv");
+    trsv_upper(h, false, n, w.R, w.ldr, z, st);
     // r = Sb - SA x
     RBX_HIP(hipMemcpyAsync(r, w.Sb, (size_t)d * 8, hipMemcpyDeviceToDevice, st));
     blas_ok(rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)d, (rocblas_int)n, &mone, w.SA, (rocblas_int)d, x64, 1,
