@@ -582,12 +582,15 @@ __global__ __launch_bounds__(BF_FTHR, 1) void bf16_skge_pair_kernel(Bf16Problem 
 // FP64 in split order.  RED_R x RED_C tiles (8 x 128): each row of a tile is
 // one 512-byte run of every partial (FP32), long enough for full HBM bursts;
 // a column-major B is written along i after a transpose through LDS.  A 1-D
-// grid of tiles, row-tile major.
+// grid of tiles, row-tile major.  With pre != nullptr the sums start from
+// pre[i*N + j], the FP64 sum of splits 0 .. s0 - 1 (bf16_presum_kernel), and
+// continue with split s0: the same additions in the same order.
 constexpr int RED_R = 8, RED_C = 128;
 template <typename PT, typename OT>
 __global__ __launch_bounds__(256) void bf16_reduce_kernel(const PT *part, int nS, int64_t M, int64_t N,
                                                           double alpha, double beta, OT *B, int64_t brs,
-                                                          int64_t bcs) {
+                                                          int64_t bcs, const double *pre = nullptr,
+                                                          int s0 = 0) {
     __shared__ double tile[RED_R][RED_C + 1];
     const int64_t ntc = (N + RED_C - 1) / RED_C;
     const int64_t i0 = (int64_t)(blockIdx.x / ntc) * RED_R, j0 = (int64_t)(blockIdx.x % ntc) * RED_C;
@@ -600,10 +603,14 @@ __global__ __launch_bounds__(256) void bf16_reduce_kernel(const PT *part, int nS
         const int64_t i = i0 + r, j = j0 + c4;
         double v[4] = {0.0, 0.0, 0.0, 0.0};
         if (i < M) {
+            if (pre != nullptr)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (j + e < N) v[e] = pre[i * N + j + e];
             const PT *p = part + i * N + j;
             if (j + 3 < N && N % 4 == 0) {
                 typedef PT pt4 __attribute__((ext_vector_type(4)));
-                int s = 0;
+                int s = s0;
                 for (; s + 8 <= nS; s += 8) {
                     pt4 x[8];
 #pragma unroll
@@ -621,7 +628,7 @@ __global__ __launch_bounds__(256) void bf16_reduce_kernel(const PT *part, int nS
             } else {
                 for (int e = 0; e < 4; ++e)
                     if (j + e < N)
-                        for (int s = 0; s < nS; ++s) v[e] += (double)p[(int64_t)s * total + e];
+                        for (int s = s0; s < nS; ++s) v[e] += (double)p[(int64_t)s * total + e];
             }
         }
 #pragma unroll
@@ -642,6 +649,45 @@ __global__ __launch_bounds__(256) void bf16_reduce_kernel(const PT *part, int nS
             if (beta != 0.0) out += beta * (double)(*b);
             *b = (OT)out;
         }
+    }
+}
+
+// pre[e] = sum of part[s][e] over s = 0 .. ns - 1 in split order, FP64: the
+// first splits of bf16_reduce_kernel's sums, run on a helper stream while the
+// last K range is still being multiplied.  Lean (four elements and four
+// partial loads in flight per thread), so its waves fit in the registers the
+// main kernel leaves free on each SIMD.
+typedef float pf32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void bf16_presum_kernel(const float *part, int ns, int64_t MN, double *pre) {
+    const int64_t n4 = MN / 4;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (int64_t)gridDim.x * blockDim.x) {
+        const pf32x4 *p = (const pf32x4 *)part + q;
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+        int s = 0;
+        for (; s + 4 <= ns; s += 4) {
+            pf32x4 x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = __builtin_nontemporal_load(p + (int64_t)(s + u) * n4);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                v0 += (double)x[u][0];
+                v1 += (double)x[u][1];
+                v2 += (double)x[u][2];
+                v3 += (double)x[u][3];
+            }
+        }
+        for (; s < ns; ++s) {
+            const pf32x4 x = __builtin_nontemporal_load(p + (int64_t)s * n4);
+            v0 += (double)x[0];
+            v1 += (double)x[1];
+            v2 += (double)x[2];
+            v3 += (double)x[3];
+        }
+        double *o = pre + 4 * q;
+        o[0] = v0;
+        o[1] = v1;
+        o[2] = v2;
+        o[3] = v3;
     }
 }
 
@@ -725,12 +771,6 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     pb.nS = (int)((pb.Kp + kw - 1) / kw);
     RBX_REQUIRE((int64_t)pb.nMB * pb.nNB * pb.nS < (1ll << 31));
 
-    const size_t plane_elems = (size_t)safe_mul(pb.Kp, pb.Np);
-    const size_t planes_bytes = ((size_t)n_terms * plane_elems * 2 + 255) / 256 * 256;
-    const size_t part_bytes = (size_t)safe_mul(safe_mul(pb.nS, M), N) * 4;
-    char *ws = (char *)scratch(SCRATCH_BF16, planes_bytes + part_bytes, stream);
-    pb.planes = (const uint16_t *)ws;
-    pb.part = (float *)(ws + planes_bytes);
 
     // interior row blocks on the two-wave kernel when the shape allows it, the
     // rest (or everything) on the bounds-checked kernel; the widest tile (256
@@ -786,6 +826,18 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
     // the reduction.
     const int64_t tiles_per_split = pb.kw / SPL_KT;
     const bool pipelined = ars == 1 && nfast > 0 && nMB_all == nfast64 && pb.nS >= 8 && pb.kw % SPL_KT == 0;
+    // the pipeline also sums the first K ranges' partials on the helper
+    // stream during the last range's product (FP64, M x N)
+    const bool presum = pipelined && (M * N) % 4 == 0;
+    const size_t plane_elems = (size_t)safe_mul(pb.Kp, pb.Np);
+    const size_t planes_bytes = ((size_t)n_terms * plane_elems * 2 + 255) / 256 * 256;
+    const size_t part_bytes = ((size_t)safe_mul(safe_mul(pb.nS, M), N) * 4 + 255) / 256 * 256;
+    const size_t pre_bytes = presum ? (size_t)safe_mul(M, N) * 8 : 0;
+    char *ws = (char *)scratch(SCRATCH_BF16, planes_bytes + part_bytes + pre_bytes, stream);
+    pb.planes = (const uint16_t *)ws;
+    pb.part = (float *)(ws + planes_bytes);
+    double *pre = presum ? (double *)(ws + planes_bytes + part_bytes) : nullptr;
+    int pre_splits = 0;  // splits already in pre
     if (pipelined) {
         // four equal K ranges: only the first range's split runs alone (at the
         // headline shape 7.34 ms per step against 7.62 without the pipeline;
@@ -807,7 +859,21 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         for (int c = 0; c < NCH; ++c) {
             if (c > 0) RBX_HIP(hipStreamWaitEvent(stream, ws_event(stream, c), 0));
             main_fast(sb[c], sb[c + 1] - sb[c]);
+            if (presum && c == NCH - 2) {
+                // partials of ranges 0 .. NCH - 2 are complete: sum them on
+                // the helper stream while the last range is multiplied
+                hipEvent_t ready = ws_event(stream, NCH);
+                RBX_HIP(hipEventRecord(ready, stream));
+                RBX_HIP(hipStreamWaitEvent(hs, ready, 0));
+                pre_splits = sb[NCH - 1];
+                ProfScope prof("bf16_presum_kernel", hs);
+                hipLaunchKernelGGL(bf16_presum_kernel, dim3((unsigned)num_cus() * 4), dim3(256), 0, hs,
+                                   (const float *)pb.part, pre_splits, M * N, pre);
+                RBX_HIP(hipGetLastError());
+                RBX_HIP(hipEventRecord(ws_event(stream, NCH + 1), hs));
+            }
         }
+        if (presum) RBX_HIP(hipStreamWaitEvent(stream, ws_event(stream, NCH + 1), 0));
     } else {
         if (ars == 1) {
             split_kunit(0, kt_total, stream);
@@ -837,7 +903,7 @@ static void bf16_split_sketch(char layout, char opS, char opA, int64_t d, int64_
         const dim3 grid = reduce_grid(M, N);
         ProfScope prof("bf16_reduce_kernel", stream);
         hipLaunchKernelGGL((bf16_reduce_kernel<float, float>), grid, dim3(256), 0, stream, pb.part, pb.nS, M, N,
-                           (double)alpha, (double)beta, B, brs, bcs);
+                           (double)alpha, (double)beta, B, brs, bcs, (const double *)pre, pre_splits);
         RBX_HIP(hipGetLastError());
     }
 }
