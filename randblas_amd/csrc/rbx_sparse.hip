@@ -385,25 +385,22 @@ __global__ __launch_bounds__(64 * NW) void csr_rows_vec_kernel(const int64_t *ro
                         for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + v * vget<T>(g, e);
                     }
                 };
-                int q = 0;
-                for (; q + 8 <= cnt; q += 8) {
+                // groups of 8 gathers in flight, the last group padded (its
+                // entries past cnt gather row 0 of G, a valid address, and are
+                // not applied; cnt is uniform, so is that test): no
+                // one-at-a-time tail paying a full memory latency per entry
+                for (int q = 0; q < cnt; q += 8) {
                     V g[8];
                     T v[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        const int64_t c = __shfl(myc, q + u);
+                        const int64_t c = __shfl(myc, q + u);  // lanes past cnt hold column 0
                         v[u] = __shfl(myv, q + u);
                         if (cl < ncols_out) g[u] = *(const V *)(G + c * ldg + cl);
                     }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) apply(v[u], g[u]);
-                }
-                for (; q < cnt; ++q) {
-                    const int64_t c = __shfl(myc, q);
-                    const T v = __shfl(myv, q);
-                    V g;
-                    if (cl < ncols_out) g = *(const V *)(G + c * ldg + cl);
-                    apply(v, g);
+                    for (int u = 0; u < 8; ++u)
+                        if (q + u < cnt) apply(v[u], g[u]);
                 }
             }
         }
@@ -1234,7 +1231,7 @@ __global__ __launch_bounds__(256) void sddmm_vec_kernel(const int64_t *rowptr, c
                                                         int64_t d, T alpha, const T *Bb, int64_t brs, int64_t bcs,
                                                         const T *G, T *dvals) {
     using V = typename VecOf<T>::type;
-    constexpr int VEC = VecOf<T>::n, DC = SddmmChunk<T>::DC, GS = 16 * VEC;
+    constexpr int VEC = VecOf<T>::n, DC = SddmmChunk<T>::DC, GS = 16 * VEC, NQ = DC / GS;
     __shared__ T tile[SD_RB][DC + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int grp = lane >> 4, gl = lane & 15;
@@ -1255,23 +1252,44 @@ __global__ __launch_bounds__(256) void sddmm_vec_kernel(const int64_t *rowptr, c
             const int64_t i = i0 + r;
             if (i >= m) break;
             const int64_t kb = rowptr[i], ke = rowptr[i + 1];
-            for (int64_t k0 = kb; k0 < ke; k0 += 4) {
-                const int64_t k = k0 + grp;
-                const bool on = k < ke;
-                const T *grow = G + (on ? colidx[k] : 0) * d + j0;
-                T p = (T)0;
-                if (on) {
-                    for (int c = gl * VEC; c < dc; c += GS) {
-                        const V gv = *(const V *)(grow + c);
+            for (int64_t pb = kb; pb < ke; pb += 64) {
+                // the batch's columns loaded once (lane e: entry pb + e);
+                // two steps of four entries (one per 16-lane group) in flight,
+                // each lane's NQ pieces of its entry's operator row loaded
+                // together
+                const int cnt = (int)(ke - pb < 64 ? ke - pb : 64);
+                const int64_t myc = lane < cnt ? colidx[pb + lane] : 0;
+                for (int q = 0; q < cnt; q += 8) {
+                    V gv[2][NQ];
 #pragma unroll
-                        for (int e = 0; e < VEC; ++e) p = fma(tile[r][c + e], vget<T>(gv, e), p);
+                    for (int h = 0; h < 2; ++h) {
+                        const int64_t c = __shfl(myc, q + 4 * h + grp);  // entries past cnt: column 0
+                        const T *grow = G + c * d + j0;
+#pragma unroll
+                        for (int u = 0; u < NQ; ++u) {
+                            const int cc = gl * VEC + u * GS;
+                            if (cc < dc) gv[h][u] = *(const V *)(grow + cc);
+                        }
                     }
-                }
 #pragma unroll
-                for (int o = 8; o >= 1; o >>= 1) p += __shfl_xor(p, o);
-                if (on && gl == 0) {
-                    const T tot = j0 == 0 ? p : dvals[k] + p;
-                    dvals[k] = last ? alpha * tot : tot;
+                    for (int h = 0; h < 2; ++h) {
+                        T p = (T)0;
+#pragma unroll
+                        for (int u = 0; u < NQ; ++u) {
+                            const int cc = gl * VEC + u * GS;
+                            if (cc < dc)
+#pragma unroll
+                                for (int e = 0; e < VEC; ++e) p = fma(tile[r][cc + e], vget<T>(gv[h][u], e), p);
+                        }
+#pragma unroll
+                        for (int o = 8; o >= 1; o >>= 1) p += __shfl_xor(p, o);
+                        const int e = q + 4 * h + grp;
+                        if (e < cnt && gl == 0) {
+                            const int64_t k = pb + e;
+                            const T tot = j0 == 0 ? p : dvals[k] + p;
+                            dvals[k] = last ? alpha * tot : tot;
+                        }
+                    }
                 }
             }
         }

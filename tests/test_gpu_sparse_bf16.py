@@ -103,7 +103,7 @@ def test_error_vs_fp64_same_operator(rb, orc, terms, tol):
 
 @pytest.mark.parametrize("layout", ["C", "R"])
 @pytest.mark.parametrize("opS", ["N", "T"])
-@pytest.mark.parametrize("d", [37, 256, 300])
+@pytest.mark.parametrize("d", [37, 128, 256, 300, 512])
 def test_variants(rb, orc, layout, opS, d):
     got, ref = _case(rb, orc, layout, opS, 150, d, 500, 3, alpha=0.75, beta=-0.5, ro=2, co=3, seed=d)
     assert rel_fro(got, ref) <= 1e-6
