@@ -1,6 +1,7 @@
 // rbx_core.hip -- error slot, RNG-state helpers, raw Philox streams and the
 // materialising fill_dense kernel.
 #include "rbx_internal.hh"
+#include "rbx_fastgen.hh"
 
 #include <math.h>
 #include <string.h>
@@ -262,14 +263,15 @@ __global__ __launch_bounds__(256) void fill_dense_vec_kernel(GenMap g, T *buff, 
     const bool gauss = g.gauss != 0;
     const int64_t cell = g.b_off / 4 + c;
     for (int64_t v = blockIdx.y; v < n_a; v += gridDim.y) {
-        const float4 w = gen4(gauss, ctr_add(g.ctr, (uint64_t)((g.a_off + v) * g.P + cell)), g.k0, g.k1);
+        const Ctr128 ctr = ctr_add(g.ctr, (uint64_t)((g.a_off + v) * g.P + cell));
         T *dst = buff + v * stride_a + 4 * c;
         if constexpr (sizeof(T) == 2) {
-            uint2 pk;
-            pk.x = (uint32_t)entry_value<uint16_t>(w.x, gauss) | ((uint32_t)entry_value<uint16_t>(w.y, gauss) << 16);
-            pk.y = (uint32_t)entry_value<uint16_t>(w.z, gauss) | ((uint32_t)entry_value<uint16_t>(w.w, gauss) << 16);
-            *(uint2 *)dst = pk;
-        } else if constexpr (sizeof(T) == 4) {
+            // bfloat16 entries: the fast generator (exact by construction)
+            *(uint2 *)dst = gauss ? gen4_bf16<true>(ctr, g.k0, g.k1) : gen4_bf16<false>(ctr, g.k0, g.k1);
+            continue;
+        }
+        const float4 w = gen4(gauss, ctr, g.k0, g.k1);
+        if constexpr (sizeof(T) == 4) {
             *(float4 *)dst = make_float4(entry_value<float>(w.x, gauss), entry_value<float>(w.y, gauss),
                                          entry_value<float>(w.z, gauss), entry_value<float>(w.w, gauss));
         } else {
@@ -300,9 +302,19 @@ __global__ __launch_bounds__(256) void fill_dense_tr_kernel(GenMap g, int64_t n_
         const int va = e / FT_C, cl = e % FT_C;  // neighbouring threads: neighbouring counters
         const int64_t v = v0 + va, cell = c0 + cl;
         if (v < n_a && cell < cell_first + n_cells) {
-            const float4 w = gen4(g.gauss != 0, ctr_add(g.ctr, (uint64_t)((g.a_off + v) * g.P + cell)), g.k0, g.k1);
+            const Ctr128 ctr = ctr_add(g.ctr, (uint64_t)((g.a_off + v) * g.P + cell));
+            if constexpr (sizeof(T) == 2) {
+                // bfloat16 entries: the fast generator (exact by construction)
+                const uint2 pk = g.gauss != 0 ? gen4_bf16<true>(ctr, g.k0, g.k1) : gen4_bf16<false>(ctr, g.k0, g.k1);
+                tile[4 * cl][va] = (T)(pk.x & 0xffffu);
+                tile[4 * cl + 1][va] = (T)(pk.x >> 16);
+                tile[4 * cl + 2][va] = (T)(pk.y & 0xffffu);
+                tile[4 * cl + 3][va] = (T)(pk.y >> 16);
+            } else {
+                const float4 w = gen4(g.gauss != 0, ctr, g.k0, g.k1);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) tile[4 * cl + t][va] = entry_value<T>(f4_get(w, t), g.gauss != 0);
+                for (int t = 0; t < 4; ++t) tile[4 * cl + t][va] = entry_value<T>(f4_get(w, t), g.gauss != 0);
+            }
         }
     }
     __syncthreads();

@@ -385,21 +385,21 @@ __global__ __launch_bounds__(64 * NW) void csr_rows_vec_kernel(const int64_t *ro
                         for (int e = 0; e < VEC; ++e) acc[e] = acc[e] + v * vget<T>(g, e);
                     }
                 };
-                // groups of 8 gathers in flight, the last group padded (its
+                // groups of 4 gathers in flight, the last group padded (its
                 // entries past cnt gather row 0 of G, a valid address, and are
                 // not applied; cnt is uniform, so is that test): no
                 // one-at-a-time tail paying a full memory latency per entry
-                for (int q = 0; q < cnt; q += 8) {
-                    V g[8];
-                    T v[8];
+                for (int q = 0; q < cnt; q += 4) {
+                    V g[4];
+                    T v[4];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
+                    for (int u = 0; u < 4; ++u) {
                         const int64_t c = __shfl(myc, q + u);  // lanes past cnt hold column 0
                         v[u] = __shfl(myv, q + u);
                         if (cl < ncols_out) g[u] = *(const V *)(G + c * ldg + cl);
                     }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u)
+                    for (int u = 0; u < 4; ++u)
                         if (q + u < cnt) apply(v[u], g[u]);
                 }
             }

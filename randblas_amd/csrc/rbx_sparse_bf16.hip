@@ -60,21 +60,21 @@ __global__ __launch_bounds__(64 * SB_NW) void csr_rows_bf16_kernel(const int64_t
             const int cnt = (int)(p1 - pb < 64 ? p1 - pb : 64);
             const int64_t myc = lane < cnt ? colidx[pb + lane] : 0;
             const float myv = lane < cnt ? split_value<TERMS>(vals[pb + lane]) : 0.0f;
-            // groups of 8 gathers in flight, the last group padded (its
+            // groups of 4 gathers in flight, the last group padded (its
             // entries past cnt gather row 0 of G, a valid address, and are
             // not applied; cnt is uniform, so is that test): no one-at-a-time
             // tail paying a full memory latency per entry
-            for (int q = 0; q < cnt; q += 8) {
-                uint2 g[8];
-                float a[8];
+            for (int q = 0; q < cnt; q += 4) {
+                uint2 g[4];
+                float a[4];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < 4; ++u) {
                     const int64_t c = __shfl(myc, q + u);  // lanes past cnt hold column 0
                     a[u] = __shfl(myv, q + u);
                     g[u] = lane_on ? *(const uint2 *)(G + c * ldg + cl) : make_uint2(0u, 0u);
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
+                for (int u = 0; u < 4; ++u)
                     if (q + u < cnt) {
                         acc[0] = __fmaf_rn(a[u], __uint_as_float(g[u].x << 16), acc[0]);
                         acc[1] = __fmaf_rn(a[u], __uint_as_float(g[u].x & 0xffff0000u), acc[1]);
