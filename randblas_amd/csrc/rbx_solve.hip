@@ -466,16 +466,22 @@ static void factor_and_solve(char layout, int64_t d, int64_t m, int64_t n, const
     // it; column-major [A b] in one pass over the operator where the wide
     // kernels take the shape, else two sketches
     const int64_t ldsa = layout == 'C' ? d : n;
-    T *SbT = w.tmpT + d * n;
-    if (layout != 'C' || !left_sketch_with_column<T>('N', d, n, m, S, ro_s, co_s, A, lda, b, w.tmpT, d, SbT, st)) {
-        inner_ok(lsk<T>(layout, d, n, m, S, ro_s, co_s, A, lda, w.tmpT, ldsa, st));
-        inner_ok(lsk<T>('C', d, 1, m, S, ro_s, co_s, b, m, SbT, d, st));
+    // FP64 column-major sketches land in SA / Sb directly; others go through
+    // tmpT and are promoted
+    const bool direct = sizeof(T) == 8 && layout == 'C';
+    T *SAo = direct ? (T *)w.SA : w.tmpT;
+    T *Sbo = direct ? (T *)w.Sb : w.tmpT + d * n;
+    if (layout != 'C' || !left_sketch_with_column<T>('N', d, n, m, S, ro_s, co_s, A, lda, b, SAo, d, Sbo, st)) {
+        inner_ok(lsk<T>(layout, d, n, m, S, ro_s, co_s, A, lda, SAo, ldsa, st));
+        inner_ok(lsk<T>('C', d, 1, m, S, ro_s, co_s, b, m, Sbo, d, st));
     }
-    hipLaunchKernelGGL((to_f64_kernel<T>), dim3(grid_of(d * n)), dim3(256), 0, st, (const T *)w.tmpT,
-                       layout == 'C' ? (int64_t)1 : ldsa, layout == 'C' ? ldsa : (int64_t)1, d, n, w.SA);
-    hipLaunchKernelGGL((to_f64_kernel<T>), dim3(grid_of(d)), dim3(256), 0, st, (const T *)SbT, (int64_t)1, d, d,
-                       (int64_t)1, w.Sb);
-    RBX_HIP(hipGetLastError());
+    if (!direct) {
+        hipLaunchKernelGGL((to_f64_kernel<T>), dim3(grid_of(d * n)), dim3(256), 0, st, (const T *)SAo,
+                           layout == 'C' ? (int64_t)1 : ldsa, layout == 'C' ? ldsa : (int64_t)1, d, n, w.SA);
+        hipLaunchKernelGGL((to_f64_kernel<T>), dim3(grid_of(d)), dim3(256), 0, st, (const T *)Sbo, (int64_t)1, d, d,
+                           (int64_t)1, w.Sb);
+        RBX_HIP(hipGetLastError());
+    }
     RBX_HIP(hipMemcpyAsync(w.QR, w.SA, (size_t)(d * n) * 8, hipMemcpyDeviceToDevice, st));
     rocblas_handle h = blas_handle(st);
     const char *qr_env = std::getenv("RBX_SOLVE_QR");
