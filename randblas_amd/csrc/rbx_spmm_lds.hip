@@ -464,9 +464,19 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
                                                            int64_t d, int64_t n, int64_t m, T alpha, T beta,
                                                            const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs,
                                                            int RW, int ncb) {
-    // chunk tiles column by column: As[buf][jj * LA_CKMAX + c] (conflict-free
-    // stores; a stored entry reads its J values at stride LA_CKMAX)
+    // Chunk tiles.  Rows of 16 bytes or more (J * sizeof(T) >= 16) are stored
+    // row by row, As[buf][c * J + ...], in 16-byte pieces whose order within
+    // the row is XOR-swizzled by c, so a stored entry reads its J values with
+    // J * sizeof(T) / 16 wide reads and the reads of 64 lanes at unrelated c
+    // spread over all bank groups (stores of consecutive c stay conflict-free).
+    // Narrower rows are stored column by column, As[buf][jj * LA_CKMAX + c].
     __shared__ __attribute__((aligned(16))) T As[2][LA_CKMAX * J];
+    constexpr bool ROWMAJ = J * (int)sizeof(T) >= 16;
+    constexpr int EPP = 16 / (int)sizeof(T);                        // elements per 16-byte piece
+    constexpr int NPC = ROWMAJ ? J / EPP : 1;                       // pieces per row
+    constexpr int RSH = ROWMAJ ? (J * (int)sizeof(T) == 64 ? 1 : J * (int)sizeof(T) == 32 ? 2 : 3) : 0;
+    typedef T piece_t __attribute__((ext_vector_type(16 / sizeof(T))));
+    auto slot_of = [](int c, int t) { return t ^ ((c >> RSH) & (NPC - 1)); };
     constexpr int PER = LA_CKMAX / NT;  // contraction indices a thread loads per chunk, at most
     const int tid = threadIdx.x;
     // XCD-aware order over a grid padded to a multiple of 8: tile tt runs on
@@ -509,12 +519,23 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
             bv[q] = Ev[o];
         }
     };
-    auto apply_one = [&](const T *bb, T v) {
+    auto fma_one = [&](int jj, T b, T v) {
+        if (FORM == SPF_KIB) acc[jj] = acc[jj] + (alpha * v) * b;
+        else acc[jj] = acc[jj] + v * (alpha * b);
+    };
+    // the entry at chunk-local column c with value v
+    auto apply_one = [&](const T *Ab, int32_t c, T v) {
+        if constexpr (ROWMAJ) {
+            piece_t pc[NPC];
 #pragma unroll
-        for (int jj = 0; jj < J; ++jj) {
-            const T b = bb[jj * LA_CKMAX];
-            if (FORM == SPF_KIB) acc[jj] = acc[jj] + (alpha * v) * b;
-            else acc[jj] = acc[jj] + v * (alpha * b);
+            for (int t = 0; t < NPC; ++t) pc[t] = *(const piece_t *)(Ab + c * J + slot_of(c, t) * EPP);
+#pragma unroll
+            for (int t = 0; t < NPC; ++t)
+#pragma unroll
+                for (int e = 0; e < EPP; ++e) fma_one(t * EPP + e, pc[t][e], v);
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) fma_one(jj, Ab[jj * LA_CKMAX + c], v);
         }
     };
     auto apply = [&](int64_t k, const int32_t *bc, const T *bv, int32_t s, int32_t e, int32_t wmax) {
@@ -523,8 +544,8 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
         const int32_t cnt = e - s;
 #pragma unroll
         for (int q = 0; q < LA_MAXE; ++q)
-            if (q < wmax && q < cnt) apply_one(Ab + (bc[q] - cbase), bv[q]);
-        for (int32_t p = s + LA_MAXE; p < e; ++p) apply_one(Ab + (col[p] - cbase), val[p]);
+            if (q < wmax && q < cnt) apply_one(Ab, bc[q] - cbase, bv[q]);
+        for (int32_t p = s + LA_MAXE; p < e; ++p) apply_one(Ab, col[p] - cbase, val[p]);
     };
 
     // one buffer descriptor per output column: m entries, none past n
@@ -548,8 +569,19 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
 #pragma unroll
         for (int u = 0; u < PER; ++u)
             if (u * NT + tid < CK) {
+                const int c = u * NT + tid;
+                if constexpr (ROWMAJ) {
 #pragma unroll
-                for (int jj = 0; jj < J; ++jj) As[buf][jj * LA_CKMAX + u * NT + tid] = ld[u][jj];
+                    for (int t = 0; t < NPC; ++t) {
+                        piece_t pc;
+#pragma unroll
+                        for (int e = 0; e < EPP; ++e) pc[e] = ld[u][t * EPP + e];
+                        *(piece_t *)(&As[buf][c * J + slot_of(c, t) * EPP]) = pc;
+                    }
+                } else {
+#pragma unroll
+                    for (int jj = 0; jj < J; ++jj) As[buf][jj * LA_CKMAX + c] = ld[u][jj];
+                }
             }
     };
 
