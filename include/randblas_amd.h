@@ -97,6 +97,12 @@ typedef struct {
     int64_t *rows;
     int64_t *cols;
     int own_memory; /* 1: arrays were allocated by rbx_[sd]fill_sparse_skop */
+    /* Library-owned prepared copy of the sampled operator (its row-sorted CSR
+     * and the LDS apply's entry layout), or NULL.  No reference counterpart.
+     * Set by rbx_[sd]fill_sparse_skop and rbx_[sd]sparse_skop_prepare, freed by
+     * rbx_sparse_skop_free; initialise the struct with rbx_sparse_skop_init (or
+     * zero it) so that this field starts NULL. */
+    void *prep;
 } rbx_sparse_skop;
 
 /* CSRMatrix<T, int64_t> (RandBLAS/sparse_data/csr_matrix.hh:73-245), zero-based,
@@ -296,11 +302,28 @@ rbx_status rbx_dfill_sparse_unpacked_nosub(const rbx_sparse_dist *D, int64_t *nn
                                            int64_t *rows, int64_t *cols,
                                            const rbx_rng_state *seed, rbx_rng_state *next_state,
                                            void *stream);
-/* fill_sparse(S) (sparse_skops.hh:587-603): allocates the arrays if NULL. */
+/* fill_sparse(S) (sparse_skops.hh:587-603): allocates the arrays if NULL, and
+ * rebuilds S->prep for the new entries (rbx_[sd]sparse_skop_prepare). */
 rbx_status rbx_sfill_sparse_skop(rbx_sparse_skop *S, void *stream);
 rbx_status rbx_dfill_sparse_skop(rbx_sparse_skop *S, void *stream);
-/* ~SparseSkOp (sparse_skops.hh:442-449) */
+/* ~SparseSkOp (sparse_skops.hh:442-449); also frees S->prep (for operators
+ * whose arrays the caller owns, only S->prep). */
 rbx_status rbx_sparse_skop_free(rbx_sparse_skop *S);
+/* No reference counterpart (the reference re-sorts the operator's COO view in
+ * every call, coo_spmm_impl.hh:69-91).  For callers that apply one sampled
+ * operator many times: builds S->prep from S's current arrays (replacing any
+ * earlier copy), so that later lskges calls over the whole operator
+ * (ro_s = co_s = 0, d x m = the operator's shape, opS = 'N') with op(A) of unit
+ * stride along the contraction skip the per-call sort; results are unchanged
+ * bit for bit.  Leaves S->prep NULL, and returns RBX_OK, when the operator is
+ * outside that path (more than 8192 rows, rows of more than 2048 entries on
+ * average, a copy above four times the operator's own bytes) or the memory
+ * cannot be had.  The copy is complete when `stream` reaches this call; it
+ * stays valid until S's arrays change, which callers that write them in place
+ * signal by calling this again or rbx_sparse_skop_free.  rbx_[sd]fill_sparse_skop
+ * calls it itself. */
+rbx_status rbx_ssparse_skop_prepare(rbx_sparse_skop *S, void *stream);
+rbx_status rbx_dsparse_skop_prepare(rbx_sparse_skop *S, void *stream);
 
 /* ---------------------------------------------------------------- sparse sketches
  * lskges (skge.hh:465-492) and rskges (skge.hh:598-626): as lskge3/rskge3 with a

@@ -90,7 +90,7 @@ class c_sparse_dist(C.Structure):
 class c_sparse_skop(C.Structure):
     _fields_ = [("dist", c_sparse_dist), ("seed_state", c_rng_state), ("next_state", c_rng_state),
                 ("nnz", C.c_int64), ("vals", C.c_void_p), ("rows", C.c_void_p),
-                ("cols", C.c_void_p), ("own_memory", C.c_int)]
+                ("cols", C.c_void_p), ("own_memory", C.c_int), ("prep", C.c_void_p)]
 
 
 class c_csr_matrix(C.Structure):
@@ -145,6 +145,8 @@ _SIG = {
     "rbx_sfill_sparse_skop": [_SS, _P],
     "rbx_dfill_sparse_skop": [_SS, _P],
     "rbx_sparse_skop_free": [_SS],
+    "rbx_ssparse_skop_prepare": [_SS, _P],
+    "rbx_dsparse_skop_prepare": [_SS, _P],
     "rbx_slskges": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _SS, _I64, _I64, _P, _I64, _F, _P, _I64, _P],
     "rbx_dlskges": [_CH, _CH, _CH, _I64, _I64, _I64, _D, _SS, _I64, _I64, _P, _I64, _D, _P, _I64, _P],
     "rbx_srskges": [_CH, _CH, _CH, _I64, _I64, _I64, _F, _P, _I64, _SS, _I64, _I64, _F, _P, _I64, _P],
@@ -556,6 +558,42 @@ class SparseSkOp:
         self.nnz = -1
         self.dtype = dtype or torch.float64
         self.vals = self.rows = self.cols = None
+        self._prep = None  # the library's prepared copy of the sampled entries
+
+    def prepare(self) -> None:
+        """Rebuild the library's prepared copy of the entries (rbx_[sd]sparse_skop_prepare):
+        lskges over the whole operator then skips its per-call sort.  fill_sparse
+        calls it; call it again after writing vals / rows / cols in place."""
+        import torch
+        self.release()
+        s = self.c()
+        pre = "d" if self.dtype == torch.float64 else "s"
+        try:
+            _call(f"rbx_{pre}sparse_skop_prepare", C.byref(s), _stream())
+        finally:
+            self._prep = s.prep
+            self._prep_key = self._key()
+
+    def release(self) -> None:
+        """Free the prepared copy (rbx_sparse_skop_free on arrays this object does not own)."""
+        prep = getattr(self, "_prep", None)
+        if prep:
+            s = self.c()
+            s.own_memory = 0
+            s.prep = prep
+            self._prep = None
+            lib().rbx_sparse_skop_free(C.byref(s))
+
+    def _key(self):
+        return (self.nnz, None if self.vals is None else self.vals.data_ptr(),
+                None if self.rows is None else self.rows.data_ptr(),
+                None if self.cols is None else self.cols.data_ptr())
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
     def c(self) -> c_sparse_skop:
         s = c_sparse_skop()
@@ -567,6 +605,9 @@ class SparseSkOp:
         s.rows = None if self.rows is None else self.rows.data_ptr()
         s.cols = None if self.cols is None else self.cols.data_ptr()
         s.own_memory = 0
+        # the prepared copy goes along only while the arrays are the ones it was built from
+        prep = getattr(self, "_prep", None)
+        s.prep = prep if prep and getattr(self, "_prep_key", None) == self._key() else None
         return s
 
 
@@ -584,9 +625,11 @@ def fill_sparse(S: SparseSkOp) -> None:
     _need("S.vals", S.vals, S.dtype, full)
     _need("S.rows", S.rows, torch.int64, full)
     _need("S.cols", S.cols, torch.int64, full)
+    S.release()
     _call(f"rbx_{pre}fill_sparse_unpacked_nosub", C.byref(S.dist.c()), C.byref(nnz), _ptr(S.vals), _ptr(S.rows),
           _ptr(S.cols), C.byref(S.seed_state.c()), C.byref(nxt), _stream())
     S.nnz = nnz.value
+    S.prepare()
 
 
 def fill_sparse_unpacked_nosub(D: SparseDist, vals, rows, cols, seed: RNGState):

@@ -358,6 +358,8 @@ struct SparseSkOp {
     T *vals;
     sint_t *rows;
     sint_t *cols;
+    // the library's prepared copy of the sampled entries (rbx_sparse_skop::prep)
+    void *prep = nullptr;
 
     SparseSkOp(SparseDist dist, const state_t &seed_state)
         : dist(dist), seed_state(seed_state), next_state(compute_next_state(dist, seed_state)),
@@ -371,19 +373,21 @@ struct SparseSkOp {
 
     SparseSkOp(SparseSkOp &&S)
         : dist(S.dist), seed_state(S.seed_state), next_state(S.next_state), n_rows(S.n_rows), n_cols(S.n_cols),
-          own_memory(S.own_memory), nnz(S.nnz), vals(S.vals), rows(S.rows), cols(S.cols) {
+          own_memory(S.own_memory), nnz(S.nnz), vals(S.vals), rows(S.rows), cols(S.cols), prep(S.prep) {
         S.vals = nullptr;
         S.rows = nullptr;
         S.cols = nullptr;
         S.nnz = -1;
+        S.prep = nullptr;
     }
     SparseSkOp(const SparseSkOp &) = delete;
     SparseSkOp &operator=(const SparseSkOp &) = delete;
 
     ~SparseSkOp() {
-        if (own_memory && (vals != nullptr || rows != nullptr || cols != nullptr)) {
+        const bool arrays = own_memory && (vals != nullptr || rows != nullptr || cols != nullptr);
+        if (arrays || prep != nullptr) {
             rbx_sparse_skop c = this->c();
-            c.own_memory = 1;
+            c.own_memory = arrays ? 1 : 0;
             (void)rbx_sparse_skop_free(&c);
         }
     }
@@ -398,6 +402,7 @@ struct SparseSkOp {
         s.rows = rows;
         s.cols = cols;
         s.own_memory = 0;
+        s.prep = prep;
         return s;
     }
 };
@@ -430,6 +435,7 @@ void fill_sparse(SparseSkOp<T, RNG, sint_t> &S) {
         st = rbx_sfill_sparse_skop(&c, gpu::stream());
     else
         st = rbx_dfill_sparse_skop(&c, gpu::stream());
+    S.prep = c.prep;  // the fill replaced the prepared copy, also when it failed
     detail::check(st, "fill_sparse");
     S.vals = (T *)c.vals;
     S.rows = c.rows;

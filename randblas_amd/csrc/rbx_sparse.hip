@@ -773,13 +773,38 @@ static void launch_rows_vec(bool colmaj, const int64_t *rowptr, const IDX *colid
 // better served by the row kernels' segment sums.
 constexpr int64_t LDS_APPLY_MAX_MEAN_ROW = 2048;
 
+// A sampled operator's prepared copy (rbx_sparse_skop::prep): its entries as
+// a CSR matrix sorted by (row, column, stored position), as left_spmm_coo
+// builds it per call, and the LDS apply's entry layout over it.  `vals`,
+// `rows`, `cols`, `nnz` and the device identify the arrays it was built from.
+struct SparsePrep {
+    uint64_t magic;
+    int dev;
+    size_t val_size;
+    int64_t d, m, nnz;
+    const void *vals;
+    const int64_t *rows, *cols;
+    char *mem;  // one allocation: row_ptr, col, val, then the layout
+    int64_t *row_ptr;
+    int32_t *col;
+    void *val;
+    char *lay;
+    size_t lay_bytes;
+};
+constexpr uint64_t PREP_MAGIC = 0x70657270787862ull;
+// the copy may take at most this many times the operator's own bytes
+constexpr size_t PREP_MAX_RATIO = 4;
+
 // left_spmm for a COO operator (spmm_dispatch.hh:52-179):
-// C = alpha * op(submat(A)) * op(B) + beta * C, A given by COO arrays.
+// C = alpha * op(submat(A)) * op(B) + beta * C, A given by COO arrays.  `prep`,
+// when given, is the prepared copy of exactly these entries (the whole
+// operator, opA = 'N'); the LDS apply then reads it instead of sorting.
 template <typename T>
 static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n, int64_t m, T alpha, int64_t A_rows,
                           int64_t A_cols, int64_t nnz, const T *vals, const int64_t *rows, const int64_t *cols,
                           int64_t ro_a, int64_t co_a, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
-                          hipStream_t st, int kib_force = -1, char *fixed_ws = nullptr, size_t fixed_ws_bytes = 0) {
+                          hipStream_t st, int kib_force = -1, char *fixed_ws = nullptr, size_t fixed_ws_bytes = 0,
+                          const SparsePrep *prep = nullptr) {
     check_layout(layout);
     check_op(opA);
     check_op(opB);
@@ -814,6 +839,13 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
     // the reference's own layout selects
     const int kib = kib_force >= 0 ? kib_force : ((layout_opB == 'R' && layout == 'R') ? 1 : 0);
     safe_mul(d + 1, m);
+    if (prep != nullptr && layout_opB == 'C') {
+        RBX_REQUIRE(opA == 'N' && ro_a == 0 && co_a == 0 && d == prep->d && m == prep->m && nnz == prep->nnz);
+        ProfScope prof("csr_lds_apply_kernel", st);
+        if (csr_lds_apply_laid<T>(prep->row_ptr, prep->col, (const T *)prep->val, d, n, m, nnz, alpha, beta, B, ldb, C,
+                                  crs, ccs, kib ? SPF_KIB : SPF_JKI, prep->lay, prep->lay_bytes, st))
+            return;
+    }
 
     // the window's entries as a CSR matrix sorted by (row, column), in scratch
     int64_t *row_ptr;
@@ -917,13 +949,87 @@ static SkopView<T> view_or_sample(const rbx_sparse_skop *S, hipStream_t st) {
     return {nnz, vals, rows, cols};
 }
 
+// S's prepared copy when it was built from S's current arrays in T on this
+// device and the call covers the whole operator with opS = 'N'; else null.
+template <typename T>
+static const SparsePrep *usable_prep(const rbx_sparse_skop *S, char opS, int64_t d, int64_t m, int64_t ro_s,
+                                     int64_t co_s) {
+    const SparsePrep *p = (const SparsePrep *)S->prep;
+    if (p == nullptr || p->magic != PREP_MAGIC || p->val_size != sizeof(T)) return nullptr;
+    if (opS != 'N' || ro_s != 0 || co_s != 0 || d != S->dist.n_rows || m != S->dist.n_cols) return nullptr;
+    if (S->nnz != p->nnz || S->vals != p->vals || S->rows != p->rows || S->cols != p->cols) return nullptr;
+    int dev = -1;
+    RBX_HIP(hipGetDevice(&dev));
+    return dev == p->dev ? p : nullptr;
+}
+
+static void prep_free(rbx_sparse_skop *S) {
+    SparsePrep *p = (SparsePrep *)S->prep;
+    S->prep = nullptr;
+    if (p == nullptr || p->magic != PREP_MAGIC) return;
+    p->magic = 0;
+    // hipFree waits for the work queued on the device, so no queued apply
+    // still reads the copy
+    if (p->mem != nullptr) (void)hipFree(p->mem);
+    delete p;
+}
+
+// rbx_[sd]sparse_skop_prepare: S->prep for S's current arrays, or null when
+// the operator is outside the LDS apply or the memory cannot be had.
+template <typename T>
+static void prepare_skop(rbx_sparse_skop *S, hipStream_t st) {
+    prep_free(S);
+    if (S->nnz <= 0 || S->vals == nullptr || S->rows == nullptr || S->cols == nullptr) return;
+    const int64_t d = S->dist.n_rows, m = S->dist.n_cols, nnz = S->nnz;
+    if (d > CSRB_MAX_ROWS || nnz > LDS_APPLY_MAX_MEAN_ROW * d || m >= (int64_t)INT32_MAX) return;
+    const size_t lay = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T));
+    const size_t bR = align256((size_t)(d + 1) * 8), bC = align256((size_t)nnz * 4);
+    const size_t bV = align256((size_t)nnz * sizeof(T));
+    const size_t total = bR + bC + bV + lay;
+    if (total > PREP_MAX_RATIO * (size_t)nnz * (sizeof(T) + 16)) return;
+    int64_t *row_ptr;
+    int32_t *ccol;
+    T *cval;
+    if (!csr_build_counting<T>(S->rows, S->cols, (const T *)S->vals, nnz, 0, 0, d, m, &row_ptr, &ccol, &cval, st))
+        return;
+    int dev = -1;
+    RBX_HIP(hipGetDevice(&dev));
+    char *mem = nullptr;
+    if (hipMalloc((void **)&mem, total) != hipSuccess) {
+        (void)hipGetLastError();
+        return;  // no copy: every lskges sorts for itself
+    }
+    SparsePrep *p = new SparsePrep();
+    p->magic = PREP_MAGIC;
+    p->dev = dev;
+    p->val_size = sizeof(T);
+    p->d = d;
+    p->m = m;
+    p->nnz = nnz;
+    p->vals = S->vals;
+    p->rows = S->rows;
+    p->cols = S->cols;
+    p->mem = mem;
+    p->row_ptr = (int64_t *)mem;
+    p->col = (int32_t *)(mem + bR);
+    p->val = mem + bR + bC;
+    p->lay = mem + bR + bC + bV;
+    p->lay_bytes = lay;
+    S->prep = p;  // owned by S from here, so prep_free releases it on any failure below
+    RBX_HIP(hipMemcpyAsync(p->row_ptr, row_ptr, (size_t)(d + 1) * 8, hipMemcpyDeviceToDevice, st));
+    RBX_HIP(hipMemcpyAsync(p->col, ccol, (size_t)nnz * 4, hipMemcpyDeviceToDevice, st));
+    RBX_HIP(hipMemcpyAsync(p->val, cval, (size_t)nnz * sizeof(T), hipMemcpyDeviceToDevice, st));
+    if (!csr_lds_layout<T>(p->row_ptr, p->col, (const T *)p->val, d, m, nnz, p->lay, lay, st)) prep_free(S);
+}
+
 template <typename T>
 static void lskges_impl(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m, T alpha,
                         const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s, const T *A, int64_t lda, T beta, T *B,
                         int64_t ldb, hipStream_t st) {
     SkopView<T> v = view_or_sample<T>(S, st);
+    const SparsePrep *prep = S->nnz >= 0 ? usable_prep<T>(S, opS, d, m, ro_s, co_s) : nullptr;
     left_spmm_coo<T>(layout, opS, opA, d, n, m, alpha, S->dist.n_rows, S->dist.n_cols, v.nnz, v.vals, v.rows, v.cols,
-                     ro_s, co_s, A, lda, beta, B, ldb, st);
+                     ro_s, co_s, A, lda, beta, B, ldb, st, -1, nullptr, 0, prep);
 }
 
 // rskges -> right_spmm -> left_spmm with flipped layout and op (spmm_dispatch.hh:181-219)
@@ -1350,7 +1456,9 @@ static void fill_sparse_skop(rbx_sparse_skop *S, hipStream_t st) {
         S->own_memory = 1;
     }
     RBX_REQUIRE(S->vals != nullptr && S->rows != nullptr && S->cols != nullptr);
+    prep_free(S);
     S->nnz = fill_sparse_impl<T>(S->dist, (T *)S->vals, S->rows, S->cols, S->seed_state, nullptr, st);
+    prepare_skop<T>(S, st);
 }
 
 // ---------------------------------------------------------------- allocation-free lskges
@@ -1393,7 +1501,8 @@ static void lskges_noalloc_impl(char layout, char opS, char opA, int64_t d, int6
     char *ws = noalloc_scratch_ptr();
     SkopView<T> v = view_or_sample<T>(S, st);
     left_spmm_coo<T>(layout, opS, opA, d, n, m, alpha, S->dist.n_rows, S->dist.n_cols, v.nnz, v.vals, v.rows, v.cols,
-                     ro_s, co_s, A, lda, beta, B, ldb, st, -1, ws, NOALLOC_SCRATCH_BYTES);
+                     ro_s, co_s, A, lda, beta, B, ldb, st, -1, ws, NOALLOC_SCRATCH_BYTES,
+                     usable_prep<T>(S, opS, d, m, ro_s, co_s));
 }
 
 }  // namespace rbx
@@ -1461,9 +1570,24 @@ rbx_status rbx_dfill_sparse_skop(rbx_sparse_skop *S, void *stream) {
     RBX_API_END
 }
 
+rbx_status rbx_ssparse_skop_prepare(rbx_sparse_skop *S, void *stream) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(S != nullptr);
+    prepare_skop<float>(S, (hipStream_t)stream);
+    RBX_API_END
+}
+
+rbx_status rbx_dsparse_skop_prepare(rbx_sparse_skop *S, void *stream) {
+    RBX_API_BEGIN
+    RBX_REQUIRE(S != nullptr);
+    prepare_skop<double>(S, (hipStream_t)stream);
+    RBX_API_END
+}
+
 rbx_status rbx_sparse_skop_free(rbx_sparse_skop *S) {
     RBX_API_BEGIN
     RBX_REQUIRE(S != nullptr);
+    prep_free(S);
     if (S->own_memory) {
         if (S->vals) RBX_HIP(hipFree(S->vals));
         if (S->rows) RBX_HIP(hipFree(S->rows));

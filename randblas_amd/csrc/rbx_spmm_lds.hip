@@ -660,13 +660,39 @@ static void launch_lds_apply_j(int J, int form, dim3 grid, const int32_t *startT
 #undef RBX_LA_ARGS
 }
 
-template <typename T>
-bool csr_lds_apply(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
-                   int64_t nnz, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int form,
-                   char *ws, size_t ws_bytes, hipStream_t st) {
+static bool lds_apply_fits(int64_t d, int64_t m, size_t val_size) {
     // buffer offsets: (m + one chunk) entries must stay below 2^31 bytes
-    if (d <= 0 || n <= 0 || m <= 0 || (m + LA_CKMAX) * (int64_t)sizeof(T) >= (1ll << 31) || ws == nullptr)
-        return false;
+    return d > 0 && m > 0 && (m + LA_CKMAX) * (int64_t)val_size < (1ll << 31);
+}
+
+template <typename T>
+bool csr_lds_layout(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t m, int64_t nnz,
+                    char *ws, size_t ws_bytes, hipStream_t st) {
+    if (!lds_apply_fits(d, m, sizeof(T)) || ws == nullptr) return false;
+    const LaPlan pl = la_plan(d, m, nnz, sizeof(T));
+    RBX_REQUIRE(ws_bytes >= pl.bytes());
+    if (pl.nk >= (1ll << 30)) return false;
+    int32_t *startT = (int32_t *)ws;
+    int32_t *Ec = (int32_t *)(ws + pl.bStart);
+    T *Ev = (T *)(ws + pl.bStart + pl.bEc);
+    const int64_t tcells = d * (pl.nk + 1);
+    hipLaunchKernelGGL(csr_chunk_table_kernel, dim3((unsigned)((tcells + 255) / 256)), dim3(256), 0, st, row_ptr, col,
+                       d, (int)pl.nk, pl.ck, startT);
+    RBX_HIP(hipGetLastError());
+    const int64_t cus = num_cus();
+    const int64_t sg = d < 4 * cus ? d : 4 * cus;
+    hipLaunchKernelGGL((csr_chunk_slots_kernel<T>), dim3((unsigned)sg), dim3(256), 0, st, row_ptr, col, val, d, pl.ck,
+                       (const int32_t *)startT, Ec, Ev);
+    RBX_HIP(hipGetLastError());
+    return true;
+}
+
+template <typename T>
+bool csr_lds_apply_laid(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
+                        int64_t nnz, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs,
+                        int form, const char *ws, size_t ws_bytes, hipStream_t st) {
+    (void)row_ptr;
+    if (n <= 0 || !lds_apply_fits(d, m, sizeof(T)) || ws == nullptr) return false;
     const LaPlan pl = la_plan(d, m, nnz, sizeof(T));
     RBX_REQUIRE(ws_bytes >= pl.bytes());
     // 512 rows per workgroup (256 for few rows), the widest column block
@@ -678,18 +704,10 @@ bool csr_lds_apply(const int64_t *row_ptr, const int32_t *col, const T *val, int
     while (J > 1 && ((n + J - 1) / J) * RW < cus * 3 / 4) J /= 2;
     const int64_t ncb = (n + J - 1) / J;
     if (ncb * RW >= (1ll << 30) || pl.nk >= (1ll << 30)) return false;
-    int32_t *startT = (int32_t *)ws;
-    int32_t *Ec = (int32_t *)(ws + pl.bStart);
-    T *Ev = (T *)(ws + pl.bStart + pl.bEc);
-    const int64_t tcells = d * (pl.nk + 1);
+    const int32_t *startT = (const int32_t *)ws;
+    const int32_t *Ec = (const int32_t *)(ws + pl.bStart);
+    const T *Ev = (const T *)(ws + pl.bStart + pl.bEc);
     const int nki = (int)pl.nk;
-    hipLaunchKernelGGL(csr_chunk_table_kernel, dim3((unsigned)((tcells + 255) / 256)), dim3(256), 0, st, row_ptr, col,
-                       d, nki, pl.ck, startT);
-    RBX_HIP(hipGetLastError());
-    const int64_t sg = d < 4 * cus ? d : 4 * cus;
-    hipLaunchKernelGGL((csr_chunk_slots_kernel<T>), dim3((unsigned)sg), dim3(256), 0, st, row_ptr, col, val, d, pl.ck,
-                       (const int32_t *)startT, Ec, Ev);
-    RBX_HIP(hipGetLastError());
     const dim3 grid((unsigned)((ncb * RW + 7) / 8 * 8));
     const int rw = (int)RW, nc = (int)ncb;
     if (NT == 512)
@@ -699,6 +717,20 @@ bool csr_lds_apply(const int64_t *row_ptr, const int32_t *col, const T *val, int
         launch_lds_apply_j<T, 256>(J, form, grid, startT, Ec, Ev, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
                                    crs, ccs, rw, nc, st);
     return true;
+}
+
+template <typename T>
+bool csr_lds_apply(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
+                   int64_t nnz, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int form,
+                   char *ws, size_t ws_bytes, hipStream_t st) {
+    if (d <= 0 || n <= 0 || m <= 0 || ws == nullptr) return false;
+    // every check of the apply before anything is launched
+    const int NT = d > 256 ? 512 : 256;
+    const int64_t RW = (d + NT - 1) / NT;
+    if (n * RW >= (1ll << 30)) return false;
+    if (!csr_lds_layout<T>(row_ptr, col, val, d, m, nnz, ws, ws_bytes, st)) return false;
+    return csr_lds_apply_laid<T>(row_ptr, col, val, d, n, m, nnz, alpha, beta, B, ldb, C, crs, ccs, form, ws, ws_bytes,
+                                 st);
 }
 
 template bool csr_build_counting_ws<float>(const int64_t *, const int64_t *, const float *, int64_t, int64_t,
@@ -714,6 +746,16 @@ template bool csr_build_counting<double>(const int64_t *, const int64_t *, const
 template bool csr_lds_apply<float>(const int64_t *, const int32_t *, const float *, int64_t, int64_t, int64_t,
                                    int64_t, float, float, const float *, int64_t, float *, int64_t, int64_t, int,
                                    char *, size_t, hipStream_t);
+template bool csr_lds_layout<float>(const int64_t *, const int32_t *, const float *, int64_t, int64_t, int64_t,
+                                    char *, size_t, hipStream_t);
+template bool csr_lds_layout<double>(const int64_t *, const int32_t *, const double *, int64_t, int64_t, int64_t,
+                                     char *, size_t, hipStream_t);
+template bool csr_lds_apply_laid<float>(const int64_t *, const int32_t *, const float *, int64_t, int64_t, int64_t,
+                                        int64_t, float, float, const float *, int64_t, float *, int64_t, int64_t, int,
+                                        const char *, size_t, hipStream_t);
+template bool csr_lds_apply_laid<double>(const int64_t *, const int32_t *, const double *, int64_t, int64_t, int64_t,
+                                         int64_t, double, double, const double *, int64_t, double *, int64_t, int64_t,
+                                         int, const char *, size_t, hipStream_t);
 template bool csr_lds_apply<double>(const int64_t *, const int32_t *, const double *, int64_t, int64_t, int64_t,
                                     int64_t, double, double, const double *, int64_t, double *, int64_t, int64_t, int,
                                     char *, size_t, hipStream_t);

@@ -625,3 +625,67 @@ def test_lskges_shuffled_operator_with_duplicates(rb, orc, dtype, layout):
     orc.left_spmm_coo("d" if dtype == np.float64 else "s", layout, "N", "N", d, n, m, 0.5, (d, m), v, r, c, 0, 0, A,
                       lda, 1.5, ref, ldb)
     assert np.array_equal(host(Bd), ref)
+
+
+def _lsk_run(rb, S, dtype, layout, opS, d, n, m, ro, co, A, lda, B0, ldb, alpha=1.0, beta=0.0, noalloc=False):
+    Bd = torch.from_numpy(B0.copy()).to(DEV)
+    rb.profile_collect()
+    rb.profile_enable(True)
+    f = rb.lskges_noalloc if noalloc else rb.lskges
+    f(layout, opS, "N", d, n, m, alpha, S, ro, co, torch.from_numpy(A).to(DEV), lda, beta, Bd, ldb)
+    torch.cuda.synchronize()
+    rb.profile_enable(False)
+    return host(Bd), rb.profile_collect()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lskges_prepared_operator(rb, orc, dtype):
+    """fill_sparse prepares the SASO operator (row-sorted copy + LDS entry layout); lskges over the
+    whole operator then skips the sort, with results bit-identical to the oracle and to the
+    per-call path; windows and opS = 'T' ignore the copy; an in-place edit followed by prepare()
+    is picked up."""
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    rng = np.random.default_rng(5)
+    d, m, n = 2000, 100000, 40
+    S = rb.SparseSkOp(rb.SparseDist(d, m, 8, rb.Axis.Short), rb.RNGState(7), dtype=tdt)
+    rb.fill_sparse(S)
+    assert S._prep
+    A = rng.uniform(-1, 1, m * n).astype(dtype)
+    B0 = rng.uniform(-1, 1, d * n).astype(dtype)
+    vals, rows, cols, _, _ = orc.fill_sparse(d, m, 8, "S", [0, 0, 0, 0], [7, 0], dtype=dtype)
+    pre = "d" if dtype == np.float64 else "s"
+
+    def oracle(vv, layout, opS, dd, mm, ro, co, alpha, beta, lda, ldb):
+        ref = B0.copy()
+        orc.left_spmm_coo(pre, layout, opS, "N", dd, n, mm, alpha, (d, m), vv, rows, cols, ro, co, A, lda, beta,
+                          ref, ldb)
+        return ref
+
+    got, prof = _lsk_run(rb, S, dtype, "C", "N", d, n, m, 0, 0, A, m, B0, d, 0.75, -0.5)
+    assert "coo_to_csr" not in prof and "csr_lds_apply_kernel" in prof
+    assert np.array_equal(got, oracle(vals, "C", "N", d, m, 0, 0, 0.75, -0.5, m, d))
+    got_na, prof = _lsk_run(rb, S, dtype, "C", "N", d, n, m, 0, 0, A, m, B0, d, 0.75, -0.5, noalloc=True)
+    assert "coo_to_csr" not in prof and np.array_equal(got_na, got)
+    # a window inside the operator: the copy does not apply
+    got, prof = _lsk_run(rb, S, dtype, "C", "N", d - 3, n, m - 5000, 2, 17, A, m, B0, d)
+    assert "coo_to_csr" in prof
+    assert np.array_equal(got, oracle(vals, "C", "N", d - 3, m - 5000, 2, 17, 1.0, 0.0, m, d))
+    # op(S) = S^T over S[500:2000, 0:1000]
+    got, prof = _lsk_run(rb, S, dtype, "C", "T", 1000, n, 1500, 500, 0, A, 1500, B0, 1000)
+    assert "coo_to_csr" in prof
+    assert np.array_equal(got, oracle(vals, "C", "T", 1000, 1500, 500, 0, 1.0, 0.0, 1500, 1000))
+    # in-place edit of the values, then prepare(): the new values are used
+    S.vals.mul_(-3.0)
+    S.prepare()
+    got, prof = _lsk_run(rb, S, dtype, "C", "N", d, n, m, 0, 0, A, m, B0, d)
+    assert "coo_to_csr" not in prof
+    assert np.array_equal(got, oracle((vals * dtype(-3.0)).astype(dtype), "C", "N", d, m, 0, 0, 1.0, 0.0, m, d))
+    # without the copy: the per-call sort gives the same bits
+    S.release()
+    got2, prof = _lsk_run(rb, S, dtype, "C", "N", d, n, m, 0, 0, A, m, B0, d)
+    assert "coo_to_csr" in prof and np.array_equal(got2, got)
+    # replaced arrays: the stale copy is not passed
+    S.prepare()
+    S.vals = S.vals.clone()
+    got3, prof = _lsk_run(rb, S, dtype, "C", "N", d, n, m, 0, 0, A, m, B0, d)
+    assert "coo_to_csr" in prof and np.array_equal(got3, got)

@@ -64,12 +64,20 @@ def test_noalloc_rejects_beyond_its_scratch(rb):
     # 1.6e6 stored entries: the FP64 CSR of the window needs ~45 MB
     S = rb.SparseSkOp(rb.SparseDist(1000, 200000, 8), rb.RNGState(5), dtype=torch.float64)
     rb.fill_sparse(S)
+    S.release()  # without the operator's prepared copy, the call sorts its entries in the scratch
     A = torch.ones(200000 * 4, dtype=torch.float64, device=DEV)
     B = torch.full((1000 * 4,), 3.0, dtype=torch.float64, device=DEV)
     with pytest.raises(rb.Error, match="scratch"):
         rb.lskges_noalloc("C", "N", "N", 1000, 4, 200000, 1.0, S, 0, 0, A, 200000, 0.0, B, 1000)
     torch.cuda.synchronize()
     assert torch.all(B == 3.0)  # nothing launched
+    # with the prepared copy the call needs no scratch, and agrees with lskges
+    S.prepare()
+    rb.lskges_noalloc("C", "N", "N", 1000, 4, 200000, 1.0, S, 0, 0, A, 200000, 0.0, B, 1000)
+    B2 = torch.zeros_like(B)
+    S.release()
+    rb.lskges("C", "N", "N", 1000, 4, 200000, 1.0, S, 0, 0, A, 200000, 0.0, B2, 1000)
+    assert torch.equal(B, B2)
     # a shape the fixed-workspace path does not take
     Bm = torch.zeros(1000 * 4, dtype=torch.float64, device=DEV)
     Sm = rb.SparseSkOp(rb.SparseDist(1000, 5000, 2), rb.RNGState(6), dtype=torch.float64)
