@@ -458,7 +458,8 @@ __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint3
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
 
-template <typename T, int J, int NT, int FORM>
+// A1: alpha == 1, so the products by alpha (exact then) are left out
+template <typename T, int J, int NT, int FORM, bool A1>
 __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT, const int32_t *Ec, const T *Ev,
                                                            int nk, int CK, const int32_t *col, const T *val,
                                                            int64_t d, int64_t n, int64_t m, T alpha, T beta,
@@ -520,15 +521,20 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
         }
     };
     auto fma_one = [&](int jj, T b, T v) {
-        if (FORM == SPF_KIB) acc[jj] = acc[jj] + (alpha * v) * b;
-        else acc[jj] = acc[jj] + v * (alpha * b);
+        if (FORM == SPF_KIB) acc[jj] = acc[jj] + (A1 ? v : alpha * v) * b;
+        else acc[jj] = acc[jj] + v * (A1 ? b : alpha * b);
     };
+    constexpr int LOGROWB = J * (int)sizeof(T) == 64 ? 6 : J * (int)sizeof(T) == 32 ? 5 : 4;
     // the entry at chunk-local column c with value v
     auto apply_one = [&](const T *Ab, int32_t c, T v) {
         if constexpr (ROWMAJ) {
+            // piece t of row c sits at byte c * ROWB + 16 * (t ^ s(c)), which is
+            // (c * ROWB | 16 * s(c)) ^ 16 * t: one XOR per piece
+            const uint32_t a0 = ((uint32_t)c << LOGROWB) | ((((uint32_t)c >> RSH) & (NPC - 1)) << 4);
+            const char *Ac = (const char *)Ab;
             piece_t pc[NPC];
 #pragma unroll
-            for (int t = 0; t < NPC; ++t) pc[t] = *(const piece_t *)(Ab + c * J + slot_of(c, t) * EPP);
+            for (int t = 0; t < NPC; ++t) pc[t] = *(const piece_t *)(Ac + (a0 ^ (uint32_t)(t << 4)));
 #pragma unroll
             for (int t = 0; t < NPC; ++t)
 #pragma unroll
@@ -638,12 +644,18 @@ static void launch_lds_apply(int form, dim3 grid, const int32_t *startT, const i
                              int ck, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m, T alpha,
                              T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int RW, int ncb,
                              hipStream_t st) {
-    if (form == SPF_KIB)
-        hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, NT, SPF_KIB>), grid, dim3(NT), 0, st, startT, Ec, Ev, nk, ck,
-                           col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb);
-    else
-        hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, NT, SPF_JKI>), grid, dim3(NT), 0, st, startT, Ec, Ev, nk, ck,
-                           col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb);
+#define RBX_LA_LAUNCH(F, A1)                                                                                  \
+    hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, NT, F, A1>), grid, dim3(NT), 0, st, startT, Ec, Ev, nk, ck, col, \
+                       val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb)
+    const bool one = alpha == (T)1;
+    if (form == SPF_KIB) {
+        if (one) RBX_LA_LAUNCH(SPF_KIB, true);
+        else RBX_LA_LAUNCH(SPF_KIB, false);
+    } else {
+        if (one) RBX_LA_LAUNCH(SPF_JKI, true);
+        else RBX_LA_LAUNCH(SPF_JKI, false);
+    }
+#undef RBX_LA_LAUNCH
     RBX_HIP(hipGetLastError());
 }
 
