@@ -105,133 +105,204 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 }
 
 // Upper Cholesky A = R^T R of an n x n column-major FP64 matrix, n <= 256,
-// in one 512-thread workgroup (the upper triangle is read and R written over
-// it; the strict lower triangle is untouched).  The upper triangle stays in
-// registers for the whole factorization: 16 x 16 tiles, slot q of wave w
-// holding tile q * 8 + w of the row-major upper triangle, lane l rows
-// l / 16 + 4 r (r = 0..3) of column l % 16 (the v_mfma_f64_16x16x4f64
-// accumulator layout); rows and columns past n are identity padding.  Per
-// 16-wide step s:
-//   1. the owners publish tile (s, s) and the tile row (s, j > s) to LDS;
-//   2. every wave factors tile (s, s) itself, lane l < 16 holding column l
-//      (pivot k read from lane k, 1 / sqrt by the hardware estimate and two
-//      Newton steps, row k's entries broadcast with v_readlane), so no
-//      barrier follows;
-//   3. R^T X = B for the tile row, one thread per column, right-looking;
-//   4. owners reload their tile-row tiles (now R) and update their trailing
-//      tiles, T -= X_a^T X_b, with v_mfma_f64_16x16x4f64.
+// in one 1024-thread workgroup (the upper triangle is read and R written over
+// it; the strict lower triangle is untouched), with 16 x 16 tiles in the
+// v_mfma_f64_16x16x4f64 accumulator layout (lane l holds rows l / 16 + 4 r,
+// r = 0..3, of column l % 16; rows and columns past n are identity padding).
+// Wave 0 keeps the diagonal tiles in LDS and works one step ahead: while the
+// other waves update the trailing tiles of step s, it updates diagonal tile
+// s + 1 and factors it, R_{s+1} and W_{s+1} = R_{s+1}^{-T} (pivot k read from
+// lane k, 1 / sqrt by the hardware estimate and two Newton steps, row k
+// broadcast with v_readlane; then W by substitution).  Waves 1..15 hold the
+// off-diagonal tiles in registers (slot q of wave w: tile q * 15 + w - 1 of
+// the row-major strict upper triangle) and per step s
+//   1. form the tile row X_b = W_s B_b (an MFMA product) and publish it to LDS;
+//   2. update the diagonal tiles past s + 1 in LDS, and their trailing tiles
+//      T_ab -= X_a^T X_b, with v_mfma_f64_16x16x4f64.
 // Global memory is read once and written once; two barriers per step (the
-// tile row's LDS buffer alternates between steps).
+// tile row's LDS buffer and W alternate between steps).
 // info = 0, or k + 1 when pivot k is not positive (R then incomplete).
-constexpr int CH_MAX = 256, CH_THR = 512, CH_WAVES = CH_THR / 64, CH_T = CH_MAX / 16;
-constexpr int CH_SLOTS = (CH_T * (CH_T + 1) / 2 + CH_WAVES - 1) / CH_WAVES;  // 17
-constexpr int CH_PLD = CH_MAX + 16;  // LDS row of the tile row (16 doubles of pad)
+constexpr int CH_MAX = 256, CH_THR = 1024, CH_WAVES = CH_THR / 64, CH_T = CH_MAX / 16;
+constexpr int CH_OFF = CH_T * (CH_T - 1) / 2;                        // 120 off-diagonal tiles
+constexpr int CH_SLOTS = (CH_OFF + CH_WAVES - 2) / (CH_WAVES - 1);  // 8 per off-diagonal wave
+constexpr int CH_PLD = CH_MAX + 16;                                 // LDS row of the tile row (16 doubles of pad)
 typedef double chf64x4 __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(CH_THR) void chol_tiles_kernel(double *A, int n, int *info) {
-    __shared__ double Dg[16][17];            // the step's diagonal tile, from its owner
-    __shared__ double Dw[CH_WAVES][16][17];  // each wave's own R of that tile
-    __shared__ double dinvw[CH_WAVES][16];   // and 1 / R(i, i)
-    __shared__ __attribute__((aligned(16))) double Pn[2][16][CH_PLD];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: slot bookkeeping in scalar registers
-    const int T = (n + 15) / 16, ntiles = T * (T + 1) / 2;
-    const int lr = lane >> 4, lc = lane & 15;
+
+// wave 0: factor the diagonal tile D (LDS, [row][col], upper part used) in
+// place, R over its upper triangle (zeros below), and W = R^{-T} into Wl
+// ([row][k]); returns the bad pivot index + 1, or 0
+__device__ __forceinline__ int chol_factor_tile(double (*D)[17], double (*Wl)[17], int lane) {
+    const int l = lane;
+    int bad = 0;
+    double c[16];  // lane l < 16: column l of the tile
+#pragma unroll
+    for (int i = 0; i < 16; ++i) c[i] = (l < 16 && i <= l) ? D[i][l] : (i == l ? 1.0 : 0.0);
+    double dinv = 1.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const double p = readlane_f64(c[k], k);
+        if (bad == 0 && !(p > 0.0)) bad = k + 1;
+        double y = __builtin_amdgcn_rsq(p);
+        y = fma(0.5 * y, fma(-p, y * y, 1.0), y);
+        y = fma(0.5 * y, fma(-p, y * y, 1.0), y);
+        if (l == k) dinv = y;
+        c[k] = l == k ? p * y : c[k] * y;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i > k) c[i] -= readlane_f64(c[k], i) * c[k];
+    }
+    // column l of R^{-1}: x = R^{-1} e_l, from the bottom (R(i, k) = lane
+    // k's c[i], 1 / R(k, k) = lane k's dinv)
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = i == l ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 15; k >= 0; --k) {
+        x[k] *= readlane_f64(dinv, k);
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i < k) x[i] -= readlane_f64(c[i], k) * x[k];
+    }
+    if (l < 16)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            D[i][l] = i <= l ? c[i] : 0.0;  // R
+            Wl[l][i] = x[i];                // W[l][i] = (R^{-1})[i][l]
+        }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    return bad;
+}
+
+struct CholShared {
+    double Dg[CH_T][16][17];  // the diagonal tiles (wave 0's)
+    double Wl[2][16][17];     // W_s = R_ss^{-T} by step parity
+    double Pn[2][16][CH_PLD];
+    int failed;
+};
+
+// wave 0's part: the diagonal tiles in LDS, updated and factored one step ahead
+__device__ __forceinline__ void chol_diag_role(CholShared &sh, double *A, int n, int *info, int T) {
+    const int lane = threadIdx.x & 63, lr = lane >> 4, lc = lane & 15;
+    for (int a = 0; a < T; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * a + lr + 4 * r, col = 16 * a + lc;
+            double v = row == col ? 1.0 : 0.0;  // identity padding past n
+            if (row < n && col < n && row <= col) v = A[row + (int64_t)col * n];
+            sh.Dg[a][lr + 4 * r][lc] = v;
+        }
+    if (lane == 0) sh.failed = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    {
+        const int bad = chol_factor_tile(sh.Dg[0], sh.Wl[0], lane);
+        if (bad && lane == 0) {
+            sh.failed = 1;
+            *info = bad;
+        }
+    }
+    __syncthreads();
+    for (int s = 0; s < T; ++s) {
+        if (sh.failed) return;  // every wave reads the flag after the same barrier
+        __syncthreads();        // the tile row published
+        double(*P)[CH_PLD] = sh.Pn[s & 1];
+        const int c0 = 16 * (s + 1);
+        if (s + 1 < T) {
+            // the critical path: ahead of the trailing updates on this SIMD
+            __builtin_amdgcn_s_setprio(3);
+            chf64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                const double v = P[4 * ks + lr][16 * (s + 1) - c0 + lc];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v, v, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sh.Dg[s + 1][lr + 4 * r][lc] -= acc[r];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            const int bad = chol_factor_tile(sh.Dg[s + 1], sh.Wl[(s + 1) & 1], lane);
+            if (bad && lane == 0) {
+                sh.failed = 1;
+                *info = 16 * (s + 1) + bad;
+            }
+            __builtin_amdgcn_s_setprio(0);
+        }
+        __syncthreads();
+    }
+    for (int a = 0; a < T; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * a + lr + 4 * r, col = 16 * a + lc;
+            if (row < n && col < n && row <= col) A[row + (int64_t)col * n] = sh.Dg[a][lr + 4 * r][lc];
+        }
+    if (lane == 0) *info = 0;
+}
+
+// waves 1..15: the off-diagonal tiles in registers
+__device__ __forceinline__ void chol_offdiag_role(CholShared &sh, double *A, int n, int T, int wave) {
+    const int lane = threadIdx.x & 63, lr = lane >> 4, lc = lane & 15;
+    const int noff = T * (T - 1) / 2;
     int ti_s[CH_SLOTS], tj_s[CH_SLOTS];
     double tile[CH_SLOTS][4];
 #pragma unroll
     for (int q = 0; q < CH_SLOTS; ++q) {
-        const int e = q * CH_WAVES + wave;
-        int a = 0, rem = e;
-        while (a < T && rem >= T - a) {  // row a of the upper triangle holds T - a tiles
-            rem -= T - a;
-            ++a;
+        int a = -1, b = -1;
+        const int e = q * (CH_WAVES - 1) + wave - 1;  // off-diagonal tile e, row-major
+        if (e < noff) {
+            int aa = 0, rem = e;
+            while (rem >= T - 1 - aa) {  // row aa holds T - 1 - aa off-diagonal tiles
+                rem -= T - 1 - aa;
+                ++aa;
+            }
+            a = aa;
+            b = aa + 1 + rem;
         }
-        ti_s[q] = e < ntiles ? a : -1;
-        tj_s[q] = e < ntiles ? a + rem : -1;
+        ti_s[q] = a;
+        tj_s[q] = b;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = 16 * a + lr + 4 * r, col = 16 * (a + rem) + lc;
-            double v = row == col ? 1.0 : 0.0;
-            if (e < ntiles && row < n && col < n && row <= col) v = A[row + (int64_t)col * n];
-            tile[q][r] = v;
+            const int row = 16 * a + lr + 4 * r, col = 16 * b + lc;
+            tile[q][r] = (a >= 0 && row < n && col < n) ? A[row + (int64_t)col * n] : 0.0;
         }
     }
+    __syncthreads();
     for (int s = 0; s < T; ++s) {
-        double(*P)[CH_PLD] = Pn[s & 1];
-        const int c0 = 16 * (s + 1), m = 16 * T - c0;
-        // 1. publish
+        if (sh.failed) return;
+        double(*P)[CH_PLD] = sh.Pn[s & 1];
+        const int c0 = 16 * (s + 1);
+        // the tile row: X = W_s B, published
+        double wa[4];
 #pragma unroll
-        for (int q = 0; q < CH_SLOTS; ++q) {
-            if (ti_s[q] == s && tj_s[q] == s)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) Dg[lr + 4 * r][lc] = tile[q][r];
-            if (ti_s[q] == s && tj_s[q] > s)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) P[lr + 4 * r][16 * tj_s[q] - c0 + lc] = tile[q][r];
-        }
-        __syncthreads();
-        // 2. factor the diagonal tile in every wave
-        {
-            const int l = lane;
-            double c[16];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) c[i] = (l < 16 && i <= l) ? Dg[i][l] : (i == l ? 1.0 : 0.0);
-            int bad = 0;
-            double dinv = 1.0;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const double p = readlane_f64(c[k], k);
-                if (bad == 0 && !(p > 0.0)) bad = 16 * s + k + 1;
-                double y = __builtin_amdgcn_rsq(p);  // 1 / sqrt(p) to full precision after two Newton steps
-                y = fma(0.5 * y, fma(-p, y * y, 1.0), y);
-                y = fma(0.5 * y, fma(-p, y * y, 1.0), y);
-                if (l == k) dinv = y;
-                c[k] = l == k ? p * y : c[k] * y;  // (entries below the diagonal, l < k, are never read)
-#pragma unroll
-                for (int i = 0; i < 16; ++i)  // constant trip count: c[] stays in registers
-                    if (i > k) c[i] -= readlane_f64(c[k], i) * c[k];
-            }
-            if (bad != 0) {  // every wave saw the same pivot: all leave here
-                if (tid == 0) *info = bad;
-                return;
-            }
-            if (l < 16) {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) Dw[wave][i][l] = c[i];
-                dinvw[wave][l] = dinv;
-            }
-        }
-        // the wave's lanes read what its lanes < 16 wrote
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int ks = 0; ks < 4; ++ks) wa[ks] = sh.Wl[s & 1][lc][4 * ks + lr];
 #pragma unroll
         for (int q = 0; q < CH_SLOTS; ++q)
-            if (ti_s[q] == s && tj_s[q] == s)
+            if (ti_s[q] == s) {
+                chf64x4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int r = 0; r < 4; ++r) tile[q][r] = (lr + 4 * r <= lc) ? Dw[wave][lr + 4 * r][lc] : 0.0;
-        // 3. the tile row: R^T x = b per column
-        if (tid < m) {
-            double x[16];
+                for (int ks = 0; ks < 4; ++ks)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[ks], tile[q][ks], acc, 0, 0, 0);
 #pragma unroll
-            for (int t = 0; t < 16; ++t) x[t] = P[t][tid];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                x[u] *= dinvw[wave][u];
-#pragma unroll
-                for (int t = 0; t < 16; ++t)
-                    if (t > u) x[t] -= Dw[wave][u][t] * x[u];
+                for (int r = 0; r < 4; ++r) {
+                    tile[q][r] = acc[r];
+                    P[lr + 4 * r][16 * tj_s[q] - c0 + lc] = acc[r];
+                }
             }
-#pragma unroll
-            for (int t = 0; t < 16; ++t) P[t][tid] = x[t];
-        }
         __syncthreads();
-        // 4. reload the tile row; update the trailing tiles
+        // the diagonal tiles past the next one (in LDS; wave 0 takes tile s + 1)
+        for (int a = s + 2; a < T; ++a)
+            if (1 + a % (CH_WAVES - 1) == wave) {
+                chf64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) {
+                    const double v = P[4 * ks + lr][16 * a - c0 + lc];
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v, v, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sh.Dg[a][lr + 4 * r][lc] -= acc[r];
+            }
+        // trailing tiles -= X_a^T X_b
 #pragma unroll
         for (int q = 0; q < CH_SLOTS; ++q) {
-            const int a = ti_s[q], b = tj_s[q];
-            if (a == s && b > s)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) tile[q][r] = P[lr + 4 * r][16 * b - c0 + lc];
-            if (a > s) {
+            if (ti_s[q] > s) {
+                const int a = ti_s[q], b = tj_s[q];
                 chf64x4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int ks = 0; ks < 4; ++ks) {
@@ -242,7 +313,9 @@ __global__ __launch_bounds__(CH_THR) void chol_tiles_kernel(double *A, int n, in
 #pragma unroll
                 for (int r = 0; r < 4; ++r) tile[q][r] -= acc[r];
             }
+            __builtin_amdgcn_sched_barrier(0);  // one slot's accumulators live at a time
         }
+        __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < CH_SLOTS; ++q) {
@@ -250,10 +323,17 @@ __global__ __launch_bounds__(CH_THR) void chol_tiles_kernel(double *A, int n, in
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * ti_s[q] + lr + 4 * r, col = 16 * tj_s[q] + lc;
-            if (row < n && col < n && row <= col) A[row + (int64_t)col * n] = tile[q][r];
+            if (row < n && col < n) A[row + (int64_t)col * n] = tile[q][r];
         }
     }
-    if (tid == 0) *info = 0;
+}
+
+__global__ __launch_bounds__(CH_THR) void chol_tiles_kernel(double *A, int n, int *info) {
+    __shared__ __attribute__((aligned(16))) CholShared sh;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int T = (n + 15) / 16;
+    if (wave == 0) chol_diag_role(sh, A, n, info, T);
+    else chol_offdiag_role(sh, A, n, T, wave);
 }
 
 // R x = y (TRANS false) or R^T x = y (TRANS true) in place, R n x n upper
