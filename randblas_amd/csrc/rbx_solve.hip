@@ -339,52 +339,37 @@ __global__ __launch_bounds__(CH_THR) void chol_tiles_kernel(double *A, int n, in
 // R x = y (TRANS false) or R^T x = y (TRANS true) in place, R n x n upper
 // triangular (column-major, leading dimension ldr, non-unit diagonal),
 // n <= TS_MAX, one 256-thread workgroup: y in LDS, 16-row blocks in the
-// order of the substitution.  Every diagonal block is staged in LDS first (one
-// coalesced pass over R).  Per block, lane i < 16 of wave 0 holds row i of the
-// block's triangle (R's row, or R's column when transposed) and substitutes
-// with x_k broadcast by v_readlane (1 / R(k, k) precomputed per lane); then
-// every thread of a remaining row subtracts the block's 16 products from its
-// y.  Those R values are loaded one block ahead, into alternating register
-// sets, so their latency overlaps the previous block's work.
+// order of the substitution.  Per block, lane i < 16 of wave 0 holds row i
+// of the block's triangle (R's row, or R's column when transposed) and
+// substitutes with x_k broadcast by v_readlane (1 / R(k, k) precomputed per
+// lane); then every thread of a remaining row subtracts the block's 16
+// products from its y, the R values loaded before the block's solve.
 constexpr int TS_MAX = 256;
 template <bool TRANS>
 __global__ __launch_bounds__(256) void trsv_upper_kernel(const double *R, int64_t ldr, double *y, int n) {
     __shared__ double ys[TS_MAX];
     __shared__ double xs[16];
-    __shared__ double Db[TS_MAX / 16][16][17];  // diagonal block b: Db[b][i][j] = R(16b + i, 16b + j)
     const int tid = threadIdx.x;
-    const int nblk = (n + 15) / 16;
     if (tid < n) ys[tid] = y[tid];
-    for (int e = tid; e < nblk * 256; e += 256) {
-        const int b = e >> 8, i = e & 15, j = (e >> 4) & 15;
-        const int r = 16 * b + i, c = 16 * b + j;
-        Db[b][i][j] = (r < n && c < n) ? R[r + (int64_t)c * ldr] : 0.0;
-    }
-    // the update operands of block position bi (the kb-th block in R); every
-    // thread loads all 16 (addresses clamped into R, unused values ignored),
-    // so no load sits under a branch and each wait covers only older loads
-    auto load_rv = [&](int bi, double *rv) {
+    __syncthreads();
+    const int nblk = (n + 15) / 16;
+    for (int bi = 0; bi < nblk; ++bi) {
         const int kb = TRANS ? bi : nblk - 1 - bi;
         const int k0 = 16 * kb, nk = n - k0 < 16 ? n - k0 : 16;
-        const int row = tid < n ? tid : n - 1;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int c = k0 + (j < nk ? j : 0);
-            rv[j] = TRANS ? R[c + (int64_t)row * ldr] : R[row + (int64_t)c * ldr];
-        }
-    };
-    auto step = [&](int bi, const double *rv) {
-        const int kb = TRANS ? bi : nblk - 1 - bi;
-        const int k0 = 16 * kb, nk = n - k0 < 16 ? n - k0 : 16;
+        // rows still to update: above the block (back substitution), below it (forward)
         const bool upd = TRANS ? (tid >= k0 + nk && tid < n) : (tid < k0);
+        double rv[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            rv[j] = (upd && j < nk) ? (TRANS ? R[(k0 + j) + (int64_t)tid * ldr] : R[tid + (int64_t)(k0 + j) * ldr]) : 0.0;
         if (tid < 64) {
-            const int i = tid & 15;
+            const int i = tid;
             double r[16];  // row i of the block's triangle T (T = R_bb, or R_bb^T)
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const bool in = i < nk && j < nk && (TRANS ? j <= i : j >= i);
-                const double v = TRANS ? Db[kb][j][i] : Db[kb][i][j];
-                r[j] = in ? v : (i == j ? 1.0 : 0.0);
+                r[j] = in ? (TRANS ? R[(k0 + j) + (int64_t)(k0 + i) * ldr] : R[(k0 + i) + (int64_t)(k0 + j) * ldr])
+                          : (i == j ? 1.0 : 0.0);
             }
             double rii = 1.0;
 #pragma unroll
@@ -399,7 +384,7 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(const double *R, int64_
                 if (i == k) x = xk;
                 b -= r[k] * xk;  // rows not yet solved; finished lanes keep x
             }
-            if (tid < nk) {
+            if (i < nk) {
                 ys[k0 + i] = x;
                 xs[i] = x;
             }
@@ -408,21 +393,10 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(const double *R, int64_
         if (upd) {
             double v = ys[tid];
 #pragma unroll
-            for (int j = 0; j < 16; ++j)
-                if (j < nk) v -= rv[j] * xs[j];
+            for (int j = 0; j < 16; ++j) v -= rv[j] * xs[j];
             ys[tid] = v;
         }
         __syncthreads();
-    };
-    double rva[16], rvb[16];
-    load_rv(0, rva);
-    __syncthreads();
-    for (int bi = 0; bi < nblk; bi += 2) {
-        if (bi + 1 < nblk) load_rv(bi + 1, rvb);
-        step(bi, rva);
-        if (bi + 1 >= nblk) break;
-        if (bi + 2 < nblk) load_rv(bi + 2, rva);
-        step(bi + 1, rvb);
     }
     if (tid < n) y[tid] = ys[tid];
 }
