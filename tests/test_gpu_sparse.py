@@ -689,3 +689,23 @@ def test_lskges_prepared_operator(rb, orc, dtype):
     S.vals = S.vals.clone()
     got3, prof = _lsk_run(rb, S, dtype, "C", "N", d, n, m, 0, 0, A, m, B0, d)
     assert "coo_to_csr" in prof and np.array_equal(got3, got)
+
+
+@pytest.mark.parametrize("axis,shape,k", [("S", (300, 4000), 6), ("L", (300, 4000), 40), ("L", (64, 9000), 30)])
+@pytest.mark.parametrize("layout,opA", [("C", "N"), ("R", "T")])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lskges_prepared_whole_operator(rb, orc, axis, shape, k, layout, opA, dtype):
+    """The prepared copy over SASO and LASO operators (LASO rows hold merged duplicates), with op(A)
+    of unit stride along the contraction in both layouts: bit-identical to the oracle, no sort."""
+    d, m = shape
+    n = 24
+    rb.profile_collect()
+    rb.profile_enable(True)
+    try:
+        got, ref = _lsk(rb, orc, dtype, layout, "N", opA, shape, k, axis, d, n, m, 0, 0, 0.5, 2.0, prefill=True)
+        torch.cuda.synchronize()
+    finally:
+        rb.profile_enable(False)
+    prof = rb.profile_collect()
+    assert "coo_to_csr" not in prof and "csr_lds_apply_kernel" in prof
+    assert np.array_equal(got, ref)
