@@ -35,6 +35,26 @@ __global__ __launch_bounds__(256) void iid_uniform_kernel(Ctr128 ctr, uint32_t k
     const uint4 w4 = philox4x32_10(ctr_add(ctr, (uint64_t)c), k0, k1);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
     const double dN = (double)n;
+    // whole counters with 16-byte aligned outputs: vector stores (a thread's
+    // samples are contiguous, so its 32 or 16 bytes go out in one or two pieces)
+    const bool vec = i0 + per <= k && ((uintptr_t)(samples + i0) & 15) == 0 &&
+                     (rademachers == nullptr || ((uintptr_t)(rademachers + i0) & (2 * sizeof(T) - 1)) == 0);
+    if (vec) {
+        typedef long long i64x2 __attribute__((ext_vector_type(2)));
+        typedef T tx2 __attribute__((ext_vector_type(2)));
+        if (rademachers) {
+            const i64x2 sv = {uniform_index(w[0], dN), uniform_index(w[2], dN)};
+            const tx2 rv = {uneg11(w[1]) >= 0.0f ? (T)1 : (T)-1, uneg11(w[3]) >= 0.0f ? (T)1 : (T)-1};
+            *(i64x2 *)(samples + i0) = sv;
+            *(tx2 *)(rademachers + i0) = rv;
+        } else {
+            const i64x2 s01 = {uniform_index(w[0], dN), uniform_index(w[1], dN)};
+            const i64x2 s23 = {uniform_index(w[2], dN), uniform_index(w[3], dN)};
+            *(i64x2 *)(samples + i0) = s01;
+            *(i64x2 *)(samples + i0 + 2) = s23;
+        }
+        return;
+    }
     for (int q = 0; q < per && i0 + q < k; ++q) {
         if (rademachers) {
             samples[i0 + q] = uniform_index(w[2 * q], dN);

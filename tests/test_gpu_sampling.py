@@ -26,6 +26,22 @@ def test_uniform_bit_exact(rb, orc, n, k, ctr):
     assert list(nxt.counter) == [int(x) for x in rnxt] and nxt.key == 42
 
 
+@pytest.mark.parametrize("signed", [False, True])
+def test_uniform_unaligned_outputs(rb, orc, signed):
+    """Outputs that start 8 (4) bytes past a 16-byte boundary take the per-entry stores."""
+    k = 4099
+    st = rb.RNGState(key=8, counter=(77, 0, 0, 0))
+    sbuf = torch.full((k + 1,), -1, dtype=torch.int64, device=DEV)
+    s = sbuf[1:]
+    rbuf = torch.zeros(k + 1, dtype=torch.float32, device=DEV)
+    r = rbuf[1:] if signed else None
+    rb.sample_indices_iid_uniform(10 ** 6, k, s, st, rademachers=r)
+    ref, rr, _ = orc.sample_indices_iid_uniform(10 ** 6, k, [77, 0, 0, 0], [8, 0], signed=signed)
+    assert np.array_equal(host(s), ref) and host(sbuf)[0] == -1
+    if signed:
+        assert np.array_equal(host(r).astype(np.float64), rr) and host(rbuf)[0] == 0
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("k", [1, 2, 3, 1001, 65536])
 def test_signed_bit_exact(rb, orc, dtype, k):
