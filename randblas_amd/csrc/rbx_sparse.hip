@@ -257,6 +257,39 @@ __global__ __launch_bounds__(256) void csr_from_sorted_kernel(const KT *keys, co
     }
 }
 
+// Row keys only, for entries whose columns already ascend within each row in
+// stored order: a stable sort by row then gives the (row, column, position)
+// order of the full key in fewer radix passes.
+__global__ __launch_bounds__(256) void coo_row_keys_kernel(const int64_t *rows, const int64_t *cols, int64_t nnz,
+                                                           int64_t ro, int64_t co, int64_t d, int64_t m,
+                                                           uint32_t *keys, int32_t *perm) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = rows[e] - ro, c = cols[e] - co;
+        const bool in = r >= 0 && r < d && c >= 0 && c < m;
+        keys[e] = (uint32_t)(in ? r : d);
+        perm[e] = (int32_t)e;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void csr_from_rowsorted_kernel(const uint32_t *keys, const int32_t *perm,
+                                                                 const int64_t *cols, const T *vals, int64_t nnz,
+                                                                 int64_t d, int64_t co, int64_t *row_ptr,
+                                                                 int32_t *col, T *val) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = keys[p];
+        const int64_t rp = p == 0 ? -1 : (int64_t)keys[p - 1];
+        for (int64_t q = rp + 1; q <= r && q <= d; ++q) row_ptr[q] = p;
+        if (p == nnz - 1)
+            for (int64_t q = r + 1; q <= d; ++q) row_ptr[q] = nnz;
+        if (r < d) {
+            const int32_t e = perm[p];
+            col[p] = (int32_t)(cols[e] - co);
+            val[p] = vals[e];
+        }
+    }
+}
+
 __global__ void zero_rowptr_kernel(int64_t *row_ptr, int64_t d) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= d; q += (int64_t)gridDim.x * blockDim.x)
         row_ptr[q] = 0;
@@ -500,6 +533,44 @@ static void coo_to_csr(const int64_t *rows, const int64_t *cols, const T *vals, 
     hipLaunchKernelGGL(zero_rowptr_kernel, dim3(64), dim3(256), 0, st, row_ptr, d);
     hipLaunchKernelGGL((csr_from_sorted_kernel<T, KT, PT>), dim3((unsigned)blocks), dim3(256), 0, st, keys_out,
                        perm_out, vals, nnz, d, m, row_ptr, ccol, cval);
+    RBX_HIP(hipGetLastError());
+    *row_ptr_out = row_ptr;
+    *col_out = ccol;
+    *val_out = cval;
+}
+
+// coo_to_csr for entries whose columns ascend within each row in stored order
+// (the expansion of a CSR matrix viewed transposed, or of a CSC matrix): the
+// same CSR from a stable radix sort on the row alone.
+template <typename T>
+static void coo_to_csr_by_row(const int64_t *rows, const int64_t *cols, const T *vals, int64_t nnz, int64_t ro,
+                              int64_t co, int64_t d, int64_t m, int64_t **row_ptr_out, int32_t **col_out,
+                              T **val_out, hipStream_t st) {
+    size_t sort_bytes = 0;
+    const int end_bit = bits_for((uint64_t)d);
+    RBX_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                      (int32_t *)nullptr, (int32_t *)nullptr, (size_t)nnz, 0, end_bit, st));
+    const size_t bk = align256((size_t)nnz * 4), bp = align256((size_t)nnz * 4);
+    const size_t brp = align256((size_t)(d + 1) * 8), bcol = align256((size_t)nnz * 4),
+                 bval = align256((size_t)nnz * sizeof(T));
+    char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, 2 * bk + 2 * bp + brp + bcol + bval + align256(sort_bytes), st);
+    uint32_t *keys_in = (uint32_t *)ws, *keys_out = (uint32_t *)(ws + bk);
+    int32_t *perm_in = (int32_t *)(ws + 2 * bk), *perm_out = (int32_t *)(ws + 2 * bk + bp);
+    char *rest = ws + 2 * bk + 2 * bp;
+    int64_t *row_ptr = (int64_t *)rest;
+    int32_t *ccol = (int32_t *)(rest + brp);
+    T *cval = (T *)(rest + brp + bcol);
+    void *sort_tmp = rest + brp + bcol + bval;
+    int64_t blocks = (nnz + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(coo_row_keys_kernel, dim3((unsigned)blocks), dim3(256), 0, st, rows, cols, nnz, ro, co, d, m,
+                       keys_in, perm_in);
+    RBX_HIP(hipGetLastError());
+    RBX_HIP(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_in, keys_out, perm_in, perm_out, (size_t)nnz, 0,
+                                      end_bit, st));
+    hipLaunchKernelGGL(zero_rowptr_kernel, dim3(64), dim3(256), 0, st, row_ptr, d);
+    hipLaunchKernelGGL((csr_from_rowsorted_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, keys_out, perm_out,
+                       cols, vals, nnz, d, co, row_ptr, ccol, cval);
     RBX_HIP(hipGetLastError());
     *row_ptr_out = row_ptr;
     *col_out = ccol;
@@ -804,7 +875,7 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
                           int64_t A_cols, int64_t nnz, const T *vals, const int64_t *rows, const int64_t *cols,
                           int64_t ro_a, int64_t co_a, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
                           hipStream_t st, int kib_force = -1, char *fixed_ws = nullptr, size_t fixed_ws_bytes = 0,
-                          const SparsePrep *prep = nullptr) {
+                          const SparsePrep *prep = nullptr, bool cols_ascend_in_rows = false) {
     check_layout(layout);
     check_op(opA);
     check_op(opB);
@@ -882,7 +953,9 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
         // few output rows: stable counting sort by row; otherwise a radix
         // sort on the (row, column) key
         if (csr_build_counting<T>(rows, cols, vals, nnz, ro_a, co_a, d, m, &row_ptr, &ccol, &cval, st)) {
-        } else if ((uint64_t)d * (uint64_t)m < (1ull << 32) && nnz < (int64_t)INT32_MAX)
+        } else if (cols_ascend_in_rows && d < (int64_t)UINT32_MAX && nnz < (int64_t)INT32_MAX)
+            coo_to_csr_by_row<T>(rows, cols, vals, nnz, ro_a, co_a, d, m, &row_ptr, &ccol, &cval, st);
+        else if ((uint64_t)d * (uint64_t)m < (1ull << 32) && nnz < (int64_t)INT32_MAX)
             coo_to_csr<T, uint32_t, int32_t>(rows, cols, vals, nnz, ro_a, co_a, d, m, &row_ptr, &ccol, &cval, st);
         else
             coo_to_csr<T, uint64_t, int64_t>(rows, cols, vals, nnz, ro_a, co_a, d, m, &row_ptr, &ccol, &cval, st);
@@ -1238,8 +1311,10 @@ static void rsksp3_impl(char layout, char opA, char opS, int64_t m, int64_t d, i
     // G is row-major n x d: op(B) = G for a row-major C, G^T of the
     // column-major d x n buffer otherwise
     const char opG = layout == 'C' ? 'T' : 'N';
+    // CSR viewed transposed and CSC (the cases that reach here): within each
+    // row of the view, the stored order already ascends in the view's column
     left_spmm_coo<T>(layout, opA, opG, m, d, n, alpha, A.n_rows, A.n_cols, A.nnz, (const T *)A.vals, rows, cols, ro_a,
-                     co_a, G, d, beta, B, ldb, st, kib);
+                     co_a, G, d, beta, B, ldb, st, kib, nullptr, 0, nullptr, A.fmt != 'O');
 }
 
 // lsksp3 (sparse_data/sksp.hh:132-180): B = alpha * op(submat(S)) * op(A) + beta * B.

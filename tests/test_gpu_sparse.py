@@ -357,17 +357,8 @@ def _sparse_in_format(rb, rng, fmt, nrows, ncols, mean_nnz, dtype, pad=(0, 0)):
     return (rb.CSCMatrix(nrows, ncols, t(parts["vals"]), t(parts["rowidx"]), t(colptr)), parts, (nrows, ncols))
 
 
-@pytest.mark.parametrize("fmt", ["csr", "csc", "coo"])
-@pytest.mark.parametrize("layout", ["C", "R"])
-@pytest.mark.parametrize("opA", ["N", "T"])
-@pytest.mark.parametrize("opS", ["N", "T"])
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_rsksp3_formats(rb, orc, fmt, layout, opA, opS, dtype):
-    """rsksp3 for CSR / CSC / COO data bit-exact against the reference's
-    dispatch (sksp.hh:277-326 -> spmm_dispatch.hh:72-176), alpha and beta
-    general; COO as a submatrix with entries outside the window."""
+def _rsksp3_formats_case(rb, orc, fmt, layout, opA, opS, dtype, m, n, d):
     rng = np.random.default_rng(41)
-    m, n, d = 260, 610, 36
     ro_s, co_s = 3, 2
     pad = (4, 6) if fmt == "coo" else (0, 0)
     rowsA, colsA = (m, n) if opA == "N" else (n, m)
@@ -390,6 +381,27 @@ def test_rsksp3_formats(rb, orc, fmt, layout, opA, opS, dtype):
     _ref_left_spmm(orc, fmt, dt, layout, opA, opS_ref, m, d, n, 0.75, shape, parts, pad[0], pad[1], Ssub, lds, -0.5,
                    ref, ldb)
     assert np.array_equal(host(Bd), ref)
+
+
+@pytest.mark.parametrize("fmt", ["csr", "csc", "coo"])
+@pytest.mark.parametrize("layout", ["C", "R"])
+@pytest.mark.parametrize("opA", ["N", "T"])
+@pytest.mark.parametrize("opS", ["N", "T"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_rsksp3_formats(rb, orc, fmt, layout, opA, opS, dtype):
+    """rsksp3 for CSR / CSC / COO data bit-exact against the reference's
+    dispatch (sksp.hh:277-326 -> spmm_dispatch.hh:72-176), alpha and beta
+    general; COO as a submatrix with entries outside the window."""
+    _rsksp3_formats_case(rb, orc, fmt, layout, opA, opS, dtype, 260, 610, 36)
+
+
+@pytest.mark.parametrize("fmt", ["csr", "csc", "coo"])
+@pytest.mark.parametrize("opA", ["N", "T"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_rsksp3_formats_many_rows(rb, orc, fmt, opA, dtype):
+    """More than 8192 rows of op(A) on the column-kernel cases: CSR viewed transposed and CSC take the
+    stable sort on the row key alone, COO the (row, column) key; all bit-exact."""
+    _rsksp3_formats_case(rb, orc, fmt, "C", opA, "N", dtype, 9000, 610, 20)
 
 
 @pytest.mark.parametrize("fmt", ["csc", "coo"])
