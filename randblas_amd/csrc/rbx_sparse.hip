@@ -1031,6 +1031,7 @@ static const SparsePrep *usable_prep(const rbx_sparse_skop *S, char opS, int64_t
     if (p == nullptr || p->magic != PREP_MAGIC || p->val_size != sizeof(T)) return nullptr;
     if (opS != 'N' || ro_s != 0 || co_s != 0 || d != S->dist.n_rows || m != S->dist.n_cols) return nullptr;
     if (S->nnz != p->nnz || S->vals != p->vals || S->rows != p->rows || S->cols != p->cols) return nullptr;
+    if (p->d != d || p->m != m) return nullptr;
     int dev = -1;
     RBX_HIP(hipGetDevice(&dev));
     return dev == p->dev ? p : nullptr;
