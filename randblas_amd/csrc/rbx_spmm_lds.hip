@@ -361,23 +361,28 @@ bool csr_build_counting(const int64_t *rows, const int64_t *cols, const T *vals,
 // streams those J columns of the dense operand through LDS in CK-deep chunks
 // (double-buffered: the next chunk's loads are in flight while the current one
 // is applied).  The stored entries are first laid out by chunk and slot:
-// Ec / Ev[(k * LA_MAXE + q) * d + r] is row r's q-th entry among those with a
+// Ec / Ev[(k * MAXE + q) * d + r] is row r's q-th entry among those with a
 // column in chunk k, and startT[k * d + r] where the row's entries of chunk k
 // start in the CSR.  A wave's load of slot q then reads 64 consecutive rows
 // (one coalesced request instead of 64 lines), and each thread loads the next
 // chunk's slots into a second register bank while it applies the current
 // chunk's; slots past the wave's largest count are skipped, and a row with
-// more than LA_MAXE entries in a chunk reads the rest from the CSR.  Every
+// more than MAXE entries in a chunk reads the rest from the CSR.  Every
 // output is one chain over its row's entries in ascending column.  The row
 // groups of one column block run back to back on one XCD, so the block's
 // dense data comes from HBM once and from that XCD's L2 after.
-constexpr int LA_CKMAX = 1024, LA_MAXE = 12;
+// Deepest chunk and slots per row and chunk, by element size: FP32 tiles take
+// twice the depth in the same LDS, so rows average up to ten entries per
+// chunk there (six in FP64), which evens out the wave's per-chunk maximum.
+constexpr int la_ckmax(size_t val_size) { return val_size == 4 ? 2048 : 1024; }
+constexpr int la_maxe(size_t val_size) { return val_size == 4 ? 20 : 12; }
 
-// Chunk depth: a power of two in [64, LA_CKMAX] giving rows about six stored
-// entries per chunk or fewer.
-static int la_chunk(int64_t d, int64_t m, int64_t nnz) {
+// Chunk depth: a power of two in [64, la_ckmax] giving rows about six (FP64)
+// or ten (FP32) stored entries per chunk or fewer.
+static int la_chunk(int64_t d, int64_t m, int64_t nnz, size_t val_size) {
+    const double per_chunk = val_size == 4 ? 10.0 : 6.0;
     int ck = 64;
-    while (ck < LA_CKMAX && (double)(2 * ck) * (double)nnz <= 6.0 * (double)d * (double)m) ck *= 2;
+    while (ck < la_ckmax(val_size) && (double)(2 * ck) * (double)nnz <= per_chunk * (double)d * (double)m) ck *= 2;
     return ck;
 }
 
@@ -389,11 +394,11 @@ struct LaPlan {
 };
 static LaPlan la_plan(int64_t d, int64_t m, int64_t nnz, size_t val_size) {
     LaPlan p;
-    p.ck = la_chunk(d, m, nnz);
+    p.ck = la_chunk(d, m, nnz, val_size);
     p.nk = (m + p.ck - 1) / p.ck;
     p.bStart = al256((size_t)(p.nk + 1) * (size_t)d * 4);
-    p.bEc = al256((size_t)p.nk * LA_MAXE * (size_t)d * 4);
-    p.bEv = al256((size_t)p.nk * LA_MAXE * (size_t)d * val_size);
+    p.bEc = al256((size_t)p.nk * la_maxe(val_size) * (size_t)d * 4);
+    p.bEv = al256((size_t)p.nk * la_maxe(val_size) * (size_t)d * val_size);
     return p;
 }
 
@@ -424,7 +429,7 @@ __global__ __launch_bounds__(256) void csr_chunk_table_kernel(const int64_t *row
 }
 
 // One workgroup per row (grid-stride): each entry p of row r goes to slot
-// q = p - startT[k * d + r] of its chunk k when q < LA_MAXE.
+// q = p - startT[k * d + r] of its chunk k when q < la_maxe(sizeof(T)).
 template <typename T>
 __global__ __launch_bounds__(256) void csr_chunk_slots_kernel(const int64_t *row_ptr, const int32_t *col,
                                                               const T *val, int64_t d, int CK,
@@ -435,8 +440,9 @@ __global__ __launch_bounds__(256) void csr_chunk_slots_kernel(const int64_t *row
             const int32_t c = col[p];
             const int64_t k = c / CK;
             const int64_t q = p - startT[k * d + r];
-            if (q < LA_MAXE) {
-                const int64_t o = (k * LA_MAXE + q) * d + r;
+            constexpr int MAXE = la_maxe(sizeof(T));
+            if (q < MAXE) {
+                const int64_t o = (k * MAXE + q) * d + r;
                 Ec[o] = c;
                 Ev[o] = val[p];
             }
@@ -471,6 +477,7 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
     // J * sizeof(T) / 16 wide reads and the reads of 64 lanes at unrelated c
     // spread over all bank groups (stores of consecutive c stay conflict-free).
     // Narrower rows are stored column by column, As[buf][jj * LA_CKMAX + c].
+    constexpr int LA_CKMAX = la_ckmax(sizeof(T)), LA_MAXE = la_maxe(sizeof(T));
     __shared__ __attribute__((aligned(16))) T As[2][LA_CKMAX * J];
     constexpr bool ROWMAJ = J * (int)sizeof(T) >= 16;
     constexpr int EPP = 16 / (int)sizeof(T);                        // elements per 16-byte piece
@@ -674,7 +681,7 @@ static void launch_lds_apply_j(int J, int form, dim3 grid, const int32_t *startT
 
 static bool lds_apply_fits(int64_t d, int64_t m, size_t val_size) {
     // buffer offsets: (m + one chunk) entries must stay below 2^31 bytes
-    return d > 0 && m > 0 && (m + LA_CKMAX) * (int64_t)val_size < (1ll << 31);
+    return d > 0 && m > 0 && (m + la_ckmax(val_size)) * (int64_t)val_size < (1ll << 31);
 }
 
 template <typename T>
