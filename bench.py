@@ -400,6 +400,7 @@ def saso_leg(rb, torch, dev, reps=10):
     800000 stored entries, 24 B each, read once)."""
     d, m, n = 2000, 100000, 500
     res = {"workload": "SASO 2000x100000 vec_nnz 8 (key 7) * A 100000x500 ColMajor Uniform(-1,1) (key 3)"}
+    outs = {}
     for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
         S = rb.SparseSkOp(rb.SparseDist(d, m, 8, rb.Axis.Short), rb.RNGState(7), dtype=dt)
         A = torch.empty(m * n, dtype=dt, device=dev)
@@ -441,8 +442,15 @@ def saso_leg(rb, torch, dev, reps=10):
                      "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}}
         log(f"saso lskges {name}: {ms:.3f} ms (prepared operator), {ms_sort:.3f} ms sorting per call, "
             f"prepare {ms_prep:.3f} ms")
+        outs[name] = B.double()
         del A, B
         torch.cuda.synchronize()
+    # the FP32 product against the FP64 one (same operator, same A up to its
+    # FP32 rounding); each is bit-identical to the reference's in its own
+    # precision (tests/test_gpu_sparse.py)
+    res["fp32_rel_err_vs_fp64"] = float(torch.linalg.norm(outs["fp32"] - outs["fp64"]) /
+                                        torch.linalg.norm(outs["fp64"]))
+    del outs
     return res
 
 
