@@ -381,10 +381,22 @@ def sparse_leg(rb, torch, dev, reps=10):
     bf_err = float(torch.linalg.norm(B.double() - B64) / torch.linalg.norm(B64))
     rb.rsksp3("C", "N", "N", m, d, n, 1.0, A, 0, 0, S, 0, 0, 0.0, B, m)
     fp32_err = float(torch.linalg.norm(B.double() - B64) / torch.linalg.norm(B64))
+    # distortion of each operator on a fixed 64-dimensional subspace of R^n (the
+    # rank of A exceeds d, so A's own row space cannot be embedded): the
+    # singular values of (1/sqrt d) Q^T S, Q an orthonormal n x 64 basis
+    # (torch's generator, seed 0), for S in FP32 and rounded to bf16
+    g = torch.Generator(device=dev).manual_seed(0)
+    Q, _ = torch.linalg.qr(torch.randn(n, 64, dtype=torch.float64, device=dev, generator=g))
+    dist = {}
+    for name, Sm in (("fp32", Sf), ("bf16", _bf16_round(torch, Sf))):
+        X = (Q.t() @ Sm.double().view(d, n).t()) / math.sqrt(d)
+        sv = torch.linalg.svdvals(X.cpu())
+        dist[name] = round(max(float(sv.max()) - 1.0, 1.0 - float(sv.min())), 6)
     res["bf16split"] = {"ms": round(bf_ms, 4), "GBps": round(bf_bytes / (bf_ms * 1e-3) / 1e9, 1),
                         "frac_hbm": round(bf_bytes / (bf_ms * 1e-3) / 8e12, 4), "kernels_ms": bf_k, "terms": 3,
                         "rel_err_vs_fp64_same_operator": bf_err,
-                        "fp32_path_rel_err_vs_fp64_bf16_operator": fp32_err}
+                        "fp32_path_rel_err_vs_fp64_bf16_operator": fp32_err,
+                        "distortion_64dim_subspace": dist}
     log(f"sparse: rsksp3 bf16-split {bf_ms:.3f} ms, rel err vs same-operator FP64 {bf_err:.2e}")
     del A, B, Bbar, dv, Sf, S64, A64, B64
     torch.cuda.synchronize()
