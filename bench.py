@@ -322,7 +322,7 @@ def libm_distance(rb, torch, dev, np, oracle, rows=64):
             "frac_bit_identical": float((dist == 0).mean())}
 
 
-def sparse_leg(rb, torch, dev, reps=10):
+def sparse_leg(rb, torch, dev, reps=20):
     """rsksp3 on a 200000 x 50000 CSR matrix with power-law column degrees
     (exponent 2.1, 4e6 draws, ~20 stored entries per row, N(0,1) FP32 values,
     generated on the device from key 17) times a
@@ -341,18 +341,7 @@ def sparse_leg(rb, torch, dev, reps=10):
     dv = torch.empty(nnz, dtype=torch.float32, device=dev)
 
     def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        rb.profile_collect()
-        rb.profile_enable(True)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        rb.profile_enable(False)
-        prof = rb.profile_collect()
-        return (t1 - t0) * 1e3 / reps, {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}
+        return _time(rb, torch, fn, reps)
 
     fwd_ms, fwd_k = timed(lambda: rb.rsksp3("C", "N", "N", m, d, n, 1.0, A, 0, 0, S, 0, 0, 0.0, B, m))
     adj_ms, adj_k = timed(lambda: rb.rsksp3_adjoint("C", "N", m, d, n, 1.0, A, S, 0, 0, Bbar, m, dv))
@@ -403,7 +392,7 @@ def sparse_leg(rb, torch, dev, reps=10):
     return res
 
 
-def saso_leg(rb, torch, dev, reps=10):
+def saso_leg(rb, torch, dev, reps=20):
     """lskges with a SASO operator: S = SparseDist(2000, 100000, vec_nnz 8,
     Short), key 7, sampled once (fill_sparse, which also prepares the
     operator's row-sorted copy); A 100000 x 500 column-major
@@ -419,31 +408,15 @@ def saso_leg(rb, torch, dev, reps=10):
         rb.fill_dense(rb.DenseDist(m, n, rb.ScalarDist.Uniform), A, rb.RNGState(3))
         B = torch.empty(d * n, dtype=dt, device=dev)
         rb.fill_sparse(S)
-        rb.lskges("C", "N", "N", d, n, m, 1.0, S, 0, 0, A, m, 0.0, B, d)
-        torch.cuda.synchronize()
-        rb.profile_collect()
-        rb.profile_enable(True)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            rb.lskges("C", "N", "N", d, n, m, 1.0, S, 0, 0, A, m, 0.0, B, d)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        rb.profile_enable(False)
-        prof = rb.profile_collect()
-        ms = (t1 - t0) * 1e3 / reps
+        call = lambda: rb.lskges("C", "N", "N", d, n, m, 1.0, S, 0, 0, A, m, 0.0, B, d)
+        ms, prof = _time(rb, torch, call, reps)
         esz = A.element_size()
         nbytes = m * n * esz + d * n * esz + 8 * m * 24
         # the same calls without the prepared copy (each call sorts the
         # operator's entries by row, as the reference's COO path does), and the
         # one-off cost of preparing it
         S.release()
-        rb.lskges("C", "N", "N", d, n, m, 1.0, S, 0, 0, A, m, 0.0, B, d)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            rb.lskges("C", "N", "N", d, n, m, 1.0, S, 0, 0, A, m, 0.0, B, d)
-        torch.cuda.synchronize()
-        ms_sort = (time.perf_counter() - t0) * 1e3 / reps
+        ms_sort, _ = _time(rb, torch, call, reps)
         t0 = time.perf_counter()
         S.prepare()
         torch.cuda.synchronize()
@@ -451,7 +424,7 @@ def saso_leg(rb, torch, dev, reps=10):
         res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1),
                      "frac_hbm": round(nbytes / (ms * 1e-3) / 8e12, 4),
                      "ms_sorting_per_call": round(ms_sort, 4), "prepare_ms": round(ms_prep, 4),
-                     "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}}
+                     "kernels_ms": prof}
         log(f"saso lskges {name}: {ms:.3f} ms (prepared operator), {ms_sort:.3f} ms sorting per call, "
             f"prepare {ms_prep:.3f} ms")
         outs[name] = B.double()
@@ -466,22 +439,33 @@ def saso_leg(rb, torch, dev, reps=10):
     return res
 
 
-def _time(rb, torch, fn, reps):
-    fn()
-    torch.cuda.synchronize()
-    rb.profile_collect()
-    rb.profile_enable(True)
-    t0 = time.perf_counter()
-    for _ in range(reps):
+def _time(rb, torch, fn, reps, warm=3, prof_reps=3):
+    """Median over `reps` calls of the GPU time between HIP events recorded
+    around each call (after `warm` untimed calls), then the per-kernel
+    breakdown from a separate pass of `prof_reps` calls with the library's
+    kernel events on (so they add nothing to the timed calls)."""
+    for _ in range(warm):
         fn()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, b in evs:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    times = sorted(a.elapsed_time(b) for a, b in evs)
+    ms = times[len(times) // 2]
+    rb.profile_collect()
+    rb.profile_enable(True)
+    for _ in range(prof_reps):
+        fn()
+    torch.cuda.synchronize()
     rb.profile_enable(False)
     prof = rb.profile_collect()
-    return (t1 - t0) * 1e3 / reps, {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}
+    return ms, {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}
 
 
-def generator_leg(rb, torch, dev, reps=5):
+def generator_leg(rb, torch, dev, reps=20):
     """The generator alone: fill_dense of the headline operator, DenseDist(4096,
     262144) Gaussian, key 42, into FP32 and FP64 buffers (4 / 8 GiB).  Bound by
     the generator's VALU work, priced against the HBM write rate."""
@@ -500,7 +484,7 @@ def generator_leg(rb, torch, dev, reps=5):
     return res
 
 
-def vector_leg(rb, torch, dev, reps=5):
+def vector_leg(rb, torch, dev, reps=20):
     """One-column sketches: sketch_vector with the headline operator,
     y = S x for S = DenseDist(4096, 262144) Gaussian (key 42) and x of 262144
     N(0,1) entries, FP32 and FP64.  Bound by generating the 1.07e9 operator
@@ -517,7 +501,7 @@ def vector_leg(rb, torch, dev, reps=5):
     return res
 
 
-def sketch_solve_leg(rb, torch, dev, reps=5):
+def sketch_solve_leg(rb, torch, dev, reps=20):
     """The sketch-and-solve product and its adjoint: S = DenseDist(1024, 65536)
     (key 42) times [A b], A 65536 x 256 N(0,1) (key 11), FP32 and FP64; the
     adjoint S^T G for G 1024 x 257 regenerates the operator.  2*1024*65536*257
@@ -560,7 +544,7 @@ def sketch_solve_leg(rb, torch, dev, reps=5):
     return res
 
 
-def formats_leg(rb, torch, dev, reps=5):
+def formats_leg(rb, torch, dev, reps=20):
     """The sparse-data path around the sketches, on the 200000 x 50000
     power-law matrix of the sparse leg (FP32 values, int64 indices):
     conversions CSR -> COO, COO -> CSC and COO -> CSR (stable sorts on the
@@ -606,7 +590,7 @@ def formats_leg(rb, torch, dev, reps=5):
     return res
 
 
-def sampling_leg(rb, torch, dev, reps=5):
+def sampling_leg(rb, torch, dev, reps=20):
     """Index sampling: 2^27 iid uniform indices in [0, 10^9) (int64 out), the
     same with FP32 Rademacher signs, and 2^26 draws from a CDF over 10^6
     indices (weights_to_cdf of power-law weights first, one sequential pass,
