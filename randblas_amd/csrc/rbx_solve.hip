@@ -120,8 +120,14 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 //   2. update the diagonal tiles past s + 1 in LDS, and their trailing tiles
 //      T_ab -= X_a^T X_b, with v_mfma_f64_16x16x4f64.
 // Global memory is read once and written once; two barriers per step (the
-// tile row's LDS buffer and W alternate between steps).
+// tile row's LDS buffer alternates between steps; W of every step is kept).
 // info = 0, or k + 1 when pivot k is not positive (R then incomplete).
+// When Rp / Wp are given, the kernel also writes, for trsm_tiles_kernel, the
+// strictly upper tiles of R (tile (k, j), k < j, at Rp + 256 (j (j - 1) / 2 +
+// k)) and W_s = R_ss^{-T} (at Wp + 256 s), each as 4 x 64 doubles: element
+// [ks][lane] is what lane `lane` passes to v_mfma_f64_16x16x4f64 at K-step ks
+// (R tiles in the accumulator layout, which is the A-operand layout of R^T;
+// W in the A-operand layout).
 constexpr int CH_MAX = 256, CH_THR = 1024, CH_WAVES = CH_THR / 64, CH_T = CH_MAX / 16;
 constexpr int CH_OFF = CH_T * (CH_T - 1) / 2;                        // 120 off-diagonal tiles
 constexpr int CH_SLOTS = (CH_OFF + CH_WAVES - 2) / (CH_WAVES - 1);  // 8 per off-diagonal wave
@@ -175,14 +181,19 @@ __device__ __forceinline__ int chol_factor_tile(double (*D)[17], double (*Wl)[17
 
 struct CholShared {
     double Dg[CH_T][16][17];  // the diagonal tiles (wave 0's)
-    double Wl[2][16][17];     // W_s = R_ss^{-T} by step parity
+    double Wl[CH_T][16][17];  // W_s = R_ss^{-T}, kept for every s (the tile solve's operands)
     double Pn[2][16][CH_PLD];
     int failed;
 };
 
 // wave 0's part: the diagonal tiles in LDS, updated and factored one step ahead
-__device__ __forceinline__ void chol_diag_role(CholShared &sh, double *A, int n, int *info, int T) {
+__device__ __forceinline__ void chol_diag_role(CholShared &sh, double *A, int n, int *info, int T, double *Wp) {
     const int lane = threadIdx.x & 63, lr = lane >> 4, lc = lane & 15;
+    auto put_w = [&](int t) {
+        if (Wp != nullptr)
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) Wp[t * 256 + ks * 64 + lane] = sh.Wl[t][lc][4 * ks + lr];
+    };
     for (int a = 0; a < T; ++a)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -218,7 +229,7 @@ __device__ __forceinline__ void chol_diag_role(CholShared &sh, double *A, int n,
 #pragma unroll
             for (int r = 0; r < 4; ++r) sh.Dg[s + 1][lr + 4 * r][lc] -= acc[r];
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            const int bad = chol_factor_tile(sh.Dg[s + 1], sh.Wl[(s + 1) & 1], lane);
+            const int bad = chol_factor_tile(sh.Dg[s + 1], sh.Wl[s + 1], lane);
             if (bad && lane == 0) {
                 sh.failed = 1;
                 *info = 16 * (s + 1) + bad;
@@ -227,17 +238,19 @@ __device__ __forceinline__ void chol_diag_role(CholShared &sh, double *A, int n,
         }
         __syncthreads();
     }
-    for (int a = 0; a < T; ++a)
+    for (int a = 0; a < T; ++a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * a + lr + 4 * r, col = 16 * a + lc;
             if (row < n && col < n && row <= col) A[row + (int64_t)col * n] = sh.Dg[a][lr + 4 * r][lc];
         }
+        put_w(a);
+    }
     if (lane == 0) *info = 0;
 }
 
 // waves 1..15: the off-diagonal tiles in registers
-__device__ __forceinline__ void chol_offdiag_role(CholShared &sh, double *A, int n, int T, int wave) {
+__device__ __forceinline__ void chol_offdiag_role(CholShared &sh, double *A, int n, int T, int wave, double *Rp) {
     const int lane = threadIdx.x & 63, lr = lane >> 4, lc = lane & 15;
     const int noff = T * (T - 1) / 2;
     int ti_s[CH_SLOTS], tj_s[CH_SLOTS];
@@ -271,7 +284,7 @@ __device__ __forceinline__ void chol_offdiag_role(CholShared &sh, double *A, int
         // the tile row: X = W_s B, published
         double wa[4];
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) wa[ks] = sh.Wl[s & 1][lc][4 * ks + lr];
+        for (int ks = 0; ks < 4; ++ks) wa[ks] = sh.Wl[s][lc][4 * ks + lr];
 #pragma unroll
         for (int q = 0; q < CH_SLOTS; ++q)
             if (ti_s[q] == s) {
@@ -325,15 +338,90 @@ __device__ __forceinline__ void chol_offdiag_role(CholShared &sh, double *A, int
             const int row = 16 * ti_s[q] + lr + 4 * r, col = 16 * tj_s[q] + lc;
             if (row < n && col < n) A[row + (int64_t)col * n] = tile[q][r];
         }
+        if (Rp != nullptr) {
+            double *rp = Rp + 256 * (tj_s[q] * (tj_s[q] - 1) / 2 + ti_s[q]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rp[r * 64 + lane] = tile[q][r];
+        }
     }
 }
 
-__global__ __launch_bounds__(CH_THR) void chol_tiles_kernel(double *A, int n, int *info) {
+__global__ __launch_bounds__(CH_THR) void chol_tiles_kernel(double *A, int n, int *info, double *Rp, double *Wp) {
     __shared__ __attribute__((aligned(16))) CholShared sh;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int T = (n + 15) / 16;
-    if (wave == 0) chol_diag_role(sh, A, n, info, T);
-    else chol_offdiag_role(sh, A, n, T, wave);
+    if (wave == 0) chol_diag_role(sh, A, n, info, T, Wp);
+    else chol_offdiag_role(sh, A, n, T, wave, Rp);
+}
+
+// X R = B in place (X = B R^{-1}) for the d x n column-major B (leading
+// dimension ldb), with R the factor chol_tiles_kernel just formed, given as
+// its Rp / Wp tiles (n <= CH_MAX).  One wave per 16 rows of B, independent of
+// the others: with Y = X^T (that wave's 16 columns of it), R^T Y = B^T is
+// solved by tile rows, Y_j = W_j (B^T_j - sum_{k<j} R_kj^T Y_k), every product
+// a v_mfma_f64_16x16x4f64 and every Y_k held in registers in the accumulator
+// layout, which is also the B-operand layout the later products need.  The
+// sum over k runs in two chains (even and odd k) added at the end.
+// (TT = (n + 15) / 16 tiles, a template parameter so that every Y index is
+// a constant and Y stays in registers)
+template <int TT>
+__global__ __launch_bounds__(64) void trsm_tiles_kernel(double *B, int64_t ldb, int d, int n, const double *Rp,
+                                                        const double *Wp) {
+    const int lane = threadIdx.x, lr = lane >> 4, lc = lane & 15;
+    const int64_t row = (int64_t)blockIdx.x * 16 + lc;  // this lane's row of B
+    const bool rok = row < d;
+    double Y[TT][4];
+#pragma unroll
+    for (int j = 0; j < TT; ++j) {
+        chf64x4 acc0, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int col = 16 * j + lr + 4 * r;
+            acc0[r] = (rok && col < n) ? B[row + (int64_t)col * ldb] : 0.0;
+        }
+        // tile row j of R^T: all of its operands are loaded before the first
+        // product (one memory latency per tile row, not one per product)
+        double ra[TT > 1 ? TT - 1 : 1][4];
+        double w4[4];
+#pragma unroll
+        for (int k = 0; k < j; ++k) {
+            const double *rp = Rp + 256 * (j * (j - 1) / 2 + k);
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) ra[k][ks] = rp[ks * 64 + lane];
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) w4[ks] = Wp[256 * j + ks * 64 + lane];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < j; ++k) {
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                if (k & 1) acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(-ra[k][ks], Y[k][ks], acc1, 0, 0, 0);
+                else acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(-ra[k][ks], Y[k][ks], acc0, 0, 0, 0);
+            }
+        }
+        chf64x4 acc = acc0 + acc1;
+        chf64x4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) y = __builtin_amdgcn_mfma_f64_16x16x4f64(w4[ks], acc[ks], y, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            Y[j][r] = y[r];
+            const int col = 16 * j + lr + 4 * r;
+            if (rok && col < n) B[row + (int64_t)col * ldb] = y[r];
+        }
+    }
+}
+
+template <int TT>
+static void launch_trsm_tiles(double *B, int64_t d, int64_t n, const double *Rp, const double *Wp, hipStream_t st) {
+    if ((n + 15) / 16 == TT) {
+        hipLaunchKernelGGL(trsm_tiles_kernel<TT>, dim3((unsigned)((d + 15) / 16)), dim3(64), 0, st, B, d, (int)d, (int)n,
+                           Rp, Wp);
+        RBX_HIP(hipGetLastError());
+    } else if constexpr (TT > 1) {
+        launch_trsm_tiles<TT - 1>(B, d, n, Rp, Wp, st);
+    }
 }
 
 // R x = y (TRANS false) or R^T x = y (TRANS true) in place, R n x n upper
@@ -461,6 +549,7 @@ template <typename T>
 struct SolveWork {
     double *SA, *QR, *Sb, *y, *tau;  // d x n, d x n, d, d, n
     double *G, *R2, *Rc;             // n x n each: R1 (CholeskyQR2), R2, R = R2 R1
+    double *Rp, *Wp;                 // the factor's tiles for trsm_tiles_kernel (n <= CH_MAX)
     int *info;                       // potrf status of the two passes, and the check flag
     T *tmpT;                         // d x (n + 1) in T (SA, then Sb: the sketches before promotion)
     // the factorization: explicit Q in QR (CholeskyQR2) or Householder
@@ -474,8 +563,9 @@ template <typename T>
 static SolveWork<T> solve_work(int64_t d, int64_t n, hipStream_t st) {
     auto a = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t dn = (size_t)safe_mul(d, n), nn = (size_t)safe_mul(n, n);
+    const size_t rp_bytes = (size_t)CH_T * (CH_T - 1) / 2 * 256 * 8, wp_bytes = (size_t)CH_T * 256 * 8;
     const size_t bytes = 2 * a(dn * 8) + 2 * a((size_t)d * 8) + a((size_t)n * 8) + 3 * a(nn * 8) + a(16) +
-                         a((dn + (size_t)d) * sizeof(T));
+                         a((dn + (size_t)d) * sizeof(T)) + a(rp_bytes) + a(wp_bytes);
     char *ws = (char *)scratch(SCRATCH_SOLVE, bytes, st);
     SolveWork<T> w;
     size_t off = 0;
@@ -494,6 +584,8 @@ static SolveWork<T> solve_work(int64_t d, int64_t n, hipStream_t st) {
     w.Rc = (double *)take(nn * 8);
     w.info = (int *)take(16);
     w.tmpT = (T *)take((dn + (size_t)d) * sizeof(T));
+    w.Rp = (double *)take(rp_bytes);
+    w.Wp = (double *)take(wp_bytes);
     return w;
 }
 
@@ -515,14 +607,17 @@ static void cholqr2(int64_t d, int64_t n, SolveWork<T> &w, hipStream_t st) {
                               D, &zero, Rk[pass], Nn),
                 "gemm");
         if (n <= CH_MAX) {
-            hipLaunchKernelGGL(chol_tiles_kernel, dim3(1), dim3(CH_THR), 0, st, Rk[pass], (int)n, w.info + pass);
+            // Q = QR R^{-1} by the factor's tiles, one wave per 16 rows
+            hipLaunchKernelGGL(chol_tiles_kernel, dim3(1), dim3(CH_THR), 0, st, Rk[pass], (int)n, w.info + pass,
+                               w.Rp, w.Wp);
             RBX_HIP(hipGetLastError());
+            launch_trsm_tiles<CH_T>(w.QR, d, n, (const double *)w.Rp, (const double *)w.Wp, st);
         } else {
             blas_ok(rocsolver_dpotrf(h, rocblas_fill_upper, Nn, Rk[pass], Nn, w.info + pass), "potrf");
+            blas_ok(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, D, Nn, &one, Rk[pass], Nn, w.QR, D),
+                    "trsm");
         }
-        blas_ok(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                              rocblas_diagonal_non_unit, D, Nn, &one, Rk[pass], Nn, w.QR, D),
-                "trsm");
     }
     hipLaunchKernelGGL(cholqr_check_kernel, dim3(1), dim3(256), 0, st, (const double *)w.R2, n, (const int *)w.info,
                        w.info + 2);

@@ -260,6 +260,8 @@ def test_factorization_path(rb, orc, monkeypatch, cond, path):
     x = torch.empty(n, dtype=torch.float64, device=DEV)
     ran = _ran(rb, lambda: rb.sketch_solve("C", d, m, n, S, 0, 0, _dev(_store(A, "C")), m, _dev(b), x))
     assert path in ran, ran
+    if path == "solve_cholqr2":
+        assert "solve_householder" not in ran, ran  # CholeskyQR2 accepted, no fallback
     err = np.linalg.norm(host(x) - x_ref) / np.linalg.norm(x_ref)
     print(f"cond {cond:.0e} via {path}: {err:.2e}")
     assert err <= (1e-11 if cond < 1e3 else 1e-4), err
