@@ -127,6 +127,7 @@ def test_joint_sketch_of_a_and_b(rb, orc, dtype, tol, n):
     ran = _ran(rb, lambda: rb.sketch_solve("C", d, m, n, S, 0, 0, Ad, m, bd, x))
     wide = "f32_wide_kernel" if dtype == torch.float32 else "f64_wide_kernel"
     assert wide in ran and "gemv_gen_kernel" not in ran, ran
+    assert "solve_householder" not in ran, ran  # well conditioned: CholeskyQR2 accepted
     err = np.linalg.norm(host(x).astype(np.float64) - x_ref) / np.linalg.norm(x_ref)
     assert err <= tol, err
     # the adjoint: [Abar bbar] = S^T [M w] in one pass
