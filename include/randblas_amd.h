@@ -360,7 +360,8 @@ rbx_status rbx_drskges(char layout, char opA, char opS, int64_t m, int64_t d, in
  * other shapes return RBX_INVALID_ARG or RBX_UNSUPPORTED with a message and
  * launch nothing.  The array is one per GPU and shared by every call on that
  * GPU: calls whose execution overlaps on one GPU (other streams, other host
- * threads) race. */
+ * threads) race.  Over a whole operator that holds its prepared copy (S->prep)
+ * the call reads that copy and uses no scratch at all, whatever its size. */
 rbx_status rbx_slskges_noalloc(char layout, char opS, char opA, int64_t d, int64_t n, int64_t m,
                                float alpha, const rbx_sparse_skop *S, int64_t ro_s, int64_t co_s,
                                const float *A, int64_t lda, float beta, float *B, int64_t ldb,
