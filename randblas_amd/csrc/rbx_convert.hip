@@ -46,8 +46,16 @@ __global__ __launch_bounds__(256) void sort_keys_kernel(const int64_t *maj, cons
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void gather3_kernel(const int64_t *perm, int64_t nnz, const T *v, const int64_t *a,
+__global__ __launch_bounds__(256) void major_keys_kernel(const int64_t *maj, int64_t nnz, uint32_t *keys,
+                                                         int32_t *perm) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        keys[e] = (uint32_t)maj[e];
+        perm[e] = (int32_t)e;
+    }
+}
+
+template <typename T, typename PT>
+__global__ __launch_bounds__(256) void gather3_kernel(const PT *perm, int64_t nnz, const T *v, const int64_t *a,
                                                       const int64_t *b, T *vo, int64_t *ao, int64_t *bo) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t s = perm[e];
@@ -69,7 +77,10 @@ __global__ __launch_bounds__(256) void order_flags_kernel(const int64_t *rows, c
         if (!(j0 < j1 || (j0 == j1 && i0 <= i1))) bad |= 1;
         if (!(i0 < i1 || (i0 == i1 && j0 <= j1))) bad |= 2;
     }
-    if (bad) atomicOr(flags, bad);
+    // one atomic per wave (one per thread serialises millions of them on the
+    // flag's cache line)
+    const int wbad = (__ballot(bad & 1) ? 1 : 0) | (__ballot(bad & 2) ? 2 : 0);
+    if ((threadIdx.x & 63) == 0 && wbad) atomicOr(flags, wbad);
 }
 
 __global__ void int_zero_kernel(int *p) { *p = 0; }
@@ -86,13 +97,66 @@ static int64_t blocks_for(int64_t n) {
     return b > 16384 ? 16384 : b;
 }
 
+// How the entries already lie: 0 ordered by (maj, mnr), 1 by (mnr, maj), 2
+// neither (one pass over the indices and a 4-byte read on the host: the
+// reference keeps this as COOMatrix::sort and skips its sort by it,
+// conversions.hh:87-117).
+static int coo_input_order(const int64_t *maj, const int64_t *mnr, int64_t nnz, hipStream_t st) {
+    if (nnz < 2) return 0;
+    int *flags = (int *)scratch(SCRATCH_SMALL, sizeof(int), st);
+    hipLaunchKernelGGL(int_zero_kernel, dim3(1), dim3(1), 0, st, flags);
+    int64_t nb = blocks_for(nnz);
+    if (nb > 4 * (int64_t)num_cus()) nb = 4 * (int64_t)num_cus();
+    hipLaunchKernelGGL(order_flags_kernel, dim3((unsigned)nb), dim3(256), 0, st, maj, mnr, nnz, flags);
+    RBX_HIP(hipGetLastError());
+    int h = 0;
+    RBX_HIP(hipMemcpyAsync(&h, flags, sizeof(h), hipMemcpyDeviceToHost, st));
+    RBX_HIP(hipStreamSynchronize(st));
+    return !(h & 2) ? 0 : (!(h & 1) ? 1 : 2);
+}
+
 // Stable sort of the COO arrays (vals, maj, mnr) by (maj, mnr) into the
-// output arrays (which may be the inputs).
+// output arrays (which may be the inputs).  Entries already in that order are
+// copied; entries ordered by (mnr, maj) need a stable sort on maj alone
+// (within a major index they already ascend in the minor one, duplicates in
+// stored order), on 32-bit keys; others sort on the full key.
 template <typename T>
 static void sort_coo(int64_t nnz, int64_t nmaj, int64_t nmnr, const T *v, const int64_t *maj, const int64_t *mnr, T *vo,
                      int64_t *majo, int64_t *mnro, hipStream_t st) {
     if (nnz == 0) return;
     RBX_REQUIRE(nmaj > 0 && nmnr > 0);
+    const int ord = coo_input_order(maj, mnr, nnz, st);
+    if (ord == 0) {
+        if (vo != v) RBX_HIP(hipMemcpyAsync(vo, v, (size_t)nnz * sizeof(T), hipMemcpyDeviceToDevice, st));
+        if (majo != maj) RBX_HIP(hipMemcpyAsync(majo, maj, (size_t)nnz * 8, hipMemcpyDeviceToDevice, st));
+        if (mnro != mnr) RBX_HIP(hipMemcpyAsync(mnro, mnr, (size_t)nnz * 8, hipMemcpyDeviceToDevice, st));
+        return;
+    }
+    if (ord == 1 && (uint64_t)nmaj <= 0xFFFFFFFFull && nnz < (int64_t)INT32_MAX) {
+        const int end_bit = bits_of((uint64_t)(nmaj - 1));
+        size_t sort_bytes = 0;
+        RBX_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                          (int32_t *)nullptr, (int32_t *)nullptr, (size_t)nnz, 0, end_bit, st));
+        const size_t b4 = ((size_t)nnz * 4 + 255) / 256 * 256, b8 = ((size_t)nnz * 8 + 255) / 256 * 256,
+                     bt = ((size_t)nnz * sizeof(T) + 255) / 256 * 256;
+        char *ws = (char *)scratch(SCRATCH_SPARSE_CSR, 4 * b4 + 2 * b8 + bt + sort_bytes, st);
+        uint32_t *k_in = (uint32_t *)ws, *k_out = (uint32_t *)(ws + b4);
+        int32_t *p_in = (int32_t *)(ws + 2 * b4), *p_out = (int32_t *)(ws + 3 * b4);
+        int64_t *a_tmp = (int64_t *)(ws + 4 * b4), *b_tmp = (int64_t *)(ws + 4 * b4 + b8);
+        T *v_tmp = (T *)(ws + 4 * b4 + 2 * b8);
+        void *sort_tmp = ws + 4 * b4 + 2 * b8 + bt;
+        const unsigned nb = (unsigned)blocks_for(nnz);
+        hipLaunchKernelGGL(major_keys_kernel, dim3(nb), dim3(256), 0, st, maj, nnz, k_in, p_in);
+        RBX_HIP(hipGetLastError());
+        RBX_HIP(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, k_in, k_out, p_in, p_out, (size_t)nnz, 0, end_bit, st));
+        hipLaunchKernelGGL((gather3_kernel<T, int32_t>), dim3(nb), dim3(256), 0, st, p_out, nnz, v, maj, mnr, v_tmp,
+                           a_tmp, b_tmp);
+        RBX_HIP(hipGetLastError());
+        RBX_HIP(hipMemcpyAsync(vo, v_tmp, (size_t)nnz * sizeof(T), hipMemcpyDeviceToDevice, st));
+        RBX_HIP(hipMemcpyAsync(majo, a_tmp, (size_t)nnz * 8, hipMemcpyDeviceToDevice, st));
+        RBX_HIP(hipMemcpyAsync(mnro, b_tmp, (size_t)nnz * 8, hipMemcpyDeviceToDevice, st));
+        return;
+    }
     const uint64_t span = (uint64_t)safe_mul(nmaj, nmnr);
     const int end_bit = bits_of(span - 1);
     size_t sort_bytes = 0;
@@ -110,7 +174,8 @@ static void sort_coo(int64_t nnz, int64_t nmaj, int64_t nmnr, const T *v, const 
     RBX_HIP(hipGetLastError());
     RBX_HIP(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, k_in, k_out, p_in, p_out, (size_t)nnz, 0, end_bit, st));
     // gather through scratch so that the output may alias the input
-    hipLaunchKernelGGL((gather3_kernel<T>), dim3(nb), dim3(256), 0, st, p_out, nnz, v, maj, mnr, v_tmp, a_tmp, b_tmp);
+    hipLaunchKernelGGL((gather3_kernel<T, int64_t>), dim3(nb), dim3(256), 0, st, p_out, nnz, v, maj, mnr, v_tmp, a_tmp,
+                       b_tmp);
     RBX_HIP(hipGetLastError());
     RBX_HIP(hipMemcpyAsync(vo, v_tmp, (size_t)nnz * sizeof(T), hipMemcpyDeviceToDevice, st));
     RBX_HIP(hipMemcpyAsync(majo, a_tmp, (size_t)nnz * 8, hipMemcpyDeviceToDevice, st));

@@ -32,9 +32,16 @@ def random_coo(rng, m, n, nnz, dup=True):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("m,n,nnz", [(50, 40, 300), (1000, 3, 2000), (7, 9000, 500), (5, 5, 0), (1, 1, 1)])
-def test_coo_to_csr_and_csc(rb, dtype, m, n, nnz):
+@pytest.mark.parametrize("given", ["shuffled", "by_row", "by_col"])
+def test_coo_to_csr_and_csc(rb, dtype, m, n, nnz, given):
+    """Entries given in any order, ordered by (row, column) (CSR order: the
+    conversion to CSR copies, to CSC sorts on the column alone) or by
+    (column, row); duplicates stay in stored order."""
     rng = np.random.default_rng(m + n + nnz)
     r, c, v = random_coo(rng, m, n, nnz)
+    if given != "shuffled":
+        o = np.lexsort((c, r)) if given == "by_row" else np.lexsort((r, c))
+        r, c, v = r[o], c[o], v[o]
     v = v.astype(dtype)
     coo = rb.COOMatrix(m, n, t(v), t(r), t(c))
     # CSR
