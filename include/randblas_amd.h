@@ -466,9 +466,13 @@ rbx_status rbx_drequire_symmetric(char layout, const double *A, int64_t n, int64
  * n_cols + 1); the sort is stable, so equal (row, column) pairs keep their
  * stored order.  coo_sort(order, coo) (coo_arrays_apply_sort,
  * sparse_data/base.hh:169-220) sorts a COO matrix in place, order 'R' (CSR)
- * or 'C' (CSC).  coo_determine_sort (base.hh:142-167) sets *order to 'C' if
- * the entries are in CSC order, else 'R' if in CSR order, else 'N', and
- * synchronises the stream.  Zero-based indices. */
+ * or 'C' (CSC).  coo_to_csr, coo_to_csc and coo_sort first find how the
+ * entries already lie (one pass and a 4-byte read, so they synchronise the
+ * stream) and copy entries already in the target order, or sort entries in
+ * the other order on the major index alone (same result).
+ * coo_determine_sort (base.hh:142-167) sets *order to 'C' if the entries are
+ * in CSC order, else 'R' if in CSR order, else 'N', and synchronises the
+ * stream.  Zero-based indices. */
 rbx_status rbx_scsr_to_coo(const rbx_csr_matrix *csr, rbx_coo_matrix *coo, void *stream);
 rbx_status rbx_dcsr_to_coo(const rbx_csr_matrix *csr, rbx_coo_matrix *coo, void *stream);
 rbx_status rbx_scsc_to_coo(const rbx_csc_matrix *csc, rbx_coo_matrix *coo, void *stream);
