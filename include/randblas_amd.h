@@ -318,10 +318,12 @@ rbx_status rbx_sparse_skop_free(rbx_sparse_skop *S);
  * bit for bit.  Leaves S->prep NULL, and returns RBX_OK, when the operator is
  * outside that path (more than 8192 rows, rows of more than 2048 entries on
  * average, a copy above four times the operator's own bytes) or the memory
- * cannot be had.  The copy is complete when `stream` reaches this call; it
- * stays valid until S's arrays change, which callers that write them in place
- * signal by calling this again or rbx_sparse_skop_free.  rbx_[sd]fill_sparse_skop
- * calls it itself. */
+ * cannot be had.  It finds whether every value is +1 or -1 (as a SASO's
+ * are), so that the applies take the signs from the copy's layout and read no
+ * values; that check synchronises `stream` once.  The copy is complete when
+ * `stream` reaches this call; it stays valid until S's arrays change, which
+ * callers that write them in place signal by calling this again or
+ * rbx_sparse_skop_free.  rbx_[sd]fill_sparse_skop calls it itself. */
 rbx_status rbx_ssparse_skop_prepare(rbx_sparse_skop *S, void *stream);
 rbx_status rbx_dsparse_skop_prepare(rbx_sparse_skop *S, void *stream);
 

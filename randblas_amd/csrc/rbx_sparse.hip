@@ -861,6 +861,7 @@ struct SparsePrep {
     void *val;
     char *lay;
     size_t lay_bytes;
+    bool unit;  // every value is exactly +1 or -1 (the layout carries the signs)
 };
 constexpr uint64_t PREP_MAGIC = 0x70657270787862ull;
 // the copy may take at most this many times the operator's own bytes
@@ -914,7 +915,7 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
         RBX_REQUIRE(opA == 'N' && ro_a == 0 && co_a == 0 && d == prep->d && m == prep->m && nnz == prep->nnz);
         ProfScope prof("csr_lds_apply_kernel", st);
         if (csr_lds_apply_laid<T>(prep->row_ptr, prep->col, (const T *)prep->val, d, n, m, nnz, alpha, beta, B, ldb, C,
-                                  crs, ccs, kib ? SPF_KIB : SPF_JKI, prep->lay, prep->lay_bytes, st))
+                                  crs, ccs, kib ? SPF_KIB : SPF_JKI, prep->lay, prep->lay_bytes, st, prep->unit))
             return;
     }
 
@@ -1048,15 +1049,40 @@ static void prep_free(rbx_sparse_skop *S) {
     delete p;
 }
 
+__global__ void prep_flag_zero_kernel(int *f) { *f = 0; }
+// *f |= 1 when some value is not exactly +1 or -1 (one atomic per wave that saw one)
+template <typename T>
+__global__ __launch_bounds__(256) void unit_values_kernel(const T *v, int64_t n, int *f) {
+    bool other = false;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        other |= !(v[i] == (T)1 || v[i] == (T)-1);
+    if (__any(other) && (threadIdx.x & 63) == 0) atomicOr(f, 1);
+}
+
 // rbx_[sd]sparse_skop_prepare: S->prep for S's current arrays, or null when
-// the operator is outside the LDS apply or the memory cannot be had.
+// the operator is outside the LDS apply or the memory cannot be had.  Finding
+// whether every value is +1 or -1 (a SASO's are) reads one flag back, so this
+// synchronises the stream.
 template <typename T>
 static void prepare_skop(rbx_sparse_skop *S, hipStream_t st) {
     prep_free(S);
     if (S->nnz <= 0 || S->vals == nullptr || S->rows == nullptr || S->cols == nullptr) return;
     const int64_t d = S->dist.n_rows, m = S->dist.n_cols, nnz = S->nnz;
     if (d > CSRB_MAX_ROWS || nnz > LDS_APPLY_MAX_MEAN_ROW * d || m >= (int64_t)INT32_MAX) return;
-    const size_t lay = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T));
+    bool unit = false;
+    {
+        int *f = (int *)scratch(SCRATCH_SMALL, sizeof(int), st);
+        hipLaunchKernelGGL(prep_flag_zero_kernel, dim3(1), dim3(1), 0, st, f);
+        int64_t nb = (nnz + 255) / 256;
+        if (nb > 4 * (int64_t)num_cus()) nb = 4 * (int64_t)num_cus();
+        hipLaunchKernelGGL((unit_values_kernel<T>), dim3((unsigned)nb), dim3(256), 0, st, (const T *)S->vals, nnz, f);
+        RBX_HIP(hipGetLastError());
+        int h = 1;
+        RBX_HIP(hipMemcpyAsync(&h, f, sizeof(h), hipMemcpyDeviceToHost, st));
+        RBX_HIP(hipStreamSynchronize(st));
+        unit = h == 0;
+    }
+    const size_t lay = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T), unit);
     const size_t bR = align256((size_t)(d + 1) * 8), bC = align256((size_t)nnz * 4);
     const size_t bV = align256((size_t)nnz * sizeof(T));
     const size_t total = bR + bC + bV + lay;
@@ -1089,11 +1115,12 @@ static void prepare_skop(rbx_sparse_skop *S, hipStream_t st) {
     p->val = mem + bR + bC;
     p->lay = mem + bR + bC + bV;
     p->lay_bytes = lay;
+    p->unit = unit;
     S->prep = p;  // owned by S from here, so prep_free releases it on any failure below
     RBX_HIP(hipMemcpyAsync(p->row_ptr, row_ptr, (size_t)(d + 1) * 8, hipMemcpyDeviceToDevice, st));
     RBX_HIP(hipMemcpyAsync(p->col, ccol, (size_t)nnz * 4, hipMemcpyDeviceToDevice, st));
     RBX_HIP(hipMemcpyAsync(p->val, cval, (size_t)nnz * sizeof(T), hipMemcpyDeviceToDevice, st));
-    if (!csr_lds_layout<T>(p->row_ptr, p->col, (const T *)p->val, d, m, nnz, p->lay, lay, st)) prep_free(S);
+    if (!csr_lds_layout<T>(p->row_ptr, p->col, (const T *)p->val, d, m, nnz, p->lay, lay, st, unit)) prep_free(S);
 }
 
 template <typename T>

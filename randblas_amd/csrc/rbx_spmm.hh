@@ -49,18 +49,20 @@ bool csr_build_counting(const int64_t *rows, const int64_t *cols, const T *vals,
 // entries laid out by chunk and slot; every byte read is written first in the
 // same call).  Returns false when the shape is outside what the kernel takes
 // (alpha == 0 and beta-only scaling are the caller's).
-size_t csr_lds_apply_ws_bytes(int64_t d, int64_t m, int64_t nnz, size_t val_size);
+size_t csr_lds_apply_ws_bytes(int64_t d, int64_t m, int64_t nnz, size_t val_size, bool unit = false);
 // csr_lds_apply in two steps, for an operator applied more than once: the
 // entry layout (chunk table and slots, depending on d, m and the entries only)
 // into `ws`, then any number of applies that read it.  Each returns false,
 // with nothing launched, when the shape is outside what the kernel takes.
+// `unit` (the layout's: every value is exactly +1 or -1) lets the applies
+// take each value's sign from its slot instead of loading the value.
 template <typename T>
 bool csr_lds_layout(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t m, int64_t nnz,
-                    char *ws, size_t ws_bytes, hipStream_t st);
+                    char *ws, size_t ws_bytes, hipStream_t st, bool unit);
 template <typename T>
 bool csr_lds_apply_laid(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
                         int64_t nnz, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs,
-                        int form, const char *ws, size_t ws_bytes, hipStream_t st);
+                        int form, const char *ws, size_t ws_bytes, hipStream_t st, bool unit);
 template <typename T>
 bool csr_lds_apply(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
                    int64_t nnz, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int form,

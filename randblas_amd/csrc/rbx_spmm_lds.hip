@@ -376,6 +376,9 @@ bool csr_build_counting(const int64_t *rows, const int64_t *cols, const T *vals,
 // chunk there (six in FP64), which evens out the wave's per-chunk maximum.
 constexpr int la_ckmax(size_t val_size) { return val_size == 4 ? 2048 : 1024; }
 constexpr int la_maxe(size_t val_size) { return val_size == 4 ? 20 : 12; }
+// the 16-bit slots of unit layouts: two per word, the column within a chunk below 2^15
+static_assert(la_maxe(4) % 2 == 0 && la_maxe(8) % 2 == 0 && la_ckmax(4) <= 0x8000 && la_ckmax(8) <= 0x8000,
+              "16-bit slot packing");
 
 // Chunk depth: a power of two in [64, la_ckmax] giving rows about six (FP64)
 // or ten (FP32) stored entries per chunk or fewer.
@@ -389,21 +392,23 @@ static int la_chunk(int64_t d, int64_t m, int64_t nnz, size_t val_size) {
 struct LaPlan {
     int ck;
     int64_t nk;
-    size_t bStart, bEc, bEv;
-    size_t bytes() const { return bStart + bEc + bEv; }
+    size_t bStart, bEc, bEv, bEp;
+    size_t bytes() const { return bStart + bEc + bEv + bEp; }
 };
-static LaPlan la_plan(int64_t d, int64_t m, int64_t nnz, size_t val_size) {
+// `unit`: the layout also holds the 16-bit slots (the last region)
+static LaPlan la_plan(int64_t d, int64_t m, int64_t nnz, size_t val_size, bool unit) {
     LaPlan p;
     p.ck = la_chunk(d, m, nnz, val_size);
     p.nk = (m + p.ck - 1) / p.ck;
     p.bStart = al256((size_t)(p.nk + 1) * (size_t)d * 4);
     p.bEc = al256((size_t)p.nk * la_maxe(val_size) * (size_t)d * 4);
     p.bEv = al256((size_t)p.nk * la_maxe(val_size) * (size_t)d * val_size);
+    p.bEp = unit ? al256((size_t)p.nk * la_maxe(val_size) * (size_t)d * 2) : 0;
     return p;
 }
 
-size_t csr_lds_apply_ws_bytes(int64_t d, int64_t m, int64_t nnz, size_t val_size) {
-    return la_plan(d, m, nnz, val_size).bytes();
+size_t csr_lds_apply_ws_bytes(int64_t d, int64_t m, int64_t nnz, size_t val_size, bool unit) {
+    return la_plan(d, m, nnz, val_size, unit).bytes();
 }
 
 // startT[k * d + r] = position of row r's first entry with column >= k*CK
@@ -430,10 +435,16 @@ __global__ __launch_bounds__(256) void csr_chunk_table_kernel(const int64_t *row
 
 // One workgroup per row (grid-stride): each entry p of row r goes to slot
 // q = p - startT[k * d + r] of its chunk k when q < la_maxe(sizeof(T)).
+// With `unit` (every value is +1 or -1) bit 31 of the slot's column holds
+// the value's sign, and the slot is also written in 16 bits, the column
+// within the chunk (below 2048) with the sign in bit 15, two slots of a row
+// to a 32-bit word (Ep[(k * MAXE / 2 + q / 2) * d + r], slot q in half q % 2),
+// so the applies that know this read one word per two slots and no values.
 template <typename T>
 __global__ __launch_bounds__(256) void csr_chunk_slots_kernel(const int64_t *row_ptr, const int32_t *col,
                                                               const T *val, int64_t d, int CK,
-                                                              const int32_t *startT, int32_t *Ec, T *Ev) {
+                                                              const int32_t *startT, int32_t *Ec, T *Ev,
+                                                              uint16_t *Ep, int unit) {
     for (int64_t r = blockIdx.x; r < d; r += gridDim.x) {
         const int64_t p1 = row_ptr[r + 1];
         for (int64_t p = row_ptr[r] + threadIdx.x; p < p1; p += blockDim.x) {
@@ -443,8 +454,12 @@ __global__ __launch_bounds__(256) void csr_chunk_slots_kernel(const int64_t *row
             constexpr int MAXE = la_maxe(sizeof(T));
             if (q < MAXE) {
                 const int64_t o = (k * MAXE + q) * d + r;
-                Ec[o] = c;
-                Ev[o] = val[p];
+                const T v = val[p];
+                Ec[o] = (unit && v < (T)0) ? (int32_t)((uint32_t)c | 0x80000000u) : c;
+                Ev[o] = v;
+                if (unit)
+                    Ep[2 * ((k * (MAXE / 2) + q / 2) * d + r) + (q & 1)] =
+                        (uint16_t)((c - k * CK) | (v < (T)0 ? 0x8000 : 0));
             }
         }
     }
@@ -464,10 +479,15 @@ __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint3
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
 
-// A1: alpha == 1, so the products by alpha (exact then) are left out
-template <typename T, int J, int NT, int FORM, bool A1>
+// A1: alpha == 1, so the products by alpha (exact then) are left out.
+// UNIT: every value is +1 or -1 (a layout built with `unit`): the slots come
+// from the 16-bit words Ep, two per load, column within the chunk and sign,
+// and no values are loaded; otherwise bit 31 of the slot's column in Ec is
+// masked off (a unit layout serves both).
+template <typename T, int J, int NT, int FORM, bool A1, bool UNIT>
 __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT, const int32_t *Ec, const T *Ev,
-                                                           int nk, int CK, const int32_t *col, const T *val,
+                                                           const uint32_t *Ep, int nk, int CK, const int32_t *col,
+                                                           const T *val,
                                                            int64_t d, int64_t n, int64_t m, T alpha, T beta,
                                                            const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs,
                                                            int RW, int ncb) {
@@ -514,17 +534,23 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
         return active ? v : 0;
     };
 
-    // register banks of slots: chunk k in bank k % 2
+    // register banks of slots: chunk k in bank k % 2 (UNIT: two 16-bit slots
+    // per word in the first LA_MAXE / 2 entries of bc)
     int32_t bc0[LA_MAXE], bc1[LA_MAXE];
     T bv0[LA_MAXE], bv1[LA_MAXE];
     // every slot is loaded (a fixed number of loads, so the waits before the
     // tile stores cover only the tile's own, older loads)
     auto load_bank = [&](int64_t k, int32_t *bc, T *bv) {
+        if constexpr (UNIT) {
 #pragma unroll
-        for (int q = 0; q < LA_MAXE; ++q) {
-            const int64_t o = (k * LA_MAXE + q) * d + rs;
-            bc[q] = Ec[o];
-            bv[q] = Ev[o];
+            for (int q = 0; q < LA_MAXE / 2; ++q) bc[q] = (int32_t)Ep[(k * (LA_MAXE / 2) + q) * d + rs];
+        } else {
+#pragma unroll
+            for (int q = 0; q < LA_MAXE; ++q) {
+                const int64_t o = (k * LA_MAXE + q) * d + rs;
+                bc[q] = Ec[o];
+                bv[q] = Ev[o];
+            }
         }
     };
     auto fma_one = [&](int jj, T b, T v) {
@@ -557,7 +583,14 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
         const int32_t cnt = e - s;
 #pragma unroll
         for (int q = 0; q < LA_MAXE; ++q)
-            if (q < wmax && q < cnt) apply_one(Ab, bc[q] - cbase, bv[q]);
+            if (q < wmax && q < cnt) {
+                if constexpr (UNIT) {
+                    const uint32_t e = ((uint32_t)bc[q >> 1] >> (16 * (q & 1))) & 0xffffu;
+                    apply_one(Ab, (int32_t)(e & 0x7fffu), (e & 0x8000u) ? (T)-1 : (T)1);
+                } else {
+                    apply_one(Ab, (int32_t)((uint32_t)bc[q] & 0x7fffffffu) - cbase, bv[q]);
+                }
+            }
         for (int32_t p = s + LA_MAXE; p < e; ++p) apply_one(Ab, col[p] - cbase, val[p]);
     };
 
@@ -647,20 +680,31 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
 }
 
 template <typename T, int J, int NT>
-static void launch_lds_apply(int form, dim3 grid, const int32_t *startT, const int32_t *Ec, const T *Ev, int nk,
+static void launch_lds_apply(int form, dim3 grid, const int32_t *startT, const int32_t *Ec, const T *Ev,
+                             const uint32_t *Ep, int nk,
                              int ck, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m, T alpha,
                              T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int RW, int ncb,
-                             hipStream_t st) {
-#define RBX_LA_LAUNCH(F, A1)                                                                                  \
-    hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, NT, F, A1>), grid, dim3(NT), 0, st, startT, Ec, Ev, nk, ck, col, \
-                       val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb)
+                             bool unit, hipStream_t st) {
+#define RBX_LA_LAUNCH(F, A1, U)                                                                                   \
+    hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, NT, F, A1, U>), grid, dim3(NT), 0, st, startT, Ec, Ev, Ep, nk, ck, \
+                       col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb)
     const bool one = alpha == (T)1;
-    if (form == SPF_KIB) {
-        if (one) RBX_LA_LAUNCH(SPF_KIB, true);
-        else RBX_LA_LAUNCH(SPF_KIB, false);
+    // the value-free variant only for the widest column blocks (the others
+    // read the values of a unit layout like any other)
+    if (unit && J == 8) {
+        if (form == SPF_KIB) {
+            if (one) RBX_LA_LAUNCH(SPF_KIB, true, true);
+            else RBX_LA_LAUNCH(SPF_KIB, false, true);
+        } else {
+            if (one) RBX_LA_LAUNCH(SPF_JKI, true, true);
+            else RBX_LA_LAUNCH(SPF_JKI, false, true);
+        }
+    } else if (form == SPF_KIB) {
+        if (one) RBX_LA_LAUNCH(SPF_KIB, true, false);
+        else RBX_LA_LAUNCH(SPF_KIB, false, false);
     } else {
-        if (one) RBX_LA_LAUNCH(SPF_JKI, true);
-        else RBX_LA_LAUNCH(SPF_JKI, false);
+        if (one) RBX_LA_LAUNCH(SPF_JKI, true, false);
+        else RBX_LA_LAUNCH(SPF_JKI, false, false);
     }
 #undef RBX_LA_LAUNCH
     RBX_HIP(hipGetLastError());
@@ -668,10 +712,11 @@ static void launch_lds_apply(int form, dim3 grid, const int32_t *startT, const i
 
 template <typename T, int NT>
 static void launch_lds_apply_j(int J, int form, dim3 grid, const int32_t *startT, const int32_t *Ec, const T *Ev,
-                               int nk, int ck, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
+                               const uint32_t *Ep, int nk, int ck, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
                                T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int RW,
-                               int ncb, hipStream_t st) {
-#define RBX_LA_ARGS form, grid, startT, Ec, Ev, nk, ck, col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb, st
+                               int ncb, bool unit, hipStream_t st) {
+#define RBX_LA_ARGS \
+    form, grid, startT, Ec, Ev, Ep, nk, ck, col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb, unit, st
     if (J == 8) launch_lds_apply<T, 8, NT>(RBX_LA_ARGS);
     else if (J == 4) launch_lds_apply<T, 4, NT>(RBX_LA_ARGS);
     else if (J == 2) launch_lds_apply<T, 2, NT>(RBX_LA_ARGS);
@@ -686,14 +731,15 @@ static bool lds_apply_fits(int64_t d, int64_t m, size_t val_size) {
 
 template <typename T>
 bool csr_lds_layout(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t m, int64_t nnz,
-                    char *ws, size_t ws_bytes, hipStream_t st) {
+                    char *ws, size_t ws_bytes, hipStream_t st, bool unit) {
     if (!lds_apply_fits(d, m, sizeof(T)) || ws == nullptr) return false;
-    const LaPlan pl = la_plan(d, m, nnz, sizeof(T));
+    const LaPlan pl = la_plan(d, m, nnz, sizeof(T), unit);
     RBX_REQUIRE(ws_bytes >= pl.bytes());
     if (pl.nk >= (1ll << 30)) return false;
     int32_t *startT = (int32_t *)ws;
     int32_t *Ec = (int32_t *)(ws + pl.bStart);
     T *Ev = (T *)(ws + pl.bStart + pl.bEc);
+    uint16_t *Ep = (uint16_t *)(ws + pl.bStart + pl.bEc + pl.bEv);
     const int64_t tcells = d * (pl.nk + 1);
     hipLaunchKernelGGL(csr_chunk_table_kernel, dim3((unsigned)((tcells + 255) / 256)), dim3(256), 0, st, row_ptr, col,
                        d, (int)pl.nk, pl.ck, startT);
@@ -701,7 +747,7 @@ bool csr_lds_layout(const int64_t *row_ptr, const int32_t *col, const T *val, in
     const int64_t cus = num_cus();
     const int64_t sg = d < 4 * cus ? d : 4 * cus;
     hipLaunchKernelGGL((csr_chunk_slots_kernel<T>), dim3((unsigned)sg), dim3(256), 0, st, row_ptr, col, val, d, pl.ck,
-                       (const int32_t *)startT, Ec, Ev);
+                       (const int32_t *)startT, Ec, Ev, Ep, (int)unit);
     RBX_HIP(hipGetLastError());
     return true;
 }
@@ -709,10 +755,10 @@ bool csr_lds_layout(const int64_t *row_ptr, const int32_t *col, const T *val, in
 template <typename T>
 bool csr_lds_apply_laid(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
                         int64_t nnz, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs,
-                        int form, const char *ws, size_t ws_bytes, hipStream_t st) {
+                        int form, const char *ws, size_t ws_bytes, hipStream_t st, bool unit) {
     (void)row_ptr;
     if (n <= 0 || !lds_apply_fits(d, m, sizeof(T)) || ws == nullptr) return false;
-    const LaPlan pl = la_plan(d, m, nnz, sizeof(T));
+    const LaPlan pl = la_plan(d, m, nnz, sizeof(T), unit);
     RBX_REQUIRE(ws_bytes >= pl.bytes());
     // 512 rows per workgroup (256 for few rows), the widest column block
     // (8, 4, 2, 1) that still gives about a workgroup per CU
@@ -726,15 +772,16 @@ bool csr_lds_apply_laid(const int64_t *row_ptr, const int32_t *col, const T *val
     const int32_t *startT = (const int32_t *)ws;
     const int32_t *Ec = (const int32_t *)(ws + pl.bStart);
     const T *Ev = (const T *)(ws + pl.bStart + pl.bEc);
+    const uint32_t *Ep = (const uint32_t *)(ws + pl.bStart + pl.bEc + pl.bEv);
     const int nki = (int)pl.nk;
     const dim3 grid((unsigned)((ncb * RW + 7) / 8 * 8));
     const int rw = (int)RW, nc = (int)ncb;
     if (NT == 512)
-        launch_lds_apply_j<T, 512>(J, form, grid, startT, Ec, Ev, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
-                                   crs, ccs, rw, nc, st);
+        launch_lds_apply_j<T, 512>(J, form, grid, startT, Ec, Ev, Ep, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
+                                   crs, ccs, rw, nc, unit, st);
     else
-        launch_lds_apply_j<T, 256>(J, form, grid, startT, Ec, Ev, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
-                                   crs, ccs, rw, nc, st);
+        launch_lds_apply_j<T, 256>(J, form, grid, startT, Ec, Ev, Ep, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
+                                   crs, ccs, rw, nc, unit, st);
     return true;
 }
 
@@ -747,9 +794,9 @@ bool csr_lds_apply(const int64_t *row_ptr, const int32_t *col, const T *val, int
     const int NT = d > 256 ? 512 : 256;
     const int64_t RW = (d + NT - 1) / NT;
     if (n * RW >= (1ll << 30)) return false;
-    if (!csr_lds_layout<T>(row_ptr, col, val, d, m, nnz, ws, ws_bytes, st)) return false;
+    if (!csr_lds_layout<T>(row_ptr, col, val, d, m, nnz, ws, ws_bytes, st, false)) return false;
     return csr_lds_apply_laid<T>(row_ptr, col, val, d, n, m, nnz, alpha, beta, B, ldb, C, crs, ccs, form, ws, ws_bytes,
-                                 st);
+                                 st, false);
 }
 
 template bool csr_build_counting_ws<float>(const int64_t *, const int64_t *, const float *, int64_t, int64_t,
@@ -766,15 +813,15 @@ template bool csr_lds_apply<float>(const int64_t *, const int32_t *, const float
                                    int64_t, float, float, const float *, int64_t, float *, int64_t, int64_t, int,
                                    char *, size_t, hipStream_t);
 template bool csr_lds_layout<float>(const int64_t *, const int32_t *, const float *, int64_t, int64_t, int64_t,
-                                    char *, size_t, hipStream_t);
+                                    char *, size_t, hipStream_t, bool);
 template bool csr_lds_layout<double>(const int64_t *, const int32_t *, const double *, int64_t, int64_t, int64_t,
-                                     char *, size_t, hipStream_t);
+                                     char *, size_t, hipStream_t, bool);
 template bool csr_lds_apply_laid<float>(const int64_t *, const int32_t *, const float *, int64_t, int64_t, int64_t,
                                         int64_t, float, float, const float *, int64_t, float *, int64_t, int64_t, int,
-                                        const char *, size_t, hipStream_t);
+                                        const char *, size_t, hipStream_t, bool);
 template bool csr_lds_apply_laid<double>(const int64_t *, const int32_t *, const double *, int64_t, int64_t, int64_t,
                                          int64_t, double, double, const double *, int64_t, double *, int64_t, int64_t,
-                                         int, const char *, size_t, hipStream_t);
+                                         int, const char *, size_t, hipStream_t, bool);
 template bool csr_lds_apply<double>(const int64_t *, const int32_t *, const double *, int64_t, int64_t, int64_t,
                                     int64_t, double, double, const double *, int64_t, double *, int64_t, int64_t, int,
                                     char *, size_t, hipStream_t);

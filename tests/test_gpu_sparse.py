@@ -721,3 +721,25 @@ def test_lskges_prepared_whole_operator(rb, orc, axis, shape, k, layout, opA, dt
     prof = rb.profile_collect()
     assert "coo_to_csr" not in prof and "csr_lds_apply_kernel" in prof
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("axis,k", [("S", 8), ("L", 40)])
+@pytest.mark.parametrize("layout,opA", [("C", "N"), ("R", "T")])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.75, -0.5)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lskges_prepared_wide(rb, orc, axis, k, layout, opA, alpha, beta, dtype):
+    """Prepared operators applied to 400 columns, so the LDS apply runs 8-column blocks: a SASO's
+    values are all +1 or -1, and its layout carries their signs (no value loads); a LASO's merged
+    duplicates are not, and its values are read.  Both bit-identical to the oracle, in both
+    summation forms, with alpha = 1 and otherwise."""
+    d, m, n = 2000, 20000, 400
+    rb.profile_collect()
+    rb.profile_enable(True)
+    try:
+        got, ref = _lsk(rb, orc, dtype, layout, "N", opA, (d, m), k, axis, d, n, m, 0, 0, alpha, beta, prefill=True)
+        torch.cuda.synchronize()
+    finally:
+        rb.profile_enable(False)
+    prof = rb.profile_collect()
+    assert "coo_to_csr" not in prof and "csr_lds_apply_kernel" in prof
+    assert np.array_equal(got, ref)
