@@ -66,11 +66,14 @@ def _lsk(rb, orc, dtype, layout, opS, opA, Sshape, k, axis, d, n, m, ro, co, alp
     return view(got, d, n, layout, ldb), view(ref, d, n, layout, ldb)
 
 
+@pytest.mark.parametrize("prefill", [False, True])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_lskges_saso_config_full_size(rb, orc, dtype):
-    """S = SparseDist(2000, 100000, vec_nnz 8, Short), key 7; A 100000 x 500 ColMajor Uniform(-1,1)."""
+def test_lskges_saso_config_full_size(rb, orc, dtype, prefill):
+    """S = SparseDist(2000, 100000, vec_nnz 8, Short), key 7; A 100000 x 500 ColMajor Uniform(-1,1).
+    Unfilled (the call samples and sorts the operator), or filled and so prepared (the apply then
+    reads the prepared copy's 16-bit slots, a few rows' chunks spilling past their 12 or 20 slots)."""
     got, ref = _lsk(rb, orc, dtype, "C", "N", "N", (2000, 100000), 8, "S", 2000, 500, 100000, 0, 0, 1.0, 0.0,
-                    prefill=False)
+                    prefill=prefill)
     assert np.array_equal(got, ref)
 
 
