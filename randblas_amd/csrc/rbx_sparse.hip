@@ -876,7 +876,7 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
                           int64_t A_cols, int64_t nnz, const T *vals, const int64_t *rows, const int64_t *cols,
                           int64_t ro_a, int64_t co_a, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
                           hipStream_t st, int kib_force = -1, char *fixed_ws = nullptr, size_t fixed_ws_bytes = 0,
-                          const SparsePrep *prep = nullptr, bool cols_ascend_in_rows = false) {
+                          const SparsePrep *prep = nullptr, bool cols_ascend_in_rows = false, bool unit = false) {
     check_layout(layout);
     check_op(opA);
     check_op(opB);
@@ -966,10 +966,10 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
     // sequential chain, op(B) streamed once through LDS
     if (layout_opB == 'C' && d <= CSRB_MAX_ROWS && nnz <= LDS_APPLY_MAX_MEAN_ROW * d) {
         ProfScope prof("csr_lds_apply_kernel", st);
-        const size_t wsb = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T));
+        const size_t wsb = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T), unit);
         char *lws = (char *)scratch(SCRATCH_SPARSE_G, wsb, st);
         if (csr_lds_apply<T>(row_ptr, ccol, cval, d, n, m, nnz, alpha, beta, B, ldb, C, crs, ccs,
-                             kib ? SPF_KIB : SPF_JKI, lws, wsb, st))
+                             kib ? SPF_KIB : SPF_JKI, lws, wsb, st, unit))
             return;
     }
     {
@@ -1002,6 +1002,7 @@ struct SkopView {
     int64_t nnz;
     const T *vals;
     const int64_t *rows, *cols;
+    bool unit;  // sampled in this call as a SASO: every value is +1 or -1
 };
 
 template <typename T>
@@ -1011,7 +1012,7 @@ static SkopView<T> view_or_sample(const rbx_sparse_skop *S, hipStream_t st) {
     if (S->nnz >= 0) {
         RBX_REQUIRE(S->nnz == 0 || (S->vals && S->rows && S->cols));
         RBX_REQUIRE(S->nnz <= g.full_nnz);
-        return {S->nnz, (const T *)S->vals, S->rows, S->cols};
+        return {S->nnz, (const T *)S->vals, S->rows, S->cols, false};
     }
     const size_t bv = align256((size_t)g.full_nnz * sizeof(T)), bi = align256((size_t)g.full_nnz * 8);
     // The sampled operator lives in the dense-operator slot: LASO sampling
@@ -1020,7 +1021,7 @@ static SkopView<T> view_or_sample(const rbx_sparse_skop *S, hipStream_t st) {
     T *vals = (T *)ws;
     int64_t *rows = (int64_t *)(ws + bv), *cols = (int64_t *)(ws + bv + bi);
     int64_t nnz = fill_sparse_impl<T>(S->dist, vals, rows, cols, S->seed_state, nullptr, st);
-    return {nnz, vals, rows, cols};
+    return {nnz, vals, rows, cols, S->dist.major_axis == 'S'};
 }
 
 // S's prepared copy when it was built from S's current arrays in T on this
@@ -1130,7 +1131,7 @@ static void lskges_impl(char layout, char opS, char opA, int64_t d, int64_t n, i
     SkopView<T> v = view_or_sample<T>(S, st);
     const SparsePrep *prep = S->nnz >= 0 ? usable_prep<T>(S, opS, d, m, ro_s, co_s) : nullptr;
     left_spmm_coo<T>(layout, opS, opA, d, n, m, alpha, S->dist.n_rows, S->dist.n_cols, v.nnz, v.vals, v.rows, v.cols,
-                     ro_s, co_s, A, lda, beta, B, ldb, st, -1, nullptr, 0, prep);
+                     ro_s, co_s, A, lda, beta, B, ldb, st, -1, nullptr, 0, prep, false, v.unit);
 }
 
 // rskges -> right_spmm -> left_spmm with flipped layout and op (spmm_dispatch.hh:181-219)
@@ -1144,7 +1145,7 @@ static void rskges_impl(char layout, char opA, char opS, int64_t m, int64_t d, i
     char trans_opS = opS == 'N' ? 'T' : 'N';
     char trans_layout = layout == 'C' ? 'R' : 'C';
     left_spmm_coo<T>(trans_layout, trans_opS, opA, d, m, n, alpha, S->dist.n_rows, S->dist.n_cols, v.nnz, v.vals,
-                     v.rows, v.cols, ro_s, co_s, A, lda, beta, B, ldb, st);
+                     v.rows, v.cols, ro_s, co_s, A, lda, beta, B, ldb, st, -1, nullptr, 0, nullptr, false, v.unit);
 }
 
 // ---------------------------------------------------------------- CSR data x dense operator

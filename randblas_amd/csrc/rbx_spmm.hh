@@ -66,6 +66,6 @@ bool csr_lds_apply_laid(const int64_t *row_ptr, const int32_t *col, const T *val
 template <typename T>
 bool csr_lds_apply(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
                    int64_t nnz, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int form,
-                   char *ws, size_t ws_bytes, hipStream_t st);
+                   char *ws, size_t ws_bytes, hipStream_t st, bool unit = false);
 
 }  // namespace rbx

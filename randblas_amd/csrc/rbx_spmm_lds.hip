@@ -788,15 +788,15 @@ bool csr_lds_apply_laid(const int64_t *row_ptr, const int32_t *col, const T *val
 template <typename T>
 bool csr_lds_apply(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
                    int64_t nnz, T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int form,
-                   char *ws, size_t ws_bytes, hipStream_t st) {
+                   char *ws, size_t ws_bytes, hipStream_t st, bool unit) {
     if (d <= 0 || n <= 0 || m <= 0 || ws == nullptr) return false;
     // every check of the apply before anything is launched
     const int NT = d > 256 ? 512 : 256;
     const int64_t RW = (d + NT - 1) / NT;
     if (n * RW >= (1ll << 30)) return false;
-    if (!csr_lds_layout<T>(row_ptr, col, val, d, m, nnz, ws, ws_bytes, st, false)) return false;
+    if (!csr_lds_layout<T>(row_ptr, col, val, d, m, nnz, ws, ws_bytes, st, unit)) return false;
     return csr_lds_apply_laid<T>(row_ptr, col, val, d, n, m, nnz, alpha, beta, B, ldb, C, crs, ccs, form, ws, ws_bytes,
-                                 st, false);
+                                 st, unit);
 }
 
 template bool csr_build_counting_ws<float>(const int64_t *, const int64_t *, const float *, int64_t, int64_t,
@@ -811,7 +811,7 @@ template bool csr_build_counting<double>(const int64_t *, const int64_t *, const
                                          int64_t, int64_t, int64_t **, int32_t **, double **, hipStream_t);
 template bool csr_lds_apply<float>(const int64_t *, const int32_t *, const float *, int64_t, int64_t, int64_t,
                                    int64_t, float, float, const float *, int64_t, float *, int64_t, int64_t, int,
-                                   char *, size_t, hipStream_t);
+                                   char *, size_t, hipStream_t, bool);
 template bool csr_lds_layout<float>(const int64_t *, const int32_t *, const float *, int64_t, int64_t, int64_t,
                                     char *, size_t, hipStream_t, bool);
 template bool csr_lds_layout<double>(const int64_t *, const int32_t *, const double *, int64_t, int64_t, int64_t,
@@ -824,6 +824,6 @@ template bool csr_lds_apply_laid<double>(const int64_t *, const int32_t *, const
                                          int, const char *, size_t, hipStream_t, bool);
 template bool csr_lds_apply<double>(const int64_t *, const int32_t *, const double *, int64_t, int64_t, int64_t,
                                     int64_t, double, double, const double *, int64_t, double *, int64_t, int64_t, int,
-                                    char *, size_t, hipStream_t);
+                                    char *, size_t, hipStream_t, bool);
 
 }  // namespace rbx
