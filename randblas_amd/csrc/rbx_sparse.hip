@@ -966,7 +966,7 @@ static void left_spmm_coo(char layout, char opA, char opB, int64_t d, int64_t n,
     // sequential chain, op(B) streamed once through LDS
     if (layout_opB == 'C' && d <= CSRB_MAX_ROWS && nnz <= LDS_APPLY_MAX_MEAN_ROW * d) {
         ProfScope prof("csr_lds_apply_kernel", st);
-        const size_t wsb = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T), unit);
+        const size_t wsb = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T));
         char *lws = (char *)scratch(SCRATCH_SPARSE_G, wsb, st);
         if (csr_lds_apply<T>(row_ptr, ccol, cval, d, n, m, nnz, alpha, beta, B, ldb, C, crs, ccs,
                              kib ? SPF_KIB : SPF_JKI, lws, wsb, st, unit))
@@ -1083,7 +1083,7 @@ static void prepare_skop(rbx_sparse_skop *S, hipStream_t st) {
         RBX_HIP(hipStreamSynchronize(st));
         unit = h == 0;
     }
-    const size_t lay = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T), unit);
+    const size_t lay = csr_lds_apply_ws_bytes(d, m, nnz, sizeof(T));
     const size_t bR = align256((size_t)(d + 1) * 8), bC = align256((size_t)nnz * 4);
     const size_t bV = align256((size_t)nnz * sizeof(T));
     const size_t total = bR + bC + bV + lay;

@@ -49,7 +49,7 @@ bool csr_build_counting(const int64_t *rows, const int64_t *cols, const T *vals,
 // entries laid out by chunk and slot; every byte read is written first in the
 // same call).  Returns false when the shape is outside what the kernel takes
 // (alpha == 0 and beta-only scaling are the caller's).
-size_t csr_lds_apply_ws_bytes(int64_t d, int64_t m, int64_t nnz, size_t val_size, bool unit = false);
+size_t csr_lds_apply_ws_bytes(int64_t d, int64_t m, int64_t nnz, size_t val_size);
 // csr_lds_apply in two steps, for an operator applied more than once: the
 // entry layout (chunk table and slots, depending on d, m and the entries only)
 // into `ws`, then any number of applies that read it.  Each returns false,

@@ -361,9 +361,11 @@ bool csr_build_counting(const int64_t *rows, const int64_t *cols, const T *vals,
 // streams those J columns of the dense operand through LDS in CK-deep chunks
 // (double-buffered: the next chunk's loads are in flight while the current one
 // is applied).  The stored entries are first laid out by chunk and slot:
-// Ec / Ev[(k * MAXE + q) * d + r] is row r's q-th entry among those with a
-// column in chunk k, and startT[k * d + r] where the row's entries of chunk k
-// start in the CSR.  A wave's load of slot q then reads 64 consecutive rows
+// slot q of row r in chunk k is the row's q-th entry among those with a
+// column in chunk k (its value at Ev[(k * MAXE + q) * d + r], its column
+// within the chunk in 16 bits, two slots per 32-bit word of Ep), and
+// startT[k * d + r] where the row's entries of chunk k start in the CSR.  A
+// wave's load of a slot word then reads 64 consecutive rows
 // (one coalesced request instead of 64 lines), and each thread loads the next
 // chunk's slots into a second register bank while it applies the current
 // chunk's; slots past the wave's largest count are skipped, and a row with
@@ -392,23 +394,21 @@ static int la_chunk(int64_t d, int64_t m, int64_t nnz, size_t val_size) {
 struct LaPlan {
     int ck;
     int64_t nk;
-    size_t bStart, bEc, bEv, bEp;
-    size_t bytes() const { return bStart + bEc + bEv + bEp; }
+    size_t bStart, bEv, bEp;
+    size_t bytes() const { return bStart + bEv + bEp; }
 };
-// `unit`: the layout also holds the 16-bit slots (the last region)
-static LaPlan la_plan(int64_t d, int64_t m, int64_t nnz, size_t val_size, bool unit) {
+static LaPlan la_plan(int64_t d, int64_t m, int64_t nnz, size_t val_size) {
     LaPlan p;
     p.ck = la_chunk(d, m, nnz, val_size);
     p.nk = (m + p.ck - 1) / p.ck;
     p.bStart = al256((size_t)(p.nk + 1) * (size_t)d * 4);
-    p.bEc = al256((size_t)p.nk * la_maxe(val_size) * (size_t)d * 4);
     p.bEv = al256((size_t)p.nk * la_maxe(val_size) * (size_t)d * val_size);
-    p.bEp = unit ? al256((size_t)p.nk * la_maxe(val_size) * (size_t)d * 2) : 0;
+    p.bEp = al256((size_t)p.nk * la_maxe(val_size) * (size_t)d * 2);
     return p;
 }
 
-size_t csr_lds_apply_ws_bytes(int64_t d, int64_t m, int64_t nnz, size_t val_size, bool unit) {
-    return la_plan(d, m, nnz, val_size, unit).bytes();
+size_t csr_lds_apply_ws_bytes(int64_t d, int64_t m, int64_t nnz, size_t val_size) {
+    return la_plan(d, m, nnz, val_size).bytes();
 }
 
 // startT[k * d + r] = position of row r's first entry with column >= k*CK
@@ -434,17 +434,17 @@ __global__ __launch_bounds__(256) void csr_chunk_table_kernel(const int64_t *row
 }
 
 // One workgroup per row (grid-stride): each entry p of row r goes to slot
-// q = p - startT[k * d + r] of its chunk k when q < la_maxe(sizeof(T)).
-// With `unit` (every value is +1 or -1) bit 31 of the slot's column holds
-// the value's sign, and the slot is also written in 16 bits, the column
-// within the chunk (below 2048) with the sign in bit 15, two slots of a row
-// to a 32-bit word (Ep[(k * MAXE / 2 + q / 2) * d + r], slot q in half q % 2),
-// so the applies that know this read one word per two slots and no values.
+// q = p - startT[k * d + r] of its chunk k when q < la_maxe(sizeof(T)): its
+// value to Ev[(k * MAXE + q) * d + r], its column within the chunk (below
+// 2048) to 16 bits, two slots of a row per 32-bit word (Ep[(k * MAXE / 2 +
+// q / 2) * d + r], slot q in half q % 2).  With `unit` (every value is +1 or
+// -1) bit 15 holds the value's sign, so the applies that know this read one
+// word per two slots and no values.
 template <typename T>
 __global__ __launch_bounds__(256) void csr_chunk_slots_kernel(const int64_t *row_ptr, const int32_t *col,
                                                               const T *val, int64_t d, int CK,
-                                                              const int32_t *startT, int32_t *Ec, T *Ev,
-                                                              uint16_t *Ep, int unit) {
+                                                              const int32_t *startT, T *Ev, uint16_t *Ep,
+                                                              int unit) {
     for (int64_t r = blockIdx.x; r < d; r += gridDim.x) {
         const int64_t p1 = row_ptr[r + 1];
         for (int64_t p = row_ptr[r] + threadIdx.x; p < p1; p += blockDim.x) {
@@ -455,11 +455,9 @@ __global__ __launch_bounds__(256) void csr_chunk_slots_kernel(const int64_t *row
             if (q < MAXE) {
                 const int64_t o = (k * MAXE + q) * d + r;
                 const T v = val[p];
-                Ec[o] = (unit && v < (T)0) ? (int32_t)((uint32_t)c | 0x80000000u) : c;
                 Ev[o] = v;
-                if (unit)
-                    Ep[2 * ((k * (MAXE / 2) + q / 2) * d + r) + (q & 1)] =
-                        (uint16_t)((c - k * CK) | (v < (T)0 ? 0x8000 : 0));
+                Ep[2 * ((k * (MAXE / 2) + q / 2) * d + r) + (q & 1)] =
+                    (uint16_t)((c - k * CK) | ((unit && v < (T)0) ? 0x8000 : 0));
             }
         }
     }
@@ -480,14 +478,13 @@ __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint3
 }
 
 // A1: alpha == 1, so the products by alpha (exact then) are left out.
-// UNIT: every value is +1 or -1 (a layout built with `unit`): the slots come
-// from the 16-bit words Ep, two per load, column within the chunk and sign,
-// and no values are loaded; otherwise bit 31 of the slot's column in Ec is
-// masked off (a unit layout serves both).
+// Slots come from the 16-bit words Ep, two per load (column within the
+// chunk).  UNIT: every value is +1 or -1 (a layout built with `unit`), its
+// sign in bit 15, and no values are loaded; otherwise the values come from
+// Ev and bit 15 is masked off (a unit layout serves both).
 template <typename T, int J, int NT, int FORM, bool A1, bool UNIT>
-__global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT, const int32_t *Ec, const T *Ev,
-                                                           const uint32_t *Ep, int nk, int CK, const int32_t *col,
-                                                           const T *val,
+__global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT, const T *Ev, const uint32_t *Ep,
+                                                           int nk, int CK, const int32_t *col, const T *val,
                                                            int64_t d, int64_t n, int64_t m, T alpha, T beta,
                                                            const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs,
                                                            int RW, int ncb) {
@@ -534,23 +531,18 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
         return active ? v : 0;
     };
 
-    // register banks of slots: chunk k in bank k % 2 (UNIT: two 16-bit slots
-    // per word in the first LA_MAXE / 2 entries of bc)
-    int32_t bc0[LA_MAXE], bc1[LA_MAXE];
+    // register banks of slots: chunk k in bank k % 2, two 16-bit slots per
+    // word of bc
+    uint32_t bc0[LA_MAXE / 2], bc1[LA_MAXE / 2];
     T bv0[LA_MAXE], bv1[LA_MAXE];
     // every slot is loaded (a fixed number of loads, so the waits before the
     // tile stores cover only the tile's own, older loads)
-    auto load_bank = [&](int64_t k, int32_t *bc, T *bv) {
-        if constexpr (UNIT) {
+    auto load_bank = [&](int64_t k, uint32_t *bc, T *bv) {
 #pragma unroll
-            for (int q = 0; q < LA_MAXE / 2; ++q) bc[q] = (int32_t)Ep[(k * (LA_MAXE / 2) + q) * d + rs];
-        } else {
+        for (int q = 0; q < LA_MAXE / 2; ++q) bc[q] = Ep[(k * (LA_MAXE / 2) + q) * d + rs];
+        if constexpr (!UNIT) {
 #pragma unroll
-            for (int q = 0; q < LA_MAXE; ++q) {
-                const int64_t o = (k * LA_MAXE + q) * d + rs;
-                bc[q] = Ec[o];
-                bv[q] = Ev[o];
-            }
+            for (int q = 0; q < LA_MAXE; ++q) bv[q] = Ev[(k * LA_MAXE + q) * d + rs];
         }
     };
     auto fma_one = [&](int jj, T b, T v) {
@@ -577,19 +569,16 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
             for (int jj = 0; jj < J; ++jj) fma_one(jj, Ab[jj * LA_CKMAX + c], v);
         }
     };
-    auto apply = [&](int64_t k, const int32_t *bc, const T *bv, int32_t s, int32_t e, int32_t wmax) {
+    auto apply = [&](int64_t k, const uint32_t *bc, const T *bv, int32_t s, int32_t e, int32_t wmax) {
         const int32_t cbase = (int32_t)(k * CK);
         const T *Ab = As[k & 1];
         const int32_t cnt = e - s;
 #pragma unroll
         for (int q = 0; q < LA_MAXE; ++q)
             if (q < wmax && q < cnt) {
-                if constexpr (UNIT) {
-                    const uint32_t e = ((uint32_t)bc[q >> 1] >> (16 * (q & 1))) & 0xffffu;
-                    apply_one(Ab, (int32_t)(e & 0x7fffu), (e & 0x8000u) ? (T)-1 : (T)1);
-                } else {
-                    apply_one(Ab, (int32_t)((uint32_t)bc[q] & 0x7fffffffu) - cbase, bv[q]);
-                }
+                const uint32_t e = (bc[q >> 1] >> (16 * (q & 1))) & 0xffffu;
+                if constexpr (UNIT) apply_one(Ab, (int32_t)(e & 0x7fffu), (e & 0x8000u) ? (T)-1 : (T)1);
+                else apply_one(Ab, (int32_t)(e & 0x7fffu), bv[q]);
             }
         for (int32_t p = s + LA_MAXE; p < e; ++p) apply_one(Ab, col[p] - cbase, val[p]);
     };
@@ -680,14 +669,13 @@ __global__ __launch_bounds__(NT) void csr_lds_apply_kernel(const int32_t *startT
 }
 
 template <typename T, int J, int NT>
-static void launch_lds_apply(int form, dim3 grid, const int32_t *startT, const int32_t *Ec, const T *Ev,
-                             const uint32_t *Ep, int nk,
+static void launch_lds_apply(int form, dim3 grid, const int32_t *startT, const T *Ev, const uint32_t *Ep, int nk,
                              int ck, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m, T alpha,
                              T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int RW, int ncb,
                              bool unit, hipStream_t st) {
 #define RBX_LA_LAUNCH(F, A1, U)                                                                                   \
-    hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, NT, F, A1, U>), grid, dim3(NT), 0, st, startT, Ec, Ev, Ep, nk, ck, \
-                       col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb)
+    hipLaunchKernelGGL((csr_lds_apply_kernel<T, J, NT, F, A1, U>), grid, dim3(NT), 0, st, startT, Ev, Ep, nk, ck, col, \
+                       val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb)
     const bool one = alpha == (T)1;
     // the value-free variant only for the widest column blocks (the others
     // read the values of a unit layout like any other)
@@ -711,12 +699,12 @@ static void launch_lds_apply(int form, dim3 grid, const int32_t *startT, const i
 }
 
 template <typename T, int NT>
-static void launch_lds_apply_j(int J, int form, dim3 grid, const int32_t *startT, const int32_t *Ec, const T *Ev,
-                               const uint32_t *Ep, int nk, int ck, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
+static void launch_lds_apply_j(int J, int form, dim3 grid, const int32_t *startT, const T *Ev, const uint32_t *Ep,
+                               int nk, int ck, const int32_t *col, const T *val, int64_t d, int64_t n, int64_t m,
                                T alpha, T beta, const T *B, int64_t ldb, T *C, int64_t crs, int64_t ccs, int RW,
                                int ncb, bool unit, hipStream_t st) {
 #define RBX_LA_ARGS \
-    form, grid, startT, Ec, Ev, Ep, nk, ck, col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb, unit, st
+    form, grid, startT, Ev, Ep, nk, ck, col, val, d, n, m, alpha, beta, B, ldb, C, crs, ccs, RW, ncb, unit, st
     if (J == 8) launch_lds_apply<T, 8, NT>(RBX_LA_ARGS);
     else if (J == 4) launch_lds_apply<T, 4, NT>(RBX_LA_ARGS);
     else if (J == 2) launch_lds_apply<T, 2, NT>(RBX_LA_ARGS);
@@ -733,13 +721,12 @@ template <typename T>
 bool csr_lds_layout(const int64_t *row_ptr, const int32_t *col, const T *val, int64_t d, int64_t m, int64_t nnz,
                     char *ws, size_t ws_bytes, hipStream_t st, bool unit) {
     if (!lds_apply_fits(d, m, sizeof(T)) || ws == nullptr) return false;
-    const LaPlan pl = la_plan(d, m, nnz, sizeof(T), unit);
+    const LaPlan pl = la_plan(d, m, nnz, sizeof(T));
     RBX_REQUIRE(ws_bytes >= pl.bytes());
     if (pl.nk >= (1ll << 30)) return false;
     int32_t *startT = (int32_t *)ws;
-    int32_t *Ec = (int32_t *)(ws + pl.bStart);
-    T *Ev = (T *)(ws + pl.bStart + pl.bEc);
-    uint16_t *Ep = (uint16_t *)(ws + pl.bStart + pl.bEc + pl.bEv);
+    T *Ev = (T *)(ws + pl.bStart);
+    uint16_t *Ep = (uint16_t *)(ws + pl.bStart + pl.bEv);
     const int64_t tcells = d * (pl.nk + 1);
     hipLaunchKernelGGL(csr_chunk_table_kernel, dim3((unsigned)((tcells + 255) / 256)), dim3(256), 0, st, row_ptr, col,
                        d, (int)pl.nk, pl.ck, startT);
@@ -747,7 +734,7 @@ bool csr_lds_layout(const int64_t *row_ptr, const int32_t *col, const T *val, in
     const int64_t cus = num_cus();
     const int64_t sg = d < 4 * cus ? d : 4 * cus;
     hipLaunchKernelGGL((csr_chunk_slots_kernel<T>), dim3((unsigned)sg), dim3(256), 0, st, row_ptr, col, val, d, pl.ck,
-                       (const int32_t *)startT, Ec, Ev, Ep, (int)unit);
+                       (const int32_t *)startT, Ev, Ep, (int)unit);
     RBX_HIP(hipGetLastError());
     return true;
 }
@@ -758,7 +745,7 @@ bool csr_lds_apply_laid(const int64_t *row_ptr, const int32_t *col, const T *val
                         int form, const char *ws, size_t ws_bytes, hipStream_t st, bool unit) {
     (void)row_ptr;
     if (n <= 0 || !lds_apply_fits(d, m, sizeof(T)) || ws == nullptr) return false;
-    const LaPlan pl = la_plan(d, m, nnz, sizeof(T), unit);
+    const LaPlan pl = la_plan(d, m, nnz, sizeof(T));
     RBX_REQUIRE(ws_bytes >= pl.bytes());
     // 512 rows per workgroup (256 for few rows), the widest column block
     // (8, 4, 2, 1) that still gives about a workgroup per CU
@@ -770,17 +757,16 @@ bool csr_lds_apply_laid(const int64_t *row_ptr, const int32_t *col, const T *val
     const int64_t ncb = (n + J - 1) / J;
     if (ncb * RW >= (1ll << 30) || pl.nk >= (1ll << 30)) return false;
     const int32_t *startT = (const int32_t *)ws;
-    const int32_t *Ec = (const int32_t *)(ws + pl.bStart);
-    const T *Ev = (const T *)(ws + pl.bStart + pl.bEc);
-    const uint32_t *Ep = (const uint32_t *)(ws + pl.bStart + pl.bEc + pl.bEv);
+    const T *Ev = (const T *)(ws + pl.bStart);
+    const uint32_t *Ep = (const uint32_t *)(ws + pl.bStart + pl.bEv);
     const int nki = (int)pl.nk;
     const dim3 grid((unsigned)((ncb * RW + 7) / 8 * 8));
     const int rw = (int)RW, nc = (int)ncb;
     if (NT == 512)
-        launch_lds_apply_j<T, 512>(J, form, grid, startT, Ec, Ev, Ep, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
+        launch_lds_apply_j<T, 512>(J, form, grid, startT, Ev, Ep, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
                                    crs, ccs, rw, nc, unit, st);
     else
-        launch_lds_apply_j<T, 256>(J, form, grid, startT, Ec, Ev, Ep, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
+        launch_lds_apply_j<T, 256>(J, form, grid, startT, Ev, Ep, nki, pl.ck, col, val, d, n, m, alpha, beta, B, ldb, C,
                                    crs, ccs, rw, nc, unit, st);
     return true;
 }
