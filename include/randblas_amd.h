@@ -358,7 +358,7 @@ rbx_status rbx_drskges(char layout, char opA, char opS, int64_t m, int64_t d, in
  * 2048 stored entries per sketch row on average, and a workspace (the
  * operator window's CSR and its chunked layout) that fits the library's fixed
  * per-GPU scratch array of rbx_lskges_noalloc_scratch_bytes() bytes (64 MiB;
- * the 2000 x 100000, vec_nnz 8 SASO operator needs 53 MB in FP64); larger or
+ * the 2000 x 100000, vec_nnz 8 SASO operator needs 48 MB in FP64); larger or
  * other shapes return RBX_INVALID_ARG or RBX_UNSUPPORTED with a message and
  * launch nothing.  The array is one per GPU and shared by every call on that
  * GPU: calls whose execution overlaps on one GPU (other streams, other host
